@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""Benchmark: BFS GTEPS on RMAT-24 (scale 24, edge factor 16, permuted ids),
+16 seeded non-isolated roots per step, plain (single-source) method on one GPU.
+
+One step = BFS from the 16 roots, inputs (CSR/CSC) resident in HBM.
+value = sum over roots of m_reach / BFS time (Graph500-style GTEPS), where
+m_reach = sum of out-degrees of the reached vertices.
+
+Also reported on the same graph (not part of `value`): PageRank (20
+iterations, FP64) GTEPS and HBM roofline fraction, and CC on the symmetrised
+graph.  The CPU baseline is the oracle (C/OpenMP restatement of the
+reference's engine) on a bounded sample of the same workload, timed on this
+host's cores, with its depths checked against the GPU's.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def pick_roots(out_degree, k, seed):
+    cand = np.nonzero(out_degree)[0]
+    rng = np.random.default_rng(seed)
+    return [int(x) for x in rng.choice(cand, size=min(k, len(cand)), replace=False)]
+
+
+def bfs_model_bytes(st, V):
+    """Algorithmic bytes of each process_edges call of one BFS:
+    push: 20F + 8X + 8N  (queue entry, offset pair, col, state probe, writes)
+    pull: 16U + 4X + 8N + 3V/8  (offsets of unvisited, col until exit, writes,
+          visited / frontier-in / frontier-out bitmaps)."""
+    push = pull = 0
+    for i, mode in enumerate(st["modes"]):
+        F, N, X = st["it_active"][i], st["it_new"][i], st["it_examined"][i]
+        U = st["it_unvisited"][i]
+        if mode == 0:
+            push += 20 * F + 8 * X + 8 * N
+        else:
+            pull += 16 * U + 4 * X + 8 * N + 3 * V / 8
+    return push, pull
+
+
+def cpu_baseline(scale, ef, seed, roots, gpu_depths, sample_roots):
+    import oracle
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(int(threads), 64))
+    oracle.set_threads(threads)
+    t0 = time.time()
+    e = oracle.rmat_edges(scale, ef, seed=seed, permute=True)
+    og = oracle.Graph(e, 1 << scale)
+    del e
+    log(f"cpu oracle graph ready in {time.time() - t0:.1f}s ({threads} threads)")
+    od = og.out_degree().astype(np.int64)
+    total_m, total_t, parity = 0, 0.0, True
+    for i, r in enumerate(roots[:sample_roots]):
+        t = oracle.wall_time()
+        _, depth, _ = og.bfs(r)
+        dt = oracle.wall_time() - t
+        reached = depth != 0xFFFFFFFF
+        total_m += int(od[reached].sum())
+        total_t += dt
+        if gpu_depths is not None and i < len(gpu_depths):
+            parity = parity and bool(np.array_equal(depth, gpu_depths[i]))
+        log(f"cpu bfs root {r}: {dt:.2f}s")
+    return {
+        "value": total_m / total_t / 1e9,
+        "unit": "GTEPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"BFS from the first {min(sample_roots, len(roots))} of the 16 roots on the "
+                  f"same RMAT-{scale} graph, compute() only (graph build excluded, as the "
+                  f"reference's exec_time excludes load)",
+    }, parity
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--roots", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=24)
+    ap.add_argument("--cpu-sample-roots", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(one partition per GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import geminigraph_amd as gg
+    gg.init(0)
+    scale, ef = args.scale, args.edge_factor
+    V, E = 1 << scale, (1 << scale) * ef
+
+    t0 = time.time()
+    g = gg.Graph.rmat(scale, ef, seed=args.seed, permute=True)
+    gg.synchronize()
+    build_s = time.time() - t0
+    log(f"RMAT-{scale} ef{ef}: V={V} E={E} built on device in {build_s:.2f}s")
+    roots = pick_roots(g.out_degree(), args.roots, args.seed + 1)
+
+    parent = gg.DeviceArray(V, np.uint32)
+    depth = gg.DeviceArray(V, np.uint32)
+
+    def step():
+        m = 0
+        for r in roots:
+            m += g.bfs_into(r, parent, depth).m_reach
+        return m
+
+    for _ in range(args.warmup):
+        step()
+    gg.synchronize()
+    gg.prof_reset()
+    gg.prof_enable(True)
+    t = time.perf_counter()
+    m_total = 0
+    for _ in range(args.steps):
+        m_total += step()
+    gg.synchronize()
+    elapsed = time.perf_counter() - t
+    gg.prof_enable(False)
+    kernels = {k: gg.prof_read(k) for k in ("bfs_pull", "bfs_pull_heavy", "push", "compact")}
+    gteps = m_total / elapsed / 1e9
+    ms_per_step = elapsed * 1e3 / args.steps
+    log(f"BFS x{len(roots)}: {ms_per_step:.2f} ms/step, {gteps:.2f} GTEPS; kernels {kernels}")
+
+    # one instrumented pass (edge counters on) for the algorithmic byte model
+    gpu_depths = []
+    push_b = pull_b = 0.0
+    with gg.options(count_examined=1):
+        for r in roots:
+            st = g.bfs_into(r, parent, depth).as_dict()
+            pb, lb = bfs_model_bytes(st, V)
+            push_b += pb
+            pull_b += lb
+            if len(gpu_depths) < args.cpu_sample_roots:
+                gpu_depths.append(depth.numpy())
+    pull_ms = kernels["bfs_pull"][0] + kernels["bfs_pull_heavy"][0]
+    pull_launches = kernels["bfs_pull"][1]
+    achieved = (pull_b * args.steps) / (pull_ms / 1e3) / 1e9 if pull_ms > 0 else 0.0
+    roofline = {
+        "bound": "hbm",
+        "kernel": "bfs_pull_group_kernel + bfs_pull_heavy_kernel (dense BFS step)",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "bytes_per_launch": round(pull_b / max(1, pull_launches / args.steps), 1),
+        "avg_launch_ms": round(kernels["bfs_pull"][0] / max(1, pull_launches), 4),
+    }
+
+    extras = {"build_s": round(build_s, 2)}
+    if not args.no_extras:
+        pr = gg.DeviceArray(V, np.float64)
+        g.pagerank_into(pr, 20)
+        gg.prof_reset()
+        gg.prof_enable(True)
+        runs = 3
+        t = time.perf_counter()
+        for _ in range(runs):
+            g.pagerank_into(pr, 20)
+        gg.synchronize()
+        pr_s = (time.perf_counter() - t) / runs
+        gg.prof_enable(False)
+        prk = {k: gg.prof_read(k) for k in ("pr_pull", "pr_heavy")}
+        pr_kernel_s = (prk["pr_pull"][0] + prk["pr_heavy"][0]) / 1e3 / runs
+        b_iter = 12 * E + 28 * V
+        extras["pagerank"] = {
+            "iterations": 20,
+            "ms": round(pr_s * 1e3, 3),
+            "gteps": round(20 * E / pr_s / 1e9, 2),
+            "ms_per_iter": round(pr_s * 1e3 / 20, 4),
+            "model_bytes_per_iter": b_iter,
+            "achieved_GBs": round(20 * b_iter / pr_kernel_s / 1e9, 1),
+            "frac": round(20 * b_iter / pr_kernel_s / 1e9 / HBM_PEAK_GBS, 4),
+        }
+        log(f"PageRank: {extras['pagerank']}")
+        pr.free()
+        del g
+        parent.free()
+        depth.free()
+        gu = gg.Graph.rmat(scale, ef, seed=args.seed, permute=True, undirected=True)
+        lab = gg.DeviceArray(V, np.uint32)
+        gu.cc_into(lab)
+        t = time.perf_counter()
+        st = gu.cc_into(lab)
+        gg.synchronize()
+        cc_s = time.perf_counter() - t
+        extras["cc"] = {"ms": round(cc_s * 1e3, 3), "iterations": st.iterations,
+                        "push_iterations": st.push_iterations,
+                        "gteps_2E": round(2 * E / cc_s / 1e9, 2)}
+        log(f"CC: {extras['cc']}")
+        del gu
+        lab.free()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu, parity = cpu_baseline(scale, ef, args.seed, roots, gpu_depths,
+                                   args.cpu_sample_roots)
+        extras["depth_parity_vs_oracle"] = parity
+        log(f"CPU baseline: {cpu}, parity={parity}")
+
+    out = {
+        "metric": "BFS GTEPS (RMAT-24, 16 roots, plain method)",
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"BFS from {len(roots)} seeded non-isolated roots, RMAT scale "
+                               f"{scale} edge factor {ef} (a=0.57 b=0.19 c=0.19), permuted ids",
+                   "vertices": V, "edges": E, "roots": len(roots), "seed": args.seed},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "extras": extras,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

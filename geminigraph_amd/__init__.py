@@ -1,0 +1,485 @@
+"""MI355X-native Gemini edge-processing engine: Python host side.
+
+This package mirrors the reference's host interface for one partition on one
+GPU (``Graph<EdgeData>`` of core/graph.hpp and the toolkits'
+``compute()``), calling the HIP kernels in ``libgg.so`` through the C ABI
+declared in ``include/gg.h``.  Nothing here computes on the CPU: if the
+library is missing or no GPU is present, the calls raise.
+
+    import geminigraph_amd as gg
+    g = gg.Graph.load_directed("graph.bin", vertices)       # core/graph.hpp:772
+    parent, depth, frontier, stats = g.bfs(root)             # toolkits/bfs.cpp
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgg.so")
+_lib = None
+
+GG_OK = 0
+RMAT_A, RMAT_B, RMAT_C = 0.57, 0.19, 0.19
+
+
+class GGError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"gg error {code}: {msg}")
+        self.code = code
+
+
+def build():
+    """Compile libgg.so for gfx950 (hipcc; runs without a GPU)."""
+    subprocess.run(["make", "-s", "-B", "-j4", "-C", _HERE], check=True)
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int),
+        ("push_iterations", ctypes.c_int),
+        ("edges_examined", ctypes.c_uint64),
+        ("m_reach", ctypes.c_uint64),
+        ("reached", ctypes.c_uint64),
+        ("device_ms", ctypes.c_double),
+        ("modes", ctypes.c_uint32 * 64),
+        ("it_active", ctypes.c_uint32 * 64),
+        ("it_new", ctypes.c_uint32 * 64),
+        ("it_examined", ctypes.c_uint64 * 64),
+        ("it_unvisited", ctypes.c_uint64 * 64),
+    ]
+
+    def as_dict(self):
+        n = min(self.iterations, 64)
+        return {
+            "iterations": self.iterations,
+            "push_iterations": self.push_iterations,
+            "edges_examined": int(self.edges_examined),
+            "m_reach": int(self.m_reach),
+            "reached": int(self.reached),
+            "device_ms": float(self.device_ms),
+            "modes": [int(x) for x in self.modes[:n]],
+            "it_active": [int(x) for x in self.it_active[:n]],
+            "it_new": [int(x) for x in self.it_new[:n]],
+            "it_examined": [int(x) for x in self.it_examined[:n]],
+            "it_unvisited": [int(x) for x in self.it_unvisited[:n]],
+        }
+
+
+class GraphInfo(ctypes.Structure):
+    _fields_ = [
+        ("vertices", ctypes.c_uint32),
+        ("edges", ctypes.c_uint64),
+        ("arcs", ctypes.c_uint64),
+        ("weighted", ctypes.c_int),
+        ("symmetric", ctypes.c_int),
+        ("transposed", ctypes.c_int),
+        ("device_bytes", ctypes.c_size_t),
+        ("d_out_degree", ctypes.c_void_p),
+        ("d_in_degree", ctypes.c_void_p),
+        ("d_out_index", ctypes.c_void_p),
+        ("d_out_nbr", ctypes.c_void_p),
+        ("d_out_w", ctypes.c_void_p),
+        ("d_in_index", ctypes.c_void_p),
+        ("d_in_nbr", ctypes.c_void_p),
+        ("d_in_w", ctypes.c_void_p),
+    ]
+
+
+# C ABI signatures: name -> (restype, argtypes)
+_vp, _u32, _u64, _i32, _i64 = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
+                               ctypes.c_int64)
+_dbl, _sz, _cp = ctypes.c_double, ctypes.c_size_t, ctypes.c_char_p
+_pp = ctypes.POINTER(ctypes.c_void_p)
+_stats = ctypes.POINTER(RunStats)
+SIGNATURES = {
+    "gg_last_error": (_cp, []),
+    "gg_init": (_i32, [_i32]),
+    "gg_synchronize": (_i32, []),
+    "gg_malloc": (_i32, [_pp, _sz]),
+    "gg_free": (_i32, [_vp]),
+    "gg_memcpy_d2h": (_i32, [_vp, _vp, _sz]),
+    "gg_memcpy_h2d": (_i32, [_vp, _vp, _sz]),
+    "gg_memcpy_d2d": (_i32, [_vp, _vp, _sz]),
+    "gg_mem_info": (_i32, [ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
+    "gg_set_option": (_i32, [_cp, _i64]),
+    "gg_get_option": (_i32, [_cp, ctypes.POINTER(_i64)]),
+    "gg_rmat_generate": (_i32, [_vp, _u64, _i32, _dbl, _dbl, _dbl, _u64, _i32, _i32]),
+    "gg_graph_build": (_i32, [_vp, _i32, _u64, _u32, _i32, _i32, _pp]),
+    "gg_graph_load": (_i32, [_cp, _u32, _i32, _i32, _pp]),
+    "gg_graph_destroy": (_i32, [_vp]),
+    "gg_graph_transpose": (_i32, [_vp]),
+    "gg_graph_get_info": (_i32, [_vp, ctypes.POINTER(GraphInfo)]),
+    "gg_subset_create": (_i32, [_vp, _pp]),
+    "gg_subset_destroy": (_i32, [_vp]),
+    "gg_subset_clear": (_i32, [_vp]),
+    "gg_subset_fill": (_i32, [_vp]),
+    "gg_subset_set_bit": (_i32, [_vp, _u32]),
+    "gg_subset_get_bit": (_i32, [_vp, _u32, ctypes.POINTER(ctypes.c_int)]),
+    "gg_subset_count": (_i32, [_vp, ctypes.POINTER(_u64)]),
+    "gg_subset_device_words": (_vp, [_vp]),
+    "gg_pagerank": (_i32, [_vp, _i32, _vp, _vp, _stats]),
+    "gg_bfs": (_i32, [_vp, _u32, _vp, _vp, _vp, _i32, _stats]),
+    "gg_cc": (_i32, [_vp, _vp, _vp, _i32, _stats]),
+    "gg_sssp": (_i32, [_vp, _u32, _vp, _vp, _i32, _stats]),
+    "gg_bc": (_i32, [_vp, _u32, _vp, _vp, _vp, _i32, _stats]),
+    "gg_prof_enable": (_i32, [_i32]),
+    "gg_prof_reset": (_i32, []),
+    "gg_prof_read": (_i32, [_cp, ctypes.POINTER(_dbl), ctypes.POINTER(_u64)]),
+}
+
+
+def lib():
+    """Load libgg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run geminigraph_amd.build() "
+                               "(or __graft_entry__.build()) first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != GG_OK:
+        raise GGError(code, lib().gg_last_error().decode(errors="replace"))
+
+
+def init(device=0):
+    _check(lib().gg_init(int(device)))
+
+
+def synchronize():
+    _check(lib().gg_synchronize())
+
+
+def set_option(name, value):
+    _check(lib().gg_set_option(name.encode(), int(value)))
+
+
+def get_option(name):
+    v = ctypes.c_int64()
+    _check(lib().gg_get_option(name.encode(), ctypes.byref(v)))
+    return v.value
+
+
+def mem_info():
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    _check(lib().gg_mem_info(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
+
+
+class options:
+    """Context manager that sets options and restores them on exit."""
+
+    def __init__(self, **kw):
+        self.kw = kw
+        self.old = {}
+
+    def __enter__(self):
+        for k, v in self.kw.items():
+            self.old[k] = get_option(k)
+            set_option(k, v)
+        return self
+
+    def __exit__(self, *exc):
+        for k, v in self.old.items():
+            set_option(k, v)
+
+
+class DeviceArray:
+    """A device buffer (alloc_vertex_array / dealloc_vertex_array analogue)."""
+
+    def __init__(self, n, dtype):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib().gg_malloc(ctypes.byref(p), max(self.nbytes, 8)))
+        self.ptr = p.value
+
+    def numpy(self):
+        """gather_vertex_array: copy to host."""
+        out = np.empty(self.n, dtype=self.dtype)
+        if self.nbytes:
+            _check(lib().gg_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.size == self.n
+        if self.nbytes:
+            _check(lib().gg_memcpy_h2d(self.ptr, arr.ctypes.data, self.nbytes))
+        return self
+
+    def free(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.gg_free(ctypes.c_void_p(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def rmat_device(scale, num_edges, seed, a=RMAT_A, b=RMAT_B, c=RMAT_C, permute=True,
+                weighted=False):
+    """Generate an RMAT edge list in device memory (packed 8/12-byte records)."""
+    rec = 12 if weighted else 8
+    buf = DeviceArray(num_edges * rec, np.uint8)
+    _check(lib().gg_rmat_generate(ctypes.c_void_p(buf.ptr), int(num_edges), int(scale),
+                                  float(a), float(b), float(c), int(seed), int(permute),
+                                  int(weighted)))
+    return buf
+
+
+class Graph:
+    """Graph<EdgeData> for one partition on one GPU (core/graph.hpp:91-1869)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        info = GraphInfo()
+        _check(lib().gg_graph_get_info(ctypes.c_void_p(handle), ctypes.byref(info)))
+        self.vertices = info.vertices
+        self.edges = info.edges
+        self.arcs = info.arcs
+        self.weighted = bool(info.weighted)
+        self.symmetric = bool(info.symmetric)
+
+    # -- construction -----------------------------------------------------
+    @classmethod
+    def build(cls, edges, vertices, undirected=False):
+        """From a host structured array of packed records (oracle.EDGE_DTYPE / WEDGE_DTYPE)."""
+        edges = np.ascontiguousarray(edges)
+        weighted = edges.dtype.itemsize == 12
+        h = ctypes.c_void_p()
+        _check(lib().gg_graph_build(ctypes.c_void_p(edges.ctypes.data), 0, len(edges),
+                                    int(vertices), int(weighted), int(undirected),
+                                    ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def build_device(cls, dev_edges, num_edges, vertices, weighted=False, undirected=False):
+        h = ctypes.c_void_p()
+        _check(lib().gg_graph_build(ctypes.c_void_p(dev_edges.ptr), 1, int(num_edges),
+                                    int(vertices), int(weighted), int(undirected),
+                                    ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor=16, seed=1, a=RMAT_A, b=RMAT_B, c=RMAT_C, permute=True,
+             weighted=False, undirected=False):
+        ne = (1 << scale) * edge_factor
+        buf = rmat_device(scale, ne, seed, a, b, c, permute, weighted)
+        try:
+            return cls.build_device(buf, ne, 1 << scale, weighted, undirected)
+        finally:
+            buf.free()
+
+    @classmethod
+    def load_directed(cls, path, vertices, weighted=False):
+        """core/graph.hpp:772 -- |E| = file size / record size."""
+        h = ctypes.c_void_p()
+        _check(lib().gg_graph_load(str(path).encode(), int(vertices), int(weighted), 0,
+                                   ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load_undirected_from_directed(cls, path, vertices, weighted=False):
+        """core/graph.hpp:350 -- symmetric CSR, CSC aliased, |E| kept."""
+        h = ctypes.c_void_p()
+        _check(lib().gg_graph_load(str(path).encode(), int(vertices), int(weighted), 1,
+                                   ctypes.byref(h)))
+        return cls(h.value)
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.gg_graph_destroy(ctypes.c_void_p(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return ctypes.c_void_p(self._h)
+
+    def info(self):
+        info = GraphInfo()
+        _check(lib().gg_graph_get_info(self.handle, ctypes.byref(info)))
+        return info
+
+    def transpose(self):
+        _check(lib().gg_graph_transpose(self.handle))
+
+    def _copy(self, ptr, n, dtype):
+        out = np.empty(n, dtype=dtype)
+        if n:
+            _check(lib().gg_memcpy_d2h(out.ctypes.data, ctypes.c_void_p(ptr), out.nbytes))
+        return out
+
+    def out_degree(self):
+        return self._copy(self.info().d_out_degree, self.vertices, np.uint32)
+
+    def in_degree(self):
+        return self._copy(self.info().d_in_degree, self.vertices, np.uint32)
+
+    def csr(self):
+        i = self.info()
+        idx = self._copy(i.d_out_index, self.vertices + 1, np.uint64)
+        nbr = self._copy(i.d_out_nbr, self.arcs, np.uint32)
+        w = self._copy(i.d_out_w, self.arcs, np.float32) if self.weighted else None
+        return idx, nbr, w
+
+    def csc(self):
+        i = self.info()
+        idx = self._copy(i.d_in_index, self.vertices + 1, np.uint64)
+        nbr = self._copy(i.d_in_nbr, self.arcs, np.uint32)
+        w = self._copy(i.d_in_w, self.arcs, np.float32) if self.weighted else None
+        return idx, nbr, w
+
+    # -- toolkits (device outputs; *_into variants keep results on the GPU) --
+    def pagerank_into(self, out, iterations=20, deltas=None):
+        st = RunStats()
+        dptr = deltas.ctypes.data if deltas is not None else None
+        _check(lib().gg_pagerank(self.handle, int(iterations), ctypes.c_void_p(out.ptr),
+                                 ctypes.c_void_p(dptr) if dptr else None, ctypes.byref(st)))
+        return st
+
+    def pagerank(self, iterations=20):
+        out = DeviceArray(self.vertices, np.float64)
+        deltas = np.zeros(iterations + 1, dtype=np.float64)
+        st = self.pagerank_into(out, iterations, deltas)
+        return out.numpy(), deltas, st.as_dict()
+
+    def bfs_into(self, root, parent, depth, frontier=None):
+        st = RunStats()
+        fr_ptr = frontier.ctypes.data if frontier is not None else None
+        cap = len(frontier) if frontier is not None else 0
+        _check(lib().gg_bfs(self.handle, int(root), ctypes.c_void_p(parent.ptr),
+                            ctypes.c_void_p(depth.ptr), ctypes.c_void_p(fr_ptr) if fr_ptr else None,
+                            cap, ctypes.byref(st)))
+        return st
+
+    def bfs(self, root):
+        parent = DeviceArray(self.vertices, np.uint32)
+        depth = DeviceArray(self.vertices, np.uint32)
+        fr = np.zeros(min(self.vertices + 2, 1 << 16), dtype=np.uint32)
+        st = self.bfs_into(root, parent, depth, fr)
+        levels = int(np.max(np.nonzero(fr)[0]) + 1) if fr.any() else 0
+        return parent.numpy(), depth.numpy(), fr[:levels].copy(), st.as_dict()
+
+    def cc_into(self, label, frontier=None):
+        st = RunStats()
+        fr_ptr = frontier.ctypes.data if frontier is not None else None
+        cap = len(frontier) if frontier is not None else 0
+        _check(lib().gg_cc(self.handle, ctypes.c_void_p(label.ptr),
+                           ctypes.c_void_p(fr_ptr) if fr_ptr else None, cap, ctypes.byref(st)))
+        return st
+
+    def cc(self):
+        label = DeviceArray(self.vertices, np.uint32)
+        fr = np.zeros(min(self.vertices + 2, 1 << 16), dtype=np.uint32)
+        st = self.cc_into(label, fr)
+        n = int(np.max(np.nonzero(fr)[0]) + 1) if fr.any() else 0
+        return label.numpy(), fr[:n].copy(), st.as_dict()
+
+    def sssp_into(self, root, dist, frontier=None):
+        st = RunStats()
+        fr_ptr = frontier.ctypes.data if frontier is not None else None
+        cap = len(frontier) if frontier is not None else 0
+        _check(lib().gg_sssp(self.handle, int(root), ctypes.c_void_p(dist.ptr),
+                             ctypes.c_void_p(fr_ptr) if fr_ptr else None, cap, ctypes.byref(st)))
+        return st
+
+    def sssp(self, root):
+        dist = DeviceArray(self.vertices, np.float32)
+        fr = np.zeros(min(self.vertices + 2, 1 << 16), dtype=np.uint32)
+        st = self.sssp_into(root, dist, fr)
+        n = int(np.max(np.nonzero(fr)[0]) + 1) if fr.any() else 0
+        return dist.numpy(), fr[:n].copy(), st.as_dict()
+
+    def bc_into(self, root, dep, inv, frontier=None):
+        st = RunStats()
+        fr_ptr = frontier.ctypes.data if frontier is not None else None
+        cap = len(frontier) if frontier is not None else 0
+        _check(lib().gg_bc(self.handle, int(root), ctypes.c_void_p(dep.ptr),
+                           ctypes.c_void_p(inv.ptr), ctypes.c_void_p(fr_ptr) if fr_ptr else None,
+                           cap, ctypes.byref(st)))
+        return st
+
+    def bc(self, root):
+        dep = DeviceArray(self.vertices, np.float64)
+        inv = DeviceArray(self.vertices, np.float64)
+        fr = np.zeros(min(self.vertices + 2, 1 << 16), dtype=np.uint32)
+        st = self.bc_into(root, dep, inv, fr)
+        n = int(np.max(np.nonzero(fr)[0]) + 1) if fr.any() else 0
+        return dep.numpy(), inv.numpy(), fr[:n].copy(), st.as_dict()
+
+
+class VertexSubset:
+    """Bitmap / VertexSubset (core/bitmap.hpp:20-61) in device memory."""
+
+    def __init__(self, graph):
+        h = ctypes.c_void_p()
+        _check(lib().gg_subset_create(graph.handle, ctypes.byref(h)))
+        self._h = h.value
+        self.size = graph.vertices
+
+    def clear(self):
+        _check(lib().gg_subset_clear(ctypes.c_void_p(self._h)))
+
+    def fill(self):
+        _check(lib().gg_subset_fill(ctypes.c_void_p(self._h)))
+
+    def set_bit(self, v):
+        _check(lib().gg_subset_set_bit(ctypes.c_void_p(self._h), int(v)))
+
+    def get_bit(self, v):
+        b = ctypes.c_int()
+        _check(lib().gg_subset_get_bit(ctypes.c_void_p(self._h), int(v), ctypes.byref(b)))
+        return b.value
+
+    def count(self):
+        c = ctypes.c_uint64()
+        _check(lib().gg_subset_count(ctypes.c_void_p(self._h), ctypes.byref(c)))
+        return c.value
+
+    def words(self):
+        n = (self.size >> 6) + 1
+        out = np.empty(n, dtype=np.uint64)
+        p = lib().gg_subset_device_words(ctypes.c_void_p(self._h))
+        _check(lib().gg_memcpy_d2h(out.ctypes.data, ctypes.c_void_p(p), out.nbytes))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.gg_subset_destroy(ctypes.c_void_p(self._h))
+            self._h = None
+
+
+def prof_enable(on=True):
+    _check(lib().gg_prof_enable(int(on)))
+
+
+def prof_reset():
+    _check(lib().gg_prof_reset())
+
+
+def prof_read(kernel):
+    ms, n = ctypes.c_double(), ctypes.c_uint64()
+    _check(lib().gg_prof_read(kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value

@@ -1,0 +1,1112 @@
+// The five toolkits on the device engine: PageRank, BFS, CC, SSSP, BC.
+//
+// Each driver mirrors the compute() of its reference toolkit: the same
+// initialisation, the same per-iteration process_edges / process_vertices
+// sequence, the same push/pull switch (core/graph.hpp:1503-1509) and the
+// same BSP visibility: a signal reads state as it stood when the iteration
+// began.  CC and SSSP read and write one array, so their pull runs in two
+// phases (gather into a message array, then apply) and their push reads a
+// snapshot of the frontier's values; BFS, BC and PageRank are hazard-free and
+// run fused.
+#include <string.h>
+
+#include <algorithm>
+
+#include "kernels.hpp"
+
+namespace gg {
+
+// --------------------------------------------------------------- utilities
+template <class T>
+__global__ void fill_kernel(T *p, T v, uint64_t n) {
+  uint64_t s = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += s) p[i] = v;
+}
+__global__ void iota_kernel(uint32_t *p, uint32_t n) {
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += s) p[i] = i;
+}
+__global__ void popcount_kernel(const uint64_t *bits, size_t words, unsigned long long *out) {
+  size_t s = (size_t)gridDim.x * blockDim.x;
+  uint64_t c = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += s)
+    c += (uint64_t)__popcll(bits[i]);
+  c = wave_sum_u64(c);
+  if (lane_id() == 0 && c) atomicAdd(out, (unsigned long long)c);
+}
+
+__global__ __launch_bounds__(kBlock) void compact_kernel(const uint64_t *bits, size_t words,
+                                                         const uint32_t *bin_deg,
+                                                         uint32_t small_max, uint32_t big_max,
+                                                         uint32_t *q0, uint32_t *q1, uint32_t *q2,
+                                                         Counters *cnt, uint64_t *visited) {
+  __shared__ uint32_t s_tot[3][kWavesPerBlock];
+  __shared__ unsigned long long s_edges[kWavesPerBlock];
+  __shared__ uint32_t s_base[3];
+  const int lane = lane_id(), wv = wave_in_block();
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  for (size_t base = (size_t)blockIdx.x * kBlock; base < words; base += stride) {
+    size_t wi = base + threadIdx.x;
+    uint64_t word = wi < words ? bits[wi] : 0;
+    if (visited && word) visited[wi] |= word;
+    uint32_t c0 = 0, c1 = 0, c2 = 0;
+    uint64_t es = 0;
+    for (uint64_t x = word; x; x &= x - 1) {
+      uint32_t v = (uint32_t)(wi * 64 + (size_t)__builtin_ctzll(x));
+      uint32_t d = bin_deg[v];
+      es += d;
+      if (d <= small_max)
+        c0++;
+      else if (d <= big_max)
+        c1++;
+      else
+        c2++;
+    }
+    uint32_t t0, t1, t2;
+    uint32_t e0 = wave_excl_scan(c0, &t0);
+    uint32_t e1 = wave_excl_scan(c1, &t1);
+    uint32_t e2 = wave_excl_scan(c2, &t2);
+    uint64_t ws = wave_sum_u64(es);
+    if (lane == 0) {
+      s_tot[0][wv] = t0;
+      s_tot[1][wv] = t1;
+      s_tot[2][wv] = t2;
+      s_edges[wv] = ws;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t b0 = 0, b1 = 0, b2 = 0;
+      unsigned long long be = 0;
+      for (int i = 0; i < kWavesPerBlock; i++) {
+        b0 += s_tot[0][i];
+        b1 += s_tot[1][i];
+        b2 += s_tot[2][i];
+        be += s_edges[i];
+      }
+      s_base[0] = b0 ? atomicAdd(&cnt->n[0], b0) : 0;
+      s_base[1] = b1 ? atomicAdd(&cnt->n[1], b1) : 0;
+      s_base[2] = b2 ? atomicAdd(&cnt->n[2], b2) : 0;
+      if (b0 + b1 + b2) {
+        atomicAdd(&cnt->total, b0 + b1 + b2);
+        atomicAdd((unsigned long long *)&cnt->edges, be);
+      }
+    }
+    __syncthreads();
+    uint32_t o0 = s_base[0] + e0, o1 = s_base[1] + e1, o2 = s_base[2] + e2;
+    for (int i = 0; i < wv; i++) {
+      o0 += s_tot[0][i];
+      o1 += s_tot[1][i];
+      o2 += s_tot[2][i];
+    }
+    for (uint64_t x = word; x; x &= x - 1) {
+      uint32_t v = (uint32_t)(wi * 64 + (size_t)__builtin_ctzll(x));
+      uint32_t d = bin_deg[v];
+      if (d <= small_max)
+        q0[o0++] = v;
+      else if (d <= big_max)
+        q1[o1++] = v;
+      else
+        q2[o2++] = v;
+    }
+    __syncthreads();
+  }
+}
+
+static unsigned grid1(uint64_t n, unsigned cap = 1u << 16) {
+  uint64_t g = (n + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+static unsigned grid_waves(uint64_t waves) {
+  uint64_t g = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+// Blocks for kernels that walk vertices in 256-aligned rounds.
+static unsigned grid_vertex_rounds(uint32_t V) { return grid1(V, 1u << 14); }
+
+template <class T>
+static void fill(T *p, T v, uint64_t n) {
+  if (n) hipLaunchKernelGGL(fill_kernel<T>, dim3(grid1(n)), dim3(kBlock), 0, stream(), p, v, n);
+}
+
+// Sets bits for one vertex per lane in 256-vertex rounds; each wave owns one
+// word per round, so one OR per wave suffices.
+__device__ __forceinline__ void set_owned_bit(uint64_t *bits, uint32_t v, bool on) {
+  uint64_t m = __ballot(on);
+  if (lane_id() == 0 && m) atomicOr((unsigned long long *)&bits[v >> 6], (unsigned long long)m);
+}
+
+__global__ void set_root_kernel(uint64_t *bits, uint32_t root) {
+  bits[root >> 6] |= 1ULL << (root & 63);
+}
+template <class A, class B>
+__global__ void put2_kernel(A *pa, A va, B *pb, B vb, uint32_t i) {
+  pa[i] = va;
+  if (pb) pb[i] = vb;
+}
+// Bitmap::fill (core/bitmap.hpp:42-52): full words, exact tail, rest zero.
+__global__ void bitmap_fill_kernel(uint64_t *bits, size_t words, uint32_t V) {
+  size_t full = (size_t)V >> 6;
+  size_t s = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += s) {
+    uint64_t x = 0;
+    if (i < full)
+      x = ~0ULL;
+    else if (i == full && (V & 63))
+      x = (1ULL << (V & 63)) - 1ULL;
+    bits[i] = x;
+  }
+}
+
+// --------------------------------------------------------------- work area
+int work_get(gg_graph *g, Work **out) {
+  if (g->work) {
+    *out = g->work;
+    return GG_OK;
+  }
+  Work *w = new Work;
+  w->V = g->vertices;
+  w->words = ((size_t)g->vertices >> 6) + 1;
+  size_t V = (size_t)g->vertices + 1;
+  auto al = [&](void **p, size_t bytes) -> int {
+    int r = device_alloc(p, bytes);
+    if (r == GG_OK) w->allocations.push_back(*p);
+    return r;
+  };
+  int r = GG_OK;
+  for (int i = 0; i < 2 && r == GG_OK; i++) {
+    Frontier &f = w->f[i];
+    r = al((void **)&f.bits, w->words * 8);
+    for (int b = 0; b < 3 && r == GG_OK; b++) r = al((void **)&f.q[b], V * 4);
+    for (int b = 0; b < 3 && r == GG_OK; b++) r = al((void **)&f.snap_f[b], V * 4);
+    if (r == GG_OK) r = al((void **)&f.cnt, sizeof(Counters));
+  }
+  if (r == GG_OK) r = al((void **)&w->visited, w->words * 8);
+  if (r == GG_OK) r = al((void **)&w->aux_bits, w->words * 8);
+  if (r == GG_OK) r = al(&w->msg, V * 8);
+  size_t nh = std::max<size_t>(std::max(g->out.n_heavy, g->in.n_heavy), 1);
+  if (r == GG_OK) r = al((void **)&w->hacc, nh * 8);
+  if (r == GG_OK) r = al((void **)&w->dacc, 4096 * 8);
+  if (r == GG_OK) r = al((void **)&w->exam, 1024 * 8);
+  if (r == GG_OK && hipHostMalloc((void **)&w->h_cnt, sizeof(Counters) * 2) != hipSuccess)
+    r = fail(GG_ENOMEM, "pinned counters");
+  if (r == GG_OK && hipHostMalloc((void **)&w->h_exam, 1024 * 8) != hipSuccess)
+    r = fail(GG_ENOMEM, "pinned counters");
+  if (r != GG_OK) {
+    work_free(w);
+    return r;
+  }
+  hipMemsetAsync(w->hacc, 0, nh * 8, stream());
+  hipMemsetAsync(w->exam, 0, 1024 * 8, stream());
+  g->work = w;
+  *out = w;
+  return GG_OK;
+}
+
+void work_free(Work *w) {
+  if (!w) return;
+  for (void *p : w->allocations) device_free(p);
+  if (w->h_cnt) hipHostFree(w->h_cnt);
+  if (w->h_exam) hipHostFree(w->h_exam);
+  delete w;
+}
+
+// Host-side engine state for one traversal.
+struct Engine {
+  gg_graph *g;
+  Work *w;
+  int cur = 0;    // w->f[cur] is the input frontier
+  Counters in{};  // host copy of the input frontier's counters
+  gg_run_stats st{};
+  uint32_t small_max, big_max;
+  unsigned long long *exam;
+
+  Engine(gg_graph *g_, Work *w_) : g(g_), w(w_) {
+    small_max = (uint32_t)options().push_small_max;
+    big_max = (uint32_t)options().push_big_max;
+    exam = options().count_examined ? w->exam : nullptr;
+  }
+  Frontier &fin() { return w->f[cur]; }
+  Frontier &fout() { return w->f[cur ^ 1]; }
+  int clear_bits(Frontier &f) {
+    GG_HIP(hipMemsetAsync(f.bits, 0, w->words * 8, stream()));
+    return GG_OK;
+  }
+  // Bitmap -> binned queues and counts; reads the counts back.
+  int compact(Frontier &f, const uint32_t *bin_deg, uint64_t *visited_or, Counters *outc) {
+    GG_HIP(hipMemsetAsync(f.cnt, 0, sizeof(Counters), stream()));
+    {
+      ProfScope p("compact");
+      hipLaunchKernelGGL(compact_kernel, dim3(grid1(w->words, 2048)), dim3(kBlock), 0, stream(),
+                         f.bits, w->words, bin_deg, small_max, big_max, f.q[0], f.q[1], f.q[2],
+                         f.cnt, visited_or);
+    }
+    GG_HIP(hipMemcpyAsync(w->h_cnt, f.cnt, sizeof(Counters), hipMemcpyDeviceToHost, stream()));
+    GG_HIP(hipStreamSynchronize(stream()));
+    GG_HIP(hipGetLastError());
+    *outc = *w->h_cnt;
+    return GG_OK;
+  }
+  int compact_in(const uint32_t *bin_deg, uint64_t *visited_or = nullptr) {
+    return compact(fin(), bin_deg, visited_or, &in);
+  }
+  int seed(uint32_t root, const uint32_t *bin_deg, uint64_t *visited_or) {
+    GG_TRY(clear_bits(fin()));
+    hipLaunchKernelGGL(set_root_kernel, dim3(1), dim3(1), 0, stream(), fin().bits, root);
+    return compact_in(bin_deg, visited_or);
+  }
+  // The reference switch: sparse iff active out-edges < |E| / 20.
+  bool sparse(uint64_t active_edges) const {
+    int64_t fm = options().force_mode;
+    if (fm == 1) return true;
+    if (fm == 2) return false;
+    return active_edges < g->edges / (uint64_t)options().switch_divisor;
+  }
+  int note_iteration(bool was_sparse) {
+    int i = st.iterations;
+    if (i < 64) {
+      st.modes[i] = was_sparse ? 0u : 1u;
+      st.it_active[i] = in.total;
+    }
+    st.iterations++;
+    if (was_sparse) st.push_iterations++;
+    if (exam) {
+      GG_HIP(hipMemcpyAsync(w->h_exam, exam, 1024 * 8, hipMemcpyDeviceToHost, stream()));
+      GG_HIP(hipMemsetAsync(exam, 0, 1024 * 8, stream()));
+      GG_HIP(hipStreamSynchronize(stream()));
+      uint64_t x = 0;
+      for (int k = 0; k < 64; k++) x += w->h_exam[k * 16];
+      st.edges_examined += x;
+      if (i < 64) st.it_examined[i] = x;
+    }
+    return GG_OK;
+  }
+  // After the edge kernels: compact the output frontier and swap.
+  int finish_iteration(bool was_sparse, const uint32_t *bin_deg, uint64_t *visited_or) {
+    GG_TRY(note_iteration(was_sparse));
+    Counters c;
+    GG_TRY(compact(fout(), bin_deg, visited_or, &c));
+    if (st.iterations - 1 < 64) st.it_new[st.iterations - 1] = c.total;
+    cur ^= 1;
+    in = c;
+    return GG_OK;
+  }
+  PushArgs push_args(const Dir &d, bool snaps) {
+    Frontier &f = fin();
+    PushArgs a;
+    a.index = d.index;
+    a.nbr = d.nbr;
+    a.w = d.w;
+    a.q0 = f.q[0];
+    a.q1 = f.q[1];
+    a.q2 = f.q[2];
+    a.s0 = snaps ? (const uint32_t *)f.snap_f[0] : nullptr;
+    a.s1 = snaps ? (const uint32_t *)f.snap_f[1] : nullptr;
+    a.s2 = snaps ? (const uint32_t *)f.snap_f[2] : nullptr;
+    a.n0 = in.n[0];
+    a.n1 = in.n[1];
+    a.n2 = in.n[2];
+    a.waves0 = (in.n[0] + 63) / 64;
+    a.waves1 = in.n[1];
+    a.waves2 = in.n[2] ? (uint32_t)options().huge_grid_waves : 0;
+    a.out = fout().bits;
+    a.exam = exam;
+    return a;
+  }
+  template <class Op>
+  int launch_push(const Dir &d, const Op &op, bool snaps) {
+    PushArgs a = push_args(d, snaps);
+    uint64_t waves = (uint64_t)a.waves0 + a.waves1 + a.waves2;
+    if (!waves) return GG_OK;
+    ProfScope p("push");
+    hipLaunchKernelGGL(push_kernel<Op>, dim3(grid_waves(waves)), dim3(kBlock), 0, stream(), a, op);
+    return GG_OK;
+  }
+  template <class Op>
+  int launch_pull_group(const Dir &d, const Op &op, double *dacc, const char *name = "pull") {
+    PullArgs a;
+    a.index = d.index;
+    a.nbr = d.nbr;
+    a.w = d.w;
+    a.V = g->vertices;
+    a.light_max = d.light_max;
+    a.ngroups = (g->vertices + 63) / 64;
+    a.out = fout().bits;
+    a.exam = exam;
+    a.dacc = dacc;
+    ProfScope p(name);
+    hipLaunchKernelGGL(pull_group_kernel<Op>, dim3(grid_waves(a.ngroups)), dim3(kBlock), 0,
+                       stream(), a, op);
+    return GG_OK;
+  }
+  template <class Op>
+  int launch_pull_heavy(const Dir &d, const Op &op, const char *name = "pull_heavy") {
+    if (!d.n_chunks) return GG_OK;
+    HeavyArgs a{d.nbr, d.w, d.chunk_v, d.chunk_h, d.chunk_beg, d.chunk_end, d.n_chunks,
+                fout().bits, exam};
+    ProfScope p(name);
+    hipLaunchKernelGGL(pull_heavy_kernel<Op>, dim3(grid_waves(d.n_chunks)), dim3(kBlock), 0,
+                       stream(), a, op);
+    return GG_OK;
+  }
+};
+
+struct NoAfter {
+  __device__ void after(bool, uint32_t) const {}
+};
+
+// ================================================================ PageRank
+// toolkits/pagerank.cpp:24-129
+struct PrPull : NoAfter {
+  using T = double;
+  const double *curr;
+  double *next;
+  const uint32_t *outdeg;
+  int last;
+  __device__ double identity() const { return 0.0; }
+  __device__ bool dst_active(uint32_t) const { return true; }
+  __device__ double value(uint32_t src, float) const { return curr[src]; }
+  __device__ double combine(double a, double b) const { return a + b; }
+  __device__ bool finish(uint32_t v, double sum, uint32_t, double &dsum) const {
+    // next = 1 - d + d * next, divided by the out-degree except in the last
+    // iteration (pagerank.cpp:77-97)
+    const double d = 0.85;
+    double nx = 1 - d + d * sum;
+    if (!last) {
+      uint32_t od = outdeg[v];
+      if (od > 0) {
+        nx /= od;
+        dsum += fabs(nx - curr[v]) * od;
+      } else {
+        dsum += fabs(nx - curr[v]);
+      }
+    }
+    next[v] = nx;
+    return false;
+  }
+};
+struct PrHeavy : NoAfter {
+  using T = double;
+  const double *curr;
+  double *hacc;
+  __device__ double identity() const { return 0.0; }
+  __device__ bool dst_active(uint32_t) const { return true; }
+  __device__ double value(uint32_t src, float) const { return curr[src]; }
+  __device__ double combine(double a, double b) const { return a + b; }
+  __device__ bool commit(uint32_t, uint32_t h, double x) const {
+    unsafeAtomicAdd(&hacc[h], x);
+    return false;
+  }
+};
+__global__ void pr_heavy_finish_kernel(const uint32_t *heavy_v, uint32_t n, double *hacc, PrPull op,
+                                       double *dacc) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double dsum = 0;
+  if (i < n) {
+    uint32_t v = heavy_v[i];
+    double s = hacc[i];
+    hacc[i] = 0.0;
+    op.finish(v, s, 0, dsum);
+  }
+  double t = wave_sum_f64(dsum);
+  if (lane_id() == 0 && t != 0.0) unsafeAtomicAdd(dacc, t);
+}
+__global__ void pr_init_kernel(double *curr, const uint32_t *outdeg, uint32_t V) {
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s) {
+    double x = 1.0;  // pagerank.cpp:36-42
+    if (outdeg[v] > 0) x /= outdeg[v];
+    curr[v] = x;
+  }
+}
+
+// ===================================================================== BFS
+// toolkits/bfs.cpp:22-109
+struct BfsPush : NoAfter {
+  using Msg = uint32_t;
+  uint32_t *parent, *depth;
+  uint32_t V, level;
+  __device__ uint32_t msg(uint32_t v, uint32_t) const { return v; }
+  __device__ bool edge(uint32_t src, uint32_t, uint32_t dst, float) const {
+    // bfs.cpp:53: plain read first, then compare-and-swap
+    if (parent[dst] == V && atomicCAS(&parent[dst], V, src) == V) {
+      depth[dst] = level;
+      return true;
+    }
+    return false;
+  }
+};
+
+struct BfsPullArgs {
+  const uint64_t *index;
+  const uint32_t *nbr;
+  const uint64_t *visited, *fin;
+  uint32_t *parent, *depth;
+  uint32_t V, level, light_max, serial_max, ngroups;
+  uint64_t *out;
+  unsigned long long *exam;
+};
+
+// Bottom-up step, one 64-vertex group per wave (bfs.cpp:60-76): an unvisited
+// vertex scans its in-list until it meets a frontier vertex.  Short lists are
+// scanned by their own lane, longer ones by the whole wave with a ballot.
+__global__ __launch_bounds__(kBlock) void bfs_pull_group_kernel(BfsPullArgs a) {
+  const uint32_t g = global_wave();
+  const int lane = lane_id();
+  if (g >= a.ngroups) return;
+  uint64_t visw = a.visited[g];
+  if (visw == ~0ULL) return;
+  uint32_t v = g * 64 + lane;
+  bool cand = v < a.V && !((visw >> lane) & 1ULL);
+  uint64_t b = 0, e1 = 0;
+  if (cand) {
+    b = a.index[v];
+    e1 = a.index[v + 1];
+  }
+  uint64_t deg = e1 - b;
+  cand = cand && deg > 0 && deg <= a.light_max;
+  bool serial = cand && deg <= a.serial_max;
+  uint32_t found = kUnset;
+  uint64_t examined = 0;
+  if (serial) {
+    for (uint64_t e = b; e < e1; e++) {
+      uint32_t u = a.nbr[e];
+      examined++;
+      if (test_bit(a.fin, u)) {
+        found = u;
+        break;
+      }
+    }
+  }
+  uint64_t mmask = __ballot(cand && !serial);
+  while (mmask) {
+    int k = __builtin_ctzll(mmask);
+    mmask &= mmask - 1;
+    uint64_t bk = shfl_u64(b, k), ek = shfl_u64(e1, k);
+    uint32_t fk = kUnset;
+    for (uint64_t e0 = bk; e0 < ek; e0 += 64) {
+      uint64_t e = e0 + lane;
+      uint32_t u = e < ek ? a.nbr[e] : 0;
+      bool hit = e < ek && test_bit(a.fin, u);
+      uint64_t hm = __ballot(hit);
+      if (lane == k)
+        examined += hm ? (uint64_t)(__builtin_ctzll(hm) + 1) : (ek - e0 < 64 ? ek - e0 : 64);
+      if (hm) {
+        fk = __shfl(u, __builtin_ctzll(hm));
+        break;
+      }
+    }
+    if (lane == k) found = fk;
+  }
+  bool got = found != kUnset;
+  if (got) {
+    a.parent[v] = found;
+    a.depth[v] = a.level;
+  }
+  uint64_t m = __ballot(got);
+  if (lane == 0 && m) atomicOr((unsigned long long *)&a.out[g], (unsigned long long)m);
+  if (a.exam) {
+    uint64_t ex = wave_sum_u64(examined);
+    if (lane == 0) count_examined(a.exam, ex);
+  }
+}
+
+struct BfsHeavyArgs {
+  const uint32_t *nbr;
+  const uint32_t *chunk_v;
+  const uint64_t *chunk_beg, *chunk_end;
+  uint32_t n_chunks;
+  const uint64_t *visited, *fin;
+  uint32_t *parent, *depth;
+  uint32_t V, level;
+  uint64_t *out;
+  unsigned long long *exam;
+};
+
+// Heavy in-lists: one wave per chunk; chunks of one vertex race and the
+// compare-and-swap on parent keeps exactly one (bfs.cpp:71, dense_slot).
+__global__ __launch_bounds__(kBlock) void bfs_pull_heavy_kernel(BfsHeavyArgs a) {
+  const uint32_t c = global_wave();
+  const int lane = lane_id();
+  if (c >= a.n_chunks) return;
+  uint32_t v = a.chunk_v[c];
+  if (test_bit(a.visited, v)) return;
+  uint64_t cb = a.chunk_beg[c], ce = a.chunk_end[c];
+  uint64_t examined = 0;
+  for (uint64_t e0 = cb; e0 < ce; e0 += 64) {
+    if (__hip_atomic_load(&a.parent[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.V) break;
+    uint64_t e = e0 + lane;
+    uint32_t u = e < ce ? a.nbr[e] : 0;
+    bool hit = e < ce && test_bit(a.fin, u);
+    uint64_t hm = __ballot(hit);
+    examined += hm ? (uint64_t)(__builtin_ctzll(hm) + 1) : (ce - e0 < 64 ? ce - e0 : 64);
+    if (hm) {
+      uint32_t fu = __shfl(u, __builtin_ctzll(hm));
+      if (lane == 0 && atomicCAS(&a.parent[v], a.V, fu) == a.V) {
+        a.depth[v] = a.level;
+        activate(a.out, v);
+      }
+      break;
+    }
+  }
+  if (lane == 0) count_examined(a.exam, examined);
+}
+
+// ====================================================================== CC
+// toolkits/cc.cpp:22-109
+struct CcPush : NoAfter {
+  using Msg = uint32_t;
+  uint32_t *label;
+  __device__ uint32_t msg(uint32_t, uint32_t snap) const { return snap; }
+  __device__ bool edge(uint32_t, uint32_t m, uint32_t dst, float) const {
+    // cc.cpp:52-56: a plain compare, then write_min; the bit is set whenever
+    // the compare succeeds.
+    if (m < label[dst]) {
+      atomicMin(&label[dst], m);
+      return true;
+    }
+    return false;
+  }
+};
+struct CcPull : NoAfter {  // phase 1 of the dense step: gather the minimum in-label
+  using T = uint32_t;
+  const uint32_t *label;
+  uint32_t *msg;
+  __device__ uint32_t identity() const { return 0xFFFFFFFFu; }
+  __device__ bool dst_active(uint32_t) const { return true; }
+  __device__ uint32_t value(uint32_t src, float) const { return label[src]; }
+  __device__ uint32_t combine(uint32_t a, uint32_t b) const { return a < b ? a : b; }
+  __device__ bool finish(uint32_t v, uint32_t acc, uint32_t deg, double &) const {
+    if (deg) msg[v] = acc;
+    return false;
+  }
+  __device__ bool commit(uint32_t v, uint32_t, uint32_t x) const {
+    atomicMin(&msg[v], x);
+    return false;
+  }
+};
+// phase 2: dense_slot (cc.cpp:72-79), one lane per vertex.
+__global__ void cc_apply_kernel(uint32_t *label, uint32_t *msg, uint32_t V, uint64_t *out) {
+  uint32_t n = (V + 255) & ~255u;
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += s) {
+    uint32_t v = base + threadIdx.x;
+    bool cand = false;
+    if (v < V) {
+      uint32_t m = msg[v];
+      if (m != 0xFFFFFFFFu) {
+        msg[v] = 0xFFFFFFFFu;
+        if (m < label[v]) {
+          label[v] = m;
+          cand = true;
+        }
+      }
+    }
+    set_owned_bit(out, v, cand);
+  }
+}
+
+// ==================================================================== SSSP
+// toolkits/sssp.cpp:22-105.  Distances are non-negative, so the IEEE bit
+// patterns order like the values and write_min is an integer atomicMin.
+constexpr uint32_t kInfBits = 0x4E6E6B28u;  // 1e9f
+struct SsspPush : NoAfter {
+  using Msg = float;
+  float *dist;
+  __device__ float msg(uint32_t, uint32_t snap) const { return __uint_as_float(snap); }
+  __device__ bool edge(uint32_t, float m, uint32_t dst, float w) const {
+    float relax = m + w;
+    if (relax < dist[dst]) {
+      uint32_t old = atomicMin((uint32_t *)&dist[dst], __float_as_uint(relax));
+      return relax < __uint_as_float(old);
+    }
+    return false;
+  }
+};
+struct SsspPull : NoAfter {
+  using T = float;
+  const float *dist;
+  uint32_t *msg;
+  __device__ float identity() const { return 1e9f; }
+  __device__ bool dst_active(uint32_t) const { return true; }
+  __device__ float value(uint32_t src, float w) const { return dist[src] + w; }
+  __device__ float combine(float a, float b) const { return b < a ? b : a; }
+  __device__ bool finish(uint32_t v, float acc, uint32_t deg, double &) const {
+    if (deg) msg[v] = __float_as_uint(acc);
+    return false;
+  }
+  __device__ bool commit(uint32_t v, uint32_t, float x) const {
+    atomicMin(&msg[v], __float_as_uint(x));
+    return false;
+  }
+};
+__global__ void sssp_apply_kernel(float *dist, uint32_t *msg, uint32_t V, uint64_t *out) {
+  uint32_t n = (V + 255) & ~255u;
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += s) {
+    uint32_t v = base + threadIdx.x;
+    bool cand = false;
+    if (v < V) {
+      uint32_t mb = msg[v];
+      if (mb != kInfBits) {
+        msg[v] = kInfBits;
+        float m = __uint_as_float(mb);
+        if (m < 1e9f && m < dist[v]) {  // sssp.cpp:71, :74-77
+          dist[v] = m;
+          cand = true;
+        }
+      }
+    }
+    set_owned_bit(out, v, cand);
+  }
+}
+// Message snapshot of the input frontier's queues (the signal phase).
+__global__ void snapshot_kernel(const uint32_t *vals, Frontier f, uint32_t n0, uint32_t n1,
+                                uint32_t n2) {
+  uint64_t n = (uint64_t)n0 + n1 + n2;
+  uint64_t s = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += s) {
+    int b = i < n0 ? 0 : (i < (uint64_t)n0 + n1 ? 1 : 2);
+    uint64_t j = b == 0 ? i : (b == 1 ? i - n0 : i - n0 - n1);
+    ((uint32_t *)f.snap_f[b])[j] = vals[f.q[b][j]];
+  }
+}
+
+// ====================================================================== BC
+// toolkits/bc.cpp:24-196
+struct BcFwdPush {
+  using Msg = double;
+  double *sigma;
+  const uint64_t *visited;
+  uint32_t *depth;
+  uint32_t level;
+  __device__ double msg(uint32_t v, uint32_t) const { return sigma[v]; }
+  __device__ bool edge(uint32_t, double m, uint32_t dst, float) const {
+    if (!test_bit(visited, dst)) {  // bc.cpp:61-66
+      unsafeAtomicAdd(&sigma[dst], m);
+      return true;
+    }
+    return false;
+  }
+  __device__ void after(bool fresh, uint32_t v) const {
+    if (fresh) depth[v] = level;
+  }
+};
+struct BcFwdPull {
+  using T = double;
+  double *sigma;
+  const uint64_t *visited, *fin;
+  uint32_t *depth;
+  uint32_t level;
+  __device__ double identity() const { return 0.0; }
+  __device__ bool dst_active(uint32_t v) const { return !test_bit(visited, v); }
+  __device__ double value(uint32_t src, float) const {
+    return test_bit(fin, src) ? sigma[src] : 0.0;
+  }
+  __device__ double combine(double a, double b) const { return a + b; }
+  __device__ bool finish(uint32_t v, double sum, uint32_t, double &) const {
+    if (sum > 0) {  // bc.cpp:77-86
+      sigma[v] += sum;
+      return true;
+    }
+    return false;
+  }
+  __device__ bool commit(uint32_t v, uint32_t, double x) const {
+    if (x > 0) {
+      unsafeAtomicAdd(&sigma[v], x);
+      return true;
+    }
+    return false;
+  }
+  __device__ void after(bool fresh, uint32_t v) const {
+    if (fresh) depth[v] = level;
+  }
+};
+// backward step from level L into level L-1 over the transposed graph
+// (bc.cpp:125-157).  Only vertices at depth L-1 can receive a non-zero sum;
+// the reference also adds into unreached vertices, which end as NaN either way.
+struct BcBwdPush : NoAfter {
+  using Msg = double;
+  double *dep;
+  const uint32_t *depth;
+  uint32_t target;  // L-1
+  __device__ double msg(uint32_t v, uint32_t) const { return dep[v]; }
+  __device__ bool edge(uint32_t, double m, uint32_t dst, float) const {
+    if (depth[dst] == target) unsafeAtomicAdd(&dep[dst], m);
+    return false;
+  }
+};
+struct BcBwdPull : NoAfter {
+  using T = double;
+  double *dep;
+  const uint32_t *depth;
+  const uint64_t *lvl;  // bitmap of level L
+  uint32_t target;
+  __device__ double identity() const { return 0.0; }
+  __device__ bool dst_active(uint32_t v) const { return depth[v] == target; }
+  __device__ double value(uint32_t src, float) const { return test_bit(lvl, src) ? dep[src] : 0.0; }
+  __device__ double combine(double a, double b) const { return a + b; }
+  __device__ bool finish(uint32_t v, double sum, uint32_t, double &) const {
+    dep[v] += sum;
+    return false;
+  }
+  __device__ bool commit(uint32_t v, uint32_t, double x) const {
+    unsafeAtomicAdd(&dep[v], x);
+    return false;
+  }
+};
+// Level-L subset by depth, with dependencies[v] += 1/num_paths[v]
+// (bc.cpp:158-167 applied to levels.back()).
+__global__ void bc_level_kernel(const uint32_t *depth, double *dep, const double *inv, uint32_t V,
+                                uint32_t L, uint64_t *out) {
+  uint32_t n = (V + 255) & ~255u;
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += s) {
+    uint32_t v = base + threadIdx.x;
+    bool on = v < V && depth[v] == L;
+    if (on) dep[v] += inv[v];
+    set_owned_bit(out, v, on);
+  }
+}
+__global__ void bc_inv_kernel(double *paths, double *dep, uint32_t V) {
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s) {
+    paths[v] = 1 / paths[v];  // bc.cpp:103-111, +inf when unreached
+    dep[v] = 0;
+  }
+}
+__global__ void bc_final_kernel(double *dep, const double *inv, uint32_t V) {
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s)
+    dep[v] = (dep[v] - inv[v]) / inv[v];  // bc.cpp:170-176
+}
+
+// ----------------------------------------------------------- event timing
+struct DeviceTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+  DeviceTimer() {
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    hipEventRecord(a, stream());
+  }
+  double stop() {
+    hipEventRecord(b, stream());
+    hipEventSynchronize(b);
+    float ms = 0;
+    hipEventElapsedTime(&ms, a, b);
+    return ms;
+  }
+  ~DeviceTimer() {
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+  }
+};
+
+static int check_graph(gg_graph *g) {
+  if (!g) return fail(GG_EINVAL, "null graph");
+  GG_TRY(ensure_init());
+  return GG_OK;
+}
+
+static void put_stats(gg_run_stats *dst, const gg_run_stats &src) {
+  if (dst) *dst = src;
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" {
+
+int gg_subset_count(const gg_subset *s, uint64_t *count) {
+  if (!s || !count) return fail(GG_EINVAL, "gg_subset_count: null argument");
+  GG_TRY(ensure_init());
+  unsigned long long *d = nullptr;
+  GG_TRY(device_alloc((void **)&d, 8));
+  hipMemsetAsync(d, 0, 8, stream());
+  hipLaunchKernelGGL(popcount_kernel, dim3(grid1(s->words, 4096)), dim3(kBlock), 0, stream(), s->d,
+                     s->words, d);
+  unsigned long long h = 0;
+  hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, stream());
+  hipError_t e = hipStreamSynchronize(stream());
+  device_free(d);
+  if (e != hipSuccess) return fail(GG_EHIP, hipGetErrorString(e));
+  *count = h;
+  return GG_OK;
+}
+
+int gg_pagerank(gg_graph *g, int iterations, double *d_curr_out, double *h_deltas,
+                gg_run_stats *stats) {
+  GG_TRY(check_graph(g));
+  if (iterations < 0 || iterations > 4000) return fail(GG_EINVAL, "iterations out of range");
+  if (!d_curr_out) return fail(GG_EINVAL, "null output");
+  if (options().force_mode == 1)
+    return fail(GG_EINVAL, "PageRank runs dense: its active set is every vertex");
+  Work *w;
+  GG_TRY(work_get(g, &w));
+  Engine en(g, w);
+  const Dir &out = out_dir(g), &in = in_dir(g);
+  uint32_t V = g->vertices;
+  DeviceTimer tm;
+  double *curr = d_curr_out;
+  double *next = (double *)w->msg;
+  hipLaunchKernelGGL(pr_init_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), curr, out.degree, V);
+  GG_HIP(hipMemsetAsync(w->dacc, 0, sizeof(double) * (iterations + 1), stream()));
+  for (int it = 0; it < iterations; it++) {
+    // active = every vertex, so active_edges = |E| and the step is dense
+    // (core/graph.hpp:1509).
+    PrPull op{{}, curr, next, out.degree, it == iterations - 1};
+    GG_TRY(en.launch_pull_group(in, op, w->dacc + it + 1, "pr_pull"));
+    PrHeavy hop{{}, curr, w->hacc};
+    GG_TRY(en.launch_pull_heavy(in, hop, "pr_heavy"));
+    if (in.n_heavy)
+      hipLaunchKernelGGL(pr_heavy_finish_kernel, dim3((in.n_heavy + kBlock - 1) / kBlock),
+                         dim3(kBlock), 0, stream(), in.heavy_v, in.n_heavy, w->hacc, op,
+                         w->dacc + it + 1);
+    if (en.st.iterations < 64) en.st.modes[en.st.iterations] = 1;
+    en.st.iterations++;
+    en.st.edges_examined += g->arcs;
+    std::swap(curr, next);
+  }
+  if (curr != d_curr_out)
+    GG_HIP(hipMemcpyAsync(d_curr_out, curr, sizeof(double) * V, hipMemcpyDeviceToDevice, stream()));
+  en.st.device_ms = tm.stop();
+  GG_HIP(hipGetLastError());
+  if (h_deltas) {
+    std::vector<double> acc(iterations + 1);
+    GG_HIP(hipMemcpyAsync(acc.data(), w->dacc, sizeof(double) * (iterations + 1),
+                          hipMemcpyDeviceToHost, stream()));
+    GG_HIP(hipStreamSynchronize(stream()));
+    h_deltas[0] = 1.0;  // every vertex contributes 1, divided by |V|
+    for (int it = 1; it <= iterations; it++) h_deltas[it] = acc[it] / V;
+  }
+  en.st.m_reach = g->arcs;
+  en.st.reached = V;
+  put_stats(stats, en.st);
+  return GG_OK;
+}
+
+int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *d_depth, uint32_t *h_frontier,
+           int cap, gg_run_stats *stats) {
+  GG_TRY(check_graph(g));
+  if (root >= g->vertices) return fail(GG_EINVAL, "root out of range");
+  if (!d_parent || !d_depth) return fail(GG_EINVAL, "null output");
+  Work *w;
+  GG_TRY(work_get(g, &w));
+  Engine en(g, w);
+  const Dir &out = out_dir(g), &in = in_dir(g);
+  uint32_t V = g->vertices;
+  DeviceTimer tm;
+  fill<uint32_t>(d_parent, V, V);  // bfs.cpp:35-36
+  fill<uint32_t>(d_depth, kUnset, V);
+  GG_HIP(hipMemsetAsync(w->visited, 0, w->words * 8, stream()));
+  hipLaunchKernelGGL((put2_kernel<uint32_t, uint32_t>), dim3(1), dim3(1), 0, stream(), d_parent,
+                     root, d_depth, 0u, root);
+  GG_TRY(en.seed(root, out.degree, w->visited));
+  uint64_t m_reach = en.in.edges, reached = 1;
+  int lv = 0;
+  if (h_frontier && cap > 0) h_frontier[0] = 1;
+  for (uint32_t level = 1; en.in.total > 0; level++) {
+    if (en.st.iterations < 64) en.st.it_unvisited[en.st.iterations] = (uint64_t)V - reached;
+    GG_TRY(en.clear_bits(en.fout()));
+    bool sp = en.sparse(en.in.edges);
+    if (sp) {
+      BfsPush op;
+      op.parent = d_parent;
+      op.depth = d_depth;
+      op.V = V;
+      op.level = level;
+      GG_TRY(en.launch_push(out, op, false));
+    } else {
+      BfsPullArgs a{in.index, in.nbr, w->visited, en.fin().bits, d_parent, d_depth, V, level,
+                    in.light_max, (uint32_t)options().pull_serial_max, (V + 63) / 64,
+                    en.fout().bits, en.exam};
+      {
+        ProfScope p("bfs_pull");
+        hipLaunchKernelGGL(bfs_pull_group_kernel, dim3(grid_waves(a.ngroups)), dim3(kBlock), 0,
+                           stream(), a);
+      }
+      if (in.n_chunks) {
+        BfsHeavyArgs h{in.nbr, in.chunk_v, in.chunk_beg, in.chunk_end, in.n_chunks, w->visited,
+                       en.fin().bits, d_parent, d_depth, V, level, en.fout().bits, en.exam};
+        ProfScope p("bfs_pull_heavy");
+        hipLaunchKernelGGL(bfs_pull_heavy_kernel, dim3(grid_waves(in.n_chunks)), dim3(kBlock), 0,
+                           stream(), h);
+      }
+    }
+    GG_TRY(en.finish_iteration(sp, out.degree, w->visited));
+    m_reach += en.in.edges;
+    reached += en.in.total;
+    lv++;
+    if (en.in.total > 0 && h_frontier && lv < cap) h_frontier[lv] = en.in.total;
+  }
+  en.st.device_ms = tm.stop();
+  en.st.m_reach = m_reach;
+  en.st.reached = reached;
+  put_stats(stats, en.st);
+  return GG_OK;
+}
+
+int gg_cc(gg_graph *g, uint32_t *d_label, uint32_t *h_frontier, int cap, gg_run_stats *stats) {
+  GG_TRY(check_graph(g));
+  if (!d_label) return fail(GG_EINVAL, "null output");
+  Work *w;
+  GG_TRY(work_get(g, &w));
+  Engine en(g, w);
+  const Dir &out = out_dir(g), &in = in_dir(g);
+  uint32_t V = g->vertices;
+  DeviceTimer tm;
+  hipLaunchKernelGGL(iota_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), d_label, V);
+  uint32_t *msg = (uint32_t *)w->msg;
+  fill<uint32_t>(msg, 0xFFFFFFFFu, V);
+  // active_in->fill() (cc.cpp:28), exact tail as in core/bitmap.hpp:42-52
+  hipLaunchKernelGGL(bitmap_fill_kernel, dim3(grid1(w->words)), dim3(kBlock), 0, stream(),
+                     en.fin().bits, w->words, V);
+  GG_TRY(en.compact_in(out.degree));
+  int lv = 0;
+  while (en.in.total > 0) {
+    if (h_frontier && lv < cap) h_frontier[lv] = en.in.total;
+    lv++;
+    GG_TRY(en.clear_bits(en.fout()));
+    bool sp = en.sparse(en.in.edges);
+    if (sp) {
+      uint64_t n = (uint64_t)en.in.n[0] + en.in.n[1] + en.in.n[2];
+      hipLaunchKernelGGL(snapshot_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(), d_label,
+                         en.fin(), en.in.n[0], en.in.n[1], en.in.n[2]);
+      CcPush op;
+      op.label = d_label;
+      GG_TRY(en.launch_push(out, op, true));
+    } else {
+      CcPull op{{}, d_label, msg};
+      GG_TRY(en.launch_pull_group(in, op, nullptr));
+      GG_TRY(en.launch_pull_heavy(in, op));
+      hipLaunchKernelGGL(cc_apply_kernel, dim3(grid_vertex_rounds(V)), dim3(kBlock), 0, stream(),
+                         d_label, msg, V, en.fout().bits);
+    }
+    GG_TRY(en.finish_iteration(sp, out.degree, nullptr));
+  }
+  en.st.device_ms = tm.stop();
+  put_stats(stats, en.st);
+  return GG_OK;
+}
+
+int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int cap,
+            gg_run_stats *stats) {
+  GG_TRY(check_graph(g));
+  if (root >= g->vertices) return fail(GG_EINVAL, "root out of range");
+  if (!g->weighted) return fail(GG_EINVAL, "SSSP needs a weighted graph");
+  if (!d_dist) return fail(GG_EINVAL, "null output");
+  Work *w;
+  GG_TRY(work_get(g, &w));
+  Engine en(g, w);
+  const Dir &out = out_dir(g), &in = in_dir(g);
+  uint32_t V = g->vertices;
+  DeviceTimer tm;
+  fill<float>(d_dist, 1e9f, V);  // sssp.cpp:163-164
+  hipLaunchKernelGGL((put2_kernel<float, float>), dim3(1), dim3(1), 0, stream(), d_dist, 0.f,
+                     (float *)nullptr, 0.f, root);
+  uint32_t *msg = (uint32_t *)w->msg;
+  fill<uint32_t>(msg, kInfBits, V);
+  GG_TRY(en.seed(root, out.degree, nullptr));
+  int lv = 0;
+  uint64_t reached_edges = en.in.edges;
+  while (en.in.total > 0) {
+    if (h_frontier && lv < cap) h_frontier[lv] = en.in.total;
+    lv++;
+    GG_TRY(en.clear_bits(en.fout()));
+    bool sp = en.sparse(en.in.edges);
+    if (sp) {
+      uint64_t n = (uint64_t)en.in.n[0] + en.in.n[1] + en.in.n[2];
+      hipLaunchKernelGGL(snapshot_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(),
+                         (const uint32_t *)d_dist, en.fin(), en.in.n[0], en.in.n[1], en.in.n[2]);
+      SsspPush op;
+      op.dist = d_dist;
+      GG_TRY(en.launch_push(out, op, true));
+    } else {
+      SsspPull op{{}, d_dist, msg};
+      GG_TRY(en.launch_pull_group(in, op, nullptr));
+      GG_TRY(en.launch_pull_heavy(in, op));
+      hipLaunchKernelGGL(sssp_apply_kernel, dim3(grid_vertex_rounds(V)), dim3(kBlock), 0, stream(),
+                         d_dist, msg, V, en.fout().bits);
+    }
+    GG_TRY(en.finish_iteration(sp, out.degree, nullptr));
+    reached_edges += en.in.edges;
+  }
+  en.st.device_ms = tm.stop();
+  en.st.m_reach = reached_edges;
+  put_stats(stats, en.st);
+  return GG_OK;
+}
+
+int gg_bc(gg_graph *g, uint32_t root, double *d_dep, double *d_inv_paths, uint32_t *h_frontier,
+          int cap, gg_run_stats *stats) {
+  GG_TRY(check_graph(g));
+  if (root >= g->vertices) return fail(GG_EINVAL, "root out of range");
+  if (!d_dep || !d_inv_paths) return fail(GG_EINVAL, "null output");
+  Work *w;
+  GG_TRY(work_get(g, &w));
+  Engine en(g, w);
+  const Dir &out = out_dir(g), &in = in_dir(g);
+  uint32_t V = g->vertices;
+  DeviceTimer tm;
+  double *sigma = d_inv_paths;  // bc.cpp:103: inv_num_paths aliases num_paths
+  uint32_t *depth = (uint32_t *)w->msg;
+  fill<double>(sigma, 0.0, V);
+  fill<uint32_t>(depth, kUnset, V);
+  hipLaunchKernelGGL((put2_kernel<double, uint32_t>), dim3(1), dim3(1), 0, stream(), sigma, 1.0,
+                     depth, 0u, root);
+  GG_HIP(hipMemsetAsync(w->visited, 0, w->words * 8, stream()));
+  GG_TRY(en.seed(root, out.degree, w->visited));
+  int lv = 0;
+  uint64_t reached = 1;
+  // forward (bc.cpp:48-101)
+  uint32_t level = 1;
+  for (; en.in.total > 0; level++) {
+    if (h_frontier && lv < cap) h_frontier[lv] = en.in.total;
+    lv++;
+    GG_TRY(en.clear_bits(en.fout()));
+    bool sp = en.sparse(en.in.edges);
+    if (sp) {
+      BcFwdPush op{sigma, w->visited, depth, level};
+      GG_TRY(en.launch_push(out, op, false));
+    } else {
+      BcFwdPull op{sigma, w->visited, en.fin().bits, depth, level};
+      GG_TRY(en.launch_pull_group(in, op, nullptr));
+      GG_TRY(en.launch_pull_heavy(in, op));
+    }
+    GG_TRY(en.finish_iteration(sp, out.degree, w->visited));
+    reached += en.in.total;
+  }
+  // levels 0 .. max_level are non-empty; the reference's trailing empty
+  // level costs one no-op process_edges in the backward pass.
+  uint32_t max_level = level - 2;
+  hipLaunchKernelGGL(bc_inv_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), sigma, d_dep, V);
+  GG_TRY(en.note_iteration(true));
+  // backward over the transposed graph (bc.cpp:121-168)
+  for (uint32_t L = max_level;; L--) {
+    GG_TRY(en.clear_bits(en.fin()));
+    hipLaunchKernelGGL(bc_level_kernel, dim3(grid_vertex_rounds(V)), dim3(kBlock), 0, stream(),
+                       depth, d_dep, sigma, V, L, en.fin().bits);
+    if (L == 0) break;
+    GG_TRY(en.compact_in(in.degree));  // transposed out-degree = in_degree
+    bool sp = en.sparse(en.in.edges);
+    if (sp) {
+      BcBwdPush op{{}, d_dep, depth, L - 1};
+      GG_TRY(en.launch_push(in, op, false));
+    } else {
+      BcBwdPull op{{}, d_dep, depth, en.fin().bits, L - 1};
+      GG_TRY(en.launch_pull_group(out, op, nullptr));
+      GG_TRY(en.launch_pull_heavy(out, op));
+    }
+    GG_TRY(en.note_iteration(sp));
+  }
+  hipLaunchKernelGGL(bc_final_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), d_dep, sigma, V);
+  en.st.device_ms = tm.stop();
+  GG_HIP(hipGetLastError());
+  en.st.reached = reached;
+  put_stats(stats, en.st);
+  return GG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,494 @@
+// Synthetic RMAT input and the on-device CSR/CSC build.
+//
+// Replaces Graph<E>::load_directed (core/graph.hpp:772-1339) and
+// Graph<E>::load_undirected_from_directed (core/graph.hpp:350-757): the
+// reference reads the file five times and builds each direction with a serial
+// count pass, a prefix sum and an atomic scatter whose neighbour order is
+// nondeterministic.  Here the edge list is uploaded once; each direction is
+// one radix sort of packed (key << vbits | neighbour) 64-bit words, so the
+// adjacency lists come out sorted by neighbour, ties in input order (stable
+// sort), which makes every run reproducible.  Degrees are an atomic
+// histogram, offsets an exclusive scan.
+#include <stdio.h>
+#include <string.h>
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <rocprim/rocprim.hpp>
+
+#include "gg_internal.hpp"
+
+namespace gg {
+
+// ------------------------------------------------------------------- RMAT
+// Definition shared with the CPU oracle (oracle/rmat.c): splitmix64 in
+// counter mode, 32 random bits per level, integer quadrant thresholds.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__host__ __device__ __forceinline__ uint64_t mix64h(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+struct PermKeys {
+  uint64_t mask, m1, a1, m2, a2, m3, a3;
+  int h;
+};
+
+static PermKeys perm_keys(int scale, uint64_t seed) {
+  const uint64_t G = 0x9E3779B97F4A7C15ULL;
+  PermKeys p;
+  p.mask = (scale >= 32) ? 0xFFFFFFFFULL : ((1ULL << scale) - 1ULL);
+  uint64_t k1 = mix64h(seed ^ 0x632BE59BD9B4E019ULL);
+  uint64_t k2 = mix64h(k1 + G);
+  uint64_t k3 = mix64h(k2 + G);
+  p.m1 = (k1 & p.mask) | 1ULL;
+  p.a1 = (k1 >> 32) & p.mask;
+  p.m2 = (k2 & p.mask) | 1ULL;
+  p.a2 = (k2 >> 32) & p.mask;
+  p.m3 = (k3 & p.mask) | 1ULL;
+  p.a3 = (k3 >> 32) & p.mask;
+  p.h = (scale + 1) / 2;
+  return p;
+}
+
+__device__ __forceinline__ uint32_t permute(uint32_t x, const PermKeys &p) {
+  uint64_t v = x;
+  v = (v * p.m1 + p.a1) & p.mask;
+  v ^= v >> p.h;
+  v = (v * p.m2 + p.a2) & p.mask;
+  v ^= v >> p.h;
+  v = (v * p.m3 + p.a3) & p.mask;
+  return (uint32_t)v;
+}
+
+__global__ void rmat_kernel(uint32_t *out, uint64_t ne, int scale, uint64_t key, uint64_t ta,
+                            uint64_t tab, uint64_t tabc, int do_perm, PermKeys pk,
+                            int weighted) {
+  const uint64_t G = 0x9E3779B97F4A7C15ULL;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += stride) {
+    uint32_t src = 0, dst = 0;
+    uint64_t w = 0;
+    for (int l = 0; l < scale; l++) {
+      if ((l & 1) == 0) w = mix64(key + (16ULL * i + (uint64_t)(l >> 1) + 1ULL) * G);
+      uint64_t r = (l & 1) ? (w >> 32) : (w & 0xFFFFFFFFULL);
+      uint32_t bit = 1u << (scale - 1 - l);
+      if (r < ta) {
+      } else if (r < tab) {
+        dst |= bit;
+      } else if (r < tabc) {
+        src |= bit;
+      } else {
+        src |= bit;
+        dst |= bit;
+      }
+    }
+    if (do_perm) {
+      src = permute(src, pk);
+      dst = permute(dst, pk);
+    }
+    if (weighted) {
+      uint64_t k = mix64(key + (16ULL * i + 15ULL + 1ULL) * G) >> 40;
+      float wt = (float)(k + 1ULL) * (1.0f / 16777216.0f);
+      uint32_t *r = out + 3 * i;
+      r[0] = src;
+      r[1] = dst;
+      r[2] = __float_as_uint(wt);
+    } else {
+      uint32_t *r = out + 2 * i;
+      r[0] = src;
+      r[1] = dst;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ build
+__global__ void degree_kernel(const uint32_t *edges, uint64_t ne, int rec_words, uint32_t *out_deg,
+                              uint32_t *in_deg, int undirected) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += stride) {
+    uint32_t s = edges[i * rec_words], d = edges[i * rec_words + 1];
+    atomicAdd(&out_deg[s], 1u);
+    if (undirected)
+      atomicAdd(&out_deg[d], 1u);
+    else
+      atomicAdd(&in_deg[d], 1u);
+  }
+}
+
+__global__ void check_range_kernel(const uint32_t *edges, uint64_t ne, int rec_words, uint32_t V,
+                                   uint32_t *bad) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += stride) {
+    uint32_t s = edges[i * rec_words], d = edges[i * rec_words + 1];
+    if (s >= V || d >= V) atomicOr(bad, 1u);
+  }
+}
+
+// keys for one direction: key vertex in the high bits, neighbour in the low
+// vbits.  mode 0: key = src, 1: key = dst, 2: both (symmetric, 2E keys).
+__global__ void make_keys_kernel(const uint32_t *edges, uint64_t ne, int rec_words, int vbits,
+                                 int mode, uint64_t *keys, float *vals) {
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += stride) {
+    uint64_t s = edges[i * rec_words], d = edges[i * rec_words + 1];
+    float w = rec_words == 3 ? __uint_as_float(edges[i * rec_words + 2]) : 0.f;
+    if (mode == 0) {
+      keys[i] = (s << vbits) | d;
+      if (vals) vals[i] = w;
+    } else if (mode == 1) {
+      keys[i] = (d << vbits) | s;
+      if (vals) vals[i] = w;
+    } else {
+      keys[2 * i] = (s << vbits) | d;
+      keys[2 * i + 1] = (d << vbits) | s;
+      if (vals) {
+        vals[2 * i] = w;
+        vals[2 * i + 1] = w;
+      }
+    }
+  }
+}
+
+__global__ void split_keys_kernel(const uint64_t *keys, uint64_t n, int vbits, uint32_t *nbr) {
+  const uint64_t mask = (1ULL << vbits) - 1ULL;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    nbr[i] = (uint32_t)(keys[i] & mask);
+}
+
+static unsigned grid_for(uint64_t n, unsigned block = 256, unsigned cap = 1u << 20) {
+  uint64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+// Exclusive scan of u32 degrees into u64 offsets (V+1 entries).
+static int scan_degrees(const uint32_t *deg, uint64_t *index, uint32_t V) {
+  size_t tmp_bytes = 0;
+  auto in = rocprim::make_transform_iterator(
+      deg, [] __device__(uint32_t x) -> uint64_t { return (uint64_t)x; });
+  GG_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, in, index, (uint64_t)0, (size_t)V + 1,
+                                 rocprim::plus<uint64_t>(), stream()));
+  void *tmp = nullptr;
+  GG_TRY(device_alloc(&tmp, tmp_bytes));
+  hipError_t e = rocprim::exclusive_scan(tmp, tmp_bytes, in, index, (uint64_t)0, (size_t)V + 1,
+                                         rocprim::plus<uint64_t>(), stream());
+  hipStreamSynchronize(stream());
+  device_free(tmp);
+  if (e != hipSuccess) return fail(GG_EHIP, std::string("exclusive_scan: ") + hipGetErrorString(e));
+  return GG_OK;
+}
+
+static int sort_keys(uint64_t *&keys, uint64_t *&keys_alt, float *&vals, float *&vals_alt,
+                     uint64_t n, int end_bit) {
+  size_t tmp_bytes = 0;
+  rocprim::double_buffer<uint64_t> kb(keys, keys_alt);
+  rocprim::double_buffer<float> vb(vals, vals_alt);
+  if (vals)
+    GG_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, kb, vb, (size_t)n, 0, end_bit, stream()));
+  else
+    GG_HIP(rocprim::radix_sort_keys(nullptr, tmp_bytes, kb, (size_t)n, 0, end_bit, stream()));
+  void *tmp = nullptr;
+  GG_TRY(device_alloc(&tmp, tmp_bytes));
+  hipError_t e;
+  if (vals)
+    e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kb, vb, (size_t)n, 0, end_bit, stream());
+  else
+    e = rocprim::radix_sort_keys(tmp, tmp_bytes, kb, (size_t)n, 0, end_bit, stream());
+  hipStreamSynchronize(stream());
+  device_free(tmp);
+  if (e != hipSuccess) return fail(GG_EHIP, std::string("radix_sort: ") + hipGetErrorString(e));
+  keys = kb.current();
+  keys_alt = kb.alternate();
+  if (vals) {
+    vals = vb.current();
+    vals_alt = vb.alternate();
+  }
+  return GG_OK;
+}
+
+static int graph_alloc(gg_graph *g, void **p, size_t bytes) {
+  GG_TRY(device_alloc(p, bytes));
+  g->allocations.push_back(*p);
+  g->device_bytes += bytes;
+  return GG_OK;
+}
+
+// Heavy-vertex chunk list of one direction (degree > light_max), computed on
+// the host from the degree array: the dense-mode analogue of tune_chunks
+// (core/graph.hpp:1341-1404), which balances threads by in-edges.
+static int build_chunks(gg_graph *g, Dir &d) {
+  uint32_t V = g->vertices;
+  d.light_max = (uint32_t)options().pull_light_max;
+  std::vector<uint32_t> deg(V);
+  std::vector<uint64_t> idx((size_t)V + 1);
+  GG_HIP(hipMemcpyAsync(deg.data(), d.degree, sizeof(uint32_t) * V, hipMemcpyDeviceToHost, stream()));
+  GG_HIP(hipMemcpyAsync(idx.data(), d.index, sizeof(uint64_t) * ((size_t)V + 1),
+                        hipMemcpyDeviceToHost, stream()));
+  GG_HIP(hipStreamSynchronize(stream()));
+  std::vector<uint32_t> cv, ch, hv;
+  std::vector<uint64_t> cb, ce;
+  for (uint32_t v = 0; v < V; v++) {
+    if (deg[v] <= d.light_max) continue;
+    uint32_t h = (uint32_t)hv.size();
+    hv.push_back(v);
+    for (uint64_t b = idx[v]; b < idx[v + 1]; b += kChunkEdges) {
+      cv.push_back(v);
+      ch.push_back(h);
+      cb.push_back(b);
+      ce.push_back(std::min<uint64_t>(b + kChunkEdges, idx[v + 1]));
+    }
+  }
+  d.n_heavy = (uint32_t)hv.size();
+  d.n_chunks = (uint32_t)cv.size();
+  size_t nh = std::max<size_t>(hv.size(), 1), nc = std::max<size_t>(cv.size(), 1);
+  GG_TRY(graph_alloc(g, (void **)&d.heavy_v, nh * 4));
+  GG_TRY(graph_alloc(g, (void **)&d.chunk_v, nc * 4));
+  GG_TRY(graph_alloc(g, (void **)&d.chunk_h, nc * 4));
+  GG_TRY(graph_alloc(g, (void **)&d.chunk_beg, nc * 8));
+  GG_TRY(graph_alloc(g, (void **)&d.chunk_end, nc * 8));
+  if (!hv.empty()) {
+    GG_HIP(hipMemcpyAsync(d.heavy_v, hv.data(), hv.size() * 4, hipMemcpyHostToDevice, stream()));
+    GG_HIP(hipMemcpyAsync(d.chunk_v, cv.data(), cv.size() * 4, hipMemcpyHostToDevice, stream()));
+    GG_HIP(hipMemcpyAsync(d.chunk_h, ch.data(), ch.size() * 4, hipMemcpyHostToDevice, stream()));
+    GG_HIP(hipMemcpyAsync(d.chunk_beg, cb.data(), cb.size() * 8, hipMemcpyHostToDevice, stream()));
+    GG_HIP(hipMemcpyAsync(d.chunk_end, ce.data(), ce.size() * 8, hipMemcpyHostToDevice, stream()));
+  }
+  GG_HIP(hipStreamSynchronize(stream()));
+  return GG_OK;
+}
+
+// One direction from the device edge list.
+static int build_dir(gg_graph *g, Dir &d, const uint32_t *edges, uint64_t ne, int rec_words,
+                     int mode) {
+  uint32_t V = g->vertices;
+  int vbits = 1;
+  while (((uint64_t)1 << vbits) < (uint64_t)V) vbits++;
+  uint64_t n = mode == 2 ? 2 * ne : ne;
+  uint64_t *keys = nullptr, *keys_alt = nullptr;
+  float *vals = nullptr, *vals_alt = nullptr;
+  int r = device_alloc((void **)&keys, n * 8);
+  if (r == GG_OK) r = device_alloc((void **)&keys_alt, n * 8);
+  if (r == GG_OK && g->weighted) r = device_alloc((void **)&vals, n * 4);
+  if (r == GG_OK && g->weighted) r = device_alloc((void **)&vals_alt, n * 4);
+  if (r == GG_OK) {
+    hipLaunchKernelGGL(make_keys_kernel, dim3(grid_for(ne)), dim3(256), 0, stream(), edges, ne,
+                       rec_words, vbits, mode, keys, vals);
+    r = sort_keys(keys, keys_alt, vals, vals_alt, n, 2 * vbits);
+  }
+  device_free(keys_alt);
+  device_free(vals_alt);
+  if (r == GG_OK) r = graph_alloc(g, (void **)&d.nbr, n * 4);
+  if (r == GG_OK) {
+    hipLaunchKernelGGL(split_keys_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), keys, n,
+                       vbits, d.nbr);
+    hipError_t e = hipStreamSynchronize(stream());
+    if (e != hipSuccess) r = fail(GG_EHIP, std::string("split_keys: ") + hipGetErrorString(e));
+  }
+  device_free(keys);
+  if (r == GG_OK && g->weighted) {
+    d.w = vals;  // sorted weights become the adjacency weights
+    g->allocations.push_back(vals);
+    g->device_bytes += n * 4;
+  } else {
+    device_free(vals);
+  }
+  return r;
+}
+
+static int graph_build_device(const uint32_t *d_edges, uint64_t ne, uint32_t V, int weighted,
+                              int undirected, gg_graph **out) {
+  if (V == 0) return fail(GG_EINVAL, "graph needs at least one vertex");
+  if (V >= 0xFFFFFFFFu) return fail(GG_EINVAL, "vertex count must be < 2^32-1 (sentinel V)");
+  int rec_words = weighted ? 3 : 2;
+  gg_graph *g = new gg_graph;
+  g->vertices = V;
+  g->edges = ne;
+  g->weighted = weighted;
+  g->symmetric = undirected;
+  g->arcs = undirected ? 2 * ne : ne;
+  int r = GG_OK;
+  uint32_t *bad = nullptr;
+  if (ne) {
+    r = device_alloc((void **)&bad, 4);
+    if (r == GG_OK) {
+      hipMemsetAsync(bad, 0, 4, stream());
+      hipLaunchKernelGGL(check_range_kernel, dim3(grid_for(ne)), dim3(256), 0, stream(), d_edges,
+                         ne, rec_words, V, bad);
+      uint32_t hb = 0;
+      hipMemcpyAsync(&hb, bad, 4, hipMemcpyDeviceToHost, stream());
+      hipStreamSynchronize(stream());
+      device_free(bad);
+      if (hb) r = fail(GG_EINVAL, "edge endpoint >= vertices");
+    }
+  }
+  if (r == GG_OK) r = graph_alloc(g, (void **)&g->out.degree, ((size_t)V + 1) * 4);
+  if (r == GG_OK) r = graph_alloc(g, (void **)&g->out.index, ((size_t)V + 1) * 8);
+  if (r == GG_OK && !undirected) r = graph_alloc(g, (void **)&g->in.degree, ((size_t)V + 1) * 4);
+  if (r == GG_OK && !undirected) r = graph_alloc(g, (void **)&g->in.index, ((size_t)V + 1) * 8);
+  if (r == GG_OK) {
+    hipMemsetAsync(g->out.degree, 0, ((size_t)V + 1) * 4, stream());
+    if (!undirected) hipMemsetAsync(g->in.degree, 0, ((size_t)V + 1) * 4, stream());
+    if (ne)
+      hipLaunchKernelGGL(degree_kernel, dim3(grid_for(ne)), dim3(256), 0, stream(), d_edges, ne,
+                         rec_words, g->out.degree, g->in.degree, undirected);
+    r = scan_degrees(g->out.degree, g->out.index, V);
+    if (r == GG_OK && !undirected) r = scan_degrees(g->in.degree, g->in.index, V);
+  }
+  if (r == GG_OK) {
+    if (ne == 0) {
+      r = graph_alloc(g, (void **)&g->out.nbr, 4);
+      if (r == GG_OK && !undirected) r = graph_alloc(g, (void **)&g->in.nbr, 4);
+      if (r == GG_OK && weighted) r = graph_alloc(g, (void **)&g->out.w, 4);
+      if (r == GG_OK && weighted && !undirected) r = graph_alloc(g, (void **)&g->in.w, 4);
+    } else if (undirected) {
+      r = build_dir(g, g->out, d_edges, ne, rec_words, 2);
+    } else {
+      r = build_dir(g, g->out, d_edges, ne, rec_words, 0);
+      if (r == GG_OK) r = build_dir(g, g->in, d_edges, ne, rec_words, 1);
+    }
+  }
+  if (r == GG_OK) r = build_chunks(g, g->out);
+  if (r == GG_OK) {
+    if (undirected)
+      g->in = g->out;
+    else
+      r = build_chunks(g, g->in);
+  }
+  if (r != GG_OK) {
+    std::string msg = gg_last_error();
+    gg_graph_destroy(g);
+    return fail(r, msg);
+  }
+  *out = g;
+  return GG_OK;
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" {
+
+int gg_rmat_generate(void *d_edges, uint64_t num_edges, int scale, double a, double b, double c,
+                     uint64_t seed, int permute_ids, int weighted) {
+  GG_TRY(ensure_init());
+  if (!d_edges && num_edges) return fail(GG_EINVAL, "gg_rmat_generate: null output");
+  if (scale < 1 || scale > 31) return fail(GG_EINVAL, "gg_rmat_generate: scale must be in [1,31]");
+  if (a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
+    return fail(GG_EINVAL, "gg_rmat_generate: bad quadrant probabilities");
+  if (!num_edges) return GG_OK;
+  uint64_t key = mix64h(seed ^ 0xD1B54A32D192ED03ULL);
+  uint64_t ta = (uint64_t)(a * 4294967296.0);
+  uint64_t tab = (uint64_t)((a + b) * 4294967296.0);
+  uint64_t tabc = (uint64_t)((a + b + c) * 4294967296.0);
+  PermKeys pk = perm_keys(scale, seed);
+  hipLaunchKernelGGL(rmat_kernel, dim3(grid_for(num_edges, 256, 1u << 16)), dim3(256), 0,
+                     stream(), (uint32_t *)d_edges, num_edges, scale, key, ta, tab, tabc,
+                     permute_ids, pk, weighted);
+  GG_HIP(hipGetLastError());
+  GG_HIP(hipStreamSynchronize(stream()));
+  return GG_OK;
+}
+
+int gg_graph_build(const void *edges, int edges_on_device, uint64_t num_edges, uint32_t vertices,
+                   int weighted, int undirected, gg_graph **out) {
+  if (!out) return fail(GG_EINVAL, "gg_graph_build: null out");
+  if (!edges && num_edges) return fail(GG_EINVAL, "gg_graph_build: null edges");
+  GG_TRY(ensure_init());
+  size_t rec = weighted ? 12 : 8;
+  if (edges_on_device || num_edges == 0)
+    return graph_build_device((const uint32_t *)edges, num_edges, vertices, weighted, undirected,
+                              out);
+  void *d = nullptr;
+  GG_TRY(device_alloc(&d, num_edges * rec));
+  hipError_t e = hipMemcpyAsync(d, edges, num_edges * rec, hipMemcpyHostToDevice, stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(stream());
+  int r = e == hipSuccess ? graph_build_device((const uint32_t *)d, num_edges, vertices, weighted,
+                                               undirected, out)
+                          : fail(GG_EHIP, std::string("upload: ") + hipGetErrorString(e));
+  device_free(d);
+  return r;
+}
+
+int gg_graph_load(const char *path, uint32_t vertices, int weighted, int undirected,
+                  gg_graph **out) {
+  if (!path || !out) return fail(GG_EINVAL, "gg_graph_load: null argument");
+  GG_TRY(ensure_init());
+  struct stat st;
+  if (stat(path, &st) != 0) return fail(GG_EIO, std::string("cannot stat ") + path);
+  size_t rec = weighted ? 12 : 8;
+  uint64_t ne = (uint64_t)st.st_size / rec;  // core/graph.hpp:781-782
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(GG_EIO, std::string("cannot open ") + path);
+  void *d = nullptr;
+  int r = device_alloc(&d, std::max<uint64_t>(ne * rec, 8));
+  void *stage = nullptr;
+  const size_t chunk = (size_t)64 << 20;
+  if (r == GG_OK && hipHostMalloc(&stage, chunk, hipHostMallocDefault) != hipSuccess)
+    r = fail(GG_ENOMEM, "pinned staging buffer");
+  uint64_t done = 0, total = ne * rec;
+  while (r == GG_OK && done < total) {
+    size_t want = (size_t)std::min<uint64_t>(chunk, total - done);
+    size_t got = fread(stage, 1, want, f);
+    if (got != want) {
+      r = fail(GG_EIO, std::string("short read from ") + path);
+      break;
+    }
+    if (hipMemcpyAsync((char *)d + done, stage, got, hipMemcpyHostToDevice, stream()) !=
+            hipSuccess ||
+        hipStreamSynchronize(stream()) != hipSuccess)
+      r = fail(GG_EHIP, "upload of edge list failed");
+    done += got;
+  }
+  fclose(f);
+  if (stage) hipHostFree(stage);
+  if (r == GG_OK)
+    r = graph_build_device((const uint32_t *)d, ne, vertices, weighted, undirected, out);
+  device_free(d);
+  return r;
+}
+
+int gg_graph_destroy(gg_graph *g) {
+  if (!g) return GG_OK;
+  if (g->work) work_free(g->work);
+  for (void *p : g->allocations) device_free(p);
+  delete g;
+  return GG_OK;
+}
+
+int gg_graph_transpose(gg_graph *g) {
+  if (!g) return fail(GG_EINVAL, "null graph");
+  if (!g->symmetric) g->transposed ^= 1;
+  return GG_OK;
+}
+
+int gg_graph_get_info(const gg_graph *g, gg_graph_info *info) {
+  if (!g || !info) return fail(GG_EINVAL, "null argument");
+  const Dir &o = out_dir(g), &i = in_dir(g);
+  info->vertices = g->vertices;
+  info->edges = g->edges;
+  info->arcs = g->arcs;
+  info->weighted = g->weighted;
+  info->symmetric = g->symmetric;
+  info->transposed = g->transposed;
+  info->device_bytes = g->device_bytes;
+  info->d_out_degree = o.degree;
+  info->d_in_degree = i.degree;
+  info->d_out_index = o.index;
+  info->d_out_nbr = o.nbr;
+  info->d_out_w = o.w;
+  info->d_in_index = i.index;
+  info->d_in_nbr = i.nbr;
+  info->d_in_w = i.w;
+  return GG_OK;
+}
+
+}  // extern "C"

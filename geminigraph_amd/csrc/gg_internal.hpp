@@ -1,0 +1,150 @@
+// Internal definitions shared by the HIP sources of libgg.
+//
+// Layout in HBM (one graph, one partition):
+//   per direction (out = CSR keyed by src, in = CSC keyed by dst):
+//     index[V+1]  u64 offsets      nbr[arcs] u32 neighbour ids
+//     w[arcs]     f32 weights (weighted graphs only, structure of arrays)
+//     degree[V]   u32
+//     heavy chunk list: vertices whose degree exceeds the pull group limit,
+//     cut into chunks of at most kChunkEdges adjacency entries
+//   frontiers: u64 bitmap words (V/64 + 1) plus three degree-binned queues.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/gg.h"
+
+namespace gg {
+struct Work;
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;                 // 4 waves per workgroup
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr uint32_t kChunkEdges = 1024;      // heavy-vertex chunk (one wave, 16 x 64)
+constexpr uint32_t kUnset = 0xFFFFFFFFu;
+
+// ------------------------------------------------------------------ errors
+void set_error(int code, const std::string &msg);
+int fail(int code, const std::string &msg);
+#define GG_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e__ = (call);                                                      \
+    if (e__ != hipSuccess)                                                        \
+      return ::gg::fail(GG_EHIP, std::string(#call) + ": " + hipGetErrorString(e__)); \
+  } while (0)
+#define GG_TRY(expr)          \
+  do {                        \
+    int r__ = (expr);         \
+    if (r__ != GG_OK) return r__; \
+  } while (0)
+
+// ----------------------------------------------------------------- options
+struct Options {
+  int64_t switch_divisor = 20;
+  int64_t force_mode = 0;        // 0 auto, 1 push, 2 pull
+  int64_t push_small_max = 64;   // in-wave load-balanced bin
+  int64_t push_big_max = 4096;   // one wave per vertex; above: shared by the grid
+  int64_t pull_light_max = 1024; // in-group; above: heavy chunks (fixed at build)
+  int64_t pull_serial_max = 32;  // BFS pull: lane-serial scan
+  int64_t huge_grid_waves = 4096;
+  int64_t count_examined = 0;    // count adjacency entries read (roofline model)
+};
+Options &options();
+
+// ----------------------------------------------------------------- runtime
+int ensure_init();
+hipStream_t stream();
+int device_alloc(void **p, size_t bytes);
+void device_free(void *p);
+
+// Profiling: event pairs around launches of named kernels.
+struct ProfScope {
+  const char *name;
+  hipEvent_t start = nullptr, stop = nullptr;
+  explicit ProfScope(const char *n);
+  ~ProfScope();
+};
+
+// ------------------------------------------------------------------- graph
+struct Dir {
+  uint64_t *index = nullptr;  // V+1
+  uint32_t *nbr = nullptr;    // arcs
+  float *w = nullptr;         // arcs or null
+  uint32_t *degree = nullptr; // V
+  // heavy chunks (degree > light limit at build time)
+  uint32_t n_chunks = 0;
+  uint32_t *chunk_v = nullptr;
+  uint64_t *chunk_beg = nullptr;
+  uint64_t *chunk_end = nullptr;
+  uint32_t n_heavy = 0;
+  uint32_t *heavy_v = nullptr;     // heavy vertex list
+  uint32_t *chunk_h = nullptr;     // heavy index of each chunk
+  uint32_t light_max = 0;
+};
+
+}  // namespace gg
+
+struct gg_graph {
+  int device = 0;
+  uint32_t vertices = 0;
+  uint64_t edges = 0;
+  uint64_t arcs = 0;
+  int weighted = 0;
+  int symmetric = 0;
+  int transposed = 0;
+  size_t device_bytes = 0;
+  gg::Dir out, in;   // in aliases out's arrays when symmetric
+  std::vector<void *> allocations;
+  // scratch owned by the graph for traversals (lazily sized)
+  gg::Work *work = nullptr;
+};
+
+struct gg_subset {
+  uint32_t size = 0;
+  size_t words = 0;
+  uint64_t *d = nullptr;
+};
+
+namespace gg {
+
+// Device counters written by the frontier compaction (one set per frontier).
+struct Counters {
+  uint32_t n[3];      // binned queue sizes
+  uint32_t total;     // active vertices (popcount)
+  uint64_t edges;     // sum of bin degrees of active vertices
+};
+
+struct Frontier {
+  uint64_t *bits = nullptr;
+  uint32_t *q[3] = {nullptr, nullptr, nullptr};
+  float *snap_f[3] = {nullptr, nullptr, nullptr};   // message snapshots (u32/f32 bits)
+  Counters *cnt = nullptr;                          // device
+};
+
+struct Work {
+  uint32_t V = 0;
+  size_t words = 0;
+  Frontier f[2];
+  uint64_t *visited = nullptr;
+  uint64_t *aux_bits = nullptr;
+  void *msg = nullptr;         // V x 8 bytes two-phase messages
+  double *hacc = nullptr;      // heavy accumulators (max of both directions)
+  double *dacc = nullptr;      // per-iteration scalar accumulators (4096)
+  unsigned long long *exam = nullptr;  // 64 slots x 16 (one 128-byte line each)
+  Counters *h_cnt = nullptr;   // pinned host copy
+  unsigned long long *h_exam = nullptr;
+  std::vector<void *> allocations;
+};
+
+int work_get(gg_graph *g, Work **w);
+void work_free(Work *w);
+
+// Host-side view of one direction, swapped by transpose.
+inline const Dir &out_dir(const gg_graph *g) { return g->transposed ? g->in : g->out; }
+inline const Dir &in_dir(const gg_graph *g) { return g->transposed ? g->out : g->in; }
+
+}  // namespace gg

@@ -1,0 +1,315 @@
+// Device building blocks of the edge engine: wave primitives, frontier
+// activation and compaction, and the degree-binned push and group/heavy pull
+// skeletons that every toolkit op plugs into.
+//
+// Mapping of the reference's OpenMP engine (core/graph.hpp:1406-1867) onto
+// CDNA4 waves:
+//   * a 64-vertex bitmap word (WORD_OFFSET) is one wavefront: lane i owns
+//     vertex 64*word + i;
+//   * edge kernels only set bits of the output frontier (atomic OR,
+//     core/bitmap.hpp:56-58); a compaction pass then turns the bitmap into
+//     three queues binned by out-degree, with one atomic per workgroup, and
+//     produces the popcount and out-degree sum that drive the push/pull
+//     switch (core/graph.hpp:1503-1509);
+//   * sparse (push) mode consumes those queues: small lists are expanded by
+//     an in-wave load-balanced search (all 64 lanes walk consecutive edges of
+//     up to 64 vertices), mid-size lists by one wave per vertex, huge lists by
+//     every wave of the grid;
+//   * dense (pull) mode sweeps the CSC one 64-vertex group per wave with the
+//     same in-wave load balancing and a segmented wave reduction, while
+//     vertices above the group limit are cut into fixed chunks at build time
+//     (the GPU form of tune_chunks, core/graph.hpp:1341-1404) that combine
+//     through one atomic per chunk -- the split-partials contract of
+//     dense_signal/dense_slot (core/graph.hpp:1767-1770);
+//   * emit + slot of the reference is a direct update of the destination
+//     state with native HBM atomics (atomicCAS, atomicMin, f64 atomicAdd).
+#pragma once
+
+#include "gg_internal.hpp"
+
+namespace gg {
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ULL << __lane_id()) - 1ULL; }
+__device__ __forceinline__ int wave_in_block() { return (int)(threadIdx.x >> 6); }
+__device__ __forceinline__ uint32_t global_wave() {
+  return blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+}
+__device__ __forceinline__ bool test_bit(const uint64_t *bits, uint32_t v) {
+  return (bits[v >> 6] >> (v & 63)) & 1ULL;
+}
+// Orders LDS traffic between lanes of one wave.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t x, int src) {
+  uint32_t lo = __shfl((uint32_t)x, src), hi = __shfl((uint32_t)(x >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ double shfl_f64(double x, int src) {
+  return __longlong_as_double((long long)shfl_u64((uint64_t)__double_as_longlong(x), src));
+}
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t x, int d) {
+  uint32_t lo = __shfl_up((uint32_t)x, d), hi = __shfl_up((uint32_t)(x >> 32), d);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+template <class T> struct Shfl;
+template <> struct Shfl<uint32_t> {
+  static __device__ __forceinline__ uint32_t at(uint32_t x, int s) { return __shfl(x, s); }
+  static __device__ __forceinline__ uint32_t up(uint32_t x, int d) { return __shfl_up(x, d); }
+  static __device__ __forceinline__ uint32_t xr(uint32_t x, int m) { return __shfl_xor(x, m); }
+};
+template <> struct Shfl<float> {
+  static __device__ __forceinline__ float at(float x, int s) { return __shfl(x, s); }
+  static __device__ __forceinline__ float up(float x, int d) { return __shfl_up(x, d); }
+  static __device__ __forceinline__ float xr(float x, int m) { return __shfl_xor(x, m); }
+};
+template <> struct Shfl<double> {
+  static __device__ __forceinline__ double at(double x, int s) { return shfl_f64(x, s); }
+  static __device__ __forceinline__ double up(double x, int d) {
+    return __longlong_as_double((long long)shfl_up_u64((uint64_t)__double_as_longlong(x), d));
+  }
+  static __device__ __forceinline__ double xr(double x, int m) {
+    return shfl_f64(x, lane_id() ^ m);
+  }
+};
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
+  for (int m = 32; m >= 1; m >>= 1) x += shfl_u64(x, lane_id() ^ m);
+  return x;
+}
+__device__ __forceinline__ double wave_sum_f64(double x) {
+  for (int m = 32; m >= 1; m >>= 1) x += shfl_f64(x, lane_id() ^ m);
+  return x;
+}
+// Exclusive prefix sum of a u32 across the wave; *total = sum of all lanes.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t x, uint32_t *total) {
+  uint32_t inc = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t y = __shfl_up(inc, d);
+    if (lane_id() >= d) inc += y;
+  }
+  *total = __shfl(inc, 63);
+  return inc - x;
+}
+// Largest lane k with excl[k] <= j (excl non-decreasing, excl[0] = 0).
+__device__ __forceinline__ int wave_owner(uint32_t excl, uint32_t j) {
+  int k = 0;
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) {
+    uint32_t ec = __shfl(excl, k + s);
+    if (ec <= j) k += s;
+  }
+  return k;
+}
+
+// Adjacency entries examined, spread over 64 cache lines (optional).
+__device__ __forceinline__ void count_examined(unsigned long long *exam, uint64_t n) {
+  if (exam && n) atomicAdd(&exam[(blockIdx.x & 63) * 16], (unsigned long long)n);
+}
+
+// Sets v in the bitmap; returns whether this call flipped the bit.
+__device__ __forceinline__ bool activate(uint64_t *bits, uint32_t v) {
+  uint64_t bit = 1ULL << (v & 63);
+  uint64_t old = atomicOr((unsigned long long *)&bits[v >> 6], (unsigned long long)bit);
+  return !(old & bit);
+}
+
+// ------------------------------------------------------------- compaction
+// Bitmap -> three queues binned by bin_deg (<= small_max, <= big_max, above),
+// plus popcount and degree sum.  Optionally ORs the words into `visited`
+// (process_vertices(visited->set_bit) over active_out, bfs.cpp:79-85).
+__global__ __launch_bounds__(kBlock) void compact_kernel(const uint64_t *bits, size_t words,
+                                                         const uint32_t *bin_deg,
+                                                         uint32_t small_max, uint32_t big_max,
+                                                         uint32_t *q0, uint32_t *q1, uint32_t *q2,
+                                                         Counters *cnt, uint64_t *visited);
+
+// ------------------------------------------------------------------- push
+struct PushArgs {
+  const uint64_t *index;
+  const uint32_t *nbr;
+  const float *w;
+  const uint32_t *q0, *q1, *q2;
+  const uint32_t *s0, *s1, *s2;  // message snapshots (raw bits) or null
+  uint32_t n0, n1, n2;
+  uint32_t waves0, waves1, waves2;
+  uint64_t *out;
+  unsigned long long *exam;
+};
+
+// Op: using Msg; Msg msg(v, snap_bits); bool edge(src, msg, dst, w) -> candidate;
+//     void after(bool fresh, dst).
+template <class Op>
+__global__ __launch_bounds__(kBlock) void push_kernel(PushArgs a, Op op) {
+  const uint32_t gw = global_wave();
+  const int lane = lane_id();
+  using Msg = typename Op::Msg;
+  if (gw < a.waves0) {
+    // small bin: in-wave load-balanced expansion of 64 frontier vertices
+    uint32_t i = gw * 64 + lane;
+    bool valid = i < a.n0;
+    uint32_t v = valid ? a.q0[i] : 0;
+    uint64_t b = valid ? a.index[v] : 0;
+    uint32_t deg = valid ? (uint32_t)(a.index[v + 1] - b) : 0;
+    Msg m = op.msg(v, (valid && a.s0) ? a.s0[i] : 0u);
+    uint32_t total;
+    uint32_t excl = wave_excl_scan(deg, &total);
+    for (uint32_t j0 = 0; j0 < total; j0 += 64) {
+      uint32_t j = j0 + lane;
+      int k = wave_owner(excl, j);
+      uint32_t src = __shfl(v, k);
+      uint64_t bb = shfl_u64(b, k);
+      uint32_t ex = __shfl(excl, k);
+      Msg mm = Shfl<Msg>::at(m, k);
+      if (j < total) {
+        uint64_t e = bb + (j - ex);
+        uint32_t dst = a.nbr[e];
+        if (op.edge(src, mm, dst, a.w ? a.w[e] : 0.f)) op.after(activate(a.out, dst), dst);
+      }
+    }
+    if (lane == 0) count_examined(a.exam, total);
+  } else if (gw < a.waves0 + a.waves1) {
+    // mid bin: one wave per vertex
+    uint32_t i = gw - a.waves0;
+    uint32_t v = a.q1[i];
+    uint64_t b = a.index[v], e1 = a.index[v + 1];
+    Msg m = op.msg(v, a.s1 ? a.s1[i] : 0u);
+    for (uint64_t e = b + lane; e < e1; e += 64) {
+      uint32_t dst = a.nbr[e];
+      if (op.edge(v, m, dst, a.w ? a.w[e] : 0.f)) op.after(activate(a.out, dst), dst);
+    }
+    if (lane == 0) count_examined(a.exam, e1 - b);
+  } else {
+    // huge bin: every remaining wave takes chunks of every huge vertex
+    uint32_t hw = gw - a.waves0 - a.waves1;
+    uint64_t done = 0;
+    for (uint32_t h = 0; h < a.n2; h++) {
+      uint32_t v = a.q2[h];
+      uint64_t b = a.index[v], e1 = a.index[v + 1];
+      Msg m = op.msg(v, a.s2 ? a.s2[h] : 0u);
+      uint64_t nch = (e1 - b + kChunkEdges - 1) / kChunkEdges;
+      for (uint64_t c = hw; c < nch; c += a.waves2) {
+        uint64_t cb = b + c * kChunkEdges;
+        uint64_t ce = cb + kChunkEdges < e1 ? cb + kChunkEdges : e1;
+        for (uint64_t e = cb + lane; e < ce; e += 64) {
+          uint32_t dst = a.nbr[e];
+          if (op.edge(v, m, dst, a.w ? a.w[e] : 0.f)) op.after(activate(a.out, dst), dst);
+        }
+        done += ce - cb;
+      }
+    }
+    if (lane == 0) count_examined(a.exam, done);
+  }
+}
+
+// ------------------------------------------------------------------- pull
+struct PullArgs {
+  const uint64_t *index;
+  const uint32_t *nbr;
+  const float *w;
+  uint32_t V;
+  uint32_t light_max;
+  uint32_t ngroups;
+  uint64_t *out;
+  unsigned long long *exam;
+  double *dacc;  // scalar accumulator of the op (delta), may be null
+};
+
+// Reduction pull over one 64-vertex group per wave.
+// Op: using T; T identity(); bool dst_active(v); T value(src, w); T combine(a, b);
+//     bool finish(v, acc, deg, double &dsum) -> activation; void after(bool, v).
+template <class Op>
+__global__ __launch_bounds__(kBlock) void pull_group_kernel(PullArgs a, Op op) {
+  using T = typename Op::T;
+  __shared__ T lacc[kWavesPerBlock][64];
+  const uint32_t g = global_wave();
+  const int lane = lane_id();
+  const int wv = wave_in_block();
+  if (g >= a.ngroups) return;  // whole wave leaves together
+  uint32_t v = g * 64 + lane;
+  bool inr = v < a.V;
+  uint64_t b = inr ? a.index[v] : 0;
+  uint64_t e1 = inr ? a.index[v + 1] : 0;
+  uint64_t deg64 = e1 - b;
+  bool light = inr && deg64 <= a.light_max && op.dst_active(v);
+  uint32_t deg = light ? (uint32_t)deg64 : 0u;
+  lacc[wv][lane] = op.identity();
+  wave_sync();
+  uint32_t total;
+  uint32_t excl = wave_excl_scan(deg, &total);
+  for (uint32_t j0 = 0; j0 < total; j0 += 64) {
+    uint32_t j = j0 + lane;
+    bool act = j < total;
+    int k = wave_owner(excl, j);
+    uint64_t bb = shfl_u64(b, k);
+    uint32_t ex = __shfl(excl, k);
+    T x = op.identity();
+    if (act) {
+      uint64_t e = bb + (j - ex);
+      x = op.value(a.nbr[e], a.w ? a.w[e] : 0.f);
+    }
+    int kk = act ? k : 64;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      T y = Shfl<T>::up(x, d);
+      int ko = __shfl_up(kk, d);
+      if (lane >= d && ko == kk) x = op.combine(x, y);
+    }
+    int nk = __shfl_down(kk, 1);
+    bool last = (lane == 63) || nk != kk;
+    if (act && last) lacc[wv][kk] = op.combine(lacc[wv][kk], x);
+    wave_sync();
+  }
+  T acc = lacc[wv][lane];
+  double dsum = 0;
+  bool cand = light ? op.finish(v, acc, (uint32_t)deg64, dsum) : false;
+  // this wave owns word g; heavy chunks may set other bits of it
+  uint64_t m = __ballot(cand);
+  if (lane == 0 && m) atomicOr((unsigned long long *)&a.out[g], (unsigned long long)m);
+  op.after(cand, v);
+  if (a.dacc) {
+    double s = wave_sum_f64(dsum);
+    if (lane == 0 && s != 0.0) unsafeAtomicAdd(a.dacc, s);
+  }
+  if (lane == 0) count_examined(a.exam, total);
+}
+
+struct HeavyArgs {
+  const uint32_t *nbr;
+  const float *w;
+  const uint32_t *chunk_v, *chunk_h;
+  const uint64_t *chunk_beg, *chunk_end;
+  uint32_t n_chunks;
+  uint64_t *out;
+  unsigned long long *exam;
+};
+
+// One wave per heavy chunk; the chunk's partial goes through op.commit.
+// Op: T identity(); bool dst_active(v); T value(src, w); T combine(a,b);
+//     bool commit(v, h, partial) -> activation; void after(fresh, v).
+template <class Op>
+__global__ __launch_bounds__(kBlock) void pull_heavy_kernel(HeavyArgs a, Op op) {
+  using T = typename Op::T;
+  const uint32_t c = global_wave();
+  const int lane = lane_id();
+  if (c >= a.n_chunks) return;
+  uint32_t v = a.chunk_v[c];
+  if (!op.dst_active(v)) return;
+  uint64_t cb = a.chunk_beg[c], ce = a.chunk_end[c];
+  T x = op.identity();
+  for (uint64_t e = cb + lane; e < ce; e += 64) x = op.combine(x, op.value(a.nbr[e], a.w ? a.w[e] : 0.f));
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) x = op.combine(x, Shfl<T>::xr(x, m));
+  if (lane == 0) {
+    if (op.commit(v, a.chunk_h[c], x)) op.after(activate(a.out, v), v);
+    count_examined(a.exam, ce - cb);
+  }
+}
+
+}  // namespace gg

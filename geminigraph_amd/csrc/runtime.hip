@@ -1,0 +1,308 @@
+// Runtime pieces of libgg: errors, options, device selection, memory,
+// profiling and vertex subsets (device bitmaps).
+#include <string.h>
+
+#include <map>
+#include <mutex>
+
+#include "gg_internal.hpp"
+
+namespace gg {
+
+static thread_local std::string t_error;
+static thread_local int t_code = 0;
+
+void set_error(int code, const std::string &msg) {
+  t_code = code;
+  t_error = msg;
+}
+int fail(int code, const std::string &msg) {
+  set_error(code, msg);
+  return code;
+}
+
+Options &options() {
+  static Options o;
+  return o;
+}
+
+static int g_device = -1;
+static hipStream_t g_stream = nullptr;
+static std::mutex g_init_mu;
+
+int init_device(int device) {
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) return fail(GG_ENODEV, "no HIP device available");
+  if (device < 0 || device >= n) return fail(GG_EINVAL, "device index out of range");
+  if (g_device == device && g_stream) return GG_OK;
+  GG_HIP(hipSetDevice(device));
+  if (g_stream) {
+    hipStreamDestroy(g_stream);
+    g_stream = nullptr;
+  }
+  GG_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  g_device = device;
+  return GG_OK;
+}
+
+int ensure_init() {
+  if (g_device >= 0 && g_stream) {
+    GG_HIP(hipSetDevice(g_device));
+    return GG_OK;
+  }
+  return init_device(0);
+}
+
+hipStream_t stream() { return g_stream; }
+
+int device_alloc(void **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(GG_ENOMEM, "hipMalloc(" + std::to_string(bytes) + ") failed: " +
+                               hipGetErrorString(e));
+  }
+  return GG_OK;
+}
+void device_free(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+// ---------------------------------------------------------------- profiling
+struct ProfEntry {
+  double total_ms = 0;
+  uint64_t launches = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+static bool g_prof_on = false;
+static std::map<std::string, ProfEntry> g_prof;
+static std::mutex g_prof_mu;
+
+ProfScope::ProfScope(const char *n) : name(n) {
+  if (!g_prof_on) return;
+  if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess) {
+    start = stop = nullptr;
+    return;
+  }
+  hipEventRecord(start, g_stream);
+}
+ProfScope::~ProfScope() {
+  if (!start) return;
+  hipEventRecord(stop, g_stream);
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof[name].pending.emplace_back(start, stop);
+}
+
+static void prof_collect() {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  for (auto &kv : g_prof) {
+    for (auto &pr : kv.second.pending) {
+      hipEventSynchronize(pr.second);
+      float ms = 0;
+      hipEventElapsedTime(&ms, pr.first, pr.second);
+      kv.second.total_ms += ms;
+      kv.second.launches++;
+      hipEventDestroy(pr.first);
+      hipEventDestroy(pr.second);
+    }
+    kv.second.pending.clear();
+  }
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+const char *gg_last_error(void) { return t_error.c_str(); }
+
+int gg_init(int device) { return init_device(device); }
+
+int gg_synchronize(void) {
+  GG_TRY(ensure_init());
+  GG_HIP(hipStreamSynchronize(g_stream));
+  return GG_OK;
+}
+
+int gg_malloc(void **d_ptr, size_t bytes) {
+  if (!d_ptr) return fail(GG_EINVAL, "gg_malloc: null out pointer");
+  GG_TRY(ensure_init());
+  return device_alloc(d_ptr, bytes);
+}
+
+int gg_free(void *d_ptr) {
+  GG_TRY(ensure_init());
+  if (d_ptr) GG_HIP(hipFree(d_ptr));
+  return GG_OK;
+}
+
+int gg_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes) {
+  GG_TRY(ensure_init());
+  if (!bytes) return GG_OK;
+  if (!h_dst || !d_src) return fail(GG_EINVAL, "gg_memcpy_d2h: null pointer");
+  GG_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, g_stream));
+  GG_HIP(hipStreamSynchronize(g_stream));
+  return GG_OK;
+}
+
+int gg_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes) {
+  GG_TRY(ensure_init());
+  if (!bytes) return GG_OK;
+  if (!h_src || !d_dst) return fail(GG_EINVAL, "gg_memcpy_h2d: null pointer");
+  GG_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, g_stream));
+  GG_HIP(hipStreamSynchronize(g_stream));
+  return GG_OK;
+}
+
+int gg_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes) {
+  GG_TRY(ensure_init());
+  if (!bytes) return GG_OK;
+  if (!d_src || !d_dst) return fail(GG_EINVAL, "gg_memcpy_d2d: null pointer");
+  GG_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, g_stream));
+  return GG_OK;
+}
+
+int gg_mem_info(size_t *free_bytes, size_t *total_bytes) {
+  GG_TRY(ensure_init());
+  size_t f = 0, t = 0;
+  GG_HIP(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return GG_OK;
+}
+
+static int64_t *option_slot(const char *name) {
+  Options &o = options();
+  if (!name) return nullptr;
+  if (!strcmp(name, "switch_divisor")) return &o.switch_divisor;
+  if (!strcmp(name, "force_mode")) return &o.force_mode;
+  if (!strcmp(name, "push_small_max")) return &o.push_small_max;
+  if (!strcmp(name, "push_big_max")) return &o.push_big_max;
+  if (!strcmp(name, "pull_light_max")) return &o.pull_light_max;
+  if (!strcmp(name, "pull_serial_max")) return &o.pull_serial_max;
+  if (!strcmp(name, "huge_grid_waves")) return &o.huge_grid_waves;
+  if (!strcmp(name, "count_examined")) return &o.count_examined;
+  return nullptr;
+}
+
+int gg_set_option(const char *name, int64_t value) {
+  int64_t *s = option_slot(name);
+  if (!s) return fail(GG_EINVAL, std::string("unknown option: ") + (name ? name : "(null)"));
+  if (!strcmp(name, "force_mode") && (value < 0 || value > 2))
+    return fail(GG_EINVAL, "force_mode must be 0, 1 or 2");
+  if (!strcmp(name, "count_examined") && (value < 0 || value > 1))
+    return fail(GG_EINVAL, "count_examined must be 0 or 1");
+  if (strcmp(name, "force_mode") && strcmp(name, "count_examined") && value <= 0)
+    return fail(GG_EINVAL, std::string(name) + " must be positive");
+  if (!strcmp(name, "push_small_max") && value > 64 * 64)
+    return fail(GG_EINVAL, "push_small_max must be <= 4096");
+  if (!strcmp(name, "pull_light_max") && value > (1 << 20))
+    return fail(GG_EINVAL, "pull_light_max must be <= 2^20");
+  *s = value;
+  return GG_OK;
+}
+
+int gg_get_option(const char *name, int64_t *value) {
+  int64_t *s = option_slot(name);
+  if (!s || !value) return fail(GG_EINVAL, "unknown option or null value");
+  *value = *s;
+  return GG_OK;
+}
+
+int gg_prof_enable(int on) {
+  GG_TRY(ensure_init());
+  if (!on) prof_collect();
+  g_prof_on = on != 0;
+  return GG_OK;
+}
+
+int gg_prof_reset(void) {
+  prof_collect();
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof.clear();
+  return GG_OK;
+}
+
+int gg_prof_read(const char *kernel, double *total_ms, uint64_t *launches) {
+  if (!kernel) return fail(GG_EINVAL, "gg_prof_read: null kernel name");
+  prof_collect();
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  auto it = g_prof.find(kernel);
+  if (total_ms) *total_ms = it == g_prof.end() ? 0.0 : it->second.total_ms;
+  if (launches) *launches = it == g_prof.end() ? 0 : it->second.launches;
+  return GG_OK;
+}
+
+// ---------------------------------------------------------- vertex subsets
+
+int gg_subset_create(const gg_graph *g, gg_subset **out) {
+  if (!g || !out) return fail(GG_EINVAL, "gg_subset_create: null argument");
+  GG_TRY(ensure_init());
+  gg_subset *s = new gg_subset;
+  s->size = g->vertices;
+  s->words = ((size_t)g->vertices >> 6) + 1;
+  int r = device_alloc((void **)&s->d, s->words * 8);
+  if (r != GG_OK) {
+    delete s;
+    return r;
+  }
+  GG_HIP(hipMemsetAsync(s->d, 0, s->words * 8, g_stream));
+  *out = s;
+  return GG_OK;
+}
+
+int gg_subset_destroy(gg_subset *s) {
+  if (!s) return GG_OK;
+  device_free(s->d);
+  delete s;
+  return GG_OK;
+}
+
+int gg_subset_clear(gg_subset *s) {
+  if (!s) return fail(GG_EINVAL, "null subset");
+  GG_HIP(hipMemsetAsync(s->d, 0, s->words * 8, g_stream));
+  return GG_OK;
+}
+
+int gg_subset_fill(gg_subset *s) {
+  // core/bitmap.hpp:42-52: full words, then the exact tail of the last word.
+  if (!s) return fail(GG_EINVAL, "null subset");
+  size_t full = (size_t)s->size >> 6;
+  if (full) GG_HIP(hipMemsetAsync(s->d, 0xFF, full * 8, g_stream));
+  uint64_t tail = 0;
+  for (size_t i = full << 6; i < s->size; i++) tail |= 1ULL << (i & 63);
+  GG_HIP(hipMemcpyAsync(s->d + full, &tail, 8, hipMemcpyHostToDevice, g_stream));
+  GG_HIP(hipStreamSynchronize(g_stream));
+  return GG_OK;
+}
+
+int gg_subset_set_bit(gg_subset *s, uint32_t v) {
+  if (!s || v >= s->size) return fail(GG_EINVAL, "gg_subset_set_bit: vertex out of range");
+  uint64_t w = 0;
+  GG_HIP(hipMemcpyAsync(&w, s->d + (v >> 6), 8, hipMemcpyDeviceToHost, g_stream));
+  GG_HIP(hipStreamSynchronize(g_stream));
+  w |= 1ULL << (v & 63);
+  GG_HIP(hipMemcpyAsync(s->d + (v >> 6), &w, 8, hipMemcpyHostToDevice, g_stream));
+  GG_HIP(hipStreamSynchronize(g_stream));
+  return GG_OK;
+}
+
+int gg_subset_get_bit(const gg_subset *s, uint32_t v, int *bit) {
+  if (!s || !bit || v >= s->size) return fail(GG_EINVAL, "gg_subset_get_bit: bad argument");
+  uint64_t w = 0;
+  GG_HIP(hipMemcpyAsync(&w, s->d + (v >> 6), 8, hipMemcpyDeviceToHost, g_stream));
+  GG_HIP(hipStreamSynchronize(g_stream));
+  *bit = (int)((w >> (v & 63)) & 1ULL);
+  return GG_OK;
+}
+
+uint64_t *gg_subset_device_words(gg_subset *s) { return s ? s->d : nullptr; }
+
+}  // extern "C"

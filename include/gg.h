@@ -1,0 +1,180 @@
+/*
+ * gg.h -- C ABI of the MI355X-native Gemini edge-processing engine.
+ *
+ * One GPU, one partition.  Every entry point returns an int status
+ * (GG_OK = 0, negative on error) and leaves a message for gg_last_error()
+ * (thread-local).  The reference aborts through assert()/exit(-1) instead
+ * (core/graph.hpp and the toolkits' main()); the C++ facade in
+ * geminigraph_amd/core turns a non-zero status back into an abort.
+ *
+ * Pointers named d_* are device pointers (hipMalloc'd, e.g. by gg_malloc);
+ * pointers named h_* are host pointers.  No torch or HIP types appear here:
+ * streams are opaque (void*), NULL meaning the library's own stream.
+ *
+ * Each declaration names the reference interface it replaces
+ * (paths relative to the reference tree).
+ */
+#ifndef GG_H
+#define GG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GG_OK 0
+#define GG_EINVAL (-1)   /* bad argument / shape */
+#define GG_ENOMEM (-2)   /* device or host allocation failed */
+#define GG_EHIP (-3)     /* HIP runtime error */
+#define GG_EIO (-4)      /* edge-list file error */
+#define GG_ENODEV (-5)   /* no usable GPU */
+#define GG_EBUDGET (-6)  /* request exceeds the device-memory budget */
+
+typedef struct gg_graph gg_graph;
+typedef struct gg_subset gg_subset;
+
+/* ------------------------------------------------------------ runtime */
+
+/* Message of the last failing call on this thread. */
+const char *gg_last_error(void);
+/* Select and initialise a device (MPI_Instance ctor, core/mpi.hpp:53-57). */
+int gg_init(int device);
+/* Block until all work queued by the library has finished. */
+int gg_synchronize(void);
+/* Device memory helpers (alloc_vertex_array / dealloc_vertex_array,
+ * core/graph.hpp:229-243; gather_vertex_array, core/graph.hpp:310-324). */
+int gg_malloc(void **d_ptr, size_t bytes);
+int gg_free(void *d_ptr);
+int gg_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
+int gg_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes);
+int gg_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes);
+/* Free / total device memory in bytes. */
+int gg_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* Tunables (names below).  The switch divisor reproduces the reference's
+ * "sparse iff active_edges < edges / 20" (core/graph.hpp:1509) at 20.
+ *   "switch_divisor"   push/pull crossover divisor (default 20)
+ *   "force_mode"       0 = auto, 1 = always push, 2 = always pull
+ *   "push_small_max"   largest out-degree handled by in-wave load balancing
+ *   "push_big_max"     largest out-degree handled by one wave per vertex
+ *   "pull_light_max"   largest in-degree handled inside a 64-vertex group
+ *   "pull_serial_max"  BFS pull: largest in-degree scanned by a single lane */
+int gg_set_option(const char *name, int64_t value);
+int gg_get_option(const char *name, int64_t *value);
+
+/* ------------------------------------------------------ synthetic input */
+
+/* RMAT edge list written to device memory: num_edges packed records of
+ * 8 bytes <u32 src,u32 dst> or 12 bytes <u32 src,u32 dst,f32 w>
+ * (EdgeUnit<E>, core/type.hpp:27-41).  The reference ships no generator;
+ * this one is counter-based so the CPU oracle reproduces it bit for bit. */
+int gg_rmat_generate(void *d_edges, uint64_t num_edges, int scale, double a, double b,
+                     double c, uint64_t seed, int permute, int weighted);
+
+/* -------------------------------------------------------------- graphs */
+
+typedef struct gg_graph_info {
+  uint32_t vertices;
+  uint64_t edges;       /* file edge count, used by the switch */
+  uint64_t arcs;        /* adjacency entries per direction */
+  int weighted;
+  int symmetric;
+  int transposed;
+  size_t device_bytes;  /* device memory held by the graph */
+  const uint32_t *d_out_degree;
+  const uint32_t *d_in_degree;
+  const uint64_t *d_out_index;   /* V+1 */
+  const uint32_t *d_out_nbr;
+  const float *d_out_w;          /* NULL when unweighted */
+  const uint64_t *d_in_index;
+  const uint32_t *d_in_nbr;
+  const float *d_in_w;
+} gg_graph_info;
+
+/* Build CSR (outgoing, keyed by src) and CSC (incoming, keyed by dst) from a
+ * packed edge list, on the device with a radix sort.  Duplicates and
+ * self-loops are kept.  undirected != 0 symmetrises first and aliases CSC to
+ * CSR, with `edges` kept at the input count.
+ * Replaces Graph<E>::load_directed (core/graph.hpp:772-1339) and
+ * Graph<E>::load_undirected_from_directed (core/graph.hpp:350-757) minus the
+ * file read.  edges_on_device != 0: `edges` is a device pointer. */
+int gg_graph_build(const void *edges, int edges_on_device, uint64_t num_edges,
+                   uint32_t vertices, int weighted, int undirected, gg_graph **out);
+/* Same, reading a binary edge-list file: |E| = file size / record size
+ * (core/graph.hpp:359-360, :781-782). */
+int gg_graph_load(const char *path, uint32_t vertices, int weighted, int undirected,
+                  gg_graph **out);
+int gg_graph_destroy(gg_graph *g);
+/* Swap directions without moving data (Graph<E>::transpose, core/graph.hpp:760-769). */
+int gg_graph_transpose(gg_graph *g);
+int gg_graph_get_info(const gg_graph *g, gg_graph_info *info);
+
+/* ------------------------------------------------------ vertex subsets */
+/* Device bitmaps of u64 words, WORD_OFFSET(i) = i >> 6
+ * (Bitmap / VertexSubset, core/bitmap.hpp:20-61;
+ *  Graph<E>::alloc_vertex_subset, core/graph.hpp:327-329). */
+int gg_subset_create(const gg_graph *g, gg_subset **out);
+int gg_subset_destroy(gg_subset *s);
+int gg_subset_clear(gg_subset *s);
+int gg_subset_fill(gg_subset *s);
+int gg_subset_set_bit(gg_subset *s, uint32_t v);
+int gg_subset_get_bit(const gg_subset *s, uint32_t v, int *bit);
+int gg_subset_count(const gg_subset *s, uint64_t *count);
+uint64_t *gg_subset_device_words(gg_subset *s);
+
+/* ------------------------------------------------------------ toolkits */
+
+typedef struct gg_run_stats {
+  int iterations;          /* process_edges calls */
+  int push_iterations;     /* of which sparse (push) */
+  uint64_t edges_examined; /* adjacency entries read by edge kernels */
+  uint64_t m_reach;        /* sum of out-degrees of reached vertices */
+  uint64_t reached;        /* reached vertices (BFS/SSSP/BC) or components (CC) */
+  double device_ms;        /* device time of the compute, by HIP events */
+  /* per process_edges call (first 64 calls): */
+  uint32_t modes[64];          /* 0 = push (sparse), 1 = pull (dense) */
+  uint32_t it_active[64];      /* active vertices passed in */
+  uint32_t it_new[64];         /* vertices activated by the call */
+  uint64_t it_examined[64];    /* adjacency entries read (option count_examined) */
+  uint64_t it_unvisited[64];   /* BFS: vertices not yet visited before the call */
+} gg_run_stats;
+
+/* PageRank, toolkits/pagerank.cpp:24-129: FP64, d = 0.85, non-normalised,
+ * `iterations` pull sweeps.  d_curr_out[V] receives the final ranks;
+ * h_deltas (iterations+1 entries, may be NULL) the printed deltas. */
+int gg_pagerank(gg_graph *g, int iterations, double *d_curr_out, double *h_deltas,
+                gg_run_stats *stats);
+/* BFS, toolkits/bfs.cpp:22-109.  d_parent[V] (sentinel V), d_depth[V]
+ * (0xFFFFFFFF when unreached, else the level); h_frontier[cap] receives the
+ * active-vertex count of every level (the reference's "active(i)" lines).
+ * Returns the number of levels in stats->iterations. */
+int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *d_depth,
+           uint32_t *h_frontier, int cap, gg_run_stats *stats);
+/* Connected components by min-label propagation, toolkits/cc.cpp:22-109,
+ * on a graph built with undirected != 0. */
+int gg_cc(gg_graph *g, uint32_t *d_label, uint32_t *h_frontier, int cap,
+          gg_run_stats *stats);
+/* Single-source shortest paths, toolkits/sssp.cpp:22-105 (f32 weights,
+ * sentinel 1e9). */
+int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int cap,
+            gg_run_stats *stats);
+/* Betweenness dependencies from one root, toolkits/bc.cpp:24-196 (FP64).
+ * d_dep[V] = dependencies, d_inv_paths[V] = 1 / num_paths. */
+int gg_bc(gg_graph *g, uint32_t root, double *d_dep, double *d_inv_paths,
+          uint32_t *h_frontier, int cap, gg_run_stats *stats);
+
+/* ----------------------------------------------------------- profiling */
+
+/* Per-kernel device time accumulated with HIP events on the launching
+ * stream while enabled.  kernel: "pull_light", "pull_heavy", "push",
+ * "pr_pull", ...  Returns total milliseconds and launch count. */
+int gg_prof_enable(int on);
+int gg_prof_reset(void);
+int gg_prof_read(const char *kernel, double *total_ms, uint64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,299 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.  Bar (from the reference's contract):
+  * BFS depths, CC labels, reachability sets and per-iteration frontier sizes
+    bit-exact; BFS parents valid;
+  * SSSP float32 distances bit-exact (the north-star tolerance is 1 ulp);
+  * PageRank and BC doubles within 1e-9 relative; BC NaN positions equal.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-9  # FP64 tolerance for PageRank and BC
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def roots_of(out_degree, k, seed):
+    cand = np.nonzero(out_degree)[0]
+    rng = np.random.default_rng(seed)
+    return [int(x) for x in rng.choice(cand, size=min(k, len(cand)), replace=False)]
+
+
+def assert_rel(got, ref, rel=REL):
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    assert got.shape == ref.shape
+    nan_g, nan_r = np.isnan(got), np.isnan(ref)
+    assert np.array_equal(nan_g, nan_r), f"NaN mismatch at {np.nonzero(nan_g != nan_r)[0][:10]}"
+    inf_g, inf_r = np.isinf(got), np.isinf(ref)
+    assert np.array_equal(inf_g, inf_r)
+    m = ~(nan_r | inf_r)
+    if m.any():
+        err = np.abs(got[m] - ref[m]) / np.maximum(np.abs(ref[m]), 1e-300)
+        bad = err > rel
+        # exact zeros must stay zero
+        assert not bad.any(), (f"max rel err {err.max():.3e} at {np.nonzero(m)[0][np.argmax(err)]}")
+
+
+def check_parents(e, V, root, parent, depth):
+    src = e["src"].astype(np.int64)
+    dst = e["dst"].astype(np.int64)
+    reached = depth != 0xFFFFFFFF
+    assert np.array_equal(parent < V, reached)
+    assert parent[root] == root
+    key = src * V + dst
+    key.sort()
+    vs = np.nonzero(reached)[0]
+    vs = vs[vs != root]
+    p = parent[vs].astype(np.int64)
+    assert (depth[p] + 1 == depth[vs]).all()
+    k = p * V + vs
+    pos = np.searchsorted(key, k)
+    assert (pos < len(key)).all() and (key[np.minimum(pos, len(key) - 1)] == k).all()
+
+
+MODES = {"auto": 0, "push": 1, "pull": 2}
+
+
+# ------------------------------------------------------------ generator/build
+
+@pytest.mark.parametrize("scale,weighted,permute", [(10, False, True), (14, True, True),
+                                                    (16, False, False)])
+def test_rmat_device_matches_oracle(gg, scale, weighted, permute):
+    ne = (1 << scale) * 16
+    say("rmat", scale, weighted, permute)
+    buf = gg.rmat_device(scale, ne, seed=77, permute=permute, weighted=weighted)
+    got = buf.numpy()
+    ref = oracle.rmat_edges(scale, seed=77, permute=permute, weighted=weighted)
+    assert np.array_equal(got, ref.view(np.uint8))
+
+
+def _segments_equal(idx_a, nbr_a, idx_b, nbr_b, w_a=None, w_b=None):
+    assert np.array_equal(idx_a, idx_b)
+    # the device build sorts every list by neighbour (ties in input order)
+    V = len(idx_a) - 1
+    keys_b = np.repeat(np.arange(V, dtype=np.int64), np.diff(idx_b).astype(np.int64))
+    order = np.lexsort((np.arange(len(nbr_b)), nbr_b, keys_b))
+    assert np.array_equal(nbr_a, nbr_b[order])
+    if w_a is not None:
+        assert np.array_equal(w_a, w_b[order])
+
+
+@pytest.mark.parametrize("weighted,undirected", [(False, False), (True, False), (False, True)])
+def test_build_matches_oracle(gg, weighted, undirected):
+    scale = 12
+    e = oracle.rmat_edges(scale, seed=5, weighted=weighted)
+    V = 1 << scale
+    say("build", weighted, undirected)
+    g = gg.Graph.build(e, V, undirected=undirected)
+    o = oracle.Graph(e, V, undirected=undirected)
+    assert g.edges == o.edges and g.arcs == o.arcs
+    assert np.array_equal(g.out_degree(), o.out_degree())
+    assert np.array_equal(g.in_degree(), o.in_degree())
+    _segments_equal(*g.csr()[:2], *o.csr()[:2], g.csr()[2], o.csr()[2])
+    _segments_equal(*g.csc()[:2], *o.csc()[:2], g.csc()[2], o.csc()[2])
+
+
+def test_load_file_matches_build(gg, tmp_path):
+    e = oracle.rmat_edges(11, seed=9, weighted=True)
+    p = tmp_path / "g.bin"
+    e.tofile(p)
+    V = 1 << 11
+    a = gg.Graph.load_directed(p, V, weighted=True)
+    b = gg.Graph.build(e, V)
+    assert a.edges == len(e)
+    for x, y in zip(a.csr(), b.csr()):
+        assert np.array_equal(x, y)
+    u = gg.Graph.load_undirected_from_directed(p, V, weighted=True)
+    assert u.edges == len(e) and u.arcs == 2 * len(e)
+
+
+def test_bad_inputs_raise(gg, tmp_path):
+    e = oracle.edges_from_pairs([(0, 5)])
+    with pytest.raises(gg.GGError):
+        gg.Graph.build(e, 4)  # endpoint >= vertices
+    with pytest.raises(gg.GGError):
+        gg.Graph.load_directed(tmp_path / "missing.bin", 10)
+    g = gg.Graph.build(oracle.edges_from_pairs([(0, 1)]), 2)
+    with pytest.raises(gg.GGError):
+        g.bfs(5)
+    with pytest.raises(gg.GGError):
+        g.sssp(0)  # unweighted
+
+
+# ----------------------------------------------------------------- PageRank
+
+@pytest.mark.parametrize("scale,ef,seed", [(12, 16, 1), (16, 16, 2024)])
+def test_pagerank_matches_oracle(gg, scale, ef, seed):
+    e = oracle.rmat_edges(scale, ef, seed=seed)
+    V = 1 << scale
+    say("pagerank", scale, ef)
+    g = gg.Graph.build(e, V)
+    pr, deltas, st = g.pagerank(20)
+    ref, ref_d = oracle.Graph(e, V).pagerank(20)
+    assert_rel(pr, ref)
+    assert_rel(deltas[:20], ref_d[:20], rel=1e-8)
+    assert st["iterations"] == 20
+
+
+def test_pagerank_heavy_vertices(gg):
+    # a hub with in-degree 5000 exercises the chunked heavy path
+    n = 6000
+    pairs = [(i, 0) for i in range(1, 5001)] + [(0, i) for i in range(1, 3000)] + \
+            [(i, i + 1) for i in range(1, n - 1)]
+    e = oracle.edges_from_pairs(pairs)
+    g = gg.Graph.build(e, n)
+    pr, _, _ = g.pagerank(10)
+    ref, _ = oracle.Graph(e, n).pagerank(10)
+    assert_rel(pr, ref)
+
+
+# ---------------------------------------------------------------------- BFS
+
+@pytest.mark.parametrize("mode", ["auto", "push", "pull"])
+@pytest.mark.parametrize("scale,seed", [(12, 3), (16, 4)])
+def test_bfs_matches_oracle(gg, scale, seed, mode):
+    e = oracle.rmat_edges(scale, seed=seed)
+    V = 1 << scale
+    g = gg.Graph.build(e, V)
+    o = oracle.Graph(e, V)
+    for root in roots_of(o.out_degree(), 3, seed):
+        say("bfs", scale, mode, root)
+        with gg.options(force_mode=MODES[mode]):
+            parent, depth, fr, st = g.bfs(root)
+        _, ref_depth, ref_fr = o.bfs(root)
+        assert np.array_equal(depth, ref_depth)
+        assert np.array_equal(fr, ref_fr)
+        check_parents(e, V, root, parent, depth)
+        reached = depth != 0xFFFFFFFF
+        assert st["m_reach"] == int(o.out_degree()[reached].astype(np.int64).sum())
+
+
+def test_bfs_edge_cases(gg):
+    # self-loops, duplicates, isolated vertices, V not a multiple of 64
+    pairs = [(0, 0), (0, 1), (0, 1), (1, 2), (2, 0), (5, 6), (70, 71), (1, 70)]
+    e = oracle.edges_from_pairs(pairs)
+    V = 77
+    g = gg.Graph.build(e, V)
+    o = oracle.Graph(e, V)
+    for root in (0, 5, 76):
+        parent, depth, fr, _ = g.bfs(root)
+        _, rd, rf = o.bfs(root)
+        assert np.array_equal(depth, rd) and np.array_equal(fr, rf)
+        check_parents(e, V, root, parent, depth)
+
+
+def test_bfs_empty_graph(gg):
+    e = oracle.edges_from_pairs(np.zeros((0, 2), dtype=np.uint32))
+    g = gg.Graph.build(e, 10)
+    parent, depth, fr, st = g.bfs(3)
+    assert fr.tolist() == [1]
+    assert depth[3] == 0 and (np.delete(depth, 3) == 0xFFFFFFFF).all()
+    assert parent[3] == 3 and (np.delete(parent, 3) == 10).all()
+
+
+def test_bfs_hub_bins(gg):
+    # out-degree 20000 (huge push bin), 3000 (mid bin), in-degree 20000 (heavy pull)
+    n = 40000
+    pairs = [(0, i) for i in range(1, 20001)] + [(1, i) for i in range(20001, 23001)] + \
+            [(i, 39999) for i in range(2, 20002)] + [(39999, 1)]
+    e = oracle.edges_from_pairs(pairs)
+    g = gg.Graph.build(e, n)
+    o = oracle.Graph(e, n)
+    for mode in ("push", "pull", "auto"):
+        with gg.options(force_mode=MODES[mode]):
+            parent, depth, fr, _ = g.bfs(0)
+        _, rd, rf = o.bfs(0)
+        assert np.array_equal(depth, rd) and np.array_equal(fr, rf), mode
+        check_parents(e, n, 0, parent, depth)
+
+
+# ----------------------------------------------------------------------- CC
+
+@pytest.mark.parametrize("mode", ["auto", "push", "pull"])
+@pytest.mark.parametrize("scale,seed", [(12, 5), (16, 6)])
+def test_cc_matches_oracle(gg, scale, seed, mode):
+    e = oracle.rmat_edges(scale, seed=seed)
+    V = 1 << scale
+    say("cc", scale, mode)
+    g = gg.Graph.build(e, V, undirected=True)
+    with gg.options(force_mode=MODES[mode]):
+        label, fr, st = g.cc()
+    ref, ref_fr = oracle.Graph(e, V, undirected=True).cc()
+    assert np.array_equal(label, ref)
+    assert np.array_equal(fr, ref_fr)
+
+
+# --------------------------------------------------------------------- SSSP
+
+@pytest.mark.parametrize("mode", ["auto", "push", "pull"])
+@pytest.mark.parametrize("scale,seed", [(12, 7), (15, 8)])
+def test_sssp_matches_oracle(gg, scale, seed, mode):
+    e = oracle.rmat_edges(scale, seed=seed, weighted=True)
+    V = 1 << scale
+    g = gg.Graph.build(e, V)
+    o = oracle.Graph(e, V)
+    for root in roots_of(o.out_degree(), 2, seed):
+        say("sssp", scale, mode, root)
+        with gg.options(force_mode=MODES[mode]):
+            dist, fr, st = g.sssp(root)
+        ref, ref_fr = o.sssp(root)
+        assert np.array_equal(dist.view(np.uint32), ref.view(np.uint32))
+        assert np.array_equal(fr, ref_fr)
+
+
+# ----------------------------------------------------------------------- BC
+
+@pytest.mark.parametrize("mode", ["auto", "push", "pull"])
+@pytest.mark.parametrize("scale,seed", [(12, 9), (15, 10)])
+def test_bc_matches_oracle(gg, scale, seed, mode):
+    e = oracle.rmat_edges(scale, seed=seed)
+    V = 1 << scale
+    g = gg.Graph.build(e, V)
+    o = oracle.Graph(e, V)
+    for root in roots_of(o.out_degree(), 2, seed):
+        say("bc", scale, mode, root)
+        with gg.options(force_mode=MODES[mode]):
+            dep, inv, fr, st = g.bc(root)
+        rdep, rinv, rfr = o.bc(root)
+        assert np.array_equal(fr, rfr)
+        assert_rel(inv, rinv)
+        assert_rel(dep, rdep)
+
+
+def test_bc_hub_graph(gg):
+    n = 30000
+    pairs = [(0, i) for i in range(1, 12001)] + [(i, 12001 + (i % 50)) for i in range(1, 12001)] + \
+            [(12001 + j, 20000) for j in range(50)] + [(i, 20000) for i in range(1, 3000)]
+    e = oracle.edges_from_pairs(pairs)
+    g = gg.Graph.build(e, n)
+    o = oracle.Graph(e, n)
+    for mode in ("push", "pull", "auto"):
+        with gg.options(force_mode=MODES[mode]):
+            dep, inv, fr, _ = g.bc(0)
+        rdep, rinv, rfr = o.bc(0)
+        assert np.array_equal(fr, rfr), mode
+        assert_rel(inv, rinv)
+        assert_rel(dep, rdep)
+
+
+def test_repeat_runs_identical(gg):
+    e = oracle.rmat_edges(14, seed=31)
+    V = 1 << 14
+    g = gg.Graph.build(e, V)
+    root = roots_of(g.out_degree(), 1, 1)[0]
+    d1 = g.bfs(root)[1]
+    d2 = g.bfs(root)[1]
+    assert np.array_equal(d1, d2)
+    gu = gg.Graph.build(e, V, undirected=True)
+    assert np.array_equal(gu.cc()[0], gu.cc()[0])
