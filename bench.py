@@ -56,7 +56,9 @@ def bfs_model_bytes(st, V):
 
 def cpu_baseline(scale, ef, seed, roots, gpu_depths, sample_roots):
     import oracle
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = os.environ.get("OMP_NUM_THREADS")
+    if not threads:
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     threads = max(1, min(int(threads), 64))
     oracle.set_threads(threads)
     t0 = time.time()
@@ -171,6 +173,20 @@ def main():
     }
 
     extras = {"build_s": round(build_s, 2)}
+    # measured stream-copy bandwidth (plain copy kernel, 1 GiB read + 1 GiB write)
+    nb = 1 << 30
+    src, dst = gg.DeviceArray(nb, np.uint8), gg.DeviceArray(nb, np.uint8)
+    gg.device_copy(dst, src, nb)
+    gg.prof_reset()
+    gg.prof_enable(True)
+    for _ in range(3):
+        gg.device_copy(dst, src, nb)
+    gg.prof_enable(False)
+    cms, cn = gg.prof_read("copy")
+    extras["stream_copy_GBs"] = round(2 * nb * cn / (cms / 1e3) / 1e9, 1)
+    roofline["frac_of_stream_copy"] = round(achieved / extras["stream_copy_GBs"], 4)
+    src.free()
+    dst.free()
     if not args.no_extras:
         pr = gg.DeviceArray(V, np.float64)
         g.pagerank_into(pr, 20)

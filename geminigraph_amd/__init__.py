@@ -105,6 +105,7 @@ SIGNATURES = {
     "gg_memcpy_h2d": (_i32, [_vp, _vp, _sz]),
     "gg_memcpy_d2d": (_i32, [_vp, _vp, _sz]),
     "gg_mem_info": (_i32, [ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
+    "gg_device_copy": (_i32, [_vp, _vp, _sz]),
     "gg_set_option": (_i32, [_cp, _i64]),
     "gg_get_option": (_i32, [_cp, ctypes.POINTER(_i64)]),
     "gg_rmat_generate": (_i32, [_vp, _u64, _i32, _dbl, _dbl, _dbl, _u64, _i32, _i32]),
@@ -169,6 +170,11 @@ def get_option(name):
     v = ctypes.c_int64()
     _check(lib().gg_get_option(name.encode(), ctypes.byref(v)))
     return v.value
+
+
+def device_copy(dst, src, nbytes):
+    """Plain streaming copy kernel between two DeviceArrays (HBM calibration)."""
+    _check(lib().gg_device_copy(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), int(nbytes)))
 
 
 def mem_info():

@@ -815,9 +815,30 @@ static void put_stats(gg_run_stats *dst, const gg_run_stats &src) {
 
 }  // namespace gg
 
+namespace gg {
+__global__ void gg_copy_kernel(uint4 *dst, const uint4 *src, size_t n) {
+  size_t s = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += s) dst[i] = src[i];
+}
+}  // namespace gg
+
 using namespace gg;
 
 extern "C" {
+
+int gg_device_copy(void *d_dst, const void *d_src, size_t bytes) {
+  GG_TRY(ensure_init());
+  if (!d_dst || !d_src || (bytes & 15) || (((uintptr_t)d_dst | (uintptr_t)d_src) & 15))
+    return fail(GG_EINVAL, "gg_device_copy: 16-byte aligned pointers and size required");
+  size_t n = bytes / 16;
+  if (n) {
+    ProfScope p("copy");
+    hipLaunchKernelGGL(gg_copy_kernel, dim3(grid1(n, 8192)), dim3(kBlock), 0, stream(),
+                       (uint4 *)d_dst, (const uint4 *)d_src, n);
+  }
+  GG_HIP(hipStreamSynchronize(stream()));
+  return GG_OK;
+}
 
 int gg_subset_count(const gg_subset *s, uint64_t *count) {
   if (!s || !count) return fail(GG_EINVAL, "gg_subset_count: null argument");

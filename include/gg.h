@@ -52,6 +52,10 @@ int gg_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes);
 int gg_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes);
 /* Free / total device memory in bytes. */
 int gg_mem_info(size_t *free_bytes, size_t *total_bytes);
+/* Streaming device copy by a plain HIP kernel ("gg_copy_kernel").  No
+ * counterpart in the reference: it calibrates HBM byte counters
+ * (FETCH_SIZE / WRITE_SIZE) against a kernel of known traffic. */
+int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
 
 /* Tunables (names below).  The switch divisor reproduces the reference's
  * "sparse iff active_edges < edges / 20" (core/graph.hpp:1509) at 20.
