@@ -86,7 +86,13 @@ class GraphInfo(ctypes.Structure):
         ("d_in_index", ctypes.c_void_p),
         ("d_in_nbr", ctypes.c_void_p),
         ("d_in_w", ctypes.c_void_p),
+        ("reordered", ctypes.c_int),
+        ("d_perm", ctypes.c_void_p),
+        ("d_iperm", ctypes.c_void_p),
     ]
+
+
+BUILD_REORDER = 1
 
 
 # C ABI signatures: name -> (restype, argtypes)
@@ -110,7 +116,9 @@ SIGNATURES = {
     "gg_get_option": (_i32, [_cp, ctypes.POINTER(_i64)]),
     "gg_rmat_generate": (_i32, [_vp, _u64, _i32, _dbl, _dbl, _dbl, _u64, _i32, _i32]),
     "gg_graph_build": (_i32, [_vp, _i32, _u64, _u32, _i32, _i32, _pp]),
+    "gg_graph_build_ex": (_i32, [_vp, _i32, _u64, _u32, _i32, _i32, ctypes.c_uint, _pp]),
     "gg_graph_load": (_i32, [_cp, _u32, _i32, _i32, _pp]),
+    "gg_graph_load_ex": (_i32, [_cp, _u32, _i32, _i32, ctypes.c_uint, _pp]),
     "gg_graph_destroy": (_i32, [_vp]),
     "gg_graph_transpose": (_i32, [_vp]),
     "gg_graph_get_info": (_i32, [_vp, ctypes.POINTER(GraphInfo)]),
@@ -264,48 +272,50 @@ class Graph:
 
     # -- construction -----------------------------------------------------
     @classmethod
-    def build(cls, edges, vertices, undirected=False):
-        """From a host structured array of packed records (oracle.EDGE_DTYPE / WEDGE_DTYPE)."""
+    def build(cls, edges, vertices, undirected=False, reorder=False):
+        """From a host structured array of packed records (oracle.EDGE_DTYPE / WEDGE_DTYPE).
+        reorder=True relabels vertices internally by descending out-degree."""
         edges = np.ascontiguousarray(edges)
         weighted = edges.dtype.itemsize == 12
         h = ctypes.c_void_p()
-        _check(lib().gg_graph_build(ctypes.c_void_p(edges.ctypes.data), 0, len(edges),
-                                    int(vertices), int(weighted), int(undirected),
-                                    ctypes.byref(h)))
+        _check(lib().gg_graph_build_ex(ctypes.c_void_p(edges.ctypes.data), 0, len(edges),
+                                       int(vertices), int(weighted), int(undirected),
+                                       BUILD_REORDER if reorder else 0, ctypes.byref(h)))
         return cls(h.value)
 
     @classmethod
-    def build_device(cls, dev_edges, num_edges, vertices, weighted=False, undirected=False):
+    def build_device(cls, dev_edges, num_edges, vertices, weighted=False, undirected=False,
+                     reorder=False):
         h = ctypes.c_void_p()
-        _check(lib().gg_graph_build(ctypes.c_void_p(dev_edges.ptr), 1, int(num_edges),
-                                    int(vertices), int(weighted), int(undirected),
-                                    ctypes.byref(h)))
+        _check(lib().gg_graph_build_ex(ctypes.c_void_p(dev_edges.ptr), 1, int(num_edges),
+                                       int(vertices), int(weighted), int(undirected),
+                                       BUILD_REORDER if reorder else 0, ctypes.byref(h)))
         return cls(h.value)
 
     @classmethod
     def rmat(cls, scale, edge_factor=16, seed=1, a=RMAT_A, b=RMAT_B, c=RMAT_C, permute=True,
-             weighted=False, undirected=False):
+             weighted=False, undirected=False, reorder=False):
         ne = (1 << scale) * edge_factor
         buf = rmat_device(scale, ne, seed, a, b, c, permute, weighted)
         try:
-            return cls.build_device(buf, ne, 1 << scale, weighted, undirected)
+            return cls.build_device(buf, ne, 1 << scale, weighted, undirected, reorder)
         finally:
             buf.free()
 
     @classmethod
-    def load_directed(cls, path, vertices, weighted=False):
+    def load_directed(cls, path, vertices, weighted=False, reorder=False):
         """core/graph.hpp:772 -- |E| = file size / record size."""
         h = ctypes.c_void_p()
-        _check(lib().gg_graph_load(str(path).encode(), int(vertices), int(weighted), 0,
-                                   ctypes.byref(h)))
+        _check(lib().gg_graph_load_ex(str(path).encode(), int(vertices), int(weighted), 0,
+                                      BUILD_REORDER if reorder else 0, ctypes.byref(h)))
         return cls(h.value)
 
     @classmethod
-    def load_undirected_from_directed(cls, path, vertices, weighted=False):
+    def load_undirected_from_directed(cls, path, vertices, weighted=False, reorder=False):
         """core/graph.hpp:350 -- symmetric CSR, CSC aliased, |E| kept."""
         h = ctypes.c_void_p()
-        _check(lib().gg_graph_load(str(path).encode(), int(vertices), int(weighted), 1,
-                                   ctypes.byref(h)))
+        _check(lib().gg_graph_load_ex(str(path).encode(), int(vertices), int(weighted), 1,
+                                      BUILD_REORDER if reorder else 0, ctypes.byref(h)))
         return cls(h.value)
 
     def close(self):
@@ -337,25 +347,51 @@ class Graph:
             _check(lib().gg_memcpy_d2h(out.ctypes.data, ctypes.c_void_p(ptr), out.nbytes))
         return out
 
+    @property
+    def reordered(self):
+        return bool(self.info().reordered)
+
+    def perm(self):
+        """perm[original id] = internal id (identity when not reordered)."""
+        i = self.info()
+        if not i.reordered:
+            return np.arange(self.vertices, dtype=np.uint32)
+        return self._copy(i.d_perm, self.vertices, np.uint32)
+
     def out_degree(self):
-        return self._copy(self.info().d_out_degree, self.vertices, np.uint32)
+        d = self._copy(self.info().d_out_degree, self.vertices, np.uint32)
+        return d[self.perm()] if self.reordered else d
 
     def in_degree(self):
-        return self._copy(self.info().d_in_degree, self.vertices, np.uint32)
+        d = self._copy(self.info().d_in_degree, self.vertices, np.uint32)
+        return d[self.perm()] if self.reordered else d
+
+    def _adjacency(self, idx_p, nbr_p, w_p):
+        idx = self._copy(idx_p, self.vertices + 1, np.uint64)
+        nbr = self._copy(nbr_p, self.arcs, np.uint32)
+        w = self._copy(w_p, self.arcs, np.float32) if self.weighted else None
+        if not self.reordered:
+            return idx, nbr, w
+        # back to original ids: each list sorted by original neighbour, ties
+        # in stored (input) order
+        iperm = np.argsort(self.perm()).astype(np.uint32)
+        deg = np.diff(idx).astype(np.int64)
+        key = iperm[np.repeat(np.arange(self.vertices, dtype=np.int64), deg)]
+        onbr = iperm[nbr]
+        order = np.lexsort((np.arange(len(onbr)), onbr, key))
+        odeg = np.bincount(key, minlength=self.vertices) if len(key) else \
+            np.zeros(self.vertices, dtype=np.int64)
+        oidx = np.zeros(self.vertices + 1, dtype=np.uint64)
+        oidx[1:] = np.cumsum(odeg)
+        return oidx, onbr[order], (w[order] if w is not None else None)
 
     def csr(self):
         i = self.info()
-        idx = self._copy(i.d_out_index, self.vertices + 1, np.uint64)
-        nbr = self._copy(i.d_out_nbr, self.arcs, np.uint32)
-        w = self._copy(i.d_out_w, self.arcs, np.float32) if self.weighted else None
-        return idx, nbr, w
+        return self._adjacency(i.d_out_index, i.d_out_nbr, i.d_out_w)
 
     def csc(self):
         i = self.info()
-        idx = self._copy(i.d_in_index, self.vertices + 1, np.uint64)
-        nbr = self._copy(i.d_in_nbr, self.arcs, np.uint32)
-        w = self._copy(i.d_in_w, self.arcs, np.float32) if self.weighted else None
-        return idx, nbr, w
+        return self._adjacency(i.d_in_index, i.d_in_nbr, i.d_in_w)
 
     # -- toolkits (device outputs; *_into variants keep results on the GPU) --
     def pagerank_into(self, out, iterations=20, deltas=None):

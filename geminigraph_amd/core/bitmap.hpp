@@ -1,0 +1,89 @@
+// Bitmap / VertexSubset in device memory (core/bitmap.hpp:20-61).  The host
+// object owns the words; device callbacks use a BitmapView (two words of
+// state, copied into the lambda) because a host pointer cannot be followed on
+// the device.
+#ifndef GG_CORE_BITMAP_HPP
+#define GG_CORE_BITMAP_HPP
+
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#define WORD_OFFSET(i) ((i) >> 6)
+#define BIT_OFFSET(i) ((i) & 0x3f)
+
+struct BitmapView {
+  size_t size;
+  unsigned long *data;
+  __device__ unsigned long get_bit(size_t i) const {
+    return data[WORD_OFFSET(i)] & (1ul << BIT_OFFSET(i));
+  }
+  __device__ void set_bit(size_t i) const {
+    atomicOr((unsigned long long *)(data + WORD_OFFSET(i)), 1ull << BIT_OFFSET(i));
+  }
+};
+
+namespace gg_facade {
+inline void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) {
+    fprintf(stderr, "gg: %s: %s\n", what, hipGetErrorString(e));
+    abort();
+  }
+}
+template <int = 0>
+__global__ void bitmap_fill_words(unsigned long *d, size_t words, size_t size) {
+  size_t full = WORD_OFFSET(size);
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < words;
+       i += (size_t)gridDim.x * blockDim.x) {
+    unsigned long x = 0;
+    if (i < full)
+      x = ~0ul;
+    else if (i == full && BIT_OFFSET(size))
+      x = (1ul << BIT_OFFSET(size)) - 1ul;
+    d[i] = x;
+  }
+}
+}  // namespace gg_facade
+
+class Bitmap {
+ public:
+  size_t size;
+  unsigned long *data;  // device words, WORD_OFFSET(size)+1 of them
+  Bitmap() : size(0), data(nullptr) {}
+  explicit Bitmap(size_t size) : size(size), data(nullptr) {
+    gg_facade::hip_check(hipMalloc((void **)&data, sizeof(unsigned long) * (WORD_OFFSET(size) + 1)),
+                         "Bitmap alloc");
+    clear();
+  }
+  ~Bitmap() {
+    if (data) (void)hipFree(data);
+  }
+  Bitmap(const Bitmap &) = delete;
+  Bitmap &operator=(const Bitmap &) = delete;
+  size_t words() const { return WORD_OFFSET(size) + 1; }
+  void clear() {
+    gg_facade::hip_check(hipMemset(data, 0, sizeof(unsigned long) * words()), "Bitmap clear");
+  }
+  void fill() {
+    hipLaunchKernelGGL(gg_facade::bitmap_fill_words<0>, dim3((words() + 255) / 256), dim3(256), 0, 0,
+                       data, words(), size);
+    gg_facade::hip_check(hipDeviceSynchronize(), "Bitmap fill");
+  }
+  // host-side single-bit access (a device round trip each)
+  unsigned long get_bit(size_t i) const {
+    unsigned long w = 0;
+    gg_facade::hip_check(hipMemcpy(&w, data + WORD_OFFSET(i), 8, hipMemcpyDeviceToHost), "get_bit");
+    return w & (1ul << BIT_OFFSET(i));
+  }
+  void set_bit(size_t i) {
+    unsigned long w = 0;
+    gg_facade::hip_check(hipMemcpy(&w, data + WORD_OFFSET(i), 8, hipMemcpyDeviceToHost), "set_bit");
+    w |= 1ul << BIT_OFFSET(i);
+    gg_facade::hip_check(hipMemcpy(data + WORD_OFFSET(i), &w, 8, hipMemcpyHostToDevice), "set_bit");
+  }
+  BitmapView view() const { return BitmapView{size, data}; }
+};
+
+typedef Bitmap VertexSubset;
+
+#endif

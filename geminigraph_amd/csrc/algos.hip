@@ -159,6 +159,26 @@ __global__ void bitmap_fill_kernel(uint64_t *bits, size_t words, uint32_t V) {
   }
 }
 
+// Boundary of reordered graphs: results computed in internal order are
+// scattered back to original ids (vertex-valued results mapped through iperm).
+template <class T>
+__global__ void unpermute_kernel(const T *in, T *out, const uint32_t *iperm, uint32_t V) {
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < V; n += s) out[iperm[n]] = in[n];
+}
+__global__ void unpermute_ids_kernel(const uint32_t *in, uint32_t *out, const uint32_t *iperm,
+                                     uint32_t V, uint32_t sentinel) {
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < V; n += s) {
+    uint32_t p = in[n];
+    out[iperm[n]] = p == sentinel ? sentinel : iperm[p];
+  }
+}
+__global__ void copy_u32_kernel(const uint32_t *in, uint32_t *out, uint32_t V) {
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < V; n += s) out[n] = in[n];
+}
+
 // --------------------------------------------------------------- work area
 int work_get(gg_graph *g, Work **out) {
   if (g->work) {
@@ -185,9 +205,10 @@ int work_get(gg_graph *g, Work **out) {
   if (r == GG_OK) r = al((void **)&w->visited, w->words * 8);
   if (r == GG_OK) r = al((void **)&w->aux_bits, w->words * 8);
   if (r == GG_OK) r = al(&w->msg, V * 8);
+  if (r == GG_OK && g->reordered) r = al(&w->res_a, V * 8);
+  if (r == GG_OK && g->reordered) r = al(&w->res_b, V * 8);
   size_t nh = std::max<size_t>(std::max(g->out.n_heavy, g->in.n_heavy), 1);
   if (r == GG_OK) r = al((void **)&w->hacc, nh * 8);
-  if (r == GG_OK) r = al((void **)&w->dacc, 4096 * 8);
   if (r == GG_OK) r = al((void **)&w->exam, 1024 * 8);
   if (r == GG_OK && hipHostMalloc((void **)&w->h_cnt, sizeof(Counters) * 2) != hipSuccess)
     r = fail(GG_ENOMEM, "pinned counters");
@@ -207,6 +228,8 @@ int work_get(gg_graph *g, Work **out) {
 void work_free(Work *w) {
   if (!w) return;
   for (void *p : w->allocations) device_free(p);
+  device_free(w->dacc);
+  device_free(w->dsum);
   if (w->h_cnt) hipHostFree(w->h_cnt);
   if (w->h_exam) hipHostFree(w->h_exam);
   delete w;
@@ -410,7 +433,22 @@ __global__ void pr_heavy_finish_kernel(const uint32_t *heavy_v, uint32_t n, doub
     op.finish(v, s, 0, dsum);
   }
   double t = wave_sum_f64(dsum);
-  if (lane_id() == 0 && t != 0.0) unsafeAtomicAdd(dacc, t);
+  if (lane_id() == 0 && t != 0.0) unsafeAtomicAdd(&dacc[((i >> 6) % kScalarSlots) * 16], t);
+}
+// One block per spread accumulator: sums its kScalarSlots partials.
+__global__ void scalar_reduce_kernel(const double *dacc, double *out) {
+  __shared__ double part[kWavesPerBlock];
+  const double *a = dacc + (size_t)blockIdx.x * kScalarStride;
+  double x = 0;
+  for (uint32_t s = threadIdx.x; s < kScalarSlots; s += blockDim.x) x += a[s * 16];
+  x = wave_sum_f64(x);
+  if (lane_id() == 0) part[wave_in_block()] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int i = 0; i < kWavesPerBlock; i++) t += part[i];
+    out[blockIdx.x] = t;
+  }
 }
 __global__ void pr_init_kernel(double *curr, const uint32_t *outdeg, uint32_t V) {
   uint32_t s = gridDim.x * blockDim.x;
@@ -870,33 +908,53 @@ int gg_pagerank(gg_graph *g, int iterations, double *d_curr_out, double *h_delta
   const Dir &out = out_dir(g), &in = in_dir(g);
   uint32_t V = g->vertices;
   DeviceTimer tm;
-  double *curr = d_curr_out;
+  double *curr = g->reordered ? (double *)w->res_a : d_curr_out;
   double *next = (double *)w->msg;
+  if (w->dacc_cap < iterations + 1) {
+    int cap = std::max(iterations + 1, 64);
+    double *a = nullptr, *s = nullptr;
+    GG_TRY(device_alloc((void **)&a, sizeof(double) * kScalarStride * cap));
+    int r = device_alloc((void **)&s, sizeof(double) * cap);
+    if (r != GG_OK) {
+      device_free(a);
+      return r;
+    }
+    device_free(w->dacc);
+    device_free(w->dsum);
+    w->dacc = a;
+    w->dsum = s;
+    w->dacc_cap = cap;
+  }
   hipLaunchKernelGGL(pr_init_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), curr, out.degree, V);
-  GG_HIP(hipMemsetAsync(w->dacc, 0, sizeof(double) * (iterations + 1), stream()));
+  GG_HIP(hipMemsetAsync(w->dacc, 0, sizeof(double) * kScalarStride * (iterations + 1), stream()));
   for (int it = 0; it < iterations; it++) {
     // active = every vertex, so active_edges = |E| and the step is dense
     // (core/graph.hpp:1509).
+    double *acc = w->dacc + (size_t)(it + 1) * kScalarStride;
     PrPull op{{}, curr, next, out.degree, it == iterations - 1};
-    GG_TRY(en.launch_pull_group(in, op, w->dacc + it + 1, "pr_pull"));
+    GG_TRY(en.launch_pull_group(in, op, acc, "pr_pull"));
     PrHeavy hop{{}, curr, w->hacc};
     GG_TRY(en.launch_pull_heavy(in, hop, "pr_heavy"));
     if (in.n_heavy)
       hipLaunchKernelGGL(pr_heavy_finish_kernel, dim3((in.n_heavy + kBlock - 1) / kBlock),
-                         dim3(kBlock), 0, stream(), in.heavy_v, in.n_heavy, w->hacc, op,
-                         w->dacc + it + 1);
+                         dim3(kBlock), 0, stream(), in.heavy_v, in.n_heavy, w->hacc, op, acc);
     if (en.st.iterations < 64) en.st.modes[en.st.iterations] = 1;
     en.st.iterations++;
     en.st.edges_examined += g->arcs;
     std::swap(curr, next);
   }
-  if (curr != d_curr_out)
+  if (g->reordered)
+    hipLaunchKernelGGL(unpermute_kernel<double>, dim3(grid1(V)), dim3(kBlock), 0, stream(), curr,
+                       d_curr_out, g->iperm, V);
+  else if (curr != d_curr_out)
     GG_HIP(hipMemcpyAsync(d_curr_out, curr, sizeof(double) * V, hipMemcpyDeviceToDevice, stream()));
   en.st.device_ms = tm.stop();
   GG_HIP(hipGetLastError());
   if (h_deltas) {
     std::vector<double> acc(iterations + 1);
-    GG_HIP(hipMemcpyAsync(acc.data(), w->dacc, sizeof(double) * (iterations + 1),
+    hipLaunchKernelGGL(scalar_reduce_kernel, dim3(iterations + 1), dim3(kBlock), 0, stream(),
+                       w->dacc, w->dsum);
+    GG_HIP(hipMemcpyAsync(acc.data(), w->dsum, sizeof(double) * (iterations + 1),
                           hipMemcpyDeviceToHost, stream()));
     GG_HIP(hipStreamSynchronize(stream()));
     h_deltas[0] = 1.0;  // every vertex contributes 1, divided by |V|
@@ -919,6 +977,12 @@ int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *d_depth, ui
   const Dir &out = out_dir(g), &in = in_dir(g);
   uint32_t V = g->vertices;
   DeviceTimer tm;
+  uint32_t *out_parent = d_parent, *out_depth = d_depth;
+  if (g->reordered) {
+    d_parent = (uint32_t *)w->res_a;
+    d_depth = (uint32_t *)w->res_b;
+    root = g->to_internal(root);
+  }
   fill<uint32_t>(d_parent, V, V);  // bfs.cpp:35-36
   fill<uint32_t>(d_depth, kUnset, V);
   GG_HIP(hipMemsetAsync(w->visited, 0, w->words * 8, stream()));
@@ -962,6 +1026,12 @@ int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *d_depth, ui
     lv++;
     if (en.in.total > 0 && h_frontier && lv < cap) h_frontier[lv] = en.in.total;
   }
+  if (g->reordered) {
+    hipLaunchKernelGGL(unpermute_ids_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), d_parent,
+                       out_parent, g->iperm, V, V);
+    hipLaunchKernelGGL(unpermute_kernel<uint32_t>, dim3(grid1(V)), dim3(kBlock), 0, stream(),
+                       d_depth, out_depth, g->iperm, V);
+  }
   en.st.device_ms = tm.stop();
   en.st.m_reach = m_reach;
   en.st.reached = reached;
@@ -978,7 +1048,15 @@ int gg_cc(gg_graph *g, uint32_t *d_label, uint32_t *h_frontier, int cap, gg_run_
   const Dir &out = out_dir(g), &in = in_dir(g);
   uint32_t V = g->vertices;
   DeviceTimer tm;
-  hipLaunchKernelGGL(iota_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), d_label, V);
+  uint32_t *out_label = d_label;
+  if (g->reordered) {
+    // labels are original ids, so the propagated minimum is the reference's
+    d_label = (uint32_t *)w->res_a;
+    hipLaunchKernelGGL(copy_u32_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), g->iperm,
+                       d_label, V);
+  } else {
+    hipLaunchKernelGGL(iota_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), d_label, V);
+  }
   uint32_t *msg = (uint32_t *)w->msg;
   fill<uint32_t>(msg, 0xFFFFFFFFu, V);
   // active_in->fill() (cc.cpp:28), exact tail as in core/bitmap.hpp:42-52
@@ -1007,6 +1085,9 @@ int gg_cc(gg_graph *g, uint32_t *d_label, uint32_t *h_frontier, int cap, gg_run_
     }
     GG_TRY(en.finish_iteration(sp, out.degree, nullptr));
   }
+  if (g->reordered)
+    hipLaunchKernelGGL(unpermute_kernel<uint32_t>, dim3(grid1(V)), dim3(kBlock), 0, stream(),
+                       d_label, out_label, g->iperm, V);
   en.st.device_ms = tm.stop();
   put_stats(stats, en.st);
   return GG_OK;
@@ -1024,6 +1105,11 @@ int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int
   const Dir &out = out_dir(g), &in = in_dir(g);
   uint32_t V = g->vertices;
   DeviceTimer tm;
+  float *out_dist = d_dist;
+  if (g->reordered) {
+    d_dist = (float *)w->res_a;
+    root = g->to_internal(root);
+  }
   fill<float>(d_dist, 1e9f, V);  // sssp.cpp:163-164
   hipLaunchKernelGGL((put2_kernel<float, float>), dim3(1), dim3(1), 0, stream(), d_dist, 0.f,
                      (float *)nullptr, 0.f, root);
@@ -1054,6 +1140,9 @@ int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int
     GG_TRY(en.finish_iteration(sp, out.degree, nullptr));
     reached_edges += en.in.edges;
   }
+  if (g->reordered)
+    hipLaunchKernelGGL(unpermute_kernel<float>, dim3(grid1(V)), dim3(kBlock), 0, stream(), d_dist,
+                       out_dist, g->iperm, V);
   en.st.device_ms = tm.stop();
   en.st.m_reach = reached_edges;
   put_stats(stats, en.st);
@@ -1071,6 +1160,12 @@ int gg_bc(gg_graph *g, uint32_t root, double *d_dep, double *d_inv_paths, uint32
   const Dir &out = out_dir(g), &in = in_dir(g);
   uint32_t V = g->vertices;
   DeviceTimer tm;
+  double *out_dep = d_dep, *out_inv = d_inv_paths;
+  if (g->reordered) {
+    d_dep = (double *)w->res_a;
+    d_inv_paths = (double *)w->res_b;
+    root = g->to_internal(root);
+  }
   double *sigma = d_inv_paths;  // bc.cpp:103: inv_num_paths aliases num_paths
   uint32_t *depth = (uint32_t *)w->msg;
   fill<double>(sigma, 0.0, V);
@@ -1123,6 +1218,12 @@ int gg_bc(gg_graph *g, uint32_t root, double *d_dep, double *d_inv_paths, uint32
     GG_TRY(en.note_iteration(sp));
   }
   hipLaunchKernelGGL(bc_final_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), d_dep, sigma, V);
+  if (g->reordered) {
+    hipLaunchKernelGGL(unpermute_kernel<double>, dim3(grid1(V)), dim3(kBlock), 0, stream(), d_dep,
+                       out_dep, g->iperm, V);
+    hipLaunchKernelGGL(unpermute_kernel<double>, dim3(grid1(V)), dim3(kBlock), 0, stream(), sigma,
+                       out_inv, g->iperm, V);
+  }
   en.st.device_ms = tm.stop();
   GG_HIP(hipGetLastError());
   en.st.reached = reached;

@@ -133,10 +133,14 @@ __global__ void check_range_kernel(const uint32_t *edges, uint64_t ne, int rec_w
 // keys for one direction: key vertex in the high bits, neighbour in the low
 // vbits.  mode 0: key = src, 1: key = dst, 2: both (symmetric, 2E keys).
 __global__ void make_keys_kernel(const uint32_t *edges, uint64_t ne, int rec_words, int vbits,
-                                 int mode, uint64_t *keys, float *vals) {
+                                 int mode, const uint32_t *perm, uint64_t *keys, float *vals) {
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += stride) {
     uint64_t s = edges[i * rec_words], d = edges[i * rec_words + 1];
+    if (perm) {
+      s = perm[s];
+      d = perm[d];
+    }
     float w = rec_words == 3 ? __uint_as_float(edges[i * rec_words + 2]) : 0.f;
     if (mode == 0) {
       keys[i] = (s << vbits) | d;
@@ -153,6 +157,25 @@ __global__ void make_keys_kernel(const uint32_t *edges, uint64_t ne, int rec_wor
       }
     }
   }
+}
+
+// Relabeling helpers: sort key = complemented out-degree (descending order,
+// ties by id through the stable sort), then perm = inverse of the sorted ids.
+__global__ void relabel_keys_kernel(const uint32_t *deg, uint32_t V, uint32_t *keys, uint32_t *ids) {
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s) {
+    keys[v] = ~deg[v];
+    ids[v] = v;
+  }
+}
+__global__ void invert_perm_kernel(const uint32_t *iperm, uint32_t V, uint32_t *perm) {
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < V; n += s) perm[iperm[n]] = n;
+}
+__global__ void gather_u32_kernel(const uint32_t *src, const uint32_t *idx, uint32_t V,
+                                  uint32_t *dst) {
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < V; n += s) dst[n] = src[idx[n]];
 }
 
 __global__ void split_keys_kernel(const uint64_t *keys, uint64_t n, int vbits, uint32_t *nbr) {
@@ -280,7 +303,7 @@ static int build_dir(gg_graph *g, Dir &d, const uint32_t *edges, uint64_t ne, in
   if (r == GG_OK && g->weighted) r = device_alloc((void **)&vals_alt, n * 4);
   if (r == GG_OK) {
     hipLaunchKernelGGL(make_keys_kernel, dim3(grid_for(ne)), dim3(256), 0, stream(), edges, ne,
-                       rec_words, vbits, mode, keys, vals);
+                       rec_words, vbits, mode, g->reordered ? g->perm : nullptr, keys, vals);
     r = sort_keys(keys, keys_alt, vals, vals_alt, n, 2 * vbits);
   }
   device_free(keys_alt);
@@ -303,8 +326,62 @@ static int build_dir(gg_graph *g, Dir &d, const uint32_t *edges, uint64_t ne, in
   return r;
 }
 
+// Degree-sorted relabeling: iperm = ids sorted by descending out-degree
+// (stable, so ties keep id order), perm its inverse; degree arrays are then
+// permuted into internal order.
+static int relabel(gg_graph *g) {
+  uint32_t V = g->vertices;
+  uint32_t *keys = nullptr, *keys_alt = nullptr, *ids = nullptr, *ids_alt = nullptr;
+  int r = device_alloc((void **)&keys, (size_t)V * 4);
+  if (r == GG_OK) r = device_alloc((void **)&keys_alt, (size_t)V * 4);
+  if (r == GG_OK) r = device_alloc((void **)&ids, (size_t)V * 4);
+  if (r == GG_OK) r = device_alloc((void **)&ids_alt, (size_t)V * 4);
+  if (r == GG_OK) r = graph_alloc(g, (void **)&g->perm, (size_t)V * 4);
+  if (r == GG_OK) r = graph_alloc(g, (void **)&g->iperm, (size_t)V * 4);
+  if (r == GG_OK) {
+    hipLaunchKernelGGL(relabel_keys_kernel, dim3(grid_for(V)), dim3(256), 0, stream(),
+                       g->out.degree, V, keys, ids);
+    rocprim::double_buffer<uint32_t> kb(keys, keys_alt), vb(ids, ids_alt);
+    size_t tmp_bytes = 0;
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, kb, vb, (size_t)V, 0, 32, stream());
+    void *tmp = nullptr;
+    if (e == hipSuccess) r = device_alloc(&tmp, tmp_bytes);
+    if (e == hipSuccess && r == GG_OK)
+      e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kb, vb, (size_t)V, 0, 32, stream());
+    if (e == hipSuccess && r == GG_OK)
+      e = hipMemcpyAsync(g->iperm, vb.current(), (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
+    if (e == hipSuccess && r == GG_OK) {
+      hipLaunchKernelGGL(invert_perm_kernel, dim3(grid_for(V)), dim3(256), 0, stream(), g->iperm, V,
+                         g->perm);
+      // degrees into internal order (keys buffer reused as scratch)
+      hipLaunchKernelGGL(gather_u32_kernel, dim3(grid_for(V)), dim3(256), 0, stream(),
+                         g->out.degree, g->iperm, V, keys);
+      e = hipMemcpyAsync(g->out.degree, keys, (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
+      if (e == hipSuccess && !g->symmetric) {
+        hipLaunchKernelGGL(gather_u32_kernel, dim3(grid_for(V)), dim3(256), 0, stream(),
+                           g->in.degree, g->iperm, V, keys);
+        e = hipMemcpyAsync(g->in.degree, keys, (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
+      }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    device_free(tmp);
+    if (e != hipSuccess && r == GG_OK) r = fail(GG_EHIP, std::string("relabel: ") + hipGetErrorString(e));
+  }
+  device_free(keys);
+  device_free(keys_alt);
+  device_free(ids);
+  device_free(ids_alt);
+  if (r == GG_OK) {
+    g->h_perm.resize(V);
+    hipError_t e = hipMemcpy(g->h_perm.data(), g->perm, (size_t)V * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) r = fail(GG_EHIP, std::string("relabel copy: ") + hipGetErrorString(e));
+  }
+  if (r == GG_OK) g->reordered = 1;
+  return r;
+}
+
 static int graph_build_device(const uint32_t *d_edges, uint64_t ne, uint32_t V, int weighted,
-                              int undirected, gg_graph **out) {
+                              int undirected, unsigned flags, gg_graph **out) {
   if (V == 0) return fail(GG_EINVAL, "graph needs at least one vertex");
   if (V >= 0xFFFFFFFFu) return fail(GG_EINVAL, "vertex count must be < 2^32-1 (sentinel V)");
   int rec_words = weighted ? 3 : 2;
@@ -339,7 +416,8 @@ static int graph_build_device(const uint32_t *d_edges, uint64_t ne, uint32_t V, 
     if (ne)
       hipLaunchKernelGGL(degree_kernel, dim3(grid_for(ne)), dim3(256), 0, stream(), d_edges, ne,
                          rec_words, g->out.degree, g->in.degree, undirected);
-    r = scan_degrees(g->out.degree, g->out.index, V);
+    if (flags & GG_BUILD_REORDER) r = relabel(g);
+    if (r == GG_OK) r = scan_degrees(g->out.degree, g->out.index, V);
     if (r == GG_OK && !undirected) r = scan_degrees(g->in.degree, g->in.index, V);
   }
   if (r == GG_OK) {
@@ -398,29 +476,43 @@ int gg_rmat_generate(void *d_edges, uint64_t num_edges, int scale, double a, dou
   return GG_OK;
 }
 
-int gg_graph_build(const void *edges, int edges_on_device, uint64_t num_edges, uint32_t vertices,
-                   int weighted, int undirected, gg_graph **out) {
+int gg_graph_build_ex(const void *edges, int edges_on_device, uint64_t num_edges,
+                      uint32_t vertices, int weighted, int undirected, unsigned flags,
+                      gg_graph **out) {
   if (!out) return fail(GG_EINVAL, "gg_graph_build: null out");
   if (!edges && num_edges) return fail(GG_EINVAL, "gg_graph_build: null edges");
+  if (flags & ~GG_BUILD_REORDER) return fail(GG_EINVAL, "gg_graph_build: unknown flags");
   GG_TRY(ensure_init());
   size_t rec = weighted ? 12 : 8;
   if (edges_on_device || num_edges == 0)
     return graph_build_device((const uint32_t *)edges, num_edges, vertices, weighted, undirected,
-                              out);
+                              flags, out);
   void *d = nullptr;
   GG_TRY(device_alloc(&d, num_edges * rec));
   hipError_t e = hipMemcpyAsync(d, edges, num_edges * rec, hipMemcpyHostToDevice, stream());
   if (e == hipSuccess) e = hipStreamSynchronize(stream());
   int r = e == hipSuccess ? graph_build_device((const uint32_t *)d, num_edges, vertices, weighted,
-                                               undirected, out)
+                                               undirected, flags, out)
                           : fail(GG_EHIP, std::string("upload: ") + hipGetErrorString(e));
   device_free(d);
   return r;
 }
 
+int gg_graph_build(const void *edges, int edges_on_device, uint64_t num_edges, uint32_t vertices,
+                   int weighted, int undirected, gg_graph **out) {
+  return gg_graph_build_ex(edges, edges_on_device, num_edges, vertices, weighted, undirected, 0u,
+                           out);
+}
+
 int gg_graph_load(const char *path, uint32_t vertices, int weighted, int undirected,
                   gg_graph **out) {
+  return gg_graph_load_ex(path, vertices, weighted, undirected, 0u, out);
+}
+
+int gg_graph_load_ex(const char *path, uint32_t vertices, int weighted, int undirected,
+                     unsigned flags, gg_graph **out) {
   if (!path || !out) return fail(GG_EINVAL, "gg_graph_load: null argument");
+  if (flags & ~GG_BUILD_REORDER) return fail(GG_EINVAL, "gg_graph_load: unknown flags");
   GG_TRY(ensure_init());
   struct stat st;
   if (stat(path, &st) != 0) return fail(GG_EIO, std::string("cannot stat ") + path);
@@ -451,7 +543,7 @@ int gg_graph_load(const char *path, uint32_t vertices, int weighted, int undirec
   fclose(f);
   if (stage) hipHostFree(stage);
   if (r == GG_OK)
-    r = graph_build_device((const uint32_t *)d, ne, vertices, weighted, undirected, out);
+    r = graph_build_device((const uint32_t *)d, ne, vertices, weighted, undirected, flags, out);
   device_free(d);
   return r;
 }
@@ -488,6 +580,9 @@ int gg_graph_get_info(const gg_graph *g, gg_graph_info *info) {
   info->d_in_index = i.index;
   info->d_in_nbr = i.nbr;
   info->d_in_w = i.w;
+  info->reordered = g->reordered;
+  info->d_perm = g->perm;
+  info->d_iperm = g->iperm;
   return GG_OK;
 }
 

@@ -26,6 +26,8 @@ constexpr int kBlock = 256;                 // 4 waves per workgroup
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr uint32_t kChunkEdges = 1024;      // heavy-vertex chunk (one wave, 16 x 64)
 constexpr uint32_t kUnset = 0xFFFFFFFFu;
+constexpr uint32_t kScalarSlots = 256;      // spread slots of one scalar accumulator
+constexpr size_t kScalarStride = kScalarSlots * 16;  // doubles per accumulator
 
 // ------------------------------------------------------------------ errors
 void set_error(int code, const std::string &msg);
@@ -98,6 +100,12 @@ struct gg_graph {
   int transposed = 0;
   size_t device_bytes = 0;
   gg::Dir out, in;   // in aliases out's arrays when symmetric
+  // degree-sorted relabeling (GG_BUILD_REORDER): internal = perm[original]
+  int reordered = 0;
+  uint32_t *perm = nullptr;   // device, original -> internal
+  uint32_t *iperm = nullptr;  // device, internal -> original
+  std::vector<uint32_t> h_perm;
+  uint32_t to_internal(uint32_t v) const { return reordered ? h_perm[v] : v; }
   std::vector<void *> allocations;
   // scratch owned by the graph for traversals (lazily sized)
   gg::Work *work = nullptr;
@@ -107,6 +115,7 @@ struct gg_subset {
   uint32_t size = 0;
   size_t words = 0;
   uint64_t *d = nullptr;
+  const gg_graph *g = nullptr;  // bits are indexed by the graph's internal ids
 };
 
 namespace gg {
@@ -132,8 +141,12 @@ struct Work {
   uint64_t *visited = nullptr;
   uint64_t *aux_bits = nullptr;
   void *msg = nullptr;         // V x 8 bytes two-phase messages
+  void *res_a = nullptr;       // V x 8 bytes results in internal order (reordered graphs)
+  void *res_b = nullptr;       // V x 8 bytes
   double *hacc = nullptr;      // heavy accumulators (max of both directions)
-  double *dacc = nullptr;      // per-iteration scalar accumulators (4096)
+  double *dacc = nullptr;      // per-iteration spread accumulators (kScalarStride each)
+  double *dsum = nullptr;      // reduced per-iteration scalars
+  int dacc_cap = 0;            // iterations the buffers hold
   unsigned long long *exam = nullptr;  // 64 slots x 16 (one 128-byte line each)
   Counters *h_cnt = nullptr;   // pinned host copy
   unsigned long long *h_exam = nullptr;

@@ -218,7 +218,10 @@ struct PullArgs {
   uint32_t ngroups;
   uint64_t *out;
   unsigned long long *exam;
-  double *dacc;  // scalar accumulator of the op (delta), may be null
+  // scalar accumulator of the op (PageRank delta), may be null: kScalarSlots
+  // partial sums 16 doubles (one 128-byte line) apart, so that the waves'
+  // atomics spread over many L2 channels instead of serialising on one line
+  double *dacc;
 };
 
 // Reduction pull over one 64-vertex group per wave.
@@ -275,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void pull_group_kernel(PullArgs a, Op op) {
   op.after(cand, v);
   if (a.dacc) {
     double s = wave_sum_f64(dsum);
-    if (lane == 0 && s != 0.0) unsafeAtomicAdd(a.dacc, s);
+    if (lane == 0 && s != 0.0) unsafeAtomicAdd(&a.dacc[(g % kScalarSlots) * 16], s);
   }
   if (lane == 0) count_examined(a.exam, total);
 }

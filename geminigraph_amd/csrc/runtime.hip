@@ -246,6 +246,7 @@ int gg_subset_create(const gg_graph *g, gg_subset **out) {
   if (!g || !out) return fail(GG_EINVAL, "gg_subset_create: null argument");
   GG_TRY(ensure_init());
   gg_subset *s = new gg_subset;
+  s->g = g;
   s->size = g->vertices;
   s->words = ((size_t)g->vertices >> 6) + 1;
   int r = device_alloc((void **)&s->d, s->words * 8);
@@ -285,6 +286,7 @@ int gg_subset_fill(gg_subset *s) {
 
 int gg_subset_set_bit(gg_subset *s, uint32_t v) {
   if (!s || v >= s->size) return fail(GG_EINVAL, "gg_subset_set_bit: vertex out of range");
+  v = s->g->to_internal(v);
   uint64_t w = 0;
   GG_HIP(hipMemcpyAsync(&w, s->d + (v >> 6), 8, hipMemcpyDeviceToHost, g_stream));
   GG_HIP(hipStreamSynchronize(g_stream));
@@ -296,6 +298,7 @@ int gg_subset_set_bit(gg_subset *s, uint32_t v) {
 
 int gg_subset_get_bit(const gg_subset *s, uint32_t v, int *bit) {
   if (!s || !bit || v >= s->size) return fail(GG_EINVAL, "gg_subset_get_bit: bad argument");
+  v = s->g->to_internal(v);
   uint64_t w = 0;
   GG_HIP(hipMemcpyAsync(&w, s->d + (v >> 6), 8, hipMemcpyDeviceToHost, g_stream));
   GG_HIP(hipStreamSynchronize(g_stream));

@@ -95,6 +95,12 @@ typedef struct gg_graph_info {
   const uint64_t *d_in_index;
   const uint32_t *d_in_nbr;
   const float *d_in_w;
+  /* GG_BUILD_REORDER: the arrays above are in internal order;
+   * d_perm[original] = internal id, d_iperm[internal] = original id
+   * (both NULL when the graph keeps original ids). */
+  int reordered;
+  const uint32_t *d_perm;
+  const uint32_t *d_iperm;
 } gg_graph_info;
 
 /* Build CSR (outgoing, keyed by src) and CSC (incoming, keyed by dst) from a
@@ -110,6 +116,18 @@ int gg_graph_build(const void *edges, int edges_on_device, uint64_t num_edges,
  * (core/graph.hpp:359-360, :781-782). */
 int gg_graph_load(const char *path, uint32_t vertices, int weighted, int undirected,
                   gg_graph **out);
+/* Build flags.  GG_BUILD_REORDER relabels vertices internally by descending
+ * out-degree (ties by id) before the CSR/CSC build, so the hot sources of the
+ * pull gathers share cache lines; every entry point still takes and returns
+ * original vertex ids (roots in, per-vertex outputs out), translating at the
+ * boundary.  The reference's loaders keep file ids (core/graph.hpp:772-1339);
+ * its locality step is the vertex chunking of core/graph.hpp:824-871. */
+#define GG_BUILD_REORDER 1u
+int gg_graph_build_ex(const void *edges, int edges_on_device, uint64_t num_edges,
+                      uint32_t vertices, int weighted, int undirected, unsigned flags,
+                      gg_graph **out);
+int gg_graph_load_ex(const char *path, uint32_t vertices, int weighted, int undirected,
+                     unsigned flags, gg_graph **out);
 int gg_graph_destroy(gg_graph *g);
 /* Swap directions without moving data (Graph<E>::transpose, core/graph.hpp:760-769). */
 int gg_graph_transpose(gg_graph *g);

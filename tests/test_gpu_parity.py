@@ -297,3 +297,72 @@ def test_repeat_runs_identical(gg):
     assert np.array_equal(d1, d2)
     gu = gg.Graph.build(e, V, undirected=True)
     assert np.array_equal(gu.cc()[0], gu.cc()[0])
+
+
+# ------------------------------------------------- degree-sorted relabeling
+
+@pytest.mark.parametrize("undirected,weighted", [(False, False), (False, True), (True, False)])
+def test_reordered_build_matches_oracle(gg, undirected, weighted):
+    e = oracle.rmat_edges(11, seed=41, weighted=weighted)
+    V = 1 << 11
+    g = gg.Graph.build(e, V, undirected=undirected, reorder=True)
+    assert g.reordered
+    perm = g.perm()
+    assert np.array_equal(np.sort(perm), np.arange(V))
+    o = oracle.Graph(e, V, undirected=undirected)
+    od = o.out_degree()
+    # internal order = descending out-degree, ties by id
+    iperm = np.argsort(perm)
+    assert (np.diff(od[iperm].astype(np.int64)) <= 0).all()
+    assert np.array_equal(g.out_degree(), od)
+    assert np.array_equal(g.in_degree(), o.in_degree())
+    _segments_equal(*g.csr()[:2], *o.csr()[:2], g.csr()[2], o.csr()[2])
+    _segments_equal(*g.csc()[:2], *o.csc()[:2], g.csc()[2], o.csc()[2])
+
+
+@pytest.mark.parametrize("mode", ["auto", "push", "pull"])
+def test_reordered_toolkits_match_oracle(gg, mode):
+    scale = 13
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, seed=43)
+    ew = oracle.rmat_edges(scale, seed=43, weighted=True)
+    g = gg.Graph.build(e, V, reorder=True)
+    gu = gg.Graph.build(e, V, undirected=True, reorder=True)
+    gw = gg.Graph.build(ew, V, reorder=True)
+    o, ou, ow = oracle.Graph(e, V), oracle.Graph(e, V, undirected=True), oracle.Graph(ew, V)
+    roots = roots_of(o.out_degree(), 2, 43)
+    say("reordered", mode)
+    with gg.options(force_mode=MODES[mode]):
+        if mode != "push":
+            pr, deltas, _ = g.pagerank(20)
+            rpr, rdeltas = o.pagerank(20)
+            assert_rel(pr, rpr)
+            assert_rel(deltas[:20], rdeltas[:20], rel=1e-8)
+        label, fr, _ = gu.cc()
+        rl, rfr = ou.cc()
+        assert np.array_equal(label, rl) and np.array_equal(fr, rfr)
+        for r in roots:
+            parent, depth, fr, st = g.bfs(r)
+            _, rd, rf = o.bfs(r)
+            assert np.array_equal(depth, rd) and np.array_equal(fr, rf)
+            check_parents(e, V, r, parent, depth)
+            dist, fr, _ = gw.sssp(r)
+            rdist, rfr = ow.sssp(r)
+            assert np.array_equal(dist.view(np.uint32), rdist.view(np.uint32))
+            assert np.array_equal(fr, rfr)
+            dep, inv, fr, _ = g.bc(r)
+            rdep, rinv, rfr = o.bc(r)
+            assert np.array_equal(fr, rfr)
+            assert_rel(inv, rinv)
+            assert_rel(dep, rdep)
+
+
+def test_reordered_subset_uses_original_ids(gg):
+    e = oracle.rmat_edges(10, seed=3)
+    g = gg.Graph.build(e, 1 << 10, reorder=True)
+    s = gg.VertexSubset(g)
+    s.set_bit(5)
+    assert s.get_bit(5) == 1 and s.count() == 1
+    internal = int(g.perm()[5])
+    words = s.words()
+    assert (int(words[internal >> 6]) >> (internal & 63)) & 1 == 1
