@@ -54,6 +54,69 @@ def bfs_model_bytes(st, V):
     return push, pull
 
 
+def batch_extra(gg, seed, reorder):
+    """RMAT-22 edge factor 32, 64 roots: one batched traversal against 64
+    sequential single-root calls on the same GPU (BFS and BC), results checked
+    identical (depths) / within 1e-9 (BC) for sampled roots."""
+    scale, ef = 22, 32
+    V = 1 << scale
+    g = gg.Graph.rmat(scale, ef, seed=seed, permute=True, reorder=reorder)
+    roots = pick_roots(g.out_degree(), 64, seed + 2)
+    parent, depth = gg.DeviceArray(V, np.uint32), gg.DeviceArray(V, np.uint32)
+    rows = gg.DeviceArray(V * 64, np.uint32)
+    g.bfs_into(roots[0], parent, depth)
+    gg.bfs_batch_into(g, roots, rows)
+    gg.synchronize()
+    t = time.perf_counter()
+    for r in roots:
+        g.bfs_into(r, parent, depth)
+    gg.synchronize()
+    seq = time.perf_counter() - t
+    t = time.perf_counter()
+    st = gg.bfs_batch_into(g, roots, rows)
+    gg.synchronize()
+    bat = time.perf_counter() - t
+    host_rows = rows.numpy().reshape(V, 64)
+    same = True
+    for i in (0, 21, 63):
+        g.bfs_into(roots[i], parent, depth)
+        same = same and bool(np.array_equal(host_rows[:, i], depth.numpy()))
+    out = {"workload": f"RMAT-{scale} edge factor {ef}, 64 roots", "bfs_sequential_ms":
+           round(seq * 1e3, 2), "bfs_batched_ms": round(bat * 1e3, 2),
+           "bfs_speedup": round(seq / bat, 2), "bfs_batched_queries_per_s": round(64 / bat, 1),
+           "bfs_batched_gteps": round(st.m_reach / bat / 1e9, 2), "bfs_depths_identical": same}
+    del host_rows
+    rows.free()
+    parent.free()
+    depth.free()
+    dep, inv = gg.DeviceArray(V, np.float64), gg.DeviceArray(V, np.float64)
+    bdep, binv = gg.DeviceArray(V * 64, np.float64), gg.DeviceArray(V * 64, np.float64)
+    g.bc_into(roots[0], dep, inv)
+    gg.bc_batch_into(g, roots, bdep, binv)
+    gg.synchronize()
+    t = time.perf_counter()
+    for r in roots:
+        g.bc_into(r, dep, inv)
+    gg.synchronize()
+    seq = time.perf_counter() - t
+    t = time.perf_counter()
+    gg.bc_batch_into(g, roots, bdep, binv)
+    gg.synchronize()
+    bat = time.perf_counter() - t
+    g.bc_into(roots[63], dep, inv)
+    a = bdep.numpy().reshape(V, 64)[:, 63]
+    b = dep.numpy()
+    fin = np.isfinite(b)
+    ok = bool(np.array_equal(np.isnan(a), np.isnan(b)) and
+              np.all(np.abs(a[fin] - b[fin]) <= 1e-9 * np.maximum(np.abs(b[fin]), 1e-300)))
+    out.update({"bc_sequential_ms": round(seq * 1e3, 2), "bc_batched_ms": round(bat * 1e3, 2),
+                "bc_speedup": round(seq / bat, 2), "bc_batched_within_1e-9": ok})
+    for x in (dep, inv, bdep, binv):
+        x.free()
+    del g
+    return out
+
+
 def cpu_baseline(scale, ef, seed, roots, gpu_depths, sample_roots):
     import oracle
     threads = os.environ.get("OMP_NUM_THREADS")
@@ -101,6 +164,8 @@ def main():
     ap.add_argument("--cpu-sample-roots", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-reorder", action="store_true",
+                    help="keep file vertex ids (no degree-sorted relabeling)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -114,7 +179,8 @@ def main():
     V, E = 1 << scale, (1 << scale) * ef
 
     t0 = time.time()
-    g = gg.Graph.rmat(scale, ef, seed=args.seed, permute=True)
+    reorder = not args.no_reorder
+    g = gg.Graph.rmat(scale, ef, seed=args.seed, permute=True, reorder=reorder)
     gg.synchronize()
     build_s = time.time() - t0
     log(f"RMAT-{scale} ef{ef}: V={V} E={E} built on device in {build_s:.2f}s")
@@ -216,7 +282,8 @@ def main():
         del g
         parent.free()
         depth.free()
-        gu = gg.Graph.rmat(scale, ef, seed=args.seed, permute=True, undirected=True)
+        gu = gg.Graph.rmat(scale, ef, seed=args.seed, permute=True, undirected=True,
+                           reorder=reorder)
         lab = gg.DeviceArray(V, np.uint32)
         gu.cc_into(lab)
         t = time.perf_counter()
@@ -229,6 +296,8 @@ def main():
         log(f"CC: {extras['cc']}")
         del gu
         lab.free()
+        extras["batched"] = batch_extra(gg, args.seed + 7, reorder)
+        log(f"batched: {extras['batched']}")
 
     cpu = None
     if not args.no_cpu:
@@ -252,7 +321,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"BFS from {len(roots)} seeded non-isolated roots, RMAT scale "
                                f"{scale} edge factor {ef} (a=0.57 b=0.19 c=0.19), permuted ids",
-                   "vertices": V, "edges": E, "roots": len(roots), "seed": args.seed},
+                   "vertices": V, "edges": E, "roots": len(roots), "seed": args.seed,
+                   "reorder": reorder,
+                   "switch_divisor": gg.get_option("switch_divisor")},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extras": extras,

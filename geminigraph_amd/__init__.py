@@ -135,6 +135,9 @@ SIGNATURES = {
     "gg_cc": (_i32, [_vp, _vp, _vp, _i32, _stats]),
     "gg_sssp": (_i32, [_vp, _u32, _vp, _vp, _i32, _stats]),
     "gg_bc": (_i32, [_vp, _u32, _vp, _vp, _vp, _i32, _stats]),
+    "gg_bfs_batch": (_i32, [_vp, _vp, _i32, _vp, _stats]),
+    "gg_sssp_batch": (_i32, [_vp, _vp, _i32, _vp, _stats]),
+    "gg_bc_batch": (_i32, [_vp, _vp, _i32, _vp, _vp, _stats]),
     "gg_prof_enable": (_i32, [_i32]),
     "gg_prof_reset": (_i32, []),
     "gg_prof_read": (_i32, [_cp, ctypes.POINTER(_dbl), ctypes.POINTER(_u64)]),
@@ -470,6 +473,69 @@ class Graph:
         st = self.bc_into(root, dep, inv, fr)
         n = int(np.max(np.nonzero(fr)[0]) + 1) if fr.any() else 0
         return dep.numpy(), inv.numpy(), fr[:n].copy(), st.as_dict()
+
+
+def _batch_roots(roots):
+    roots = np.ascontiguousarray(roots, dtype=np.uint32)
+    if not 1 <= len(roots) <= 64:
+        raise ValueError("1 to 64 roots per batch")
+    return roots
+
+
+def bfs_batch_into(graph, roots, depth):
+    """Up to 64 BFS roots in one traversal; depth is a DeviceArray of V*64
+    uint32, vertex-major (depth[v*64 + i] = level of v from roots[i])."""
+    roots = _batch_roots(roots)
+    st = RunStats()
+    _check(lib().gg_bfs_batch(graph.handle, ctypes.c_void_p(roots.ctypes.data), len(roots),
+                              ctypes.c_void_p(depth.ptr), ctypes.byref(st)))
+    return st
+
+
+def bfs_batch(graph, roots):
+    """Returns depths as a (len(roots), V) array (one row per root) and stats."""
+    roots = _batch_roots(roots)
+    depth = DeviceArray(graph.vertices * 64, np.uint32)
+    st = bfs_batch_into(graph, roots, depth)
+    d = depth.numpy().reshape(graph.vertices, 64)[:, :len(roots)].T.copy()
+    return d, st.as_dict()
+
+
+def sssp_batch_into(graph, roots, dist):
+    roots = _batch_roots(roots)
+    st = RunStats()
+    _check(lib().gg_sssp_batch(graph.handle, ctypes.c_void_p(roots.ctypes.data), len(roots),
+                               ctypes.c_void_p(dist.ptr), ctypes.byref(st)))
+    return st
+
+
+def sssp_batch(graph, roots):
+    """Distances as a (len(roots), V) float32 array and stats."""
+    roots = _batch_roots(roots)
+    dist = DeviceArray(graph.vertices * 64, np.float32)
+    st = sssp_batch_into(graph, roots, dist)
+    d = dist.numpy().reshape(graph.vertices, 64)[:, :len(roots)].T.copy()
+    return d, st.as_dict()
+
+
+def bc_batch_into(graph, roots, dep, inv):
+    roots = _batch_roots(roots)
+    st = RunStats()
+    _check(lib().gg_bc_batch(graph.handle, ctypes.c_void_p(roots.ctypes.data), len(roots),
+                             ctypes.c_void_p(dep.ptr), ctypes.c_void_p(inv.ptr), ctypes.byref(st)))
+    return st
+
+
+def bc_batch(graph, roots):
+    """(dependencies, inv_num_paths) as (len(roots), V) float64 arrays and stats."""
+    roots = _batch_roots(roots)
+    dep = DeviceArray(graph.vertices * 64, np.float64)
+    inv = DeviceArray(graph.vertices * 64, np.float64)
+    st = bc_batch_into(graph, roots, dep, inv)
+    k = len(roots)
+    d = dep.numpy().reshape(graph.vertices, 64)[:, :k].T.copy()
+    i = inv.numpy().reshape(graph.vertices, 64)[:, :k].T.copy()
+    return d, i, st.as_dict()
 
 
 class VertexSubset:

@@ -39,7 +39,8 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(const uint64_t *bits, s
                                                          const uint32_t *bin_deg,
                                                          uint32_t small_max, uint32_t big_max,
                                                          uint32_t *q0, uint32_t *q1, uint32_t *q2,
-                                                         Counters *cnt, uint64_t *visited) {
+                                                         Counters *cnt, uint64_t *visited,
+                                                         uint64_t *clear) {
   __shared__ uint32_t s_tot[3][kWavesPerBlock];
   __shared__ unsigned long long s_edges[kWavesPerBlock];
   __shared__ uint32_t s_base[3];
@@ -49,6 +50,7 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(const uint64_t *bits, s
     size_t wi = base + threadIdx.x;
     uint64_t word = wi < words ? bits[wi] : 0;
     if (visited && word) visited[wi] |= word;
+    if (clear && wi < words) clear[wi] = 0;
     uint32_t c0 = 0, c1 = 0, c2 = 0;
     uint64_t es = 0;
     for (uint64_t x = word; x; x &= x - 1) {
@@ -230,6 +232,14 @@ void work_free(Work *w) {
   for (void *p : w->allocations) device_free(p);
   device_free(w->dacc);
   device_free(w->dsum);
+  device_free(w->ms_vis);
+  device_free(w->ms_f[0]);
+  device_free(w->ms_f[1]);
+  device_free(w->ms_roots);
+  device_free(w->ms_tmp);
+  device_free(w->mb_a);
+  device_free(w->mb_b);
+  device_free(w->ms_levels);
   if (w->h_cnt) hipHostFree(w->h_cnt);
   if (w->h_exam) hipHostFree(w->h_exam);
   delete w;
@@ -256,14 +266,16 @@ struct Engine {
     GG_HIP(hipMemsetAsync(f.bits, 0, w->words * 8, stream()));
     return GG_OK;
   }
-  // Bitmap -> binned queues and counts; reads the counts back.
+  // Bitmap -> binned queues and counts; reads the counts back.  The other
+  // frontier's bitmap is cleared in the same pass: it is the next output.
   int compact(Frontier &f, const uint32_t *bin_deg, uint64_t *visited_or, Counters *outc) {
+    Frontier &other = (&f == &w->f[0]) ? w->f[1] : w->f[0];
     GG_HIP(hipMemsetAsync(f.cnt, 0, sizeof(Counters), stream()));
     {
       ProfScope p("compact");
       hipLaunchKernelGGL(compact_kernel, dim3(grid1(w->words, 2048)), dim3(kBlock), 0, stream(),
                          f.bits, w->words, bin_deg, small_max, big_max, f.q[0], f.q[1], f.q[2],
-                         f.cnt, visited_or);
+                         f.cnt, visited_or, other.bits);
     }
     GG_HIP(hipMemcpyAsync(w->h_cnt, f.cnt, sizeof(Counters), hipMemcpyDeviceToHost, stream()));
     GG_HIP(hipStreamSynchronize(stream()));
@@ -851,6 +863,383 @@ static void put_stats(gg_run_stats *dst, const gg_run_stats &src) {
   if (dst) *dst = src;
 }
 
+// ========================================================= batched BFS (64)
+// Bit-parallel multi-source BFS: vis[v] and f[v] are 64-bit masks, bit i for
+// root i.  A level is the plain BFS level of every root at once: push ORs a
+// source's frontier mask into its out-neighbours, pull ORs the in-neighbours'
+// masks; `apply` then keeps the bits not yet visited and records depths.
+struct MsPush : NoAfter {
+  using Msg = uint64_t;
+  const uint64_t *fcur, *vis;
+  uint64_t *fnext;
+  __device__ uint64_t msg(uint32_t v, uint32_t) const { return fcur[v]; }
+  __device__ bool edge(uint32_t, uint64_t m, uint32_t dst, float) const {
+    uint64_t f = m & ~vis[dst];
+    if (!f) return false;
+    if ((fnext[dst] & f) != f) atomicOr((unsigned long long *)&fnext[dst], (unsigned long long)f);
+    return true;
+  }
+};
+struct MsPull : NoAfter {
+  using T = uint64_t;
+  const uint64_t *fcur, *vis;
+  uint64_t *fnext;
+  uint64_t full;
+  __device__ uint64_t identity() const { return 0ULL; }
+  __device__ bool dst_active(uint32_t v) const { return (vis[v] & full) != full; }
+  __device__ uint64_t value(uint32_t src, float) const { return fcur[src]; }
+  __device__ uint64_t combine(uint64_t a, uint64_t b) const { return a | b; }
+  __device__ bool finish(uint32_t v, uint64_t acc, uint32_t, double &) const {
+    uint64_t nw = acc & ~vis[v] & full;
+    if (nw) fnext[v] = nw;
+    return nw != 0;
+  }
+  __device__ bool commit(uint32_t v, uint32_t, uint64_t x) const {
+    x &= ~vis[v] & full;
+    if (!x) return false;
+    atomicOr((unsigned long long *)&fnext[v], (unsigned long long)x);
+    return true;
+  }
+};
+// For every vertex activated this level: keep the new bits, mark them
+// visited, write their depths (vertex-major, 64 entries per vertex).
+__global__ void ms_apply_kernel(const uint64_t *active, uint32_t V, uint64_t *vis, uint64_t *fnext,
+                                uint32_t *depth, uint32_t level) {
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s) {
+    if (!test_bit(active, v)) continue;
+    uint64_t nw = fnext[v] & ~vis[v];
+    fnext[v] = nw;
+    vis[v] |= nw;
+    uint32_t *d = depth + (size_t)v * 64;
+    for (uint64_t x = nw; x; x &= x - 1) d[__builtin_ctzll(x)] = level;
+  }
+}
+__global__ void ms_seed_kernel(const uint32_t *roots, int n, uint64_t *vis, uint64_t *f,
+                               uint64_t *bits, uint32_t *depth) {
+  for (int i = 0; i < n; i++) {  // one thread: roots may repeat
+    uint32_t r = roots[i];
+    vis[r] |= 1ULL << i;
+    f[r] |= 1ULL << i;
+    bits[r >> 6] |= 1ULL << (r & 63);
+    depth[(size_t)r * 64 + i] = 0;
+  }
+}
+// sum over roots of the out-degrees of reached vertices
+__global__ void ms_reach_kernel(const uint64_t *vis, const uint32_t *outdeg, uint32_t V,
+                                unsigned long long *acc) {
+  uint32_t s = gridDim.x * blockDim.x;
+  uint64_t x = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s)
+    x += (uint64_t)__popcll(vis[v]) * outdeg[v];
+  x = wave_sum_u64(x);
+  if (lane_id() == 0 && x) atomicAdd(&acc[(global_wave() & 63) * 16], (unsigned long long)x);
+}
+template <class T>
+__global__ void unpermute_rows_kernel(const T *in, T *out, const uint32_t *iperm, uint32_t V) {
+  uint64_t n = (uint64_t)V * 64, s = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += s)
+    out[(uint64_t)iperm[k >> 6] * 64 + (k & 63)] = in[k];
+}
+
+// ------------------------------------------- lane = root kernels (64 roots)
+// One wave per vertex, lane i carries root i; per-(vertex, root) state is
+// vertex-major, so every edge step moves one coalesced 256/512-byte row.
+struct LaneArgs {
+  const uint64_t *index;
+  const uint32_t *nbr;
+  const float *w;
+  const uint32_t *list;  // vertices to visit (null: 0..n-1)
+  uint32_t n;
+  uint32_t light_max;   // longer lists go to lane_heavy_kernel
+};
+// Op: T; bool active(v); T init(v, lane); void edge(v, lane, u, w, T&);
+//     void finish(v, lane, T).
+template <class Op>
+__global__ __launch_bounds__(kBlock) void lane_vertex_kernel(LaneArgs a, Op op) {
+  const uint32_t gw = global_wave();
+  const int lane = lane_id();
+  if (gw >= a.n) return;
+  uint32_t v = a.list ? a.list[gw] : gw;
+  if (!op.active(v)) return;
+  uint64_t b = a.index[v], e = a.index[v + 1];
+  if (e - b > a.light_max) return;
+  typename Op::T acc = op.init(v, lane);
+  for (uint64_t k = b; k < e; k++) op.edge(v, lane, a.nbr[k], a.w ? a.w[k] : 0.f, acc);
+  op.finish(v, lane, acc);
+}
+// Heavy lists: one wave per chunk; partials through op.commit (atomics).
+template <class Op>
+__global__ __launch_bounds__(kBlock) void lane_heavy_kernel(HeavyArgs a, Op op) {
+  const uint32_t c = global_wave();
+  const int lane = lane_id();
+  if (c >= a.n_chunks) return;
+  uint32_t v = a.chunk_v[c];
+  if (!op.active(v)) return;
+  typename Op::T acc = op.init(v, lane);
+  for (uint64_t k = a.chunk_beg[c]; k < a.chunk_end[c]; k++)
+    op.edge(v, lane, a.nbr[k], a.w ? a.w[k] : 0.f, acc);
+  op.commit(v, lane, acc);
+}
+
+// Batched SSSP, dense step: dist row of v = min over in-edges of (row of u + w).
+struct MssPull {
+  using T = float;
+  float *dist;
+  uint64_t *cm_next, *out_bits;
+  __device__ bool active(uint32_t) const { return true; }
+  __device__ float init(uint32_t, int) const { return 1e9f; }
+  __device__ void edge(uint32_t, int lane, uint32_t u, float w, float &acc) const {
+    float r = dist[(size_t)u * 64 + lane] + w;
+    if (r < acc) acc = r;
+  }
+  __device__ void finish(uint32_t v, int lane, float acc) const {
+    float *p = &dist[(size_t)v * 64 + lane];
+    bool imp = acc < *p;
+    if (imp) *p = acc;
+    uint64_t m = __ballot(imp);
+    if (m && lane == 0) {
+      cm_next[v] = m;
+      activate(out_bits, v);
+    }
+  }
+  __device__ void commit(uint32_t v, int lane, float acc) const {
+    float *p = &dist[(size_t)v * 64 + lane];
+    bool imp = false;
+    if (acc < *p) imp = acc < __uint_as_float(atomicMin((uint32_t *)p, __float_as_uint(acc)));
+    uint64_t m = __ballot(imp);
+    if (m && lane == 0) {
+      atomicOr((unsigned long long *)&cm_next[v], (unsigned long long)m);
+      activate(out_bits, v);
+    }
+  }
+};
+// Batched SSSP, sparse step: each changed (vertex, root) relaxes its out-edges.
+__global__ __launch_bounds__(kBlock) void mss_push_kernel(const uint64_t *index, const uint32_t *nbr,
+                                                           const float *wt, const uint32_t *q0,
+                                                           uint32_t n0, const uint32_t *q1,
+                                                           uint32_t n1, const uint32_t *q2,
+                                                           uint32_t n2, uint32_t waves2,
+                                                           float *dist, const uint64_t *cm_cur,
+                                                           uint64_t *cm_next, uint64_t *out_bits) {
+  const uint32_t gw = global_wave();
+  const int lane = lane_id();
+  auto relax_range = [&](uint32_t u, uint64_t b, uint64_t e) {
+    uint64_t m = cm_cur[u];
+    bool act = (m >> lane) & 1ULL;
+    float du = dist[(size_t)u * 64 + lane];
+    for (uint64_t k = b; k < e; k++) {
+      uint32_t v = nbr[k];
+      bool imp = false;
+      if (act) {
+        float c = du + wt[k];
+        float *p = &dist[(size_t)v * 64 + lane];
+        if (c < *p) imp = c < __uint_as_float(atomicMin((uint32_t *)p, __float_as_uint(c)));
+      }
+      uint64_t im = __ballot(imp);
+      if (im && lane == 0) {
+        atomicOr((unsigned long long *)&cm_next[v], (unsigned long long)im);
+        activate(out_bits, v);
+      }
+    }
+  };
+  if (gw < n0 + n1) {
+    uint32_t u = gw < n0 ? q0[gw] : q1[gw - n0];
+    relax_range(u, index[u], index[u + 1]);
+  } else {
+    uint32_t hw = gw - n0 - n1;
+    for (uint32_t h = 0; h < n2; h++) {
+      uint32_t u = q2[h];
+      uint64_t b = index[u], e = index[u + 1];
+      uint64_t nch = (e - b + kChunkEdges - 1) / kChunkEdges;
+      for (uint64_t c = hw; c < nch; c += waves2) {
+        uint64_t cb = b + c * kChunkEdges;
+        relax_range(u, cb, cb + kChunkEdges < e ? cb + kChunkEdges : e);
+      }
+    }
+  }
+}
+__global__ void mss_seed_kernel(const uint32_t *roots, int n, float *dist, uint64_t *cm,
+                                uint64_t *bits) {
+  for (int i = 0; i < n; i++) {
+    uint32_t r = roots[i];
+    dist[(size_t)r * 64 + i] = 0.f;
+    cm[r] |= 1ULL << i;
+    bits[r >> 6] |= 1ULL << (r & 63);
+  }
+}
+
+// Batched BC, forward: sigma row of each newly reached vertex = sum of the
+// sigma of its in-neighbours in the previous level, per root (bc.cpp:58-89).
+struct MsbcSigma {
+  using T = double;
+  double *sigma;
+  const uint64_t *fprev, *fnew;
+  __device__ bool active(uint32_t v) const { return fnew[v] != 0; }
+  __device__ double init(uint32_t, int) const { return 0.0; }
+  __device__ void edge(uint32_t v, int lane, uint32_t u, float, double &acc) const {
+    uint64_t m = fprev[u] & fnew[v];
+    if ((m >> lane) & 1ULL) acc += sigma[(size_t)u * 64 + lane];
+  }
+  __device__ void finish(uint32_t v, int lane, double acc) const {
+    if ((fnew[v] >> lane) & 1ULL) sigma[(size_t)v * 64 + lane] = acc;
+  }
+  __device__ void commit(uint32_t v, int lane, double acc) const {
+    if (((fnew[v] >> lane) & 1ULL) && acc != 0.0) unsafeAtomicAdd(&sigma[(size_t)v * 64 + lane], acc);
+  }
+};
+// Batched BC, backward: for v at depth L-1 of root i,
+// D(v) += sum of D(w) over out-edges v->w with w at depth L (bc.cpp:125-167).
+struct MsbcBack {
+  struct T {
+    double s;
+    bool on;  // this lane's root has v at depth L-1
+  };
+  double *dd;
+  const uint32_t *depth;
+  const uint64_t *level_bits;  // vertices at depth L-1 for some root
+  uint32_t L;
+  __device__ bool active(uint32_t v) const { return test_bit(level_bits, v); }
+  __device__ T init(uint32_t v, int lane) const {
+    return T{0.0, depth[(size_t)v * 64 + lane] == L - 1};
+  }
+  __device__ void edge(uint32_t, int lane, uint32_t u, float, T &t) const {
+    if (t.on && depth[(size_t)u * 64 + lane] == L) t.s += dd[(size_t)u * 64 + lane];
+  }
+  __device__ void finish(uint32_t v, int lane, T t) const {
+    if (t.on) dd[(size_t)v * 64 + lane] += t.s;
+  }
+  __device__ void commit(uint32_t v, int lane, T t) const {
+    if (t.on && t.s != 0.0) unsafeAtomicAdd(&dd[(size_t)v * 64 + lane], t.s);
+  }
+};
+__global__ void msbc_seed_sigma_kernel(const uint32_t *roots, int n, double *sigma) {
+  for (int i = 0; i < n; i++) sigma[(size_t)roots[i] * 64 + i] = 1.0;
+}
+// inv = 1/sigma in place (+inf where unreached), D = inv
+__global__ void msbc_inv_kernel(double *sigma_inv, double *dd, uint64_t n) {
+  uint64_t s = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += s) {
+    double inv = 1 / sigma_inv[k];
+    sigma_inv[k] = inv;
+    dd[k] = inv;
+  }
+}
+__global__ void msbc_final_kernel(double *dd, const double *inv, uint64_t n) {
+  uint64_t s = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += s)
+    dd[k] = (dd[k] - inv[k]) / inv[k];  // bc.cpp:170-176
+}
+
+// ------------------------------------------------- batched-query plumbing
+static int batch_roots(const gg_graph *g, const uint32_t *h_roots, int n_roots, uint32_t *out) {
+  if (!h_roots || n_roots < 1 || n_roots > 64)
+    return fail(GG_EINVAL, "batched query: 1 to 64 roots");
+  for (int i = 0; i < n_roots; i++) {
+    if (h_roots[i] >= g->vertices) return fail(GG_EINVAL, "batched query: root out of range");
+    out[i] = g->to_internal(h_roots[i]);
+  }
+  return GG_OK;
+}
+
+static int ms_alloc(gg_graph *g, Work *w, bool bc) {
+  size_t V = g->vertices;
+  if (!w->ms_vis) {
+    GG_TRY(device_alloc((void **)&w->ms_vis, V * 8 + 8));
+    GG_TRY(device_alloc((void **)&w->ms_f[0], V * 8 + 8));
+    GG_TRY(device_alloc((void **)&w->ms_f[1], V * 8 + 8));
+    GG_TRY(device_alloc((void **)&w->ms_roots, 64 * 4));
+    GG_TRY(device_alloc(&w->ms_tmp, V * 64 * 4));
+  }
+  if (bc && g->reordered && !w->mb_a) {
+    GG_TRY(device_alloc(&w->mb_a, V * 64 * 8));
+    GG_TRY(device_alloc(&w->mb_b, V * 64 * 8));
+  }
+  return GG_OK;
+}
+
+// Keeps a copy of one level's frontier bitmap (BC's `levels`, bc.cpp:33, :99).
+static int ms_store_level(Work *w, const uint64_t *bits, uint32_t level) {
+  if (level >= w->ms_levels_cap) {
+    size_t cap = std::max<size_t>(64, 2 * (size_t)level + 2);
+    uint64_t *p = nullptr;
+    GG_TRY(device_alloc((void **)&p, cap * w->words * 8));
+    if (w->ms_levels)
+      GG_HIP(hipMemcpyAsync(p, w->ms_levels, w->ms_levels_cap * w->words * 8,
+                            hipMemcpyDeviceToDevice, stream()));
+    GG_HIP(hipStreamSynchronize(stream()));
+    device_free(w->ms_levels);
+    w->ms_levels = p;
+    w->ms_levels_cap = cap;
+  }
+  GG_HIP(hipMemcpyAsync(w->ms_levels + (size_t)level * w->words, bits, w->words * 8,
+                        hipMemcpyDeviceToDevice, stream()));
+  return GG_OK;
+}
+
+// Multi-root BFS levels (bit-parallel masks); hook(level, fprev, fnew) runs
+// after every non-empty level with the level's frontier compacted in en.fin().
+template <class Hook>
+static int ms_bfs_run(gg_graph *g, Work *w, Engine &en, const uint32_t *roots, int n_roots,
+                      uint32_t *depth, Hook hook, bool store_levels = false) {
+  const Dir &out = out_dir(g), &in = in_dir(g);
+  uint32_t V = g->vertices;
+  GG_HIP(hipMemsetAsync(depth, 0xFF, (size_t)V * 64 * 4, stream()));
+  GG_HIP(hipMemsetAsync(w->ms_vis, 0, (size_t)V * 8, stream()));
+  GG_HIP(hipMemsetAsync(w->ms_f[0], 0, (size_t)V * 8, stream()));
+  GG_HIP(hipMemcpyAsync(w->ms_roots, roots, 4 * (size_t)n_roots, hipMemcpyHostToDevice, stream()));
+  GG_TRY(en.clear_bits(en.fin()));
+  hipLaunchKernelGGL(ms_seed_kernel, dim3(1), dim3(1), 0, stream(), w->ms_roots, n_roots, w->ms_vis,
+                     w->ms_f[0], en.fin().bits, depth);
+  GG_TRY(en.compact_in(out.degree));
+  if (store_levels) GG_TRY(ms_store_level(w, en.fin().bits, 0));
+  const uint64_t full = n_roots == 64 ? ~0ULL : ((1ULL << n_roots) - 1ULL);
+  int fc = 0;
+  for (uint32_t level = 1; en.in.total > 0; level++) {
+    uint64_t *fcur = w->ms_f[fc], *fnext = w->ms_f[fc ^ 1];
+    GG_HIP(hipMemsetAsync(fnext, 0, (size_t)V * 8, stream()));
+    bool sp = en.sparse(en.in.edges);
+    if (sp) {
+      MsPush op;
+      op.fcur = fcur;
+      op.vis = w->ms_vis;
+      op.fnext = fnext;
+      GG_TRY(en.launch_push(out, op, false));
+    } else {
+      MsPull op;
+      op.fcur = fcur;
+      op.vis = w->ms_vis;
+      op.fnext = fnext;
+      op.full = full;
+      GG_TRY(en.launch_pull_group(in, op, nullptr, "ms_pull"));
+      GG_TRY(en.launch_pull_heavy(in, op, "ms_pull_heavy"));
+    }
+    {
+      ProfScope p("ms_apply");
+      hipLaunchKernelGGL(ms_apply_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(),
+                         en.fout().bits, V, w->ms_vis, fnext, depth, level);
+    }
+    GG_TRY(en.finish_iteration(sp, out.degree, nullptr));
+    if (en.in.total > 0) GG_TRY(hook(level, fcur, fnext));
+    fc ^= 1;
+  }
+  return GG_OK;
+}
+
+// Sum over roots of the out-degrees of the vertices each reached.
+static int ms_reach(gg_graph *g, Work *w, uint64_t *m) {
+  GG_HIP(hipMemsetAsync(w->exam, 0, 1024 * 8, stream()));
+  hipLaunchKernelGGL(ms_reach_kernel, dim3(grid1(g->vertices, 4096)), dim3(kBlock), 0, stream(),
+                     w->ms_vis, out_dir(g).degree, g->vertices, w->exam);
+  GG_HIP(hipMemcpyAsync(w->h_exam, w->exam, 1024 * 8, hipMemcpyDeviceToHost, stream()));
+  GG_HIP(hipMemsetAsync(w->exam, 0, 1024 * 8, stream()));
+  GG_HIP(hipStreamSynchronize(stream()));
+  uint64_t s = 0;
+  for (int k = 0; k < 64; k++) s += w->h_exam[k * 16];
+  *m = s;
+  return GG_OK;
+}
+
 }  // namespace gg
 
 namespace gg {
@@ -994,7 +1383,6 @@ int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *d_depth, ui
   if (h_frontier && cap > 0) h_frontier[0] = 1;
   for (uint32_t level = 1; en.in.total > 0; level++) {
     if (en.st.iterations < 64) en.st.it_unvisited[en.st.iterations] = (uint64_t)V - reached;
-    GG_TRY(en.clear_bits(en.fout()));
     bool sp = en.sparse(en.in.edges);
     if (sp) {
       BfsPush op;
@@ -1067,7 +1455,6 @@ int gg_cc(gg_graph *g, uint32_t *d_label, uint32_t *h_frontier, int cap, gg_run_
   while (en.in.total > 0) {
     if (h_frontier && lv < cap) h_frontier[lv] = en.in.total;
     lv++;
-    GG_TRY(en.clear_bits(en.fout()));
     bool sp = en.sparse(en.in.edges);
     if (sp) {
       uint64_t n = (uint64_t)en.in.n[0] + en.in.n[1] + en.in.n[2];
@@ -1121,7 +1508,6 @@ int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int
   while (en.in.total > 0) {
     if (h_frontier && lv < cap) h_frontier[lv] = en.in.total;
     lv++;
-    GG_TRY(en.clear_bits(en.fout()));
     bool sp = en.sparse(en.in.edges);
     if (sp) {
       uint64_t n = (uint64_t)en.in.n[0] + en.in.n[1] + en.in.n[2];
@@ -1181,7 +1567,6 @@ int gg_bc(gg_graph *g, uint32_t root, double *d_dep, double *d_inv_paths, uint32
   for (; en.in.total > 0; level++) {
     if (h_frontier && lv < cap) h_frontier[lv] = en.in.total;
     lv++;
-    GG_TRY(en.clear_bits(en.fout()));
     bool sp = en.sparse(en.in.edges);
     if (sp) {
       BcFwdPush op{sigma, w->visited, depth, level};
@@ -1227,6 +1612,175 @@ int gg_bc(gg_graph *g, uint32_t root, double *d_dep, double *d_inv_paths, uint32
   en.st.device_ms = tm.stop();
   GG_HIP(hipGetLastError());
   en.st.reached = reached;
+  put_stats(stats, en.st);
+  return GG_OK;
+}
+
+int gg_bfs_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, uint32_t *d_depth,
+                 gg_run_stats *stats) {
+  GG_TRY(check_graph(g));
+  if (!d_depth) return fail(GG_EINVAL, "gg_bfs_batch: null output");
+  uint32_t roots[64];
+  GG_TRY(batch_roots(g, h_roots, n_roots, roots));
+  Work *w;
+  GG_TRY(work_get(g, &w));
+  GG_TRY(ms_alloc(g, w, false));
+  Engine en(g, w);
+  uint32_t V = g->vertices;
+  DeviceTimer tm;
+  uint32_t *depth = g->reordered ? (uint32_t *)w->ms_tmp : d_depth;
+  GG_TRY(ms_bfs_run(g, w, en, roots, n_roots, depth,
+                    [](uint32_t, const uint64_t *, const uint64_t *) { return GG_OK; }));
+  uint64_t m = 0;
+  GG_TRY(ms_reach(g, w, &m));
+  if (g->reordered)
+    hipLaunchKernelGGL(unpermute_rows_kernel<uint32_t>, dim3(grid1((uint64_t)V * 64)), dim3(kBlock),
+                       0, stream(), depth, d_depth, g->iperm, V);
+  en.st.device_ms = tm.stop();
+  GG_HIP(hipGetLastError());
+  en.st.m_reach = m;
+  en.st.reached = (uint64_t)n_roots;
+  put_stats(stats, en.st);
+  return GG_OK;
+}
+
+int gg_sssp_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, float *d_dist,
+                  gg_run_stats *stats) {
+  GG_TRY(check_graph(g));
+  if (!g->weighted) return fail(GG_EINVAL, "SSSP needs a weighted graph");
+  if (!d_dist) return fail(GG_EINVAL, "gg_sssp_batch: null output");
+  uint32_t roots[64];
+  GG_TRY(batch_roots(g, h_roots, n_roots, roots));
+  Work *w;
+  GG_TRY(work_get(g, &w));
+  GG_TRY(ms_alloc(g, w, false));
+  Engine en(g, w);
+  const Dir &out = out_dir(g), &in = in_dir(g);
+  uint32_t V = g->vertices;
+  DeviceTimer tm;
+  float *dist = g->reordered ? (float *)w->ms_tmp : d_dist;
+  fill<uint32_t>((uint32_t *)dist, kInfBits, (uint64_t)V * 64);  // 1e9 (sssp.cpp:163)
+  GG_HIP(hipMemsetAsync(w->ms_f[0], 0, (size_t)V * 8, stream()));
+  GG_HIP(hipMemcpyAsync(w->ms_roots, roots, 4 * (size_t)n_roots, hipMemcpyHostToDevice, stream()));
+  GG_TRY(en.clear_bits(en.fin()));
+  hipLaunchKernelGGL(mss_seed_kernel, dim3(1), dim3(1), 0, stream(), w->ms_roots, n_roots, dist,
+                     w->ms_f[0], en.fin().bits);
+  GG_TRY(en.compact_in(out.degree));
+  int fc = 0;
+  while (en.in.total > 0) {
+    uint64_t *cm_cur = w->ms_f[fc], *cm_next = w->ms_f[fc ^ 1];
+    GG_HIP(hipMemsetAsync(cm_next, 0, (size_t)V * 8, stream()));
+    bool sp = en.sparse(en.in.edges);
+    if (sp) {
+      uint32_t waves2 = en.in.n[2] ? (uint32_t)options().huge_grid_waves : 0;
+      uint64_t waves = (uint64_t)en.in.n[0] + en.in.n[1] + waves2;
+      ProfScope p("mss_push");
+      hipLaunchKernelGGL(mss_push_kernel, dim3(grid_waves(waves)), dim3(kBlock), 0, stream(),
+                         out.index, out.nbr, out.w, en.fin().q[0], en.in.n[0], en.fin().q[1],
+                         en.in.n[1], en.fin().q[2], en.in.n[2], waves2, dist, cm_cur, cm_next,
+                         en.fout().bits);
+    } else {
+      MssPull op{dist, cm_next, en.fout().bits};
+      ProfScope p("mss_pull");
+      LaneArgs a{in.index, in.nbr, in.w, nullptr, V, in.light_max};
+      hipLaunchKernelGGL(lane_vertex_kernel<MssPull>, dim3(grid_waves(V)), dim3(kBlock), 0,
+                         stream(), a, op);
+      if (in.n_chunks) {
+        HeavyArgs h{in.nbr, in.w, in.chunk_v, in.chunk_h, in.chunk_beg, in.chunk_end, in.n_chunks,
+                    en.fout().bits, nullptr};
+        hipLaunchKernelGGL(lane_heavy_kernel<MssPull>, dim3(grid_waves(in.n_chunks)), dim3(kBlock),
+                           0, stream(), h, op);
+      }
+    }
+    GG_TRY(en.finish_iteration(sp, out.degree, nullptr));
+    fc ^= 1;
+  }
+  if (g->reordered)
+    hipLaunchKernelGGL(unpermute_rows_kernel<float>, dim3(grid1((uint64_t)V * 64)), dim3(kBlock), 0,
+                       stream(), dist, d_dist, g->iperm, V);
+  en.st.device_ms = tm.stop();
+  GG_HIP(hipGetLastError());
+  en.st.reached = (uint64_t)n_roots;
+  put_stats(stats, en.st);
+  return GG_OK;
+}
+
+int gg_bc_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_dep,
+                double *d_inv_paths, gg_run_stats *stats) {
+  GG_TRY(check_graph(g));
+  if (!d_dep || !d_inv_paths) return fail(GG_EINVAL, "gg_bc_batch: null output");
+  uint32_t roots[64];
+  GG_TRY(batch_roots(g, h_roots, n_roots, roots));
+  Work *w;
+  GG_TRY(work_get(g, &w));
+  GG_TRY(ms_alloc(g, w, true));
+  Engine en(g, w);
+  const Dir &out = out_dir(g), &in = in_dir(g);
+  uint32_t V = g->vertices;
+  const uint64_t n = (uint64_t)V * 64;
+  DeviceTimer tm;
+  uint32_t *depth = (uint32_t *)w->ms_tmp;
+  double *sigma = g->reordered ? (double *)w->mb_a : d_inv_paths;
+  double *dd = g->reordered ? (double *)w->mb_b : d_dep;
+  fill<double>(sigma, 0.0, n);
+  GG_HIP(hipMemcpyAsync(w->ms_roots, roots, 4 * (size_t)n_roots, hipMemcpyHostToDevice, stream()));
+  hipLaunchKernelGGL(msbc_seed_sigma_kernel, dim3(1), dim3(1), 0, stream(), w->ms_roots, n_roots,
+                     sigma);
+  // forward: the multi-root BFS levels, with sigma accumulated per level
+  uint32_t levels = 0;
+  auto hook = [&](uint32_t level, const uint64_t *fprev, const uint64_t *fnew) -> int {
+    GG_TRY(ms_store_level(w, en.fin().bits, level));
+    levels = level;
+    MsbcSigma op{sigma, fprev, fnew};
+    ProfScope p("msbc_sigma");
+    for (int b = 0; b < 3; b++) {
+      if (!en.in.n[b]) continue;
+      LaneArgs a{in.index, in.nbr, nullptr, en.fin().q[b], en.in.n[b], in.light_max};
+      hipLaunchKernelGGL(lane_vertex_kernel<MsbcSigma>, dim3(grid_waves(en.in.n[b])), dim3(kBlock),
+                         0, stream(), a, op);
+    }
+    if (in.n_chunks && en.in.total) {
+      HeavyArgs h{in.nbr, nullptr, in.chunk_v, in.chunk_h, in.chunk_beg, in.chunk_end, in.n_chunks,
+                  nullptr, nullptr};
+      hipLaunchKernelGGL(lane_heavy_kernel<MsbcSigma>, dim3(grid_waves(in.n_chunks)), dim3(kBlock),
+                         0, stream(), h, op);
+    }
+    return GG_OK;
+  };
+  GG_TRY(ms_bfs_run(g, w, en, roots, n_roots, depth, hook, true));
+  hipLaunchKernelGGL(msbc_inv_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(), sigma, dd, n);
+  // backward: level L-1 pulls D from level L along out-edges (the reference
+  // pushes along the transposed graph, bc.cpp:121-168)
+  for (uint32_t L = levels; L >= 1; L--) {
+    const uint64_t *lb = w->ms_levels + (size_t)(L - 1) * w->words;
+    GG_HIP(hipMemcpyAsync(en.fin().bits, lb, w->words * 8, hipMemcpyDeviceToDevice, stream()));
+    GG_TRY(en.compact_in(out.degree));
+    MsbcBack op{dd, depth, lb, L};
+    ProfScope p("msbc_back");
+    for (int b = 0; b < 3; b++) {
+      if (!en.in.n[b]) continue;
+      LaneArgs a{out.index, out.nbr, nullptr, en.fin().q[b], en.in.n[b], out.light_max};
+      hipLaunchKernelGGL(lane_vertex_kernel<MsbcBack>, dim3(grid_waves(en.in.n[b])), dim3(kBlock),
+                         0, stream(), a, op);
+    }
+    if (out.n_chunks) {
+      HeavyArgs h{out.nbr, nullptr, out.chunk_v, out.chunk_h, out.chunk_beg, out.chunk_end,
+                  out.n_chunks, nullptr, nullptr};
+      hipLaunchKernelGGL(lane_heavy_kernel<MsbcBack>, dim3(grid_waves(out.n_chunks)), dim3(kBlock),
+                         0, stream(), h, op);
+    }
+    GG_TRY(en.note_iteration(false));
+  }
+  hipLaunchKernelGGL(msbc_final_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(), dd, sigma, n);
+  if (g->reordered) {
+    hipLaunchKernelGGL(unpermute_rows_kernel<double>, dim3(grid1(n)), dim3(kBlock), 0, stream(), dd,
+                       d_dep, g->iperm, V);
+    hipLaunchKernelGGL(unpermute_rows_kernel<double>, dim3(grid1(n)), dim3(kBlock), 0, stream(),
+                       sigma, d_inv_paths, g->iperm, V);
+  }
+  en.st.device_ms = tm.stop();
+  GG_HIP(hipGetLastError());
+  en.st.reached = (uint64_t)n_roots;
   put_stats(stats, en.st);
   return GG_OK;
 }

@@ -46,7 +46,11 @@ int fail(int code, const std::string &msg);
 
 // ----------------------------------------------------------------- options
 struct Options {
-  int64_t switch_divisor = 20;
+  // push/pull crossover: sparse iff active out-edges < |E| / switch_divisor.
+  // The reference uses 20 (core/graph.hpp:1509); on gfx950 the pull sweep is
+  // cheap relative to push atomics and the measured optimum for RMAT-24 BFS
+  // lies between 200 and 2000 (results do not depend on the mode).
+  int64_t switch_divisor = 200;
   int64_t force_mode = 0;        // 0 auto, 1 push, 2 pull
   int64_t push_small_max = 64;   // in-wave load-balanced bin
   int64_t push_big_max = 4096;   // one wave per vertex; above: shared by the grid
@@ -148,6 +152,14 @@ struct Work {
   double *dsum = nullptr;      // reduced per-iteration scalars
   int dacc_cap = 0;            // iterations the buffers hold
   unsigned long long *exam = nullptr;  // 64 slots x 16 (one 128-byte line each)
+  // batched (64-root) traversals, allocated on first use
+  uint64_t *ms_vis = nullptr;          // V visited masks
+  uint64_t *ms_f[2] = {nullptr, nullptr};  // V frontier masks (current, next)
+  uint32_t *ms_roots = nullptr;        // 64 roots
+  void *ms_tmp = nullptr;              // V x 64 x 4 bytes (depth / distance rows)
+  void *mb_a = nullptr, *mb_b = nullptr;  // V x 64 x 8 bytes each (BC rows, reordered outputs)
+  uint64_t *ms_levels = nullptr;       // per-level frontier bitmaps (BC backward)
+  size_t ms_levels_cap = 0;            // levels the store holds
   Counters *h_cnt = nullptr;   // pinned host copy
   unsigned long long *h_exam = nullptr;
   std::vector<void *> allocations;

@@ -68,6 +68,13 @@ template <> struct Shfl<float> {
   static __device__ __forceinline__ float up(float x, int d) { return __shfl_up(x, d); }
   static __device__ __forceinline__ float xr(float x, int m) { return __shfl_xor(x, m); }
 };
+template <> struct Shfl<uint64_t> {
+  static __device__ __forceinline__ uint64_t at(uint64_t x, int s) { return shfl_u64(x, s); }
+  static __device__ __forceinline__ uint64_t up(uint64_t x, int d) { return shfl_up_u64(x, d); }
+  static __device__ __forceinline__ uint64_t xr(uint64_t x, int m) {
+    return shfl_u64(x, lane_id() ^ m);
+  }
+};
 template <> struct Shfl<double> {
   static __device__ __forceinline__ double at(double x, int s) { return shfl_f64(x, s); }
   static __device__ __forceinline__ double up(double x, int d) {
@@ -128,7 +135,8 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(const uint64_t *bits, s
                                                          const uint32_t *bin_deg,
                                                          uint32_t small_max, uint32_t big_max,
                                                          uint32_t *q0, uint32_t *q1, uint32_t *q2,
-                                                         Counters *cnt, uint64_t *visited);
+                                                         Counters *cnt, uint64_t *visited,
+                                                         uint64_t *clear);
 
 // ------------------------------------------------------------------- push
 struct PushArgs {
@@ -306,7 +314,20 @@ __global__ __launch_bounds__(kBlock) void pull_heavy_kernel(HeavyArgs a, Op op) 
   if (!op.dst_active(v)) return;
   uint64_t cb = a.chunk_beg[c], ce = a.chunk_end[c];
   T x = op.identity();
-  for (uint64_t e = cb + lane; e < ce; e += 64) x = op.combine(x, op.value(a.nbr[e], a.w ? a.w[e] : 0.f));
+  // chunks hold at most kChunkEdges = 16 x 64 entries: load all neighbour ids
+  // first, then gather, so each lane keeps 16 independent loads in flight
+  constexpr int kPer = kChunkEdges / 64;
+  uint32_t u[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    uint64_t e = cb + lane + 64 * k;
+    u[k] = e < ce ? a.nbr[e] : 0xFFFFFFFFu;
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    uint64_t e = cb + lane + 64 * k;
+    if (u[k] != 0xFFFFFFFFu) x = op.combine(x, op.value(u[k], a.w ? a.w[e] : 0.f));
+  }
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) x = op.combine(x, Shfl<T>::xr(x, m));
   if (lane == 0) {

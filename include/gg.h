@@ -187,6 +187,28 @@ int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int
 int gg_bc(gg_graph *g, uint32_t root, double *d_dep, double *d_inv_paths,
           uint32_t *h_frontier, int cap, gg_run_stats *stats);
 
+/* ------------------------------------------------------- batched queries */
+/* Up to 64 BFS roots in one traversal: every vertex carries a 64-bit visited
+ * mask and a 64-bit frontier mask (bit i = root i), so one sweep of the
+ * CSR/CSC serves all roots.  No counterpart in the reference, which runs one
+ * root per compute() (toolkits/bfs.cpp:22-109); each root's depths equal what
+ * gg_bfs returns for it.  d_depth is vertex-major: d_depth[v * 64 + i] = level
+ * of v from roots[i], 0xFFFFFFFF when unreached (64 entries per vertex
+ * whatever n_roots is).  stats->m_reach sums over the roots. */
+int gg_bfs_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, uint32_t *d_depth,
+                 gg_run_stats *stats);
+/* Up to 64 SSSP roots in one traversal (toolkits/sssp.cpp semantics per
+ * root): one wave per vertex, lane i relaxes root i, so each edge moves one
+ * 256-byte row of distances.  d_dist[v * 64 + i] (f32, 1e9 when unreached)
+ * equals what gg_sssp returns for roots[i], bit for bit. */
+int gg_sssp_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, float *d_dist,
+                  gg_run_stats *stats);
+/* Up to 64 BC roots in one traversal (toolkits/bc.cpp semantics per root):
+ * d_dep[v * 64 + i] and d_inv_paths[v * 64 + i] (FP64) as gg_bc returns them
+ * for roots[i] (within 1e-9 relative; NaN where unreached). */
+int gg_bc_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_dep,
+                double *d_inv_paths, gg_run_stats *stats);
+
 /* ----------------------------------------------------------- profiling */
 
 /* Per-kernel device time accumulated with HIP events on the launching

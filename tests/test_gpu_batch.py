@@ -1,0 +1,77 @@
+"""Batched queries (up to 64 roots per traversal): every root's result must
+equal the single-root method's -- the oracle's, bit for bit for SSSP
+distances, within 1e-9 relative for BC."""
+import numpy as np
+import pytest
+
+import oracle
+from test_gpu_parity import MODES, assert_rel, roots_of, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("reorder", [False, True])
+@pytest.mark.parametrize("mode", ["auto", "push", "pull"])
+def test_sssp_batch_matches_oracle(gg, reorder, mode):
+    scale = 12
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, seed=81, weighted=True)
+    g = gg.Graph.build(e, V, reorder=reorder)
+    o = oracle.Graph(e, V)
+    roots = roots_of(o.out_degree(), 40, 81) + [0]
+    say("sssp_batch", reorder, mode)
+    with gg.options(force_mode=MODES[mode]):
+        dist, _ = gg.sssp_batch(g, roots)
+    for i, r in enumerate(roots):
+        ref, _ = o.sssp(r)
+        assert np.array_equal(dist[i].view(np.uint32), ref.view(np.uint32)), (i, r)
+
+
+def test_sssp_batch_hub_graph(gg):
+    # out-degree 20000 root (grid-shared push chunks), heavy in-degree sink
+    n = 30000
+    pairs = [(0, i) for i in range(1, 20001)] + [(i, 29999) for i in range(1, 12001)] + \
+            [(29999, 1)] + [(i, i + 1) for i in range(20001, 29998)]
+    w = (np.arange(len(pairs)) % 97 + 1).astype(np.float32) / 97
+    e = oracle.edges_from_pairs(pairs, w)
+    g = gg.Graph.build(e, n)
+    o = oracle.Graph(e, n)
+    roots = [0, 1, 29999, 20001]
+    for mode in ("push", "pull", "auto"):
+        with gg.options(force_mode=MODES[mode]):
+            dist, _ = gg.sssp_batch(g, roots)
+        for i, r in enumerate(roots):
+            assert np.array_equal(dist[i].view(np.uint32), o.sssp(r)[0].view(np.uint32)), mode
+
+
+@pytest.mark.parametrize("reorder", [False, True])
+@pytest.mark.parametrize("mode", ["auto", "push", "pull"])
+def test_bc_batch_matches_oracle(gg, reorder, mode):
+    scale = 12
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, seed=82)
+    g = gg.Graph.build(e, V, reorder=reorder)
+    o = oracle.Graph(e, V)
+    roots = roots_of(o.out_degree(), 24, 82)
+    say("bc_batch", reorder, mode)
+    with gg.options(force_mode=MODES[mode]):
+        dep, inv, _ = gg.bc_batch(g, roots)
+    for i, r in enumerate(roots):
+        rdep, rinv, _ = o.bc(r)
+        assert_rel(inv[i], rinv)
+        assert_rel(dep[i], rdep)
+
+
+def test_bc_batch_hub_graph(gg):
+    n = 30000
+    pairs = [(0, i) for i in range(1, 12001)] + [(i, 12001 + (i % 50)) for i in range(1, 12001)] + \
+            [(12001 + j, 20000) for j in range(50)] + [(i, 20000) for i in range(1, 3000)]
+    e = oracle.edges_from_pairs(pairs)
+    g = gg.Graph.build(e, n)
+    o = oracle.Graph(e, n)
+    roots = [0, 5, 12001, 20000]
+    dep, inv, _ = gg.bc_batch(g, roots)
+    for i, r in enumerate(roots):
+        rdep, rinv, _ = o.bc(r)
+        assert_rel(inv[i], rinv)
+        assert_rel(dep[i], rdep)

@@ -366,3 +366,40 @@ def test_reordered_subset_uses_original_ids(gg):
     internal = int(g.perm()[5])
     words = s.words()
     assert (int(words[internal >> 6]) >> (internal & 63)) & 1 == 1
+
+
+# ------------------------------------------------------- batched 64-root BFS
+
+@pytest.mark.parametrize("reorder", [False, True])
+@pytest.mark.parametrize("mode", ["auto", "push", "pull"])
+def test_bfs_batch_matches_plain(gg, reorder, mode):
+    scale = 13
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, 32, seed=71)
+    g = gg.Graph.build(e, V, reorder=reorder)
+    o = oracle.Graph(e, V)
+    roots = roots_of(o.out_degree(), 64, 71)
+    say("bfs_batch", reorder, mode)
+    with gg.options(force_mode=MODES[mode]):
+        depths, st = gg.bfs_batch(g, roots)
+    m = 0
+    od = o.out_degree().astype(np.int64)
+    for i, r in enumerate(roots):
+        _, rd, _ = o.bfs(r)
+        assert np.array_equal(depths[i], rd), (i, r)
+        m += int(od[rd != 0xFFFFFFFF].sum())
+    assert st["m_reach"] == m
+
+
+def test_bfs_batch_partial_and_repeated_roots(gg):
+    e = oracle.rmat_edges(11, seed=72)
+    V = 1 << 11
+    g = gg.Graph.build(e, V)
+    o = oracle.Graph(e, V)
+    r = roots_of(o.out_degree(), 3, 72)
+    roots = [r[0], r[1], r[0], r[2], 0]  # repeats and a possibly isolated vertex
+    depths, _ = gg.bfs_batch(g, roots)
+    for i, root in enumerate(roots):
+        assert np.array_equal(depths[i], o.bfs(root)[1])
+    with pytest.raises(ValueError):
+        gg.bfs_batch(g, list(range(65)))
