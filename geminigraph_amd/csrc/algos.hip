@@ -521,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_group_kernel(BfsPullArgs a) {
   uint64_t examined = 0;
   if (serial) {
     for (uint64_t e = b; e < e1; e++) {
-      uint32_t u = a.nbr[e];
+      uint32_t u = stream_load(&a.nbr[e]);
       examined++;
       if (test_bit(a.fin, u)) {
         found = u;
@@ -537,7 +537,7 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_group_kernel(BfsPullArgs a) {
     uint32_t fk = kUnset;
     for (uint64_t e0 = bk; e0 < ek; e0 += 64) {
       uint64_t e = e0 + lane;
-      uint32_t u = e < ek ? a.nbr[e] : 0;
+      uint32_t u = e < ek ? stream_load(&a.nbr[e]) : 0;
       bool hit = e < ek && test_bit(a.fin, u);
       uint64_t hm = __ballot(hit);
       if (lane == k)
@@ -587,7 +587,7 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_heavy_kernel(BfsHeavyArgs a) 
   for (uint64_t e0 = cb; e0 < ce; e0 += 64) {
     if (__hip_atomic_load(&a.parent[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.V) break;
     uint64_t e = e0 + lane;
-    uint32_t u = e < ce ? a.nbr[e] : 0;
+    uint32_t u = e < ce ? stream_load(&a.nbr[e]) : 0;
     bool hit = e < ce && test_bit(a.fin, u);
     uint64_t hm = __ballot(hit);
     examined += hm ? (uint64_t)(__builtin_ctzll(hm) + 1) : (ce - e0 < 64 ? ce - e0 : 64);
@@ -903,16 +903,87 @@ struct MsPull : NoAfter {
 };
 // For every vertex activated this level: keep the new bits, mark them
 // visited, write their depths (vertex-major, 64 entries per vertex).
-__global__ void ms_apply_kernel(const uint64_t *active, uint32_t V, uint64_t *vis, uint64_t *fnext,
-                                uint32_t *depth, uint32_t level) {
-  uint32_t s = gridDim.x * blockDim.x;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s) {
-    if (!test_bit(active, v)) continue;
-    uint64_t nw = fnext[v] & ~vis[v];
-    fnext[v] = nw;
-    vis[v] |= nw;
-    uint32_t *d = depth + (size_t)v * 64;
-    for (uint64_t x = nw; x; x &= x - 1) d[__builtin_ctzll(x)] = level;
+// Dense multi-root step with early exit: a vertex stops scanning its in-list
+// once every root still missing from it has been found (the bottom-up BFS
+// break of bfs.cpp:66-67, per root mask).  Short lists: one lane each; longer
+// ones: the whole wave, 64 entries at a time.  Heavy lists go to the chunked
+// kernel.
+struct MsPullArgs {
+  const uint64_t *index;
+  const uint32_t *nbr;
+  const uint64_t *fcur, *vis;
+  uint64_t *fnext, *out;
+  uint64_t full;
+  uint32_t V, light_max, serial_max, ngroups;
+};
+__global__ __launch_bounds__(kBlock) void ms_pull_group_kernel(MsPullArgs a) {
+  const uint32_t g = global_wave();
+  const int lane = lane_id();
+  if (g >= a.ngroups) return;
+  uint32_t v = g * 64 + lane;
+  uint64_t need = 0, b = 0, e1 = 0;
+  if (v < a.V) {
+    need = a.full & ~a.vis[v];
+    if (need) {
+      b = a.index[v];
+      e1 = a.index[v + 1];
+    }
+  }
+  uint64_t deg = e1 - b;
+  bool cand = need && deg > 0 && deg <= a.light_max;
+  bool serial = cand && deg <= a.serial_max;
+  uint64_t acc = 0;
+  if (serial) {
+    for (uint64_t e = b; e < e1; e++) {
+      acc |= a.fcur[stream_load(&a.nbr[e])];
+      if ((acc & need) == need) break;
+    }
+  }
+  uint64_t mmask = __ballot(cand && !serial);
+  while (mmask) {
+    int k = __builtin_ctzll(mmask);
+    mmask &= mmask - 1;
+    uint64_t bk = shfl_u64(b, k), ek = shfl_u64(e1, k), nk = shfl_u64(need, k);
+    uint64_t ak = 0;
+    for (uint64_t e0 = bk; e0 < ek; e0 += 64) {
+      uint64_t e = e0 + lane;
+      uint64_t x = e < ek ? a.fcur[stream_load(&a.nbr[e])] : 0ULL;
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) x |= shfl_u64(x, lane ^ m);
+      ak |= x;
+      if ((ak & nk) == nk) break;
+    }
+    if (lane == k) acc = ak;
+  }
+  uint64_t nw = acc & need;
+  if (nw) a.fnext[v] = nw;
+  uint64_t m = __ballot(nw != 0);
+  if (lane == 0 && m) atomicOr((unsigned long long *)&a.out[g], (unsigned long long)m);
+}
+
+// One wave per 64-vertex bitmap word: lane k settles vertex 64w+k's masks,
+// then the wave writes each activated vertex's depth row with lane = root
+// (one coalesced 256-byte masked store per vertex).
+__global__ __launch_bounds__(kBlock) void ms_apply_kernel(const uint64_t *active, size_t words,
+                                                          uint64_t *vis, uint64_t *fnext,
+                                                          uint32_t *depth, uint32_t level) {
+  const int lane = lane_id();
+  const size_t nw_waves = (size_t)gridDim.x * kWavesPerBlock;
+  for (size_t wi = global_wave(); wi < words; wi += nw_waves) {
+    uint64_t act = active[wi];
+    if (!act) continue;
+    uint32_t v = (uint32_t)(wi * 64 + lane);
+    uint64_t nw = 0;
+    if ((act >> lane) & 1ULL) {
+      nw = fnext[v] & ~vis[v];
+      fnext[v] = nw;
+      vis[v] |= nw;
+    }
+    for (uint64_t x = act; x; x &= x - 1) {
+      int k = __builtin_ctzll(x);
+      uint64_t m = shfl_u64(nw, k);
+      if ((m >> lane) & 1ULL) depth[(size_t)(wi * 64 + k) * 64 + lane] = level;
+    }
   }
 }
 __global__ void ms_seed_kernel(const uint32_t *roots, int n, uint64_t *vis, uint64_t *f,
@@ -1211,13 +1282,20 @@ static int ms_bfs_run(gg_graph *g, Work *w, Engine &en, const uint32_t *roots, i
       op.vis = w->ms_vis;
       op.fnext = fnext;
       op.full = full;
-      GG_TRY(en.launch_pull_group(in, op, nullptr, "ms_pull"));
+      MsPullArgs pa{in.index, in.nbr, fcur, w->ms_vis, fnext, en.fout().bits, full, V,
+                    in.light_max, (uint32_t)options().pull_serial_max, (V + 63) / 64};
+      {
+        ProfScope p("ms_pull");
+        hipLaunchKernelGGL(ms_pull_group_kernel, dim3(grid_waves(pa.ngroups)), dim3(kBlock), 0,
+                           stream(), pa);
+      }
       GG_TRY(en.launch_pull_heavy(in, op, "ms_pull_heavy"));
     }
     {
       ProfScope p("ms_apply");
-      hipLaunchKernelGGL(ms_apply_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(),
-                         en.fout().bits, V, w->ms_vis, fnext, depth, level);
+      hipLaunchKernelGGL(ms_apply_kernel, dim3(grid_waves(std::min<size_t>(w->words, 1u << 16))),
+                         dim3(kBlock), 0, stream(), en.fout().bits, w->words, w->ms_vis, fnext,
+                         depth, level);
     }
     GG_TRY(en.finish_iteration(sp, out.degree, nullptr));
     if (en.in.total > 0) GG_TRY(hook(level, fcur, fnext));

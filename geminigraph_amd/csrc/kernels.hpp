@@ -38,6 +38,12 @@ __device__ __forceinline__ uint32_t global_wave() {
 __device__ __forceinline__ bool test_bit(const uint64_t *bits, uint32_t v) {
   return (bits[v >> 6] >> (v & 63)) & 1ULL;
 }
+// Adjacency arrays are streamed once per sweep: load them non-temporally so
+// they do not evict the gathered vertex state (the hot sources) from L2.
+template <class T>
+__device__ __forceinline__ T stream_load(const T *p) {
+  return __builtin_nontemporal_load(p);
+}
 // Orders LDS traffic between lanes of one wave.
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -263,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void pull_group_kernel(PullArgs a, Op op) {
     T x = op.identity();
     if (act) {
       uint64_t e = bb + (j - ex);
-      x = op.value(a.nbr[e], a.w ? a.w[e] : 0.f);
+      x = op.value(stream_load(&a.nbr[e]), a.w ? stream_load(&a.w[e]) : 0.f);
     }
     int kk = act ? k : 64;
 #pragma unroll
@@ -321,12 +327,12 @@ __global__ __launch_bounds__(kBlock) void pull_heavy_kernel(HeavyArgs a, Op op) 
 #pragma unroll
   for (int k = 0; k < kPer; k++) {
     uint64_t e = cb + lane + 64 * k;
-    u[k] = e < ce ? a.nbr[e] : 0xFFFFFFFFu;
+    u[k] = e < ce ? stream_load(&a.nbr[e]) : 0xFFFFFFFFu;
   }
 #pragma unroll
   for (int k = 0; k < kPer; k++) {
     uint64_t e = cb + lane + 64 * k;
-    if (u[k] != 0xFFFFFFFFu) x = op.combine(x, op.value(u[k], a.w ? a.w[e] : 0.f));
+    if (u[k] != 0xFFFFFFFFu) x = op.combine(x, op.value(u[k], a.w ? stream_load(&a.w[e]) : 0.f));
   }
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) x = op.combine(x, Shfl<T>::xr(x, m));
