@@ -95,6 +95,21 @@ class GraphInfo(ctypes.Structure):
 BUILD_REORDER = 1
 
 
+class ServerInfo(ctypes.Structure):
+    """gg_server_info of include/gg.h."""
+    _fields_ = [("budget_bytes", ctypes.c_size_t), ("used_bytes", ctypes.c_size_t),
+                ("graphs", ctypes.c_int), ("workers", ctypes.c_int),
+                ("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("max_batch", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+QUERY_BFS, QUERY_SSSP, QUERY_BC = 0, 1, 2
+_QUERY_DTYPE = {QUERY_BFS: np.uint32, QUERY_SSSP: np.float32, QUERY_BC: np.float64}
+
+
 # C ABI signatures: name -> (restype, argtypes)
 _vp, _u32, _u64, _i32, _i64 = (ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
                                ctypes.c_int64)
@@ -138,6 +153,13 @@ SIGNATURES = {
     "gg_bfs_batch": (_i32, [_vp, _vp, _i32, _vp, _stats]),
     "gg_sssp_batch": (_i32, [_vp, _vp, _i32, _vp, _stats]),
     "gg_bc_batch": (_i32, [_vp, _vp, _i32, _vp, _vp, _stats]),
+    "gg_server_create": (_i32, [_sz, _i32, _pp]),
+    "gg_server_destroy": (_i32, [_vp]),
+    "gg_server_add_graph": (_i32, [_vp, _vp, ctypes.POINTER(ctypes.c_int)]),
+    "gg_server_submit": (_i32, [_vp, _i32, _i32, _u32, _i32, ctypes.POINTER(_u64)]),
+    "gg_server_wait": (_i32, [_vp, _u64, _vp, _sz, ctypes.POINTER(_dbl)]),
+    "gg_server_replay": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_dbl)]),
+    "gg_server_get_info": (_i32, [_vp, ctypes.POINTER(ServerInfo)]),
     "gg_prof_enable": (_i32, [_i32]),
     "gg_prof_reset": (_i32, []),
     "gg_prof_read": (_i32, [_cp, ctypes.POINTER(_dbl), ctypes.POINTER(_u64)]),
@@ -577,6 +599,83 @@ class VertexSubset:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.gg_subset_destroy(ctypes.c_void_p(self._h))
             self._h = None
+
+
+class Server:
+    """Query server over resident graphs (native, in libgg): continuous
+    batching of same-graph, same-kind queries into batched traversals of up
+    to 64 roots, on `workers` streams, with device memory admitted against
+    `budget_bytes`.  Graphs must outlive the server."""
+
+    def __init__(self, budget_bytes, workers=4):
+        h = ctypes.c_void_p()
+        _check(lib().gg_server_create(int(budget_bytes), int(workers), ctypes.byref(h)))
+        self._h = h.value
+        self._graphs = []
+
+    def add_graph(self, graph):
+        gid = ctypes.c_int()
+        _check(lib().gg_server_add_graph(ctypes.c_void_p(self._h), graph.handle,
+                                         ctypes.byref(gid)))
+        self._graphs.append(graph)
+        return gid.value
+
+    def submit(self, graph_id, kind, root, want_result=True):
+        qid = ctypes.c_uint64()
+        _check(lib().gg_server_submit(ctypes.c_void_p(self._h), int(graph_id), int(kind),
+                                      int(root), int(bool(want_result)), ctypes.byref(qid)))
+        return qid.value, graph_id, kind
+
+    def wait(self, ticket):
+        """Blocks until the query is done; returns (result or None, latency_ms)."""
+        qid, graph_id, kind = ticket
+        lat = ctypes.c_double()
+        out = np.empty(self._graphs[graph_id].vertices, dtype=_QUERY_DTYPE[kind])
+        _check(lib().gg_server_wait(ctypes.c_void_p(self._h), qid, out.ctypes.data, out.nbytes,
+                                    ctypes.byref(lat)))
+        return out, lat.value
+
+    def discard(self, ticket):
+        """Waits for a query submitted with want_result=False; returns latency_ms."""
+        lat = ctypes.c_double()
+        _check(lib().gg_server_wait(ctypes.c_void_p(self._h), ticket[0], None, 0,
+                                    ctypes.byref(lat)))
+        return lat.value
+
+    def replay(self, arrival_s, graph_ids, kinds, roots):
+        """Open-loop replay of a query trace; returns (latencies_ms, wall_s)."""
+        arr = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        gid = np.ascontiguousarray(graph_ids, dtype=np.int32)
+        knd = np.ascontiguousarray(kinds, dtype=np.int32)
+        rts = np.ascontiguousarray(roots, dtype=np.uint32)
+        n = len(arr)
+        if not (len(gid) == len(knd) == len(rts) == n):
+            raise ValueError("replay: trace columns differ in length")
+        lat = np.empty(n, dtype=np.float64)
+        wall = ctypes.c_double()
+        _check(lib().gg_server_replay(ctypes.c_void_p(self._h), n, arr.ctypes.data, gid.ctypes.data,
+                                      knd.ctypes.data, rts.ctypes.data, lat.ctypes.data,
+                                      ctypes.byref(wall)))
+        return lat, wall.value
+
+    def info(self):
+        i = ServerInfo()
+        _check(lib().gg_server_get_info(ctypes.c_void_p(self._h), ctypes.byref(i)))
+        return i.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.gg_server_destroy(ctypes.c_void_p(self._h))
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
 
 
 def prof_enable(on=True):

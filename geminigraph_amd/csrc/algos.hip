@@ -183,8 +183,9 @@ __global__ void copy_u32_kernel(const uint32_t *in, uint32_t *out, uint32_t V) {
 
 // --------------------------------------------------------------- work area
 int work_get(gg_graph *g, Work **out) {
-  if (g->work) {
-    *out = g->work;
+  const int slot = context_slot();
+  if (g->works[slot]) {
+    *out = g->works[slot];
     return GG_OK;
   }
   Work *w = new Work;
@@ -193,7 +194,10 @@ int work_get(gg_graph *g, Work **out) {
   size_t V = (size_t)g->vertices + 1;
   auto al = [&](void **p, size_t bytes) -> int {
     int r = device_alloc(p, bytes);
-    if (r == GG_OK) w->allocations.push_back(*p);
+    if (r == GG_OK) {
+      w->allocations.push_back(*p);
+      w->bytes += bytes;
+    }
     return r;
   };
   int r = GG_OK;
@@ -222,7 +226,7 @@ int work_get(gg_graph *g, Work **out) {
   }
   hipMemsetAsync(w->hacc, 0, nh * 8, stream());
   hipMemsetAsync(w->exam, 0, 1024 * 8, stream());
-  g->work = w;
+  g->works[slot] = w;
   *out = w;
   return GG_OK;
 }
@@ -1211,6 +1215,20 @@ static int batch_roots(const gg_graph *g, const uint32_t *h_roots, int n_roots, 
     out[i] = g->to_internal(h_roots[i]);
   }
   return GG_OK;
+}
+
+size_t work_footprint(const gg_graph *g, int kind) {
+  // mirrors work_get and ms_alloc (+ the first ms_store_level growth for BC)
+  size_t V = (size_t)g->vertices + 1, words = ((size_t)g->vertices >> 6) + 1;
+  size_t nh = std::max<size_t>(std::max(g->out.n_heavy, g->in.n_heavy), 1);
+  size_t b = 2 * (words * 8 + 6 * V * 4 + sizeof(Counters)) + 2 * words * 8 + V * 8 +
+             (g->reordered ? 2 * V * 8 : 0) + nh * 8 + 1024 * 8;
+  b += 3 * (V * 8 + 8) + 64 * 4 + V * 64 * 4;
+  if (kind == GG_QUERY_BC) {
+    if (g->reordered) b += 2 * V * 64 * 8;
+    b += 64 * words * 8;
+  }
+  return b;
 }
 
 static int ms_alloc(gg_graph *g, Work *w, bool bc) {

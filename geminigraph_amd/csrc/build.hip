@@ -550,7 +550,8 @@ int gg_graph_load_ex(const char *path, uint32_t vertices, int weighted, int undi
 
 int gg_graph_destroy(gg_graph *g) {
   if (!g) return GG_OK;
-  if (g->work) work_free(g->work);
+  for (Work *w : g->works)
+    if (w) work_free(w);
   for (void *p : g->allocations) device_free(p);
   delete g;
   return GG_OK;

@@ -63,7 +63,13 @@ Options &options();
 
 // ----------------------------------------------------------------- runtime
 int ensure_init();
+// The stream and traversal-context slot of the calling thread: slot 0 and
+// the library stream by default; serving workers bind their own so that
+// their traversals run concurrently on separate streams and buffers.
 hipStream_t stream();
+int context_slot();
+void bind_thread(hipStream_t s, int slot);
+constexpr int kMaxSlots = 33;
 int device_alloc(void **p, size_t bytes);
 void device_free(void *p);
 
@@ -111,8 +117,8 @@ struct gg_graph {
   std::vector<uint32_t> h_perm;
   uint32_t to_internal(uint32_t v) const { return reordered ? h_perm[v] : v; }
   std::vector<void *> allocations;
-  // scratch owned by the graph for traversals (lazily sized)
-  gg::Work *work = nullptr;
+  // traversal contexts owned by the graph, one per context slot (lazily sized)
+  gg::Work *works[gg::kMaxSlots] = {};
 };
 
 struct gg_subset {
@@ -160,6 +166,7 @@ struct Work {
   void *mb_a = nullptr, *mb_b = nullptr;  // V x 64 x 8 bytes each (BC rows, reordered outputs)
   uint64_t *ms_levels = nullptr;       // per-level frontier bitmaps (BC backward)
   size_t ms_levels_cap = 0;            // levels the store holds
+  size_t bytes = 0;                    // device bytes held (fixed part)
   Counters *h_cnt = nullptr;   // pinned host copy
   unsigned long long *h_exam = nullptr;
   std::vector<void *> allocations;
@@ -167,6 +174,9 @@ struct Work {
 
 int work_get(gg_graph *g, Work **w);
 void work_free(Work *w);
+// Device bytes of one traversal context that runs batched queries of `kind`
+// (GG_QUERY_*) on g, counting the first 64 stored BFS levels.
+size_t work_footprint(const gg_graph *g, int kind);
 
 // Host-side view of one direction, swapped by transpose.
 inline const Dir &out_dir(const gg_graph *g) { return g->transposed ? g->in : g->out; }

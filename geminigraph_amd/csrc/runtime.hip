@@ -55,7 +55,15 @@ int ensure_init() {
   return init_device(0);
 }
 
-hipStream_t stream() { return g_stream; }
+static thread_local hipStream_t t_stream = nullptr;
+static thread_local int t_slot = 0;
+
+hipStream_t stream() { return t_stream ? t_stream : g_stream; }
+int context_slot() { return t_slot; }
+void bind_thread(hipStream_t s, int slot) {
+  t_stream = s;
+  t_slot = slot;
+}
 
 int device_alloc(void **p, size_t bytes) {
   *p = nullptr;
@@ -89,11 +97,11 @@ ProfScope::ProfScope(const char *n) : name(n) {
     start = stop = nullptr;
     return;
   }
-  hipEventRecord(start, g_stream);
+  hipEventRecord(start, stream());
 }
 ProfScope::~ProfScope() {
   if (!start) return;
-  hipEventRecord(stop, g_stream);
+  hipEventRecord(stop, stream());
   std::lock_guard<std::mutex> lk(g_prof_mu);
   g_prof[name].pending.emplace_back(start, stop);
 }
@@ -127,7 +135,7 @@ int gg_init(int device) { return init_device(device); }
 
 int gg_synchronize(void) {
   GG_TRY(ensure_init());
-  GG_HIP(hipStreamSynchronize(g_stream));
+  GG_HIP(hipStreamSynchronize(stream()));
   return GG_OK;
 }
 
@@ -147,8 +155,8 @@ int gg_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes) {
   GG_TRY(ensure_init());
   if (!bytes) return GG_OK;
   if (!h_dst || !d_src) return fail(GG_EINVAL, "gg_memcpy_d2h: null pointer");
-  GG_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, g_stream));
-  GG_HIP(hipStreamSynchronize(g_stream));
+  GG_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, stream()));
+  GG_HIP(hipStreamSynchronize(stream()));
   return GG_OK;
 }
 
@@ -156,8 +164,8 @@ int gg_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes) {
   GG_TRY(ensure_init());
   if (!bytes) return GG_OK;
   if (!h_src || !d_dst) return fail(GG_EINVAL, "gg_memcpy_h2d: null pointer");
-  GG_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, g_stream));
-  GG_HIP(hipStreamSynchronize(g_stream));
+  GG_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, stream()));
+  GG_HIP(hipStreamSynchronize(stream()));
   return GG_OK;
 }
 
@@ -165,7 +173,7 @@ int gg_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes) {
   GG_TRY(ensure_init());
   if (!bytes) return GG_OK;
   if (!d_src || !d_dst) return fail(GG_EINVAL, "gg_memcpy_d2d: null pointer");
-  GG_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, g_stream));
+  GG_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, stream()));
   return GG_OK;
 }
 
@@ -254,7 +262,7 @@ int gg_subset_create(const gg_graph *g, gg_subset **out) {
     delete s;
     return r;
   }
-  GG_HIP(hipMemsetAsync(s->d, 0, s->words * 8, g_stream));
+  GG_HIP(hipMemsetAsync(s->d, 0, s->words * 8, stream()));
   *out = s;
   return GG_OK;
 }
@@ -268,7 +276,7 @@ int gg_subset_destroy(gg_subset *s) {
 
 int gg_subset_clear(gg_subset *s) {
   if (!s) return fail(GG_EINVAL, "null subset");
-  GG_HIP(hipMemsetAsync(s->d, 0, s->words * 8, g_stream));
+  GG_HIP(hipMemsetAsync(s->d, 0, s->words * 8, stream()));
   return GG_OK;
 }
 
@@ -276,11 +284,11 @@ int gg_subset_fill(gg_subset *s) {
   // core/bitmap.hpp:42-52: full words, then the exact tail of the last word.
   if (!s) return fail(GG_EINVAL, "null subset");
   size_t full = (size_t)s->size >> 6;
-  if (full) GG_HIP(hipMemsetAsync(s->d, 0xFF, full * 8, g_stream));
+  if (full) GG_HIP(hipMemsetAsync(s->d, 0xFF, full * 8, stream()));
   uint64_t tail = 0;
   for (size_t i = full << 6; i < s->size; i++) tail |= 1ULL << (i & 63);
-  GG_HIP(hipMemcpyAsync(s->d + full, &tail, 8, hipMemcpyHostToDevice, g_stream));
-  GG_HIP(hipStreamSynchronize(g_stream));
+  GG_HIP(hipMemcpyAsync(s->d + full, &tail, 8, hipMemcpyHostToDevice, stream()));
+  GG_HIP(hipStreamSynchronize(stream()));
   return GG_OK;
 }
 
@@ -288,11 +296,11 @@ int gg_subset_set_bit(gg_subset *s, uint32_t v) {
   if (!s || v >= s->size) return fail(GG_EINVAL, "gg_subset_set_bit: vertex out of range");
   v = s->g->to_internal(v);
   uint64_t w = 0;
-  GG_HIP(hipMemcpyAsync(&w, s->d + (v >> 6), 8, hipMemcpyDeviceToHost, g_stream));
-  GG_HIP(hipStreamSynchronize(g_stream));
+  GG_HIP(hipMemcpyAsync(&w, s->d + (v >> 6), 8, hipMemcpyDeviceToHost, stream()));
+  GG_HIP(hipStreamSynchronize(stream()));
   w |= 1ULL << (v & 63);
-  GG_HIP(hipMemcpyAsync(s->d + (v >> 6), &w, 8, hipMemcpyHostToDevice, g_stream));
-  GG_HIP(hipStreamSynchronize(g_stream));
+  GG_HIP(hipMemcpyAsync(s->d + (v >> 6), &w, 8, hipMemcpyHostToDevice, stream()));
+  GG_HIP(hipStreamSynchronize(stream()));
   return GG_OK;
 }
 
@@ -300,8 +308,8 @@ int gg_subset_get_bit(const gg_subset *s, uint32_t v, int *bit) {
   if (!s || !bit || v >= s->size) return fail(GG_EINVAL, "gg_subset_get_bit: bad argument");
   v = s->g->to_internal(v);
   uint64_t w = 0;
-  GG_HIP(hipMemcpyAsync(&w, s->d + (v >> 6), 8, hipMemcpyDeviceToHost, g_stream));
-  GG_HIP(hipStreamSynchronize(g_stream));
+  GG_HIP(hipMemcpyAsync(&w, s->d + (v >> 6), 8, hipMemcpyDeviceToHost, stream()));
+  GG_HIP(hipStreamSynchronize(stream()));
   *bit = (int)((w >> (v & 63)) & 1ULL);
   return GG_OK;
 }

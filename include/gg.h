@@ -209,6 +209,51 @@ int gg_sssp_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, float *d_di
 int gg_bc_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_dep,
                 double *d_inv_paths, gg_run_stats *stats);
 
+/* ---------------------------------------------------------------- serving */
+/* A query server over graphs resident in HBM (no counterpart in the
+ * reference, whose Graph is not re-entrant: core/graph.hpp:132-141).
+ * Queries (BFS depths, SSSP distances, BC dependencies from one root) are
+ * admitted as they arrive; each worker, on its own HIP stream and its own
+ * traversal buffers, takes the oldest pending query and every pending query
+ * of the same graph and kind (up to 64) and runs them as one batched
+ * traversal, so batches from different workers overlap on the device while
+ * new queries queue.  Device memory is admitted against `budget_bytes`: the
+ * resident graphs plus each worker's buffers for the graphs it serves. */
+typedef struct gg_server gg_server;
+#define GG_QUERY_BFS 0
+#define GG_QUERY_SSSP 1
+#define GG_QUERY_BC 2
+typedef struct gg_server_info {
+  size_t budget_bytes;
+  size_t used_bytes;     /* graphs + worker buffers admitted so far */
+  int graphs;
+  int workers;
+  uint64_t submitted;
+  uint64_t completed;
+  uint64_t batches;      /* batched traversals run */
+  uint64_t max_batch;    /* largest batch formed */
+} gg_server_info;
+int gg_server_create(size_t budget_bytes, int workers, gg_server **out);
+int gg_server_destroy(gg_server *s);
+/* Makes a graph resident; GG_EBUDGET if it does not fit the budget.  The
+ * server does not own the graph (destroy it after the server). */
+int gg_server_add_graph(gg_server *s, gg_graph *g, int *graph_id);
+/* Queues one query; want_result != 0 keeps its per-vertex result for
+ * gg_server_wait (BFS: u32 depth, SSSP: f32 distance, BC: f64 dependency). */
+int gg_server_submit(gg_server *s, int graph_id, int kind, uint32_t root, int want_result,
+                     uint64_t *query_id);
+/* Waits for a query; copies its result (if kept) into h_result
+ * (result_bytes >= V * element size) and reports arrival-to-completion
+ * latency.  The query is released afterwards. */
+int gg_server_wait(gg_server *s, uint64_t query_id, void *h_result, size_t result_bytes,
+                   double *latency_ms);
+/* Open-loop replay: query i arrives at arrival_s[i] seconds after the call
+ * starts; latency_ms[i] = completion - arrival; *wall_s = time to the last
+ * completion.  Results are not kept. */
+int gg_server_replay(gg_server *s, int n, const double *arrival_s, const int *graph_id,
+                     const int *kind, const uint32_t *root, double *latency_ms, double *wall_s);
+int gg_server_get_info(gg_server *s, gg_server_info *info);
+
 /* ----------------------------------------------------------- profiling */
 
 /* Per-kernel device time accumulated with HIP events on the launching
