@@ -117,6 +117,52 @@ def batch_extra(gg, seed, reorder):
     return out
 
 
+def serve_extra(gg, seed, reorder, seconds, rate=2000.0, workers=4, budget=200e9):
+    """Serving: a Poisson stream at `rate` queries/s, BFS 50% / SSSP 30% /
+    BC 20%, uniformly random roots, over 4 resident weighted RMAT graphs
+    (scales 18, 20, 22, 24, edge factor 16) admitted by a 200 GB budget; the
+    server coalesces same-graph, same-kind queries into batches of up to 64
+    on `workers` streams.  Latency = scheduled arrival to completion."""
+    scales = (18, 20, 22, 24)
+    graphs = [gg.Graph.rmat(s, 16, seed=seed + s, permute=True, weighted=True, reorder=reorder)
+              for s in scales]
+    rng = np.random.default_rng(seed)
+
+    def trace(n):
+        gi = rng.integers(0, len(scales), n)
+        kind = rng.choice(3, size=n, p=[0.5, 0.3, 0.2])
+        root = np.array([rng.integers(0, 1 << scales[i]) for i in gi], dtype=np.uint32)
+        arrival = np.cumsum(rng.exponential(1.0 / rate, n))
+        return arrival, gi, kind, root
+
+    with gg.Server(int(budget), workers=workers) as s:
+        ids = [s.add_graph(g) for g in graphs]
+        # warm-up: every worker allocates its buffers for each (graph, kind)
+        arr, gi, kind, root = trace(400)
+        s.replay(arr * 0, [ids[i] for i in gi], kind, root)
+        before = s.info()
+        n = int(rate * seconds)
+        arr, gi, kind, root = trace(n)
+        lat, wall = s.replay(arr, [ids[i] for i in gi], kind, root)
+        info = s.info()
+    batches = info["batches"] - before["batches"]
+    out = {"workload": f"Poisson {rate:.0f} q/s for {seconds} s, BFS/SSSP/BC 50/30/20, "
+                       f"weighted RMAT scales {list(scales)} edge factor 16",
+           "queries": n, "workers": workers, "budget_GB": budget / 1e9,
+           "queries_per_s": round(n / wall, 1), "offered_per_s": rate,
+           "p50_ms": round(float(np.percentile(lat, 50)), 2),
+           "p99_ms": round(float(np.percentile(lat, 99)), 2),
+           "mean_ms": round(float(lat.mean()), 2),
+           "batches": batches, "mean_batch": round(n / max(batches, 1), 2),
+           "max_batch": info["max_batch"], "used_GB": round(info["used_bytes"] / 1e9, 2)}
+    for k, name in enumerate(("bfs", "sssp", "bc")):
+        m = kind == k
+        out[f"{name}_p50_ms"] = round(float(np.percentile(lat[m], 50)), 2)
+        out[f"{name}_p99_ms"] = round(float(np.percentile(lat[m], 99)), 2)
+    del graphs
+    return out
+
+
 def cpu_baseline(scale, ef, seed, roots, gpu_depths, sample_roots):
     import oracle
     threads = os.environ.get("OMP_NUM_THREADS")
@@ -166,6 +212,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-reorder", action="store_true",
                     help="keep file vertex ids (no degree-sorted relabeling)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0,
+                    help="length of the serving trace (0 skips the serving leg)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -298,6 +346,9 @@ def main():
         lab.free()
         extras["batched"] = batch_extra(gg, args.seed + 7, reorder)
         log(f"batched: {extras['batched']}")
+        if args.serve_seconds > 0:
+            extras["serving"] = serve_extra(gg, args.seed + 11, reorder, args.serve_seconds)
+            log(f"serving: {extras['serving']}")
 
     cpu = None
     if not args.no_cpu:

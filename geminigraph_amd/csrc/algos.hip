@@ -237,6 +237,7 @@ void work_free(Work *w) {
   device_free(w->dacc);
   device_free(w->dsum);
   device_free(w->ms_vis);
+  device_free(w->ms_lmask);
   device_free(w->ms_f[0]);
   device_free(w->ms_f[1]);
   device_free(w->ms_roots);
@@ -1028,8 +1029,64 @@ struct LaneArgs {
   uint32_t n;
   uint32_t light_max;   // longer lists go to lane_heavy_kernel
 };
-// Op: T; bool active(v); T init(v, lane); void edge(v, lane, u, w, T&);
-//     void finish(v, lane, T).
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t x, int j) {
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, j);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), j);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// One adjacency range of vertex v, 64 roots per wave (lane = root).  The
+// lanes first load 64 arcs at once -- neighbour, weight and the neighbour's
+// selection mask (the roots for which the arc matters, e.g. the roots whose
+// distance at the neighbour changed) -- then walk only the selected arcs, in
+// arc order, four row loads in flight at a time.
+// Op: T, L; uint64_t select(u); L load(u, lane); void combine(T&, L, w).
+template <class Op>
+__device__ __forceinline__ void lane_range(const Op &op, const uint32_t *nbr, const float *wt,
+                                           uint64_t b, uint64_t e, uint64_t vm, int lane,
+                                           typename Op::T &acc) {
+  for (uint64_t base = b; base < e; base += 64) {
+    const uint64_t k = base + lane;
+    uint32_t u = 0;
+    float w = 0.f;
+    uint64_t sel = 0;
+    if (k < e) {
+      u = stream_load(nbr + k);
+      if (wt) w = stream_load(wt + k);
+      sel = op.select(u) & vm;
+    }
+    uint64_t any = __ballot(sel != 0);
+    while (any) {
+      uint32_t uj[4];
+      float wj[4];
+      bool on[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        uj[t] = 0;
+        wj[t] = 0.f;
+        on[t] = false;
+        if (any) {
+          const int j = __builtin_ctzll(any);
+          any &= any - 1;
+          uj[t] = (uint32_t)__builtin_amdgcn_readlane((int)u, j);
+          wj[t] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w), j));
+          on[t] = (readlane_u64(sel, j) >> lane) & 1ULL;
+        }
+      }
+      typename Op::L x[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) x[t] = on[t] ? op.load(uj[t], lane) : typename Op::L{};
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+        if (on[t]) op.combine(acc, x[t], wj[t]);
+    }
+  }
+}
+
+// One wave per listed vertex.  Op adds: bool active(v) (wave-uniform);
+// T init(v, lane); bool on(v, lane, T) (root takes part at v);
+// void finish(v, lane, T).
 template <class Op>
 __global__ __launch_bounds__(kBlock) void lane_vertex_kernel(LaneArgs a, Op op) {
   const uint32_t gw = global_wave();
@@ -1038,9 +1095,11 @@ __global__ __launch_bounds__(kBlock) void lane_vertex_kernel(LaneArgs a, Op op) 
   uint32_t v = a.list ? a.list[gw] : gw;
   if (!op.active(v)) return;
   uint64_t b = a.index[v], e = a.index[v + 1];
-  if (e - b > a.light_max) return;
+  if (e - b > a.light_max || e == b) return;
   typename Op::T acc = op.init(v, lane);
-  for (uint64_t k = b; k < e; k++) op.edge(v, lane, a.nbr[k], a.w ? a.w[k] : 0.f, acc);
+  const uint64_t vm = __ballot(op.on(v, lane, acc));
+  if (!vm) return;
+  lane_range(op, a.nbr, a.w, b, e, vm, lane, acc);
   op.finish(v, lane, acc);
 }
 // Heavy lists: one wave per chunk; partials through op.commit (atomics).
@@ -1052,20 +1111,28 @@ __global__ __launch_bounds__(kBlock) void lane_heavy_kernel(HeavyArgs a, Op op) 
   uint32_t v = a.chunk_v[c];
   if (!op.active(v)) return;
   typename Op::T acc = op.init(v, lane);
-  for (uint64_t k = a.chunk_beg[c]; k < a.chunk_end[c]; k++)
-    op.edge(v, lane, a.nbr[k], a.w ? a.w[k] : 0.f, acc);
+  const uint64_t vm = __ballot(op.on(v, lane, acc));
+  if (!vm) return;
+  lane_range(op, a.nbr, a.w, a.chunk_beg[c], a.chunk_end[c], vm, lane, acc);
   op.commit(v, lane, acc);
 }
 
 // Batched SSSP, dense step: dist row of v = min over in-edges of (row of u + w).
 struct MssPull {
   using T = float;
+  using L = float;
   float *dist;
+  const uint64_t *cm_cur;  // roots whose distance at a vertex changed last step
   uint64_t *cm_next, *out_bits;
   __device__ bool active(uint32_t) const { return true; }
   __device__ float init(uint32_t, int) const { return 1e9f; }
-  __device__ void edge(uint32_t, int lane, uint32_t u, float w, float &acc) const {
-    float r = dist[(size_t)u * 64 + lane] + w;
+  __device__ bool on(uint32_t, int, float) const { return true; }
+  // only neighbours whose distance changed can lower v's: the others were
+  // relaxed into v when they last changed (same fixed point, fewer rows read)
+  __device__ uint64_t select(uint32_t u) const { return cm_cur[u]; }
+  __device__ float load(uint32_t u, int lane) const { return dist[(size_t)u * 64 + lane]; }
+  __device__ void combine(float &acc, float du, float w) const {
+    float r = du + w;
     if (r < acc) acc = r;
   }
   __device__ void finish(uint32_t v, int lane, float acc) const {
@@ -1148,14 +1215,15 @@ __global__ void mss_seed_kernel(const uint32_t *roots, int n, float *dist, uint6
 // sigma of its in-neighbours in the previous level, per root (bc.cpp:58-89).
 struct MsbcSigma {
   using T = double;
+  using L = double;
   double *sigma;
   const uint64_t *fprev, *fnew;
   __device__ bool active(uint32_t v) const { return fnew[v] != 0; }
   __device__ double init(uint32_t, int) const { return 0.0; }
-  __device__ void edge(uint32_t v, int lane, uint32_t u, float, double &acc) const {
-    uint64_t m = fprev[u] & fnew[v];
-    if ((m >> lane) & 1ULL) acc += sigma[(size_t)u * 64 + lane];
-  }
+  __device__ bool on(uint32_t v, int lane, double) const { return (fnew[v] >> lane) & 1ULL; }
+  __device__ uint64_t select(uint32_t u) const { return fprev[u]; }
+  __device__ double load(uint32_t u, int lane) const { return sigma[(size_t)u * 64 + lane]; }
+  __device__ void combine(double &acc, double x, float) const { acc += x; }
   __device__ void finish(uint32_t v, int lane, double acc) const {
     if ((fnew[v] >> lane) & 1ULL) sigma[(size_t)v * 64 + lane] = acc;
   }
@@ -1166,26 +1234,37 @@ struct MsbcSigma {
 // Batched BC, backward: for v at depth L-1 of root i,
 // D(v) += sum of D(w) over out-edges v->w with w at depth L (bc.cpp:125-167).
 struct MsbcBack {
-  struct T {
-    double s;
-    bool on;  // this lane's root has v at depth L-1
+  using T = double;
+  struct L {
+    uint32_t d;
+    double dd;
   };
   double *dd;
   const uint32_t *depth;
   const uint64_t *level_bits;  // vertices at depth L-1 for some root
-  uint32_t L;
+  const uint64_t *next_bits;   // vertices at depth L for some root
+  // per-vertex root masks of levels L-1 and L (null: tested on the depth rows)
+  const uint64_t *mask_prev, *mask_next;
+  uint32_t lvl;
   __device__ bool active(uint32_t v) const { return test_bit(level_bits, v); }
-  __device__ T init(uint32_t v, int lane) const {
-    return T{0.0, depth[(size_t)v * 64 + lane] == L - 1};
+  __device__ double init(uint32_t, int) const { return 0.0; }
+  __device__ bool on(uint32_t v, int lane, double = 0.0) const {
+    return mask_prev ? (mask_prev[v] >> lane) & 1ULL : depth[(size_t)v * 64 + lane] == lvl - 1;
   }
-  __device__ void edge(uint32_t, int lane, uint32_t u, float, T &t) const {
-    if (t.on && depth[(size_t)u * 64 + lane] == L) t.s += dd[(size_t)u * 64 + lane];
+  __device__ uint64_t select(uint32_t u) const {
+    return mask_next ? mask_next[u] : test_bit(next_bits, u) ? ~0ULL : 0ULL;
   }
-  __device__ void finish(uint32_t v, int lane, T t) const {
-    if (t.on) dd[(size_t)v * 64 + lane] += t.s;
+  __device__ L load(uint32_t u, int lane) const {
+    return L{mask_next ? lvl : depth[(size_t)u * 64 + lane], dd[(size_t)u * 64 + lane]};
   }
-  __device__ void commit(uint32_t v, int lane, T t) const {
-    if (t.on && t.s != 0.0) unsafeAtomicAdd(&dd[(size_t)v * 64 + lane], t.s);
+  __device__ void combine(double &acc, L x, float) const {
+    if (x.d == lvl) acc += x.dd;
+  }
+  __device__ void finish(uint32_t v, int lane, double acc) const {
+    if (on(v, lane)) dd[(size_t)v * 64 + lane] += acc;
+  }
+  __device__ void commit(uint32_t v, int lane, double acc) const {
+    if (acc != 0.0 && on(v, lane)) unsafeAtomicAdd(&dd[(size_t)v * 64 + lane], acc);
   }
 };
 __global__ void msbc_seed_sigma_kernel(const uint32_t *roots, int n, double *sigma) {
@@ -1217,21 +1296,23 @@ static int batch_roots(const gg_graph *g, const uint32_t *h_roots, int n_roots, 
   return GG_OK;
 }
 
+// Store of per-level root masks (BC backward): initial levels, size limit.
+constexpr size_t kMaskLevels0 = 16;
+constexpr size_t kMaskStoreMax = size_t(4) << 30;
+
 size_t work_footprint(const gg_graph *g, int kind) {
-  // mirrors work_get and ms_alloc (+ the first ms_store_level growth for BC)
+  // mirrors work_get and ms_alloc with results kept in the context (+ the
+  // first ms_store_level growth for BC)
   size_t V = (size_t)g->vertices + 1, words = ((size_t)g->vertices >> 6) + 1;
   size_t nh = std::max<size_t>(std::max(g->out.n_heavy, g->in.n_heavy), 1);
   size_t b = 2 * (words * 8 + 6 * V * 4 + sizeof(Counters)) + 2 * words * 8 + V * 8 +
              (g->reordered ? 2 * V * 8 : 0) + nh * 8 + 1024 * 8;
   b += 3 * (V * 8 + 8) + 64 * 4 + V * 64 * 4;
-  if (kind == GG_QUERY_BC) {
-    if (g->reordered) b += 2 * V * 64 * 8;
-    b += 64 * words * 8;
-  }
+  if (kind == GG_QUERY_BC) b += 2 * V * 64 * 8 + 64 * words * 8 + kMaskLevels0 * (V - 1) * 8;
   return b;
 }
 
-static int ms_alloc(gg_graph *g, Work *w, bool bc) {
+static int ms_alloc(gg_graph *g, Work *w, bool bc, bool keep = false) {
   size_t V = g->vertices;
   if (!w->ms_vis) {
     GG_TRY(device_alloc((void **)&w->ms_vis, V * 8 + 8));
@@ -1240,7 +1321,7 @@ static int ms_alloc(gg_graph *g, Work *w, bool bc) {
     GG_TRY(device_alloc((void **)&w->ms_roots, 64 * 4));
     GG_TRY(device_alloc(&w->ms_tmp, V * 64 * 4));
   }
-  if (bc && g->reordered && !w->mb_a) {
+  if (bc && (g->reordered || keep) && !w->mb_a) {
     GG_TRY(device_alloc(&w->mb_a, V * 64 * 8));
     GG_TRY(device_alloc(&w->mb_b, V * 64 * 8));
   }
@@ -1266,6 +1347,34 @@ static int ms_store_level(Work *w, const uint64_t *bits, uint32_t level) {
   return GG_OK;
 }
 
+// Keeps one level's per-vertex root masks (BC backward selects exactly the
+// roots for which an arc joins consecutive levels).  Gives up (*ok = false)
+// when the store would exceed kMaskStoreMax, e.g. on long-diameter graphs;
+// the backward pass then tests the depth rows instead.
+static int ms_store_mask(Work *w, const uint64_t *mask, uint32_t level, bool *ok) {
+  if (!*ok) return GG_OK;
+  const size_t row = (size_t)w->V * 8;
+  if (level >= w->ms_lmask_cap) {
+    size_t cap = std::max<size_t>(kMaskLevels0, 2 * (size_t)level + 2);
+    if (cap * row > kMaskStoreMax) {
+      *ok = false;
+      return GG_OK;
+    }
+    uint64_t *p = nullptr;
+    GG_TRY(device_alloc((void **)&p, cap * row));
+    if (w->ms_lmask)
+      GG_HIP(hipMemcpyAsync(p, w->ms_lmask, w->ms_lmask_cap * row, hipMemcpyDeviceToDevice,
+                            stream()));
+    GG_HIP(hipStreamSynchronize(stream()));
+    device_free(w->ms_lmask);
+    w->ms_lmask = p;
+    w->ms_lmask_cap = cap;
+  }
+  GG_HIP(hipMemcpyAsync(w->ms_lmask + (size_t)level * w->V, mask, row, hipMemcpyDeviceToDevice,
+                        stream()));
+  return GG_OK;
+}
+
 // Multi-root BFS levels (bit-parallel masks); hook(level, fprev, fnew) runs
 // after every non-empty level with the level's frontier compacted in en.fin().
 template <class Hook>
@@ -1281,7 +1390,7 @@ static int ms_bfs_run(gg_graph *g, Work *w, Engine &en, const uint32_t *roots, i
   hipLaunchKernelGGL(ms_seed_kernel, dim3(1), dim3(1), 0, stream(), w->ms_roots, n_roots, w->ms_vis,
                      w->ms_f[0], en.fin().bits, depth);
   GG_TRY(en.compact_in(out.degree));
-  if (store_levels) GG_TRY(ms_store_level(w, en.fin().bits, 0));
+  if (store_levels) GG_TRY(hook(0, nullptr, w->ms_f[0]));
   const uint64_t full = n_roots == 64 ? ~0ULL : ((1ULL << n_roots) - 1ULL);
   int fc = 0;
   for (uint32_t level = 1; en.in.total > 0; level++) {
@@ -1712,10 +1821,14 @@ int gg_bc(gg_graph *g, uint32_t root, double *d_dep, double *d_inv_paths, uint32
   return GG_OK;
 }
 
-int gg_bfs_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, uint32_t *d_depth,
-                 gg_run_stats *stats) {
+}  // extern "C"
+
+namespace gg {
+
+int bfs_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, uint32_t *d_depth,
+                  gg_run_stats *stats, const void **rows) {
   GG_TRY(check_graph(g));
-  if (!d_depth) return fail(GG_EINVAL, "gg_bfs_batch: null output");
+  const bool keep = d_depth == nullptr;
   uint32_t roots[64];
   GG_TRY(batch_roots(g, h_roots, n_roots, roots));
   Work *w;
@@ -1724,12 +1837,13 @@ int gg_bfs_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, uint32_t *d_
   Engine en(g, w);
   uint32_t V = g->vertices;
   DeviceTimer tm;
-  uint32_t *depth = g->reordered ? (uint32_t *)w->ms_tmp : d_depth;
+  uint32_t *depth = g->reordered || keep ? (uint32_t *)w->ms_tmp : d_depth;
   GG_TRY(ms_bfs_run(g, w, en, roots, n_roots, depth,
                     [](uint32_t, const uint64_t *, const uint64_t *) { return GG_OK; }));
   uint64_t m = 0;
   GG_TRY(ms_reach(g, w, &m));
-  if (g->reordered)
+  if (rows) *rows = depth;
+  if (g->reordered && !keep)
     hipLaunchKernelGGL(unpermute_rows_kernel<uint32_t>, dim3(grid1((uint64_t)V * 64)), dim3(kBlock),
                        0, stream(), depth, d_depth, g->iperm, V);
   en.st.device_ms = tm.stop();
@@ -1740,11 +1854,11 @@ int gg_bfs_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, uint32_t *d_
   return GG_OK;
 }
 
-int gg_sssp_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, float *d_dist,
-                  gg_run_stats *stats) {
+int sssp_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, float *d_dist,
+                   gg_run_stats *stats, const void **rows) {
   GG_TRY(check_graph(g));
   if (!g->weighted) return fail(GG_EINVAL, "SSSP needs a weighted graph");
-  if (!d_dist) return fail(GG_EINVAL, "gg_sssp_batch: null output");
+  const bool keep = d_dist == nullptr;
   uint32_t roots[64];
   GG_TRY(batch_roots(g, h_roots, n_roots, roots));
   Work *w;
@@ -1754,7 +1868,8 @@ int gg_sssp_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, float *d_di
   const Dir &out = out_dir(g), &in = in_dir(g);
   uint32_t V = g->vertices;
   DeviceTimer tm;
-  float *dist = g->reordered ? (float *)w->ms_tmp : d_dist;
+  float *dist = g->reordered || keep ? (float *)w->ms_tmp : d_dist;
+  if (rows) *rows = dist;
   fill<uint32_t>((uint32_t *)dist, kInfBits, (uint64_t)V * 64);  // 1e9 (sssp.cpp:163)
   GG_HIP(hipMemsetAsync(w->ms_f[0], 0, (size_t)V * 8, stream()));
   GG_HIP(hipMemcpyAsync(w->ms_roots, roots, 4 * (size_t)n_roots, hipMemcpyHostToDevice, stream()));
@@ -1776,7 +1891,7 @@ int gg_sssp_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, float *d_di
                          en.in.n[1], en.fin().q[2], en.in.n[2], waves2, dist, cm_cur, cm_next,
                          en.fout().bits);
     } else {
-      MssPull op{dist, cm_next, en.fout().bits};
+      MssPull op{dist, cm_cur, cm_next, en.fout().bits};
       ProfScope p("mss_pull");
       LaneArgs a{in.index, in.nbr, in.w, nullptr, V, in.light_max};
       hipLaunchKernelGGL(lane_vertex_kernel<MssPull>, dim3(grid_waves(V)), dim3(kBlock), 0,
@@ -1791,7 +1906,7 @@ int gg_sssp_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, float *d_di
     GG_TRY(en.finish_iteration(sp, out.degree, nullptr));
     fc ^= 1;
   }
-  if (g->reordered)
+  if (g->reordered && !keep)
     hipLaunchKernelGGL(unpermute_rows_kernel<float>, dim3(grid1((uint64_t)V * 64)), dim3(kBlock), 0,
                        stream(), dist, d_dist, g->iperm, V);
   en.st.device_ms = tm.stop();
@@ -1801,32 +1916,36 @@ int gg_sssp_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, float *d_di
   return GG_OK;
 }
 
-int gg_bc_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_dep,
-                double *d_inv_paths, gg_run_stats *stats) {
+int bc_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_dep,
+                 double *d_inv_paths, gg_run_stats *stats, const void **rows) {
   GG_TRY(check_graph(g));
-  if (!d_dep || !d_inv_paths) return fail(GG_EINVAL, "gg_bc_batch: null output");
+  const bool keep = d_dep == nullptr;
   uint32_t roots[64];
   GG_TRY(batch_roots(g, h_roots, n_roots, roots));
   Work *w;
   GG_TRY(work_get(g, &w));
-  GG_TRY(ms_alloc(g, w, true));
+  GG_TRY(ms_alloc(g, w, true, keep));
   Engine en(g, w);
   const Dir &out = out_dir(g), &in = in_dir(g);
   uint32_t V = g->vertices;
   const uint64_t n = (uint64_t)V * 64;
   DeviceTimer tm;
   uint32_t *depth = (uint32_t *)w->ms_tmp;
-  double *sigma = g->reordered ? (double *)w->mb_a : d_inv_paths;
-  double *dd = g->reordered ? (double *)w->mb_b : d_dep;
+  double *sigma = g->reordered || keep ? (double *)w->mb_a : d_inv_paths;
+  double *dd = g->reordered || keep ? (double *)w->mb_b : d_dep;
+  if (rows) *rows = dd;
   fill<double>(sigma, 0.0, n);
   GG_HIP(hipMemcpyAsync(w->ms_roots, roots, 4 * (size_t)n_roots, hipMemcpyHostToDevice, stream()));
   hipLaunchKernelGGL(msbc_seed_sigma_kernel, dim3(1), dim3(1), 0, stream(), w->ms_roots, n_roots,
                      sigma);
   // forward: the multi-root BFS levels, with sigma accumulated per level
   uint32_t levels = 0;
+  bool masks = options().bc_level_masks != 0;
   auto hook = [&](uint32_t level, const uint64_t *fprev, const uint64_t *fnew) -> int {
     GG_TRY(ms_store_level(w, en.fin().bits, level));
+    GG_TRY(ms_store_mask(w, fnew, level, &masks));
     levels = level;
+    if (level == 0) return GG_OK;
     MsbcSigma op{sigma, fprev, fnew};
     ProfScope p("msbc_sigma");
     for (int b = 0; b < 3; b++) {
@@ -1851,7 +1970,9 @@ int gg_bc_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_dep
     const uint64_t *lb = w->ms_levels + (size_t)(L - 1) * w->words;
     GG_HIP(hipMemcpyAsync(en.fin().bits, lb, w->words * 8, hipMemcpyDeviceToDevice, stream()));
     GG_TRY(en.compact_in(out.degree));
-    MsbcBack op{dd, depth, lb, L};
+    MsbcBack op{dd, depth, lb, w->ms_levels + (size_t)L * w->words,
+                masks ? w->ms_lmask + (size_t)(L - 1) * V : nullptr,
+                masks ? w->ms_lmask + (size_t)L * V : nullptr, L};
     ProfScope p("msbc_back");
     for (int b = 0; b < 3; b++) {
       if (!en.in.n[b]) continue;
@@ -1868,7 +1989,7 @@ int gg_bc_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_dep
     GG_TRY(en.note_iteration(false));
   }
   hipLaunchKernelGGL(msbc_final_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(), dd, sigma, n);
-  if (g->reordered) {
+  if (g->reordered && !keep) {
     hipLaunchKernelGGL(unpermute_rows_kernel<double>, dim3(grid1(n)), dim3(kBlock), 0, stream(), dd,
                        d_dep, g->iperm, V);
     hipLaunchKernelGGL(unpermute_rows_kernel<double>, dim3(grid1(n)), dim3(kBlock), 0, stream(),
@@ -1879,6 +2000,28 @@ int gg_bc_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_dep
   en.st.reached = (uint64_t)n_roots;
   put_stats(stats, en.st);
   return GG_OK;
+}
+
+}  // namespace gg
+
+extern "C" {
+
+int gg_bfs_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, uint32_t *d_depth,
+                 gg_run_stats *stats) {
+  if (!d_depth) return fail(GG_EINVAL, "gg_bfs_batch: null output");
+  return bfs_batch_run(g, h_roots, n_roots, d_depth, stats, nullptr);
+}
+
+int gg_sssp_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, float *d_dist,
+                  gg_run_stats *stats) {
+  if (!d_dist) return fail(GG_EINVAL, "gg_sssp_batch: null output");
+  return sssp_batch_run(g, h_roots, n_roots, d_dist, stats, nullptr);
+}
+
+int gg_bc_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_dep,
+                double *d_inv_paths, gg_run_stats *stats) {
+  if (!d_dep || !d_inv_paths) return fail(GG_EINVAL, "gg_bc_batch: null output");
+  return bc_batch_run(g, h_roots, n_roots, d_dep, d_inv_paths, stats, nullptr);
 }
 
 }  // extern "C"

@@ -58,6 +58,7 @@ struct Options {
   int64_t pull_serial_max = 32;  // BFS pull: lane-serial scan
   int64_t huge_grid_waves = 4096;
   int64_t count_examined = 0;    // count adjacency entries read (roofline model)
+  int64_t bc_level_masks = 1;    // batched BC: keep per-level root masks (else depth rows)
 };
 Options &options();
 
@@ -166,6 +167,8 @@ struct Work {
   void *mb_a = nullptr, *mb_b = nullptr;  // V x 64 x 8 bytes each (BC rows, reordered outputs)
   uint64_t *ms_levels = nullptr;       // per-level frontier bitmaps (BC backward)
   size_t ms_levels_cap = 0;            // levels the store holds
+  uint64_t *ms_lmask = nullptr;        // per-level root masks, V each (BC backward)
+  size_t ms_lmask_cap = 0;
   size_t bytes = 0;                    // device bytes held (fixed part)
   Counters *h_cnt = nullptr;   // pinned host copy
   unsigned long long *h_exam = nullptr;
@@ -177,6 +180,17 @@ void work_free(Work *w);
 // Device bytes of one traversal context that runs batched queries of `kind`
 // (GG_QUERY_*) on g, counting the first 64 stored BFS levels.
 size_t work_footprint(const gg_graph *g, int kind);
+// Batched queries (gg_bfs_batch, gg_sssp_batch, gg_bc_batch).  With a null
+// output the results stay in the calling context's buffers, in internal
+// vertex order (row of internal vertex n at rows[n * 64]), and *rows points
+// at them (BC: the dependencies); they stay valid until the context's next
+// batched query on g.
+int bfs_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, uint32_t *d_depth,
+                  gg_run_stats *stats, const void **rows);
+int sssp_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, float *d_dist,
+                   gg_run_stats *stats, const void **rows);
+int bc_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_dep,
+                 double *d_inv_paths, gg_run_stats *stats, const void **rows);
 
 // Host-side view of one direction, swapped by transpose.
 inline const Dir &out_dir(const gg_graph *g) { return g->transposed ? g->in : g->out; }

@@ -13,6 +13,7 @@
 // GG_EBUDGET if that is not enough.
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -30,11 +31,16 @@ static double now_s() {
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
+// Column `col` of 64-root result rows kept in internal vertex order, in
+// original vertex order: out[v] = rows[perm[v] * 64 + col].
 template <class T>
-__global__ void extract_column_kernel(const T *rows, uint32_t V, int col, T *out) {
+__global__ void extract_column_kernel(const T *rows, const uint32_t *perm, uint32_t V, int col,
+                                      T *out) {
   uint32_t s = gridDim.x * blockDim.x;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s)
-    out[v] = rows[(size_t)v * 64 + col];
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s) {
+    uint32_t n = perm ? perm[v] : v;
+    out[v] = rows[(size_t)n * 64 + col];
+  }
 }
 
 struct SQuery {
@@ -50,10 +56,7 @@ struct SQuery {
 };
 
 struct WorkerBuf {
-  void *a = nullptr;    // 64-root rows: depth u32 / distance f32 / dependency f64
-  void *b = nullptr;    // BC inv_num_paths rows
-  void *col = nullptr;  // one extracted result column
-  size_t a_bytes = 0;
+  void *col = nullptr;  // one extracted result column (8 x V bytes)
   size_t charged = 0;   // budget charged for this worker on this graph
 };
 
@@ -79,17 +82,13 @@ namespace gg {
 
 static size_t elem_bytes(int kind) { return kind == GG_QUERY_BC ? 8 : 4; }
 
-// Device bytes worker buffers need to serve `kind` on g: the traversal
-// context plus the 64-root result rows and one result column.
+// Device bytes a worker needs to serve `kind` on g: its traversal context
+// (which keeps the 64-root result rows) plus one result column.
 static size_t serve_bytes(const gg_graph *g, int kind) {
-  size_t V = g->vertices;
-  return work_footprint(g, kind) + 64 * elem_bytes(kind) * V +
-         (kind == GG_QUERY_BC ? 64 * 8 * V : 0) + 8 * V;
+  return work_footprint(g, kind) + 8 * (size_t)g->vertices;
 }
 
 static void release_buf(WorkerBuf &b) {
-  device_free(b.a);
-  device_free(b.b);
   device_free(b.col);
   b = WorkerBuf{};
 }
@@ -122,46 +121,33 @@ static int admit_locked(gg_server *s, int k, int gi, int kind) {
   return GG_OK;
 }
 
-static int ensure_buffers(gg_graph *g, WorkerBuf &b, int kind) {
-  size_t V = g->vertices;
-  size_t ab = 64 * elem_bytes(kind) * V;
-  if (b.a_bytes < ab) {
-    device_free(b.a);
-    b.a = nullptr;
-    b.a_bytes = 0;
-    GG_TRY(device_alloc(&b.a, ab));
-    b.a_bytes = ab;
-  }
-  if (kind == GG_QUERY_BC && !b.b) GG_TRY(device_alloc(&b.b, 64 * 8 * V));
-  if (!b.col) GG_TRY(device_alloc(&b.col, 8 * V));
-  return GG_OK;
-}
-
 static int run_batch(gg_graph *g, WorkerBuf &b, int kind, std::vector<SQuery *> &batch) {
-  GG_TRY(ensure_buffers(g, b, kind));
+  if (!b.col) GG_TRY(device_alloc(&b.col, 8 * (size_t)g->vertices));
   std::vector<uint32_t> roots(batch.size());
   for (size_t i = 0; i < batch.size(); i++) roots[i] = batch[i]->root;
   int n = (int)roots.size();
-  int r;
+  const void *rows = nullptr;
   if (kind == GG_QUERY_BFS)
-    r = gg_bfs_batch(g, roots.data(), n, (uint32_t *)b.a, nullptr);
+    GG_TRY(bfs_batch_run(g, roots.data(), n, nullptr, nullptr, &rows));
   else if (kind == GG_QUERY_SSSP)
-    r = gg_sssp_batch(g, roots.data(), n, (float *)b.a, nullptr);
+    GG_TRY(sssp_batch_run(g, roots.data(), n, nullptr, nullptr, &rows));
   else
-    r = gg_bc_batch(g, roots.data(), n, (double *)b.a, (double *)b.b, nullptr);
-  if (r != GG_OK) return r;
-  uint32_t V = g->vertices;
+    GG_TRY(bc_batch_run(g, roots.data(), n, nullptr, nullptr, nullptr, &rows));
+  const uint32_t V = g->vertices;
+  const uint32_t *perm = g->reordered ? g->perm : nullptr;
+  const size_t eb = elem_bytes(kind);
+  const dim3 grid(std::min<uint32_t>((V + 255) / 256, 65536));
   for (size_t i = 0; i < batch.size(); i++) {
     if (!batch[i]->want) continue;
-    size_t eb = elem_bytes(kind);
     if (eb == 8)
-      hipLaunchKernelGGL(extract_column_kernel<double>, dim3((V + 255) / 256 < 65536 ? (V + 255) / 256 : 65536),
-                         dim3(256), 0, stream(), (const double *)b.a, V, (int)i, (double *)b.col);
+      hipLaunchKernelGGL(extract_column_kernel<double>, grid, dim3(256), 0, stream(),
+                         (const double *)rows, perm, V, (int)i, (double *)b.col);
     else
-      hipLaunchKernelGGL(extract_column_kernel<uint32_t>, dim3((V + 255) / 256 < 65536 ? (V + 255) / 256 : 65536),
-                         dim3(256), 0, stream(), (const uint32_t *)b.a, V, (int)i, (uint32_t *)b.col);
+      hipLaunchKernelGGL(extract_column_kernel<uint32_t>, grid, dim3(256), 0, stream(),
+                         (const uint32_t *)rows, perm, V, (int)i, (uint32_t *)b.col);
     batch[i]->result.resize(eb * V);
-    GG_HIP(hipMemcpyAsync(batch[i]->result.data(), b.col, eb * V, hipMemcpyDeviceToHost, stream()));
+    GG_HIP(hipMemcpyAsync(batch[i]->result.data(), b.col, eb * V, hipMemcpyDeviceToHost,
+                          stream()));
     GG_HIP(hipStreamSynchronize(stream()));
   }
   GG_HIP(hipStreamSynchronize(stream()));

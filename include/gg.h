@@ -58,13 +58,19 @@ int gg_mem_info(size_t *free_bytes, size_t *total_bytes);
 int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
 
 /* Tunables (names below).  The switch divisor reproduces the reference's
- * "sparse iff active_edges < edges / 20" (core/graph.hpp:1509) at 20.
- *   "switch_divisor"   push/pull crossover divisor (default 20)
+ * "sparse iff active_edges < edges / 20" (core/graph.hpp:1509) at 20; the
+ * default, 200, is the crossover measured on MI355X (results do not depend
+ * on it).
+ *   "switch_divisor"   push/pull crossover divisor (default 200)
  *   "force_mode"       0 = auto, 1 = always push, 2 = always pull
  *   "push_small_max"   largest out-degree handled by in-wave load balancing
  *   "push_big_max"     largest out-degree handled by one wave per vertex
  *   "pull_light_max"   largest in-degree handled inside a 64-vertex group
- *   "pull_serial_max"  BFS pull: largest in-degree scanned by a single lane */
+ *   "pull_serial_max"  BFS pull: largest in-degree scanned by a single lane
+ *   "huge_grid_waves"  waves sharing the adjacency of the largest-degree bin
+ *   "count_examined"   1 = count adjacency entries read (gg_run_stats)
+ *   "bc_level_masks"   batched BC: 1 = keep per-level root masks (default),
+ *                      0 = select backward arcs from the depth rows */
 int gg_set_option(const char *name, int64_t value);
 int gg_get_option(const char *name, int64_t *value);
 

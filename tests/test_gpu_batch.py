@@ -75,3 +75,20 @@ def test_bc_batch_hub_graph(gg):
         rdep, rinv, _ = o.bc(r)
         assert_rel(inv[i], rinv)
         assert_rel(dep[i], rdep)
+
+
+def test_bc_batch_depth_row_fallback(gg):
+    # backward arcs selected from the depth rows (the path taken when the
+    # per-level root masks would not fit) give the same dependencies
+    scale = 11
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, seed=83)
+    g = gg.Graph.build(e, V, reorder=True)
+    o = oracle.Graph(e, V)
+    roots = roots_of(o.out_degree(), 17, 83)
+    with gg.options(bc_level_masks=0):
+        dep, inv, _ = gg.bc_batch(g, roots)
+    for i, r in enumerate(roots):
+        rdep, rinv, _ = o.bc(r)
+        assert_rel(inv[i], rinv)
+        assert_rel(dep[i], rdep)

@@ -120,19 +120,26 @@ def test_serve_replay_and_budget(gg, resident):
 
 
 def test_serve_budget_evicts_idle_buffers(gg, resident):
-    # room for every graph but for only one graph's query buffers at a time:
-    # the worker drops the buffers of the graph it served before.
-    graphs_bytes = sum(g.info().device_bytes for g, _ in resident)
-    V = max(1 << sc for sc, _, _, _ in GRAPHS)
-    one = 64 * 8 * V * 2 + 64 * 4 * V + 64 * V + 40 * V + 16 * V * 64 + (1 << 20)
-    with gg.Server(graphs_bytes + one, workers=1) as s:
+    # room for every graph but for only the largest graph's query buffers: a
+    # worker drops the buffers of the graph it served before.
+    big = 1  # the scale-13 graph
+    with gg.Server(64 << 30, workers=1) as s:
+        ids = [s.add_graph(g) for g, _ in resident]
+        graphs_bytes = s.info()["used_bytes"]
+        g, o = resident[big]
+        s.wait(s.submit(ids[big], gg.QUERY_BC, roots_of(o.out_degree(), 1, 3)[0]))
+        per_worker = s.info()["used_bytes"] - graphs_bytes
+    with gg.Server(graphs_bytes + per_worker + (64 << 10), workers=1) as s:
         ids = [s.add_graph(g) for g, _ in resident]
         for gi in (0, 1, 2, 0):
             g, o = resident[gi]
             r = roots_of(o.out_degree(), 1, gi)[0]
             res, _ = s.wait(s.submit(ids[gi], gg.QUERY_BC, r))
             assert_rel(res, o.bc(r)[0])
-            assert s.info()["used_bytes"] <= s.info()["budget_bytes"]
+            info = s.info()
+            assert info["used_bytes"] <= info["budget_bytes"]
+            if gi == big:  # the other graphs' buffers were dropped
+                assert info["used_bytes"] == graphs_bytes + per_worker
 
 
 def test_serve_bad_requests(gg, resident):
