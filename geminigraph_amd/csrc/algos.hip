@@ -41,89 +41,9 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(const uint64_t *bits, s
                                                          uint32_t *q0, uint32_t *q1, uint32_t *q2,
                                                          Counters *cnt, uint64_t *visited,
                                                          uint64_t *clear) {
-  __shared__ uint32_t s_tot[3][kWavesPerBlock];
-  __shared__ unsigned long long s_edges[kWavesPerBlock];
-  __shared__ uint32_t s_base[3];
-  const int lane = lane_id(), wv = wave_in_block();
-  const size_t stride = (size_t)gridDim.x * kBlock;
-  for (size_t base = (size_t)blockIdx.x * kBlock; base < words; base += stride) {
-    size_t wi = base + threadIdx.x;
-    uint64_t word = wi < words ? bits[wi] : 0;
-    if (visited && word) visited[wi] |= word;
-    if (clear && wi < words) clear[wi] = 0;
-    uint32_t c0 = 0, c1 = 0, c2 = 0;
-    uint64_t es = 0;
-    for (uint64_t x = word; x; x &= x - 1) {
-      uint32_t v = (uint32_t)(wi * 64 + (size_t)__builtin_ctzll(x));
-      uint32_t d = bin_deg[v];
-      es += d;
-      if (d <= small_max)
-        c0++;
-      else if (d <= big_max)
-        c1++;
-      else
-        c2++;
-    }
-    uint32_t t0, t1, t2;
-    uint32_t e0 = wave_excl_scan(c0, &t0);
-    uint32_t e1 = wave_excl_scan(c1, &t1);
-    uint32_t e2 = wave_excl_scan(c2, &t2);
-    uint64_t ws = wave_sum_u64(es);
-    if (lane == 0) {
-      s_tot[0][wv] = t0;
-      s_tot[1][wv] = t1;
-      s_tot[2][wv] = t2;
-      s_edges[wv] = ws;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t b0 = 0, b1 = 0, b2 = 0;
-      unsigned long long be = 0;
-      for (int i = 0; i < kWavesPerBlock; i++) {
-        b0 += s_tot[0][i];
-        b1 += s_tot[1][i];
-        b2 += s_tot[2][i];
-        be += s_edges[i];
-      }
-      s_base[0] = b0 ? atomicAdd(&cnt->n[0], b0) : 0;
-      s_base[1] = b1 ? atomicAdd(&cnt->n[1], b1) : 0;
-      s_base[2] = b2 ? atomicAdd(&cnt->n[2], b2) : 0;
-      if (b0 + b1 + b2) {
-        atomicAdd(&cnt->total, b0 + b1 + b2);
-        atomicAdd((unsigned long long *)&cnt->edges, be);
-      }
-    }
-    __syncthreads();
-    uint32_t o0 = s_base[0] + e0, o1 = s_base[1] + e1, o2 = s_base[2] + e2;
-    for (int i = 0; i < wv; i++) {
-      o0 += s_tot[0][i];
-      o1 += s_tot[1][i];
-      o2 += s_tot[2][i];
-    }
-    for (uint64_t x = word; x; x &= x - 1) {
-      uint32_t v = (uint32_t)(wi * 64 + (size_t)__builtin_ctzll(x));
-      uint32_t d = bin_deg[v];
-      if (d <= small_max)
-        q0[o0++] = v;
-      else if (d <= big_max)
-        q1[o1++] = v;
-      else
-        q2[o2++] = v;
-    }
-    __syncthreads();
-  }
+  compact_words(bits, words, bin_deg, small_max, big_max, q0, q1, q2, cnt, visited, clear);
 }
 
-static unsigned grid1(uint64_t n, unsigned cap = 1u << 16) {
-  uint64_t g = (n + kBlock - 1) / kBlock;
-  if (g < 1) g = 1;
-  if (g > cap) g = cap;
-  return (unsigned)g;
-}
-static unsigned grid_waves(uint64_t waves) {
-  uint64_t g = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
-  return (unsigned)(g < 1 ? 1 : g);
-}
 // Blocks for kernels that walk vertices in 256-aligned rounds.
 static unsigned grid_vertex_rounds(uint32_t V) { return grid1(V, 1u << 14); }
 
@@ -238,6 +158,7 @@ void work_free(Work *w) {
   device_free(w->dsum);
   device_free(w->ms_vis);
   device_free(w->ms_lmask);
+  bfs_state_free(w->bfs);
   device_free(w->ms_f[0]);
   device_free(w->ms_f[1]);
   device_free(w->ms_roots);
@@ -476,137 +397,8 @@ __global__ void pr_init_kernel(double *curr, const uint32_t *outdeg, uint32_t V)
   }
 }
 
-// ===================================================================== BFS
-// toolkits/bfs.cpp:22-109
-struct BfsPush : NoAfter {
-  using Msg = uint32_t;
-  uint32_t *parent, *depth;
-  uint32_t V, level;
-  __device__ uint32_t msg(uint32_t v, uint32_t) const { return v; }
-  __device__ bool edge(uint32_t src, uint32_t, uint32_t dst, float) const {
-    // bfs.cpp:53: plain read first, then compare-and-swap
-    if (parent[dst] == V && atomicCAS(&parent[dst], V, src) == V) {
-      depth[dst] = level;
-      return true;
-    }
-    return false;
-  }
-};
+// BFS (toolkits/bfs.cpp) has a device-driven loop of its own: bfs.hip.
 
-struct BfsPullArgs {
-  const uint64_t *index;
-  const uint32_t *nbr;
-  const uint64_t *visited, *fin;
-  uint32_t *parent, *depth;
-  uint32_t V, level, light_max, serial_max, ngroups;
-  uint64_t *out;
-  unsigned long long *exam;
-};
-
-// Bottom-up step, one 64-vertex group per wave (bfs.cpp:60-76): an unvisited
-// vertex scans its in-list until it meets a frontier vertex.  Short lists are
-// scanned by their own lane, longer ones by the whole wave with a ballot.
-__global__ __launch_bounds__(kBlock) void bfs_pull_group_kernel(BfsPullArgs a) {
-  const uint32_t g = global_wave();
-  const int lane = lane_id();
-  if (g >= a.ngroups) return;
-  uint64_t visw = a.visited[g];
-  if (visw == ~0ULL) return;
-  uint32_t v = g * 64 + lane;
-  bool cand = v < a.V && !((visw >> lane) & 1ULL);
-  uint64_t b = 0, e1 = 0;
-  if (cand) {
-    b = a.index[v];
-    e1 = a.index[v + 1];
-  }
-  uint64_t deg = e1 - b;
-  cand = cand && deg > 0 && deg <= a.light_max;
-  bool serial = cand && deg <= a.serial_max;
-  uint32_t found = kUnset;
-  uint64_t examined = 0;
-  if (serial) {
-    for (uint64_t e = b; e < e1; e++) {
-      uint32_t u = stream_load(&a.nbr[e]);
-      examined++;
-      if (test_bit(a.fin, u)) {
-        found = u;
-        break;
-      }
-    }
-  }
-  uint64_t mmask = __ballot(cand && !serial);
-  while (mmask) {
-    int k = __builtin_ctzll(mmask);
-    mmask &= mmask - 1;
-    uint64_t bk = shfl_u64(b, k), ek = shfl_u64(e1, k);
-    uint32_t fk = kUnset;
-    for (uint64_t e0 = bk; e0 < ek; e0 += 64) {
-      uint64_t e = e0 + lane;
-      uint32_t u = e < ek ? stream_load(&a.nbr[e]) : 0;
-      bool hit = e < ek && test_bit(a.fin, u);
-      uint64_t hm = __ballot(hit);
-      if (lane == k)
-        examined += hm ? (uint64_t)(__builtin_ctzll(hm) + 1) : (ek - e0 < 64 ? ek - e0 : 64);
-      if (hm) {
-        fk = __shfl(u, __builtin_ctzll(hm));
-        break;
-      }
-    }
-    if (lane == k) found = fk;
-  }
-  bool got = found != kUnset;
-  if (got) {
-    a.parent[v] = found;
-    a.depth[v] = a.level;
-  }
-  uint64_t m = __ballot(got);
-  if (lane == 0 && m) atomicOr((unsigned long long *)&a.out[g], (unsigned long long)m);
-  if (a.exam) {
-    uint64_t ex = wave_sum_u64(examined);
-    if (lane == 0) count_examined(a.exam, ex);
-  }
-}
-
-struct BfsHeavyArgs {
-  const uint32_t *nbr;
-  const uint32_t *chunk_v;
-  const uint64_t *chunk_beg, *chunk_end;
-  uint32_t n_chunks;
-  const uint64_t *visited, *fin;
-  uint32_t *parent, *depth;
-  uint32_t V, level;
-  uint64_t *out;
-  unsigned long long *exam;
-};
-
-// Heavy in-lists: one wave per chunk; chunks of one vertex race and the
-// compare-and-swap on parent keeps exactly one (bfs.cpp:71, dense_slot).
-__global__ __launch_bounds__(kBlock) void bfs_pull_heavy_kernel(BfsHeavyArgs a) {
-  const uint32_t c = global_wave();
-  const int lane = lane_id();
-  if (c >= a.n_chunks) return;
-  uint32_t v = a.chunk_v[c];
-  if (test_bit(a.visited, v)) return;
-  uint64_t cb = a.chunk_beg[c], ce = a.chunk_end[c];
-  uint64_t examined = 0;
-  for (uint64_t e0 = cb; e0 < ce; e0 += 64) {
-    if (__hip_atomic_load(&a.parent[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != a.V) break;
-    uint64_t e = e0 + lane;
-    uint32_t u = e < ce ? stream_load(&a.nbr[e]) : 0;
-    bool hit = e < ce && test_bit(a.fin, u);
-    uint64_t hm = __ballot(hit);
-    examined += hm ? (uint64_t)(__builtin_ctzll(hm) + 1) : (ce - e0 < 64 ? ce - e0 : 64);
-    if (hm) {
-      uint32_t fu = __shfl(u, __builtin_ctzll(hm));
-      if (lane == 0 && atomicCAS(&a.parent[v], a.V, fu) == a.V) {
-        a.depth[v] = a.level;
-        activate(a.out, v);
-      }
-      break;
-    }
-  }
-  if (lane == 0) count_examined(a.exam, examined);
-}
 
 // ====================================================================== CC
 // toolkits/cc.cpp:22-109
@@ -835,37 +627,6 @@ __global__ void bc_final_kernel(double *dep, const double *inv, uint32_t V) {
   uint32_t s = gridDim.x * blockDim.x;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s)
     dep[v] = (dep[v] - inv[v]) / inv[v];  // bc.cpp:170-176
-}
-
-// ----------------------------------------------------------- event timing
-struct DeviceTimer {
-  hipEvent_t a = nullptr, b = nullptr;
-  DeviceTimer() {
-    hipEventCreate(&a);
-    hipEventCreate(&b);
-    hipEventRecord(a, stream());
-  }
-  double stop() {
-    hipEventRecord(b, stream());
-    hipEventSynchronize(b);
-    float ms = 0;
-    hipEventElapsedTime(&ms, a, b);
-    return ms;
-  }
-  ~DeviceTimer() {
-    hipEventDestroy(a);
-    hipEventDestroy(b);
-  }
-};
-
-static int check_graph(gg_graph *g) {
-  if (!g) return fail(GG_EINVAL, "null graph");
-  GG_TRY(ensure_init());
-  return GG_OK;
-}
-
-static void put_stats(gg_run_stats *dst, const gg_run_stats &src) {
-  if (dst) *dst = src;
 }
 
 // ========================================================= batched BFS (64)
@@ -1556,78 +1317,6 @@ int gg_pagerank(gg_graph *g, int iterations, double *d_curr_out, double *h_delta
   }
   en.st.m_reach = g->arcs;
   en.st.reached = V;
-  put_stats(stats, en.st);
-  return GG_OK;
-}
-
-int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *d_depth, uint32_t *h_frontier,
-           int cap, gg_run_stats *stats) {
-  GG_TRY(check_graph(g));
-  if (root >= g->vertices) return fail(GG_EINVAL, "root out of range");
-  if (!d_parent || !d_depth) return fail(GG_EINVAL, "null output");
-  Work *w;
-  GG_TRY(work_get(g, &w));
-  Engine en(g, w);
-  const Dir &out = out_dir(g), &in = in_dir(g);
-  uint32_t V = g->vertices;
-  DeviceTimer tm;
-  uint32_t *out_parent = d_parent, *out_depth = d_depth;
-  if (g->reordered) {
-    d_parent = (uint32_t *)w->res_a;
-    d_depth = (uint32_t *)w->res_b;
-    root = g->to_internal(root);
-  }
-  fill<uint32_t>(d_parent, V, V);  // bfs.cpp:35-36
-  fill<uint32_t>(d_depth, kUnset, V);
-  GG_HIP(hipMemsetAsync(w->visited, 0, w->words * 8, stream()));
-  hipLaunchKernelGGL((put2_kernel<uint32_t, uint32_t>), dim3(1), dim3(1), 0, stream(), d_parent,
-                     root, d_depth, 0u, root);
-  GG_TRY(en.seed(root, out.degree, w->visited));
-  uint64_t m_reach = en.in.edges, reached = 1;
-  int lv = 0;
-  if (h_frontier && cap > 0) h_frontier[0] = 1;
-  for (uint32_t level = 1; en.in.total > 0; level++) {
-    if (en.st.iterations < 64) en.st.it_unvisited[en.st.iterations] = (uint64_t)V - reached;
-    bool sp = en.sparse(en.in.edges);
-    if (sp) {
-      BfsPush op;
-      op.parent = d_parent;
-      op.depth = d_depth;
-      op.V = V;
-      op.level = level;
-      GG_TRY(en.launch_push(out, op, false));
-    } else {
-      BfsPullArgs a{in.index, in.nbr, w->visited, en.fin().bits, d_parent, d_depth, V, level,
-                    in.light_max, (uint32_t)options().pull_serial_max, (V + 63) / 64,
-                    en.fout().bits, en.exam};
-      {
-        ProfScope p("bfs_pull");
-        hipLaunchKernelGGL(bfs_pull_group_kernel, dim3(grid_waves(a.ngroups)), dim3(kBlock), 0,
-                           stream(), a);
-      }
-      if (in.n_chunks) {
-        BfsHeavyArgs h{in.nbr, in.chunk_v, in.chunk_beg, in.chunk_end, in.n_chunks, w->visited,
-                       en.fin().bits, d_parent, d_depth, V, level, en.fout().bits, en.exam};
-        ProfScope p("bfs_pull_heavy");
-        hipLaunchKernelGGL(bfs_pull_heavy_kernel, dim3(grid_waves(in.n_chunks)), dim3(kBlock), 0,
-                           stream(), h);
-      }
-    }
-    GG_TRY(en.finish_iteration(sp, out.degree, w->visited));
-    m_reach += en.in.edges;
-    reached += en.in.total;
-    lv++;
-    if (en.in.total > 0 && h_frontier && lv < cap) h_frontier[lv] = en.in.total;
-  }
-  if (g->reordered) {
-    hipLaunchKernelGGL(unpermute_ids_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), d_parent,
-                       out_parent, g->iperm, V, V);
-    hipLaunchKernelGGL(unpermute_kernel<uint32_t>, dim3(grid1(V)), dim3(kBlock), 0, stream(),
-                       d_depth, out_depth, g->iperm, V);
-  }
-  en.st.device_ms = tm.stop();
-  en.st.m_reach = m_reach;
-  en.st.reached = reached;
   put_stats(stats, en.st);
   return GG_OK;
 }

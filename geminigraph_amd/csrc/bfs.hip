@@ -1,0 +1,615 @@
+// BFS (toolkits/bfs.cpp) with its iteration loop driven from the device.
+//
+// Every iteration is the same fixed sequence of launches, issued ahead of
+// time on one stream:
+//   decide -- one block: reduces the size and out-degree sum of the frontier
+//             (counted by the kernels that produced it), applies the
+//             push/pull switch (core/graph.hpp:1503-1509), records the
+//             iteration's statistics, or marks the traversal done;
+//   bin    -- push only: frontier bitmap -> three degree-binned queues, and
+//             clears the output bitmap;
+//   push   -- push only: a persistent grid over the queues (bfs.cpp:49-58);
+//   pull   -- pull only: bottom-up over 64-vertex groups, then heavy chunks
+//             (bfs.cpp:60-76).
+// Launches of the mode not chosen return at once, so the host never waits
+// for an iteration before issuing the next one: it stays one iteration ahead
+// and reads each iteration's done flag through an event-ordered copy.
+//
+// Claims: a push edge claims its target by being the first to set the
+// target's bit in the output frontier; chunks of one heavy in-list race with a
+// compare-and-swap on the vertex's depth entry, which sits at its original
+// id (relabelling scatters those, so polling them does not pile onto a few
+// cache lines).  The reference claims with a compare-and-swap on parent
+// (bfs.cpp:53, 71); each way keeps exactly one valid parent.  Results are
+// written in original vertex ids as vertices are found (parent[iperm[v]] =
+// iperm[u]), so a relabelled graph needs no pass over V at the end.  A
+// frontier joins `visited` one iteration later, in the kernel that reads it
+// next (bin for push, the group pull for pull); until then the readers test
+// visited | frontier.
+#include <string.h>
+
+#include <algorithm>
+
+#include "kernels.hpp"
+
+namespace gg {
+
+enum : uint32_t { kBfsPush = 0, kBfsPull = 1, kBfsDone = 2 };
+constexpr int kCountStride = 16;  // u64 per count slot: one 128-byte line
+
+// Device control block of one traversal (done and iter first: the host
+// reads those 8 bytes after every decide).
+struct BfsCtl {
+  uint32_t done, iter;  // iter = iterations started
+  uint32_t mode, level, push_iters, V;
+  int32_t force;
+  uint32_t lv_cap;
+  uint64_t reached, m_reach, examined, threshold;
+  uint32_t modes[64];
+  uint64_t active[64], fresh[64], exam[64], unvisited[64];
+};
+
+// Where found vertices go.
+struct BfsOut {
+  uint32_t *parent, *depth;  // indexed by original ids
+  const uint32_t *iperm;     // internal -> original (null: not relabelled)
+  const uint32_t *deg;       // out-degree by internal id
+  uint64_t *visited, *fout;
+};
+
+__device__ __forceinline__ uint32_t orig_id(const uint32_t *iperm, uint32_t v) {
+  return iperm ? iperm[v] : v;
+}
+
+// Lane 0 of a wave adds its counts to one of kScalarSlots spread slots.
+__device__ __forceinline__ void bfs_count(uint64_t *set, uint32_t slot, uint64_t n, uint64_t e) {
+  if (!n) return;
+  uint64_t *p = set + (size_t)(slot % kScalarSlots) * kCountStride;
+  atomicAdd((unsigned long long *)p, (unsigned long long)n);
+  atomicAdd((unsigned long long *)(p + 1), (unsigned long long)e);
+}
+
+__device__ __forceinline__ void bfs_wave_flush(uint64_t *set, uint32_t gw, uint64_t cnt,
+                                               uint64_t edg, unsigned long long *exam,
+                                               uint64_t ex) {
+  cnt = wave_sum_u64(cnt);
+  edg = wave_sum_u64(edg);
+  ex = wave_sum_u64(ex);
+  if (lane_id() == 0) {
+    bfs_count(set, gw, cnt, edg);
+    count_examined(exam, ex);
+  }
+}
+
+// Push claim: `visited` holds every vertex of the earlier levels and of the
+// frontier (the bin kernel adds the frontier before the push), so the first
+// edge to set dst's bit in the output frontier claims it; only the winner
+// touches the scattered outputs.
+__device__ __forceinline__ bool bfs_claim(const BfsOut &o, uint32_t src, uint32_t dst,
+                                          uint32_t level) {
+  const uint64_t bit = 1ULL << (dst & 63);
+  if ((o.visited[dst >> 6] | o.fout[dst >> 6]) & bit) return false;  // plain reads first
+  if (atomicOr((unsigned long long *)&o.fout[dst >> 6], (unsigned long long)bit) & bit)
+    return false;
+  const uint32_t od = orig_id(o.iperm, dst);
+  o.parent[od] = orig_id(o.iperm, src);
+  o.depth[od] = level;
+  return true;
+}
+
+// Progress the host polls, in coherent pinned memory: the deciding thread
+// publishes each decision with a system-scope store.
+struct BfsHostFlag {
+  uint32_t decided;  // iterations decided so far
+  uint32_t done;
+};
+
+// Reduces the counts of the frontier just produced (kScalarSlots spread
+// slots) and decides the next iteration: the push/pull switch
+// (core/graph.hpp:1509), the iteration's records, or the end of the
+// traversal.  Called by all kBlock threads of one block.
+__device__ void bfs_decide(BfsCtl *c, uint64_t *set, Counters *qc, unsigned long long *exam,
+                           uint32_t *lv, BfsHostFlag *hf) {
+  __shared__ uint64_t sn[kBlock], se[kBlock], sx[kBlock];
+  __shared__ uint32_t s_done;
+  const int t = threadIdx.x;  // kBlock == kScalarSlots: one slot per thread
+  if (t == 0) s_done = c->done;
+  __syncthreads();
+  if (s_done) return;
+  uint64_t *p = set + (size_t)t * kCountStride;
+  sn[t] = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  se[t] = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  sx[t] = 0;
+  if (exam && t < 64) {
+    sx[t] = __hip_atomic_load(&exam[t * 16], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&exam[t * 16], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      sn[t] += sn[t + s];
+      se[t] += se[t + s];
+      sx[t] += sx[t + s];
+    }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  const uint64_t total = sn[0], edges = se[0];
+  const uint32_t i = c->iter;
+  if (i > 0 && i - 1 < 64) {
+    c->fresh[i - 1] = total;
+    c->exam[i - 1] = sx[0];
+  }
+  c->examined += sx[0];
+  if (total == 0) {
+    c->done = 1;
+    c->mode = kBfsDone;
+    __threadfence_system();
+    __hip_atomic_store(&hf->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    return;
+  }
+  c->reached += total;
+  c->m_reach += edges;
+  if (i < c->lv_cap) lv[i] = (uint32_t)total;
+  // core/graph.hpp:1509: sparse iff active out-edges < |E| / divisor
+  uint32_t mode = c->force == 1   ? kBfsPush
+                  : c->force == 2 ? kBfsPull
+                  : edges < c->threshold ? kBfsPush
+                                         : kBfsPull;
+  if (i < 64) {
+    c->modes[i] = mode == kBfsPush ? 0u : 1u;
+    c->active[i] = total;
+    c->unvisited[i] = (uint64_t)c->V - c->reached;
+  }
+  c->mode = mode;
+  c->level = i + 1;
+  c->iter = i + 1;
+  if (mode == kBfsPush) c->push_iters++;
+  qc->n[0] = qc->n[1] = qc->n[2] = 0;
+  qc->total = 0;
+  qc->edges = 0;
+  __threadfence_system();
+  __hip_atomic_store(&hf->decided, i + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Output arrays to "unreached" (bfs.cpp:35-36), bitmaps and count slots to 0.
+__global__ void bfs_init_kernel(uint32_t *parent, uint32_t *depth, uint32_t V, uint64_t *visited,
+                                uint64_t *f0, uint64_t *f1, size_t words, uint64_t *sets,
+                                size_t set_words, unsigned long long *exam) {
+  const size_t s = (size_t)gridDim.x * blockDim.x;
+  const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (size_t i = i0; i < V; i += s) {
+    parent[i] = V;
+    depth[i] = kUnset;
+  }
+  for (size_t i = i0; i < words; i += s) {
+    visited[i] = 0;
+    f0[i] = 0;
+    f1[i] = 0;
+  }
+  for (size_t i = i0; i < set_words; i += s) sets[i] = 0;
+  if (exam)
+    for (size_t i = i0; i < 1024; i += s) exam[i] = 0;
+}
+
+// One block: the root (bfs.cpp:37-39), the control block, and the decision
+// for the first iteration.
+__global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
+    BfsCtl *c, uint64_t *set0, Counters *qc, unsigned long long *exam, uint32_t *lv,
+    BfsHostFlag *hf, uint64_t *visited, uint64_t *fin, uint32_t root, uint32_t root_orig,
+    uint32_t *parent, uint32_t *depth, const uint32_t *deg, uint32_t V, int32_t force,
+    uint64_t threshold, uint32_t lv_cap) {
+  if (threadIdx.x == 0) {
+    c->done = 0;
+    c->iter = 0;
+    c->mode = kBfsDone;
+    c->level = 0;
+    c->push_iters = 0;
+    c->V = V;
+    c->force = force;
+    c->lv_cap = lv_cap;
+    c->reached = 0;
+    c->m_reach = 0;
+    c->examined = 0;
+    c->threshold = threshold;
+    const uint64_t bit = 1ULL << (root & 63);
+    visited[root >> 6] |= bit;
+    fin[root >> 6] |= bit;
+    parent[root_orig] = root_orig;
+    depth[root_orig] = 0;
+    set0[0] = 1;
+    set0[1] = deg[root];
+    __threadfence();
+  }
+  __syncthreads();
+  bfs_decide(c, set0, qc, exam, lv, hf);
+}
+
+__global__ __launch_bounds__(kBlock) void bfs_decide_kernel(BfsCtl *c, uint64_t *set,
+                                                            Counters *qc,
+                                                            unsigned long long *exam,
+                                                            uint32_t *lv, BfsHostFlag *hf) {
+  bfs_decide(c, set, qc, exam, lv, hf);
+}
+
+__global__ __launch_bounds__(kBlock) void bfs_bin_kernel(const BfsCtl *c, const uint64_t *fin,
+                                                         uint64_t *fout, uint64_t *visited,
+                                                         size_t words,
+                                                         const uint32_t *deg, uint32_t small_max,
+                                                         uint32_t big_max, uint32_t *q0,
+                                                         uint32_t *q1, uint32_t *q2,
+                                                         Counters *qc) {
+  if (c->mode != kBfsPush) return;
+  compact_words(fin, words, deg, small_max, big_max, q0, q1, q2, qc, visited, fout);
+}
+
+struct BfsPushArgs {
+  const uint64_t *index;
+  const uint32_t *nbr;
+  const uint32_t *q0, *q1, *q2;
+  const Counters *qc;
+};
+
+// Persistent grid over the three bins: 64 small vertices per wave step
+// (in-wave load balancing), one mid vertex per wave step, and chunks of every
+// huge vertex shared by all waves.
+__global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOut o,
+                                                          BfsPushArgs a, uint64_t *set_out,
+                                                          unsigned long long *exam) {
+  if (c->mode != kBfsPush) return;
+  const uint32_t level = c->level;
+  const uint32_t W = gridDim.x * kWavesPerBlock, gw = global_wave();
+  const int lane = lane_id();
+  const uint32_t n0 = a.qc->n[0], n1 = a.qc->n[1], n2 = a.qc->n[2];
+  const uint32_t waves0 = (n0 + 63) / 64;
+  uint64_t cnt = 0, edg = 0, ex = 0;
+  auto visit = [&](uint32_t src, uint32_t dst) {
+    if (bfs_claim(o, src, dst, level)) {
+      cnt++;
+      edg += o.deg[dst];
+    }
+  };
+  for (uint32_t vw = gw; vw < waves0 + n1; vw += W) {
+    if (vw < waves0) {
+      const uint32_t i = vw * 64 + lane;
+      const bool valid = i < n0;
+      const uint32_t v = valid ? a.q0[i] : 0;
+      const uint64_t b = valid ? a.index[v] : 0;
+      const uint32_t d = valid ? (uint32_t)(a.index[v + 1] - b) : 0;
+      uint32_t total;
+      const uint32_t excl = wave_excl_scan(d, &total);
+      for (uint32_t j0 = 0; j0 < total; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        const int k = wave_owner(excl, j);
+        const uint32_t src = __shfl(v, k);
+        const uint64_t bb = shfl_u64(b, k);
+        const uint32_t exk = __shfl(excl, k);
+        if (j < total) visit(src, stream_load(&a.nbr[bb + (j - exk)]));
+      }
+      if (lane == 0) ex += total;
+    } else {
+      const uint32_t v = a.q1[vw - waves0];
+      const uint64_t b = a.index[v], e1 = a.index[v + 1];
+      for (uint64_t e = b + lane; e < e1; e += 64) visit(v, stream_load(&a.nbr[e]));
+      if (lane == 0) ex += e1 - b;
+    }
+  }
+  for (uint32_t h = 0; h < n2; h++) {
+    const uint32_t v = a.q2[h];
+    const uint64_t b = a.index[v], e1 = a.index[v + 1];
+    const uint64_t nch = (e1 - b + kChunkEdges - 1) / kChunkEdges;
+    for (uint64_t ch = gw; ch < nch; ch += W) {
+      const uint64_t cb = b + ch * kChunkEdges;
+      const uint64_t ce = cb + kChunkEdges < e1 ? cb + kChunkEdges : e1;
+      for (uint64_t e = cb + lane; e < ce; e += 64) visit(v, stream_load(&a.nbr[e]));
+      if (lane == 0) ex += ce - cb;
+    }
+  }
+  bfs_wave_flush(set_out, gw, cnt, edg, exam, ex);
+}
+
+struct BfsPullArgs {
+  const uint64_t *index;
+  const uint32_t *nbr;
+  const uint64_t *fin;
+  uint32_t V, light_max, serial_max, ngroups;
+};
+
+// Bottom-up step (bfs.cpp:60-76) over 64-vertex groups, a persistent grid:
+// an unvisited vertex scans its in-neighbours until one is in the frontier.
+// Short lists are scanned by their own lane, longer ones by the whole wave
+// 64 entries at a time; lists above light_max are left to the chunk kernel.
+// The wave owns its group's words of visited and of the output frontier
+// (the chunk kernel runs after it and only ORs bits in).
+__global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOut o,
+                                                          BfsPullArgs a, uint64_t *set_out,
+                                                          unsigned long long *exam) {
+  if (c->mode != kBfsPull) return;
+  const uint32_t level = c->level;
+  const uint32_t W = gridDim.x * kWavesPerBlock;
+  const int lane = lane_id();
+  uint64_t cnt = 0, edg = 0, examined = 0;
+  for (uint32_t g = global_wave(); g < a.ngroups; g += W) {
+    const uint64_t visw = o.visited[g] | a.fin[g];
+    if (visw == ~0ULL) {
+      if (lane == 0) {
+        o.fout[g] = 0;
+        o.visited[g] = visw;
+      }
+      continue;
+    }
+    const uint32_t v = g * 64 + lane;
+    bool cand = v < a.V && !((visw >> lane) & 1ULL);
+    uint64_t b = 0, e1 = 0;
+    if (cand) {
+      b = a.index[v];
+      e1 = a.index[v + 1];
+    }
+    const uint64_t deg = e1 - b;
+    cand = cand && deg > 0 && deg <= a.light_max;
+    const bool serial = cand && deg <= a.serial_max;
+    uint32_t found = kUnset;
+    if (serial) {
+      // four neighbours and their frontier words in flight per step
+      for (uint64_t e = b; e < e1 && found == kUnset; e += 4) {
+        uint32_t u[4];
+        bool h[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) u[t] = e + t < e1 ? stream_load(&a.nbr[e + t]) : 0;
+#pragma unroll
+        for (int t = 0; t < 4; t++) h[t] = e + t < e1 && test_bit(a.fin, u[t]);
+#pragma unroll
+        for (int t = 3; t >= 0; t--)
+          if (h[t]) found = u[t];
+        uint64_t n = e1 - e < 4 ? e1 - e : 4;
+        examined += n;
+      }
+    }
+    uint64_t mmask = __ballot(cand && !serial);
+    while (mmask) {
+      const int k = __builtin_ctzll(mmask);
+      mmask &= mmask - 1;
+      const uint64_t bk = shfl_u64(b, k), ek = shfl_u64(e1, k);
+      uint32_t fk = kUnset;
+      for (uint64_t e0 = bk; e0 < ek; e0 += 64) {
+        const uint64_t e = e0 + lane;
+        const uint32_t u = e < ek ? stream_load(&a.nbr[e]) : 0;
+        const bool hit = e < ek && test_bit(a.fin, u);
+        const uint64_t hm = __ballot(hit);
+        if (lane == k)
+          examined += hm ? (uint64_t)(__builtin_ctzll(hm) + 1) : (ek - e0 < 64 ? ek - e0 : 64);
+        if (hm) {
+          fk = __shfl(u, __builtin_ctzll(hm));
+          break;
+        }
+      }
+      if (lane == k) found = fk;
+    }
+    const bool got = found != kUnset;
+    const uint64_t m = __ballot(got);
+    if (got) {
+      const uint32_t ov = orig_id(o.iperm, v);
+      o.parent[ov] = orig_id(o.iperm, found);
+      o.depth[ov] = level;
+      edg += o.deg[v];
+    }
+    if (lane == 0) {
+      o.fout[g] = m;
+      o.visited[g] = visw;  // this level joins visited; m joins next iteration
+      cnt += (uint64_t)__popcll(m);
+    }
+  }
+  bfs_wave_flush(set_out, global_wave(), cnt, edg, exam, examined);
+}
+
+struct BfsHeavyArgs {
+  const uint32_t *nbr;
+  const uint32_t *chunk_v;
+  const uint64_t *chunk_beg, *chunk_end;
+  const uint64_t *fin;
+  uint32_t n_chunks;
+};
+
+// In-lists above light_max, one wave per chunk (grid-stride); chunks of one
+// vertex race and the compare-and-swap on its depth keeps one parent.  The
+// chunks poll that depth entry to stop early once another chunk has won.
+__global__ __launch_bounds__(kBlock) void bfs_pull_heavy_kernel(const BfsCtl *c, BfsOut o,
+                                                                BfsHeavyArgs a, uint64_t *set_out,
+                                                                unsigned long long *exam) {
+  if (c->mode != kBfsPull) return;
+  const uint32_t level = c->level;
+  const uint32_t W = gridDim.x * kWavesPerBlock;
+  const int lane = lane_id();
+  uint64_t cnt = 0, edg = 0, examined = 0;
+  for (uint32_t ch = global_wave(); ch < a.n_chunks; ch += W) {
+    const uint32_t v = a.chunk_v[ch];
+    const uint64_t bit = 1ULL << (v & 63);
+    if ((o.visited[v >> 6] | a.fin[v >> 6]) & bit) continue;
+    const uint32_t ov = orig_id(o.iperm, v);
+    uint32_t *dv = &o.depth[ov];
+    const uint64_t cb = a.chunk_beg[ch], ce = a.chunk_end[ch];
+    for (uint64_t e0 = cb; e0 < ce; e0 += 64) {
+      if (__hip_atomic_load(dv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kUnset) break;
+      const uint64_t e = e0 + lane;
+      const uint32_t u = e < ce ? stream_load(&a.nbr[e]) : 0;
+      const bool hit = e < ce && test_bit(a.fin, u);
+      const uint64_t hm = __ballot(hit);
+      examined += hm ? (uint64_t)(__builtin_ctzll(hm) + 1) : (ce - e0 < 64 ? ce - e0 : 64);
+      if (hm) {
+        const uint32_t fu = __shfl(u, __builtin_ctzll(hm));
+        if (lane == 0 && atomicCAS(dv, kUnset, level) == kUnset) {
+          o.parent[ov] = orig_id(o.iperm, fu);
+          atomicOr((unsigned long long *)&o.fout[v >> 6], (unsigned long long)bit);
+          cnt++;
+          edg += o.deg[v];
+        }
+        break;
+      }
+    }
+  }
+  if (lane != 0) examined = 0;  // counted once per wave
+  bfs_wave_flush(set_out, global_wave(), cnt, edg, exam, examined);
+}
+
+// Device state of the BFS loop, kept in the traversal context.
+struct BfsState {
+  BfsCtl *ctl = nullptr;
+  uint64_t *sets = nullptr;   // 2 x kScalarSlots x kCountStride
+  Counters *qc = nullptr;
+  uint32_t *lv = nullptr;
+  uint32_t lv_cap = 0;
+  BfsCtl *h_ctl = nullptr;      // pinned
+  BfsHostFlag *h_flag = nullptr;  // pinned, coherent: written by the device
+};
+
+void bfs_state_free(void *p) {
+  BfsState *s = (BfsState *)p;
+  if (!s) return;
+  device_free(s->ctl);
+  device_free(s->sets);
+  device_free(s->qc);
+  device_free(s->lv);
+  if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+  if (s->h_flag) (void)hipHostFree(s->h_flag);
+  delete s;
+}
+
+static int bfs_state(Work *w, uint32_t lv_need, BfsState **out) {
+  BfsState *s = (BfsState *)w->bfs;
+  if (!s) {
+    s = new BfsState;
+    w->bfs = s;
+    GG_TRY(device_alloc((void **)&s->ctl, sizeof(BfsCtl)));
+    GG_TRY(device_alloc((void **)&s->sets, 2 * kScalarSlots * kCountStride * 8));
+    GG_TRY(device_alloc((void **)&s->qc, sizeof(Counters)));
+    GG_HIP(hipHostMalloc((void **)&s->h_ctl, sizeof(BfsCtl)));
+    GG_HIP(hipHostMalloc((void **)&s->h_flag, sizeof(BfsHostFlag), hipHostMallocCoherent));
+  }
+  if (lv_need > s->lv_cap) {
+    device_free(s->lv);
+    s->lv = nullptr;
+    s->lv_cap = 0;
+    GG_TRY(device_alloc((void **)&s->lv, (size_t)lv_need * 4));
+    s->lv_cap = lv_need;
+  }
+  *out = s;
+  return GG_OK;
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *d_depth,
+                      uint32_t *h_frontier, int cap, gg_run_stats *stats) {
+  GG_TRY(check_graph(g));
+  if (root >= g->vertices) return fail(GG_EINVAL, "root out of range");
+  if (!d_parent || !d_depth) return fail(GG_EINVAL, "null output");
+  Work *w;
+  GG_TRY(work_get(g, &w));
+  const Dir &out = out_dir(g), &in = in_dir(g);
+  const uint32_t V = g->vertices;
+  const uint32_t lv_need = (uint32_t)std::min<int64_t>(std::max(cap, 1), (int64_t)V + 1);
+  BfsState *s;
+  GG_TRY(bfs_state(w, lv_need, &s));
+  const Options &opt = options();
+  unsigned long long *exam = opt.count_examined ? w->exam : nullptr;
+  const uint32_t *iperm = g->reordered ? g->iperm : nullptr;
+  const uint32_t r_int = g->to_internal(root);
+  const size_t words = w->words;
+  const uint64_t threshold = g->edges / (uint64_t)opt.switch_divisor;
+  hipStream_t st = stream();
+  const size_t set_words = (size_t)kScalarSlots * kCountStride;
+  volatile BfsHostFlag *hf = s->h_flag;
+  hf->decided = 0;
+  hf->done = 0;
+  DeviceTimer tm;
+  hipLaunchKernelGGL(bfs_init_kernel, dim3(grid1(V)), dim3(kBlock), 0, st, d_parent, d_depth, V,
+                     w->visited, w->f[0].bits, w->f[1].bits, words, s->sets, 2 * set_words, exam);
+  hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(kBlock), 0, st, s->ctl, s->sets, s->qc, exam,
+                     s->lv, s->h_flag, w->visited, w->f[0].bits, r_int, root, d_parent, d_depth,
+                     out.degree, V, (int32_t)opt.force_mode, threshold, s->lv_cap);
+  // fixed launch shapes: the frontier sizes live on the device
+  const unsigned bin_grid = grid1(words, 2048);
+  const unsigned push_grid = grid_waves((uint64_t)opt.huge_grid_waves);
+  const uint32_t ngroups = (V + 63) / 64;
+  const uint64_t pw = (uint64_t)opt.bfs_pull_waves;
+  const unsigned pull_grid = grid_waves(pw ? std::min<uint64_t>(ngroups, pw) : ngroups);
+  const uint64_t nch = std::max(in.n_chunks, 1u);
+  const unsigned heavy_grid = grid_waves(pw ? std::min<uint64_t>(nch, pw) : nch);
+  for (uint32_t n = 0;; n++) {
+    // keep one iteration queued: before issuing iteration n, wait until
+    // iteration n-1 has been decided (so iteration n-2 has finished)
+    if (n >= 1) {
+      uint64_t spins = 0;
+      while (!hf->done && hf->decided < n) {
+        if ((++spins & 0xFFF) == 0 && hipStreamQuery(st) == hipSuccess && !hf->done &&
+            hf->decided < n)
+          return fail(GG_EHIP, "gg_bfs: device loop stopped without a decision");
+      }
+      if (hf->done) break;
+    }
+    if (n > V + 2) return fail(GG_EHIP, "gg_bfs: traversal did not terminate");
+    uint64_t *fin = w->f[n & 1].bits, *fout = w->f[(n + 1) & 1].bits;
+    uint64_t *set_out = s->sets + (size_t)((n + 1) & 1) * set_words;
+    BfsOut o{d_parent, d_depth, iperm, out.degree, w->visited, fout};
+    {
+      ProfScope p("compact");
+      hipLaunchKernelGGL(bfs_bin_kernel, dim3(bin_grid), dim3(kBlock), 0, st, s->ctl, fin, fout,
+                         w->visited, words, out.degree, (uint32_t)opt.push_small_max,
+                         (uint32_t)opt.push_big_max, w->f[0].q[0], w->f[0].q[1], w->f[0].q[2],
+                         s->qc);
+    }
+    {
+      ProfScope p("push");
+      BfsPushArgs a{out.index, out.nbr, w->f[0].q[0], w->f[0].q[1], w->f[0].q[2], s->qc};
+      hipLaunchKernelGGL(bfs_push_kernel, dim3(push_grid), dim3(kBlock), 0, st, s->ctl, o, a,
+                         set_out, exam);
+    }
+    {
+      ProfScope p("bfs_pull");
+      BfsPullArgs a{in.index, in.nbr, fin, V, in.light_max, (uint32_t)opt.pull_serial_max,
+                    ngroups};
+      hipLaunchKernelGGL(bfs_pull_kernel, dim3(pull_grid), dim3(kBlock), 0, st, s->ctl, o, a,
+                         set_out, exam);
+    }
+    if (in.n_chunks) {
+      ProfScope p("bfs_pull_heavy");
+      BfsHeavyArgs a{in.nbr, in.chunk_v, in.chunk_beg, in.chunk_end, fin, in.n_chunks};
+      hipLaunchKernelGGL(bfs_pull_heavy_kernel, dim3(heavy_grid), dim3(kBlock), 0, st, s->ctl, o,
+                         a, set_out, exam);
+    }
+    hipLaunchKernelGGL(bfs_decide_kernel, dim3(1), dim3(kBlock), 0, st, s->ctl, set_out, s->qc,
+                       exam, s->lv, s->h_flag);
+    GG_HIP(hipGetLastError());
+  }
+  GG_HIP(hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, st));
+  const double ms = tm.stop();
+  const BfsCtl &c = *s->h_ctl;
+  if (h_frontier && cap > 0) {
+    const uint32_t levels = std::min<uint32_t>(c.iter, (uint32_t)cap);
+    if (levels) {
+      GG_HIP(hipMemcpyAsync(h_frontier, s->lv, (size_t)levels * 4, hipMemcpyDeviceToHost, st));
+      GG_HIP(hipStreamSynchronize(st));
+    }
+  }
+  gg_run_stats r;
+  memset(&r, 0, sizeof(r));
+  r.iterations = c.iter;
+  r.push_iterations = c.push_iters;
+  r.edges_examined = c.examined;
+  r.m_reach = c.m_reach;
+  r.reached = c.reached;
+  r.device_ms = ms;
+  for (uint32_t i = 0; i < std::min<uint32_t>(c.iter, 64); i++) {
+    r.modes[i] = c.modes[i];
+    r.it_active[i] = c.active[i];
+    r.it_new[i] = c.fresh[i];
+    r.it_examined[i] = c.exam[i];
+    r.it_unvisited[i] = c.unvisited[i];
+  }
+  put_stats(stats, r);
+  return GG_OK;
+}

@@ -55,10 +55,11 @@ struct Options {
   int64_t push_small_max = 64;   // in-wave load-balanced bin
   int64_t push_big_max = 4096;   // one wave per vertex; above: shared by the grid
   int64_t pull_light_max = 1024; // in-group; above: heavy chunks (fixed at build)
-  int64_t pull_serial_max = 32;  // BFS pull: lane-serial scan
+  int64_t pull_serial_max = 128; // BFS pull: lane-serial scan (4 probes in flight)
   int64_t huge_grid_waves = 4096;
   int64_t count_examined = 0;    // count adjacency entries read (roofline model)
   int64_t bc_level_masks = 1;    // batched BC: keep per-level root masks (else depth rows)
+  int64_t bfs_pull_waves = 8192; // BFS pull grids: waves (0 = one per group / chunk)
 };
 Options &options();
 
@@ -170,6 +171,7 @@ struct Work {
   uint64_t *ms_lmask = nullptr;        // per-level root masks, V each (BC backward)
   size_t ms_lmask_cap = 0;
   size_t bytes = 0;                    // device bytes held (fixed part)
+  void *bfs = nullptr;         // device-driven BFS loop state (bfs.hip)
   Counters *h_cnt = nullptr;   // pinned host copy
   unsigned long long *h_exam = nullptr;
   std::vector<void *> allocations;
@@ -177,6 +179,7 @@ struct Work {
 
 int work_get(gg_graph *g, Work **w);
 void work_free(Work *w);
+void bfs_state_free(void *state);
 // Device bytes of one traversal context that runs batched queries of `kind`
 // (GG_QUERY_*) on g, counting the first 64 stored BFS levels.
 size_t work_footprint(const gg_graph *g, int kind);

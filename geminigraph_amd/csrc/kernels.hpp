@@ -133,10 +133,129 @@ __device__ __forceinline__ bool activate(uint64_t *bits, uint32_t v) {
   return !(old & bit);
 }
 
+// ------------------------------------------------------------ launch sizes
+inline unsigned grid1(uint64_t n, unsigned cap = 1u << 16) {
+  uint64_t g = (n + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+inline unsigned grid_waves(uint64_t waves) {
+  uint64_t g = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+// ----------------------------------------------------------- event timing
+struct DeviceTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+  DeviceTimer() {
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    hipEventRecord(a, stream());
+  }
+  double stop() {
+    hipEventRecord(b, stream());
+    hipEventSynchronize(b);
+    float ms = 0;
+    hipEventElapsedTime(&ms, a, b);
+    return ms;
+  }
+  ~DeviceTimer() {
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+  }
+};
+
+inline int check_graph(gg_graph *g) {
+  if (!g) return fail(GG_EINVAL, "null graph");
+  GG_TRY(ensure_init());
+  return GG_OK;
+}
+
+inline void put_stats(gg_run_stats *dst, const gg_run_stats &src) {
+  if (dst) *dst = src;
+}
+
 // ------------------------------------------------------------- compaction
 // Bitmap -> three queues binned by bin_deg (<= small_max, <= big_max, above),
 // plus popcount and degree sum.  Optionally ORs the words into `visited`
 // (process_vertices(visited->set_bit) over active_out, bfs.cpp:79-85).
+__device__ __forceinline__ void compact_words(const uint64_t *bits, size_t words,
+                                                   const uint32_t *bin_deg, uint32_t small_max,
+                                                   uint32_t big_max, uint32_t *q0, uint32_t *q1,
+                                                   uint32_t *q2, Counters *cnt, uint64_t *visited,
+                                                   uint64_t *clear) {
+  __shared__ uint32_t s_tot[3][kWavesPerBlock];
+  __shared__ unsigned long long s_edges[kWavesPerBlock];
+  __shared__ uint32_t s_base[3];
+  const int lane = lane_id(), wv = wave_in_block();
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  for (size_t base = (size_t)blockIdx.x * kBlock; base < words; base += stride) {
+    size_t wi = base + threadIdx.x;
+    uint64_t word = wi < words ? bits[wi] : 0;
+    if (visited && word) visited[wi] |= word;
+    if (clear && wi < words) clear[wi] = 0;
+    uint32_t c0 = 0, c1 = 0, c2 = 0;
+    uint64_t es = 0;
+    for (uint64_t x = word; x; x &= x - 1) {
+      uint32_t v = (uint32_t)(wi * 64 + (size_t)__builtin_ctzll(x));
+      uint32_t d = bin_deg[v];
+      es += d;
+      if (d <= small_max)
+        c0++;
+      else if (d <= big_max)
+        c1++;
+      else
+        c2++;
+    }
+    uint32_t t0, t1, t2;
+    uint32_t e0 = wave_excl_scan(c0, &t0);
+    uint32_t e1 = wave_excl_scan(c1, &t1);
+    uint32_t e2 = wave_excl_scan(c2, &t2);
+    uint64_t ws = wave_sum_u64(es);
+    if (lane == 0) {
+      s_tot[0][wv] = t0;
+      s_tot[1][wv] = t1;
+      s_tot[2][wv] = t2;
+      s_edges[wv] = ws;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t b0 = 0, b1 = 0, b2 = 0;
+      unsigned long long be = 0;
+      for (int i = 0; i < kWavesPerBlock; i++) {
+        b0 += s_tot[0][i];
+        b1 += s_tot[1][i];
+        b2 += s_tot[2][i];
+        be += s_edges[i];
+      }
+      s_base[0] = b0 ? atomicAdd(&cnt->n[0], b0) : 0;
+      s_base[1] = b1 ? atomicAdd(&cnt->n[1], b1) : 0;
+      s_base[2] = b2 ? atomicAdd(&cnt->n[2], b2) : 0;
+      if (b0 + b1 + b2) {
+        atomicAdd(&cnt->total, b0 + b1 + b2);
+        atomicAdd((unsigned long long *)&cnt->edges, be);
+      }
+    }
+    __syncthreads();
+    uint32_t o0 = s_base[0] + e0, o1 = s_base[1] + e1, o2 = s_base[2] + e2;
+    for (int i = 0; i < wv; i++) {
+      o0 += s_tot[0][i];
+      o1 += s_tot[1][i];
+      o2 += s_tot[2][i];
+    }
+    for (uint64_t x = word; x; x &= x - 1) {
+      uint32_t v = (uint32_t)(wi * 64 + (size_t)__builtin_ctzll(x));
+      uint32_t d = bin_deg[v];
+      if (d <= small_max)
+        q0[o0++] = v;
+      else if (d <= big_max)
+        q1[o1++] = v;
+      else
+        q2[o2++] = v;
+    }
+    __syncthreads();
+  }
+}
 __global__ __launch_bounds__(kBlock) void compact_kernel(const uint64_t *bits, size_t words,
                                                          const uint32_t *bin_deg,
                                                          uint32_t small_max, uint32_t big_max,

@@ -198,6 +198,7 @@ static int64_t *option_slot(const char *name) {
   if (!strcmp(name, "huge_grid_waves")) return &o.huge_grid_waves;
   if (!strcmp(name, "count_examined")) return &o.count_examined;
   if (!strcmp(name, "bc_level_masks")) return &o.bc_level_masks;
+  if (!strcmp(name, "bfs_pull_waves")) return &o.bfs_pull_waves;
   return nullptr;
 }
 
@@ -208,7 +209,9 @@ int gg_set_option(const char *name, int64_t value) {
     return fail(GG_EINVAL, "force_mode must be 0, 1 or 2");
   const bool flag = !strcmp(name, "count_examined") || !strcmp(name, "bc_level_masks");
   if (flag && (value < 0 || value > 1)) return fail(GG_EINVAL, std::string(name) + " must be 0 or 1");
-  if (strcmp(name, "force_mode") && !flag && value <= 0)
+  if (!strcmp(name, "bfs_pull_waves") && (value < 0 || value > (1 << 24)))
+    return fail(GG_EINVAL, "bfs_pull_waves must be in [0, 2^24]");
+  if (strcmp(name, "force_mode") && strcmp(name, "bfs_pull_waves") && !flag && value <= 0)
     return fail(GG_EINVAL, std::string(name) + " must be positive");
   if (!strcmp(name, "push_small_max") && value > 64 * 64)
     return fail(GG_EINVAL, "push_small_max must be <= 4096");
