@@ -630,6 +630,12 @@ __global__ void bc_final_kernel(double *dep, const double *inv, uint32_t V) {
 }
 
 // ========================================================= batched BFS (64)
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t x, int j) {
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, j);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), j);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Bit-parallel multi-source BFS: vis[v] and f[v] are 64-bit masks, bit i for
 // root i.  A level is the plain BFS level of every root at once: push ORs a
 // source's frontier mask into its out-neighbours, pull ORs the in-neighbours'
@@ -700,8 +706,13 @@ __global__ __launch_bounds__(kBlock) void ms_pull_group_kernel(MsPullArgs a) {
   bool serial = cand && deg <= a.serial_max;
   uint64_t acc = 0;
   if (serial) {
-    for (uint64_t e = b; e < e1; e++) {
-      acc |= a.fcur[stream_load(&a.nbr[e])];
+    // four neighbours and their masks in flight per step
+    for (uint64_t e = b; e < e1; e += 4) {
+      uint32_t u[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) u[t] = e + t < e1 ? stream_load(&a.nbr[e + t]) : 0;
+#pragma unroll
+      for (int t = 0; t < 4; t++) acc |= e + t < e1 ? a.fcur[u[t]] : 0ULL;
       if ((acc & need) == need) break;
     }
   }
@@ -745,6 +756,7 @@ __global__ __launch_bounds__(kBlock) void ms_apply_kernel(const uint64_t *active
       fnext[v] = nw;
       vis[v] |= nw;
     }
+    if (!depth) continue;  // depths come from the per-level masks
     for (uint64_t x = act; x; x &= x - 1) {
       int k = __builtin_ctzll(x);
       uint64_t m = shfl_u64(nw, k);
@@ -754,12 +766,45 @@ __global__ __launch_bounds__(kBlock) void ms_apply_kernel(const uint64_t *active
 }
 __global__ void ms_seed_kernel(const uint32_t *roots, int n, uint64_t *vis, uint64_t *f,
                                uint64_t *bits, uint32_t *depth) {
-  for (int i = 0; i < n; i++) {  // one thread: roots may repeat
-    uint32_t r = roots[i];
-    vis[r] |= 1ULL << i;
-    f[r] |= 1ULL << i;
-    bits[r >> 6] |= 1ULL << (r & 63);
-    depth[(size_t)r * 64 + i] = 0;
+  const int i = threadIdx.x;  // one thread per root; roots may repeat
+  if (i >= n) return;
+  const uint32_t r = roots[i];
+  atomicOr((unsigned long long *)&vis[r], 1ULL << i);
+  atomicOr((unsigned long long *)&f[r], 1ULL << i);
+  atomicOr((unsigned long long *)&bits[r >> 6], 1ULL << (r & 63));
+  if (depth) depth[(size_t)r * 64 + i] = 0;
+}
+
+// Depth rows from the per-level root masks, one wave per 64 vertices: the
+// lanes load the 64 vertices' mask of each level (coalesced), then for each
+// vertex lane i reads the vertex's masks and keeps the level whose mask has
+// bit i (kUnset if none).  Row of vertex v goes to row_of[v] (null: v).
+__global__ __launch_bounds__(kBlock) void ms_rows_from_masks_kernel(const uint64_t *lmask,
+                                                                    uint32_t V, uint32_t levels,
+                                                                    const uint32_t *row_of,
+                                                                    uint32_t *depth) {
+  const int lane = lane_id();
+  const uint32_t W = gridDim.x * kWavesPerBlock;
+  for (uint32_t g = global_wave(); (size_t)g * 64 < V; g += W) {
+    const uint32_t v = g * 64 + lane;
+    uint32_t d[64];
+#pragma unroll
+    for (int j = 0; j < 64; j++) d[j] = kUnset;
+    uint64_t next = v < V && levels ? lmask[v] : 0;
+    for (uint32_t L = 0; L < levels; L++) {
+      const uint64_t m = next;
+      next = v < V && L + 1 < levels ? lmask[(size_t)(L + 1) * V + v] : 0;
+      if (!__ballot(m != 0)) continue;
+#pragma unroll
+      for (int j = 0; j < 64; j++)
+        if ((readlane_u64(m, j) >> lane) & 1ULL) d[j] = L;
+    }
+    const uint32_t r = v < V ? (row_of ? row_of[v] : v) : 0;
+#pragma unroll
+    for (int j = 0; j < 64; j++) {
+      const uint32_t rj = (uint32_t)__builtin_amdgcn_readlane((int)r, j);
+      if (g * 64 + j < V) depth[(size_t)rj * 64 + lane] = d[j];
+    }
   }
 }
 // sum over roots of the out-degrees of reached vertices
@@ -790,12 +835,6 @@ struct LaneArgs {
   uint32_t n;
   uint32_t light_max;   // longer lists go to lane_heavy_kernel
 };
-
-__device__ __forceinline__ uint64_t readlane_u64(uint64_t x, int j) {
-  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, j);
-  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), j);
-  return ((uint64_t)hi << 32) | lo;
-}
 
 // One adjacency range of vertex v, 64 roots per wave (lane = root).  The
 // lanes first load 64 arcs at once -- neighbour, weight and the neighbour's
@@ -1057,9 +1096,8 @@ static int batch_roots(const gg_graph *g, const uint32_t *h_roots, int n_roots, 
   return GG_OK;
 }
 
-// Store of per-level root masks (BC backward): initial levels, size limit.
+// Store of per-level root masks (batched BFS/BC): initial levels.
 constexpr size_t kMaskLevels0 = 16;
-constexpr size_t kMaskStoreMax = size_t(4) << 30;
 
 size_t work_footprint(const gg_graph *g, int kind) {
   // mirrors work_get and ms_alloc with results kept in the context (+ the
@@ -1069,7 +1107,8 @@ size_t work_footprint(const gg_graph *g, int kind) {
   size_t b = 2 * (words * 8 + 6 * V * 4 + sizeof(Counters)) + 2 * words * 8 + V * 8 +
              (g->reordered ? 2 * V * 8 : 0) + nh * 8 + 1024 * 8;
   b += 3 * (V * 8 + 8) + 64 * 4 + V * 64 * 4;
-  if (kind == GG_QUERY_BC) b += 2 * V * 64 * 8 + 64 * words * 8 + kMaskLevels0 * (V - 1) * 8;
+  b += kMaskLevels0 * (V - 1) * 8;  // per-level root masks
+  if (kind == GG_QUERY_BC) b += 2 * V * 64 * 8 + 64 * words * 8;
   return b;
 }
 
@@ -1108,53 +1147,73 @@ static int ms_store_level(Work *w, const uint64_t *bits, uint32_t level) {
   return GG_OK;
 }
 
-// Keeps one level's per-vertex root masks (BC backward selects exactly the
-// roots for which an arc joins consecutive levels).  Gives up (*ok = false)
-// when the store would exceed kMaskStoreMax, e.g. on long-diameter graphs;
-// the backward pass then tests the depth rows instead.
-static int ms_store_mask(Work *w, const uint64_t *mask, uint32_t level, bool *ok) {
-  if (!*ok) return GG_OK;
+// Room for level `level` in the store of per-level root masks (grown by
+// doubling).  False when the store would exceed kMaskStoreMax (e.g. a
+// long-diameter graph): the caller then falls back to depth rows.
+static int ms_mask_room(Work *w, uint32_t level, bool *ok) {
   const size_t row = (size_t)w->V * 8;
-  if (level >= w->ms_lmask_cap) {
-    size_t cap = std::max<size_t>(kMaskLevels0, 2 * (size_t)level + 2);
-    if (cap * row > kMaskStoreMax) {
-      *ok = false;
-      return GG_OK;
-    }
-    uint64_t *p = nullptr;
-    GG_TRY(device_alloc((void **)&p, cap * row));
-    if (w->ms_lmask)
-      GG_HIP(hipMemcpyAsync(p, w->ms_lmask, w->ms_lmask_cap * row, hipMemcpyDeviceToDevice,
-                            stream()));
-    GG_HIP(hipStreamSynchronize(stream()));
-    device_free(w->ms_lmask);
-    w->ms_lmask = p;
-    w->ms_lmask_cap = cap;
-  }
-  GG_HIP(hipMemcpyAsync(w->ms_lmask + (size_t)level * w->V, mask, row, hipMemcpyDeviceToDevice,
-                        stream()));
+  const size_t limit = (size_t)options().level_mask_bytes;
+  const size_t need = (size_t)level + 1;
+  *ok = need * row <= limit;
+  if (!*ok || level < w->ms_lmask_cap) return GG_OK;
+  // grow (never shrink): the old capacity is <= level < the new one
+  const size_t cap = std::min(std::max<size_t>(kMaskLevels0, 2 * need), limit / row);
+  uint64_t *p = nullptr;
+  GG_TRY(device_alloc((void **)&p, cap * row));
+  if (w->ms_lmask)
+    GG_HIP(hipMemcpyAsync(p, w->ms_lmask, w->ms_lmask_cap * row, hipMemcpyDeviceToDevice,
+                          stream()));
+  GG_HIP(hipStreamSynchronize(stream()));
+  device_free(w->ms_lmask);
+  w->ms_lmask = p;
+  w->ms_lmask_cap = cap;
   return GG_OK;
 }
 
+static int ms_rows_from_masks(Work *w, uint32_t levels, const uint32_t *row_of, uint32_t *depth) {
+  const uint32_t V = w->V;
+  hipLaunchKernelGGL(ms_rows_from_masks_kernel, dim3(grid_waves(std::min<uint64_t>((V + 63) / 64, 16384))),
+                     dim3(kBlock), 0, stream(), w->ms_lmask, V, levels, row_of, depth);
+  GG_HIP(hipGetLastError());
+  return GG_OK;
+}
+
+// What a multi-root BFS run leaves: the number of levels and whether their
+// root masks are in the store (else the depth rows were written).
+struct MsLevels {
+  uint32_t levels = 0;
+  bool masks = false;
+};
+
 // Multi-root BFS levels (bit-parallel masks); hook(level, fprev, fnew) runs
-// after every non-empty level with the level's frontier compacted in en.fin().
+// after every non-empty level with the level's frontier compacted in en.fin()
+// (and for level 0 when store_levels).  With level masks on, each level's
+// root masks go to the store and `depth` (rows in internal order) is left
+// alone; if the store runs out of room, the rows of the levels so far are
+// built from it and the remaining levels write rows directly.
 template <class Hook>
 static int ms_bfs_run(gg_graph *g, Work *w, Engine &en, const uint32_t *roots, int n_roots,
-                      uint32_t *depth, Hook hook, bool store_levels = false) {
+                      uint32_t *depth, Hook hook, bool store_levels, MsLevels *res) {
   const Dir &out = out_dir(g), &in = in_dir(g);
   uint32_t V = g->vertices;
-  GG_HIP(hipMemsetAsync(depth, 0xFF, (size_t)V * 64 * 4, stream()));
+  bool masks = options().level_masks != 0;
+  if (masks) GG_TRY(ms_mask_room(w, 0, &masks));
+  if (!masks) GG_HIP(hipMemsetAsync(depth, 0xFF, (size_t)V * 64 * 4, stream()));
   GG_HIP(hipMemsetAsync(w->ms_vis, 0, (size_t)V * 8, stream()));
   GG_HIP(hipMemsetAsync(w->ms_f[0], 0, (size_t)V * 8, stream()));
   GG_HIP(hipMemcpyAsync(w->ms_roots, roots, 4 * (size_t)n_roots, hipMemcpyHostToDevice, stream()));
   GG_TRY(en.clear_bits(en.fin()));
-  hipLaunchKernelGGL(ms_seed_kernel, dim3(1), dim3(1), 0, stream(), w->ms_roots, n_roots, w->ms_vis,
-                     w->ms_f[0], en.fin().bits, depth);
+  hipLaunchKernelGGL(ms_seed_kernel, dim3(1), dim3(64), 0, stream(), w->ms_roots, n_roots,
+                     w->ms_vis, w->ms_f[0], en.fin().bits, masks ? nullptr : depth);
   GG_TRY(en.compact_in(out.degree));
+  if (masks)
+    GG_HIP(hipMemcpyAsync(w->ms_lmask, w->ms_f[0], (size_t)V * 8, hipMemcpyDeviceToDevice,
+                          stream()));
   if (store_levels) GG_TRY(hook(0, nullptr, w->ms_f[0]));
   const uint64_t full = n_roots == 64 ? ~0ULL : ((1ULL << n_roots) - 1ULL);
   int fc = 0;
-  for (uint32_t level = 1; en.in.total > 0; level++) {
+  uint32_t level = 1;
+  for (; en.in.total > 0; level++) {
     uint64_t *fcur = w->ms_f[fc], *fnext = w->ms_f[fc ^ 1];
     GG_HIP(hipMemsetAsync(fnext, 0, (size_t)V * 8, stream()));
     bool sp = en.sparse(en.in.edges);
@@ -1179,16 +1238,29 @@ static int ms_bfs_run(gg_graph *g, Work *w, Engine &en, const uint32_t *roots, i
       }
       GG_TRY(en.launch_pull_heavy(in, op, "ms_pull_heavy"));
     }
+    if (masks) {
+      bool room;
+      GG_TRY(ms_mask_room(w, level, &room));
+      if (!room) {  // rows of levels [0, level) from the store, then rows directly
+        GG_TRY(ms_rows_from_masks(w, level, nullptr, depth));
+        masks = false;
+      }
+    }
     {
       ProfScope p("ms_apply");
       hipLaunchKernelGGL(ms_apply_kernel, dim3(grid_waves(std::min<size_t>(w->words, 1u << 16))),
                          dim3(kBlock), 0, stream(), en.fout().bits, w->words, w->ms_vis, fnext,
-                         depth, level);
+                         masks ? nullptr : depth, level);
     }
+    if (masks)
+      GG_HIP(hipMemcpyAsync(w->ms_lmask + (size_t)level * V, fnext, (size_t)V * 8,
+                            hipMemcpyDeviceToDevice, stream()));
     GG_TRY(en.finish_iteration(sp, out.degree, nullptr));
     if (en.in.total > 0) GG_TRY(hook(level, fcur, fnext));
     fc ^= 1;
   }
+  res->levels = level;  // levels 0 .. level-1 hold vertices (the last one may be empty)
+  res->masks = masks;
   return GG_OK;
 }
 
@@ -1527,12 +1599,17 @@ int bfs_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, uint32_t *d
   uint32_t V = g->vertices;
   DeviceTimer tm;
   uint32_t *depth = g->reordered || keep ? (uint32_t *)w->ms_tmp : d_depth;
-  GG_TRY(ms_bfs_run(g, w, en, roots, n_roots, depth,
-                    [](uint32_t, const uint64_t *, const uint64_t *) { return GG_OK; }));
+  MsLevels lv;
+  GG_TRY(ms_bfs_run(
+      g, w, en, roots, n_roots, depth,
+      [](uint32_t, const uint64_t *, const uint64_t *) { return GG_OK; }, false, &lv));
   uint64_t m = 0;
   GG_TRY(ms_reach(g, w, &m));
   if (rows) *rows = depth;
-  if (g->reordered && !keep)
+  if (lv.masks)  // every row once, in internal order (kept) or in original ids
+    GG_TRY(ms_rows_from_masks(w, lv.levels, keep || !g->reordered ? nullptr : g->iperm,
+                              keep ? depth : d_depth));
+  else if (g->reordered && !keep)
     hipLaunchKernelGGL(unpermute_rows_kernel<uint32_t>, dim3(grid1((uint64_t)V * 64)), dim3(kBlock),
                        0, stream(), depth, d_depth, g->iperm, V);
   en.st.device_ms = tm.stop();
@@ -1629,10 +1706,8 @@ int bc_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_de
                      sigma);
   // forward: the multi-root BFS levels, with sigma accumulated per level
   uint32_t levels = 0;
-  bool masks = options().bc_level_masks != 0;
   auto hook = [&](uint32_t level, const uint64_t *fprev, const uint64_t *fnew) -> int {
     GG_TRY(ms_store_level(w, en.fin().bits, level));
-    GG_TRY(ms_store_mask(w, fnew, level, &masks));
     levels = level;
     if (level == 0) return GG_OK;
     MsbcSigma op{sigma, fprev, fnew};
@@ -1651,7 +1726,9 @@ int bc_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_de
     }
     return GG_OK;
   };
-  GG_TRY(ms_bfs_run(g, w, en, roots, n_roots, depth, hook, true));
+  MsLevels lv;
+  GG_TRY(ms_bfs_run(g, w, en, roots, n_roots, depth, hook, true, &lv));
+  const bool masks = lv.masks;
   hipLaunchKernelGGL(msbc_inv_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(), sigma, dd, n);
   // backward: level L-1 pulls D from level L along out-edges (the reference
   // pushes along the transposed graph, bc.cpp:121-168)

@@ -58,7 +58,8 @@ struct Options {
   int64_t pull_serial_max = 128; // BFS pull: lane-serial scan (4 probes in flight)
   int64_t huge_grid_waves = 4096;
   int64_t count_examined = 0;    // count adjacency entries read (roofline model)
-  int64_t bc_level_masks = 1;    // batched BC: keep per-level root masks (else depth rows)
+  int64_t level_masks = 1;       // batched BFS/BC: per-level root masks (else depth rows)
+  int64_t level_mask_bytes = int64_t(4) << 30;  // size limit of that store
   int64_t bfs_pull_waves = 8192; // BFS pull grids: waves (0 = one per group / chunk)
 };
 Options &options();

@@ -197,7 +197,8 @@ static int64_t *option_slot(const char *name) {
   if (!strcmp(name, "pull_serial_max")) return &o.pull_serial_max;
   if (!strcmp(name, "huge_grid_waves")) return &o.huge_grid_waves;
   if (!strcmp(name, "count_examined")) return &o.count_examined;
-  if (!strcmp(name, "bc_level_masks")) return &o.bc_level_masks;
+  if (!strcmp(name, "level_masks")) return &o.level_masks;
+  if (!strcmp(name, "level_mask_bytes")) return &o.level_mask_bytes;
   if (!strcmp(name, "bfs_pull_waves")) return &o.bfs_pull_waves;
   return nullptr;
 }
@@ -207,7 +208,7 @@ int gg_set_option(const char *name, int64_t value) {
   if (!s) return fail(GG_EINVAL, std::string("unknown option: ") + (name ? name : "(null)"));
   if (!strcmp(name, "force_mode") && (value < 0 || value > 2))
     return fail(GG_EINVAL, "force_mode must be 0, 1 or 2");
-  const bool flag = !strcmp(name, "count_examined") || !strcmp(name, "bc_level_masks");
+  const bool flag = !strcmp(name, "count_examined") || !strcmp(name, "level_masks");
   if (flag && (value < 0 || value > 1)) return fail(GG_EINVAL, std::string(name) + " must be 0 or 1");
   if (!strcmp(name, "bfs_pull_waves") && (value < 0 || value > (1 << 24)))
     return fail(GG_EINVAL, "bfs_pull_waves must be in [0, 2^24]");

@@ -69,8 +69,10 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *   "pull_serial_max"  BFS pull: largest in-degree scanned by a single lane
  *   "huge_grid_waves"  waves sharing the adjacency of the largest-degree bin
  *   "count_examined"   1 = count adjacency entries read (gg_run_stats)
- *   "bc_level_masks"   batched BC: 1 = keep per-level root masks (default),
- *                      0 = select backward arcs from the depth rows */
+ *   "level_masks"      batched BFS/BC: 1 = keep per-level root masks
+ *                      (default; depth rows are built once at the end and
+ *                      BC selects backward arcs exactly), 0 = write depth
+ *                      rows level by level */
 int gg_set_option(const char *name, int64_t value);
 int gg_get_option(const char *name, int64_t *value);
 

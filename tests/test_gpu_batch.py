@@ -86,9 +86,27 @@ def test_bc_batch_depth_row_fallback(gg):
     g = gg.Graph.build(e, V, reorder=True)
     o = oracle.Graph(e, V)
     roots = roots_of(o.out_degree(), 17, 83)
-    with gg.options(bc_level_masks=0):
+    with gg.options(level_masks=0):
         dep, inv, _ = gg.bc_batch(g, roots)
     for i, r in enumerate(roots):
         rdep, rinv, _ = o.bc(r)
         assert_rel(inv[i], rinv)
         assert_rel(dep[i], rdep)
+
+
+@pytest.mark.parametrize("case", ["rows", "overflow"])
+def test_bfs_batch_depth_row_paths(gg, case):
+    # depth rows written level by level (level masks off), and a mask store
+    # too small for the traversal (rows of the stored levels are built from
+    # it, then written directly): same depths as the oracle
+    n = 6000
+    pairs = [(i, i + 1) for i in range(n - 1)] + [(0, 3000), (3000, 5999), (17, 4)]
+    e = oracle.edges_from_pairs(pairs)
+    g = gg.Graph.build(e, n, reorder=True)
+    o = oracle.Graph(e, n)
+    roots = [0, 5, 17, 2999, 3000, 5998]
+    opts = {"level_masks": 0} if case == "rows" else {"level_mask_bytes": 40 * n * 8}
+    with gg.options(**opts):
+        depths, _ = gg.bfs_batch(g, roots)
+    for i, r in enumerate(roots):
+        assert np.array_equal(depths[i], o.bfs(r)[1]), (case, r)
