@@ -158,6 +158,10 @@ void work_free(Work *w) {
   device_free(w->dsum);
   device_free(w->ms_vis);
   device_free(w->ms_lmask);
+  device_free(w->pr_vals);
+  device_free(w->pr_cold);
+  device_free(w->pr_hot);
+  device_free(w->hpart);
   bfs_state_free(w->bfs);
   device_free(w->ms_f[0]);
   device_free(w->ms_f[1]);
@@ -285,8 +289,10 @@ struct Engine {
     return GG_OK;
   }
   template <class Op>
-  int launch_pull_group(const Dir &d, const Op &op, double *dacc, const char *name = "pull") {
+  int launch_pull_group(const Dir &d, const Op &op, double *dacc, const char *name = "pull",
+                        const uint32_t *count = nullptr) {
     PullArgs a;
+    a.count = count;
     a.index = d.index;
     a.nbr = d.nbr;
     a.w = d.w;
@@ -304,49 +310,26 @@ struct Engine {
   template <class Op>
   int launch_pull_heavy(const Dir &d, const Op &op, const char *name = "pull_heavy") {
     if (!d.n_chunks) return GG_OK;
+    if (w->hpart_cap < d.n_chunks) {
+      device_free(w->hpart);
+      w->hpart = nullptr;
+      w->hpart_cap = 0;
+      GG_TRY(device_alloc(&w->hpart, (size_t)d.n_chunks * 8));
+      w->hpart_cap = d.n_chunks;
+    }
     HeavyArgs a{d.nbr, d.w, d.chunk_v, d.chunk_h, d.chunk_beg, d.chunk_end, d.n_chunks,
-                fout().bits, exam};
+                fout().bits, exam, w->hpart};
     ProfScope p(name);
-    hipLaunchKernelGGL(pull_heavy_kernel<Op>, dim3(grid_waves(d.n_chunks)), dim3(kBlock), 0,
-                       stream(), a, op);
+    const uint64_t hw = options().heavy_waves;
+    hipLaunchKernelGGL(pull_heavy_kernel<Op>,
+                       dim3(grid_waves(hw ? std::min<uint64_t>(d.n_chunks, hw) : d.n_chunks)),
+                       dim3(kBlock), 0, stream(), a, op);
+    hipLaunchKernelGGL(heavy_reduce_kernel<Op>, dim3(grid_waves(d.n_heavy)), dim3(kBlock), 0,
+                       stream(), d.heavy_v, d.heavy_first, d.n_heavy, w->hpart, fout().bits, op);
     return GG_OK;
   }
 };
 
-struct NoAfter {
-  __device__ void after(bool, uint32_t) const {}
-};
-
-// ================================================================ PageRank
-// toolkits/pagerank.cpp:24-129
-struct PrPull : NoAfter {
-  using T = double;
-  const double *curr;
-  double *next;
-  const uint32_t *outdeg;
-  int last;
-  __device__ double identity() const { return 0.0; }
-  __device__ bool dst_active(uint32_t) const { return true; }
-  __device__ double value(uint32_t src, float) const { return curr[src]; }
-  __device__ double combine(double a, double b) const { return a + b; }
-  __device__ bool finish(uint32_t v, double sum, uint32_t, double &dsum) const {
-    // next = 1 - d + d * next, divided by the out-degree except in the last
-    // iteration (pagerank.cpp:77-97)
-    const double d = 0.85;
-    double nx = 1 - d + d * sum;
-    if (!last) {
-      uint32_t od = outdeg[v];
-      if (od > 0) {
-        nx /= od;
-        dsum += fabs(nx - curr[v]) * od;
-      } else {
-        dsum += fabs(nx - curr[v]);
-      }
-    }
-    next[v] = nx;
-    return false;
-  }
-};
 struct PrHeavy : NoAfter {
   using T = double;
   const double *curr;
@@ -1352,19 +1335,26 @@ int gg_pagerank(gg_graph *g, int iterations, double *d_curr_out, double *h_delta
     w->dsum = s;
     w->dacc_cap = cap;
   }
+  // relabelled graphs: hot arcs and cold arcs both streamed (prplan.hip)
+  const PrPlan *plan = nullptr;
+  GG_TRY(pr_plan_get(g, &plan));
   hipLaunchKernelGGL(pr_init_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), curr, out.degree, V);
   GG_HIP(hipMemsetAsync(w->dacc, 0, sizeof(double) * kScalarStride * (iterations + 1), stream()));
   for (int it = 0; it < iterations; it++) {
     // active = every vertex, so active_edges = |E| and the step is dense
     // (core/graph.hpp:1509).
     double *acc = w->dacc + (size_t)(it + 1) * kScalarStride;
-    PrPull op{{}, curr, next, out.degree, it == iterations - 1};
-    GG_TRY(en.launch_pull_group(in, op, acc, "pr_pull"));
-    PrHeavy hop{{}, curr, w->hacc};
-    GG_TRY(en.launch_pull_heavy(in, hop, "pr_heavy"));
-    if (in.n_heavy)
-      hipLaunchKernelGGL(pr_heavy_finish_kernel, dim3((in.n_heavy + kBlock - 1) / kBlock),
-                         dim3(kBlock), 0, stream(), in.heavy_v, in.n_heavy, w->hacc, op, acc);
+    PrPull op{{}, curr, next, out.degree, it == iterations - 1, nullptr};
+    if (plan) {
+      GG_TRY(pr_plan_iteration(g, plan, w, op, acc));
+    } else {
+      GG_TRY(en.launch_pull_group(in, op, acc, "pr_pull"));
+      PrHeavy hop{{}, curr, w->hacc};
+      GG_TRY(en.launch_pull_heavy(in, hop, "pr_heavy"));
+      if (in.n_heavy)
+        hipLaunchKernelGGL(pr_heavy_finish_kernel, dim3((in.n_heavy + kBlock - 1) / kBlock),
+                           dim3(kBlock), 0, stream(), in.heavy_v, in.n_heavy, w->hacc, op, acc);
+    }
     if (en.st.iterations < 64) en.st.modes[en.st.iterations] = 1;
     en.st.iterations++;
     en.st.edges_examined += g->arcs;

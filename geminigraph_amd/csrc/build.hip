@@ -256,12 +256,13 @@ static int build_chunks(gg_graph *g, Dir &d) {
   GG_HIP(hipMemcpyAsync(idx.data(), d.index, sizeof(uint64_t) * ((size_t)V + 1),
                         hipMemcpyDeviceToHost, stream()));
   GG_HIP(hipStreamSynchronize(stream()));
-  std::vector<uint32_t> cv, ch, hv;
+  std::vector<uint32_t> cv, ch, hv, hf;
   std::vector<uint64_t> cb, ce;
   for (uint32_t v = 0; v < V; v++) {
     if (deg[v] <= d.light_max) continue;
     uint32_t h = (uint32_t)hv.size();
     hv.push_back(v);
+    hf.push_back((uint32_t)cv.size());
     for (uint64_t b = idx[v]; b < idx[v + 1]; b += kChunkEdges) {
       cv.push_back(v);
       ch.push_back(h);
@@ -271,8 +272,11 @@ static int build_chunks(gg_graph *g, Dir &d) {
   }
   d.n_heavy = (uint32_t)hv.size();
   d.n_chunks = (uint32_t)cv.size();
+  hf.push_back((uint32_t)cv.size());
   size_t nh = std::max<size_t>(hv.size(), 1), nc = std::max<size_t>(cv.size(), 1);
   GG_TRY(graph_alloc(g, (void **)&d.heavy_v, nh * 4));
+  GG_TRY(graph_alloc(g, (void **)&d.heavy_first, hf.size() * 4));
+  GG_HIP(hipMemcpyAsync(d.heavy_first, hf.data(), hf.size() * 4, hipMemcpyHostToDevice, stream()));
   GG_TRY(graph_alloc(g, (void **)&d.chunk_v, nc * 4));
   GG_TRY(graph_alloc(g, (void **)&d.chunk_h, nc * 4));
   GG_TRY(graph_alloc(g, (void **)&d.chunk_beg, nc * 8));
@@ -552,6 +556,7 @@ int gg_graph_destroy(gg_graph *g) {
   if (!g) return GG_OK;
   for (Work *w : g->works)
     if (w) work_free(w);
+  pr_plan_free(g->pr_plan);
   for (void *p : g->allocations) device_free(p);
   delete g;
   return GG_OK;

@@ -60,6 +60,8 @@ struct Options {
   int64_t count_examined = 0;    // count adjacency entries read (roofline model)
   int64_t level_masks = 1;       // batched BFS/BC: per-level root masks (else depth rows)
   int64_t level_mask_bytes = int64_t(4) << 30;  // size limit of that store
+  int64_t pr_hot_sources = 0;    // PageRank streamed plan: hot sources (0 = plain pull)
+  int64_t heavy_waves = 8192;    // dense heavy-chunk grids: waves (0 = one per chunk)
   int64_t bfs_pull_waves = 8192; // BFS pull grids: waves (0 = one per group / chunk)
 };
 Options &options();
@@ -97,8 +99,35 @@ struct Dir {
   uint64_t *chunk_end = nullptr;
   uint32_t n_heavy = 0;
   uint32_t *heavy_v = nullptr;     // heavy vertex list
+  uint32_t *heavy_first = nullptr; // n_heavy + 1: first chunk of each heavy vertex
   uint32_t *chunk_h = nullptr;     // heavy index of each chunk
   uint32_t light_max = 0;
+};
+
+// PageRank hot/cold plan of a relabelled graph (built on first use, see
+// prplan.hip).  Sources with internal id < H are hot: their contributions
+// fit in L2.  Hot arcs are packed into one array streamed in fixed chunks;
+// the out-arcs of the other sources are cold: each iteration streams their
+// contributions into per-destination-range bins, which workgroups sum in LDS.
+struct PrPlan {
+  int transposed = 0;          // orientation it was built for
+  uint32_t H = 0;              // hot sources: internal id < H
+  uint32_t nbins = 0;          // bins of kPrBinDst destinations
+  uint32_t nseg = 0;           // accumulate segments (<= kPrSegArcs arcs, one bin each)
+  uint64_t cold_base = 0;      // first cold out-arc (out.index[H])
+  uint64_t n_cold = 0;         // cold arcs
+  uint64_t n_hot = 0;          // hot arcs; the packed arrays are padded to whole chunks
+  uint32_t n_hchunks = 0;      // hot chunks of kPrHotChunk arcs
+  uint32_t *slot = nullptr;       // per cold out-arc: position in its bin's list
+  uint16_t *dst_local = nullptr;  // per bin-list position: destination - bin start
+  uint64_t *seg = nullptr;        // per segment: bin << 40 | first position (+ end in seg_end)
+  uint64_t *seg_end = nullptr;
+  uint8_t *bin_multi = nullptr;   // per bin: more than one segment
+  uint32_t *hot_src = nullptr;    // packed hot arcs (source), dst order
+  uint16_t *hot_dst = nullptr;    // destination - first destination of the chunk
+  uint32_t *hchunk_first = nullptr;  // first destination of each hot chunk
+  uint32_t xcd_g[9] = {};            // cold source groups of each XCD (balanced by arcs)
+  std::vector<void *> allocations;
 };
 
 }  // namespace gg
@@ -122,6 +151,7 @@ struct gg_graph {
   std::vector<void *> allocations;
   // traversal contexts owned by the graph, one per context slot (lazily sized)
   gg::Work *works[gg::kMaxSlots] = {};
+  gg::PrPlan *pr_plan = nullptr;
 };
 
 struct gg_subset {
@@ -157,6 +187,8 @@ struct Work {
   void *res_a = nullptr;       // V x 8 bytes results in internal order (reordered graphs)
   void *res_b = nullptr;       // V x 8 bytes
   double *hacc = nullptr;      // heavy accumulators (max of both directions)
+  void *hpart = nullptr;       // per-chunk partials of a dense heavy pass (8 bytes each)
+  uint64_t hpart_cap = 0;
   double *dacc = nullptr;      // per-iteration spread accumulators (kScalarStride each)
   double *dsum = nullptr;      // reduced per-iteration scalars
   int dacc_cap = 0;            // iterations the buffers hold
@@ -173,6 +205,10 @@ struct Work {
   size_t ms_lmask_cap = 0;
   size_t bytes = 0;                    // device bytes held (fixed part)
   void *bfs = nullptr;         // device-driven BFS loop state (bfs.hip)
+  double *pr_vals = nullptr;   // PageRank cold contributions, bin order
+  double *pr_cold = nullptr;   // PageRank cold sums per destination (kept at 0 between)
+  double *pr_hot = nullptr;    // PageRank hot sums per destination (kept at 0 between)
+  uint64_t pr_vals_cap = 0;
   Counters *h_cnt = nullptr;   // pinned host copy
   unsigned long long *h_exam = nullptr;
   std::vector<void *> allocations;
@@ -181,6 +217,15 @@ struct Work {
 int work_get(gg_graph *g, Work **w);
 void work_free(Work *w);
 void bfs_state_free(void *state);
+// PageRank plan (prplan.hip): *out = null when the graph gets the plain pull
+// (not relabelled, option pr_hot_sources = 0, or no cold arcs).
+int pr_plan_get(gg_graph *g, const PrPlan **out);
+void pr_plan_free(PrPlan *p);
+// One PageRank iteration on the plan: hot and cold sums, then
+// finish(v, hot + cold) of the PageRank op for every vertex.
+struct PrPull;
+int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull &op,
+                      double *dacc);
 // Device bytes of one traversal context that runs batched queries of `kind`
 // (GG_QUERY_*) on g, counting the first 64 stored BFS levels.
 size_t work_footprint(const gg_graph *g, int kind);

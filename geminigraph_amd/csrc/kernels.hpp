@@ -38,6 +38,8 @@ __device__ __forceinline__ uint32_t global_wave() {
 __device__ __forceinline__ bool test_bit(const uint64_t *bits, uint32_t v) {
   return (bits[v >> 6] >> (v & 63)) & 1ULL;
 }
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // Adjacency arrays are streamed once per sweep: load them non-temporally so
 // they do not evict the gathered vertex state (the hot sources) from L2.
 template <class T>
@@ -341,6 +343,42 @@ __global__ __launch_bounds__(kBlock) void push_kernel(PushArgs a, Op op) {
   }
 }
 
+struct NoAfter {
+  __device__ void after(bool, uint32_t) const {}
+};
+
+// ================================================================ PageRank
+// toolkits/pagerank.cpp:24-129
+struct PrPull : NoAfter {
+  using T = double;
+  const double *curr;
+  double *next;
+  const uint32_t *outdeg;
+  int last;
+  const double *cold;  // sums over the cold arcs (hot/cold plan), or null
+  __device__ double identity() const { return 0.0; }
+  __device__ bool dst_active(uint32_t) const { return true; }
+  __device__ double value(uint32_t src, float) const { return curr[src]; }
+  __device__ double combine(double a, double b) const { return a + b; }
+  __device__ bool finish(uint32_t v, double sum, uint32_t, double &dsum) const {
+    // next = 1 - d + d * next, divided by the out-degree except in the last
+    // iteration (pagerank.cpp:77-97)
+    const double d = 0.85;
+    if (cold) sum += cold[v];
+    double nx = 1 - d + d * sum;
+    if (!last) {
+      uint32_t od = outdeg[v];
+      if (od > 0) {
+        nx /= od;
+        dsum += fabs(nx - curr[v]) * od;
+      } else {
+        dsum += fabs(nx - curr[v]);
+      }
+    }
+    next[v] = nx;
+    return false;
+  }
+};
 // ------------------------------------------------------------------- pull
 struct PullArgs {
   const uint64_t *index;
@@ -355,6 +393,8 @@ struct PullArgs {
   // partial sums 16 doubles (one 128-byte line) apart, so that the waves'
   // atomics spread over many L2 channels instead of serialising on one line
   double *dacc;
+  // per-vertex prefix length of the in-list to read (null: the whole list)
+  const uint32_t *count;
 };
 
 // Reduction pull over one 64-vertex group per wave.
@@ -371,7 +411,7 @@ __global__ __launch_bounds__(kBlock) void pull_group_kernel(PullArgs a, Op op) {
   uint32_t v = g * 64 + lane;
   bool inr = v < a.V;
   uint64_t b = inr ? a.index[v] : 0;
-  uint64_t e1 = inr ? a.index[v + 1] : 0;
+  uint64_t e1 = inr ? (a.count ? b + a.count[v] : a.index[v + 1]) : 0;
   uint64_t deg64 = e1 - b;
   bool light = inr && deg64 <= a.light_max && op.dst_active(v);
   uint32_t deg = light ? (uint32_t)deg64 : 0u;
@@ -379,28 +419,45 @@ __global__ __launch_bounds__(kBlock) void pull_group_kernel(PullArgs a, Op op) {
   wave_sync();
   uint32_t total;
   uint32_t excl = wave_excl_scan(deg, &total);
-  for (uint32_t j0 = 0; j0 < total; j0 += 64) {
-    uint32_t j = j0 + lane;
-    bool act = j < total;
-    int k = wave_owner(excl, j);
-    uint64_t bb = shfl_u64(b, k);
-    uint32_t ex = __shfl(excl, k);
-    T x = op.identity();
-    if (act) {
-      uint64_t e = bb + (j - ex);
-      x = op.value(stream_load(&a.nbr[e]), a.w ? stream_load(&a.w[e]) : 0.f);
-    }
-    int kk = act ? k : 64;
+  // Four 64-entry steps per round: their adjacency loads, then their
+  // gathers, are issued back to back; the reductions then run in step order
+  // (the same combination order as one step at a time).
+  constexpr int kSteps = 4;
+  for (uint32_t j0 = 0; j0 < total; j0 += 64 * kSteps) {
+    int ks[kSteps];
+    uint32_t us[kSteps];
+    float ws[kSteps];
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      T y = Shfl<T>::up(x, d);
-      int ko = __shfl_up(kk, d);
-      if (lane >= d && ko == kk) x = op.combine(x, y);
+    for (int t = 0; t < kSteps; t++) {
+      const uint32_t j = j0 + t * 64 + lane;
+      const bool act = j < total;
+      const int k = wave_owner(excl, j);
+      const uint64_t bb = shfl_u64(b, k);
+      const uint32_t ex = __shfl(excl, k);
+      const uint64_t e = bb + (j - ex);
+      ks[t] = act ? k : 64;
+      us[t] = act ? stream_load(&a.nbr[e]) : 0u;
+      ws[t] = act && a.w ? stream_load(&a.w[e]) : 0.f;
     }
-    int nk = __shfl_down(kk, 1);
-    bool last = (lane == 63) || nk != kk;
-    if (act && last) lacc[wv][kk] = op.combine(lacc[wv][kk], x);
-    wave_sync();
+    T xs[kSteps];
+#pragma unroll
+    for (int t = 0; t < kSteps; t++) xs[t] = ks[t] < 64 ? op.value(us[t], ws[t]) : op.identity();
+#pragma unroll
+    for (int t = 0; t < kSteps; t++) {
+      if (j0 + t * 64 >= total) break;  // wave-uniform
+      T x = xs[t];
+      const int kk = ks[t];
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        T y = Shfl<T>::up(x, d);
+        int ko = __shfl_up(kk, d);
+        if (lane >= d && ko == kk) x = op.combine(x, y);
+      }
+      const int nk = __shfl_down(kk, 1);
+      const bool last = (lane == 63) || nk != kk;
+      if (kk < 64 && last) lacc[wv][kk] = op.combine(lacc[wv][kk], x);
+      wave_sync();
+    }
   }
   T acc = lacc[wv][lane];
   double dsum = 0;
@@ -424,6 +481,9 @@ struct HeavyArgs {
   uint32_t n_chunks;
   uint64_t *out;
   unsigned long long *exam;
+  // per-chunk partials (8 bytes each): when set, chunks store their partial
+  // and heavy_reduce_kernel commits once per vertex
+  void *part;
 };
 
 // One wave per heavy chunk; the chunk's partial goes through op.commit.
@@ -432,33 +492,69 @@ struct HeavyArgs {
 template <class Op>
 __global__ __launch_bounds__(kBlock) void pull_heavy_kernel(HeavyArgs a, Op op) {
   using T = typename Op::T;
-  const uint32_t c = global_wave();
   const int lane = lane_id();
-  if (c >= a.n_chunks) return;
-  uint32_t v = a.chunk_v[c];
+  const uint32_t W = gridDim.x * kWavesPerBlock;
+  // persistent grid: each wave walks chunks c, c + W, ...
+  for (uint32_t c = global_wave(); c < a.n_chunks; c += W) {
+    const uint32_t v = a.chunk_v[c];
+    if (!op.dst_active(v)) {
+      if (a.part && lane == 0)
+        reinterpret_cast<T *>(static_cast<uint64_t *>(a.part) + c)[0] = op.identity();
+      continue;
+    }
+    const uint64_t cb = a.chunk_beg[c], ce = a.chunk_end[c];
+    T x = op.identity();
+    // chunks hold at most kChunkEdges = 16 x 64 entries: load all neighbour
+    // ids first, then gather, so each lane keeps 16 independent loads in flight
+    constexpr int kPer = kChunkEdges / 64;
+    uint32_t u[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint64_t e = cb + lane + 64 * k;
+      u[k] = e < ce ? stream_load(&a.nbr[e]) : 0xFFFFFFFFu;
+    }
+    T y[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint64_t e = cb + lane + 64 * k;
+      y[k] = u[k] != 0xFFFFFFFFu ? op.value(u[k], a.w ? stream_load(&a.w[e]) : 0.f)
+                                 : op.identity();
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) x = op.combine(x, y[k]);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) x = op.combine(x, Shfl<T>::xr(x, m));
+    if (lane == 0) {
+      if (a.part)
+        reinterpret_cast<T *>(static_cast<uint64_t *>(a.part) + c)[0] = x;
+      else if (op.commit(v, a.chunk_h[c], x))
+        op.after(activate(a.out, v), v);
+      count_examined(a.exam, ce - cb);
+    }
+  }
+}
+
+// One wave per heavy vertex: combines its chunks' partials (lane-strided,
+// then a fixed xor tree: the same order every run) and commits once, so
+// chunks of one hub never contend on one address.
+template <class Op>
+__global__ __launch_bounds__(kBlock) void heavy_reduce_kernel(const uint32_t *heavy_v,
+                                                             const uint32_t *heavy_first,
+                                                             uint32_t n_heavy, const void *part,
+                                                             uint64_t *out, Op op) {
+  using T = typename Op::T;
+  const uint32_t h = global_wave();
+  const int lane = lane_id();
+  if (h >= n_heavy) return;
+  const uint32_t v = heavy_v[h];
   if (!op.dst_active(v)) return;
-  uint64_t cb = a.chunk_beg[c], ce = a.chunk_end[c];
+  const T *p = reinterpret_cast<const T *>(static_cast<const uint64_t *>(part));
   T x = op.identity();
-  // chunks hold at most kChunkEdges = 16 x 64 entries: load all neighbour ids
-  // first, then gather, so each lane keeps 16 independent loads in flight
-  constexpr int kPer = kChunkEdges / 64;
-  uint32_t u[kPer];
-#pragma unroll
-  for (int k = 0; k < kPer; k++) {
-    uint64_t e = cb + lane + 64 * k;
-    u[k] = e < ce ? stream_load(&a.nbr[e]) : 0xFFFFFFFFu;
-  }
-#pragma unroll
-  for (int k = 0; k < kPer; k++) {
-    uint64_t e = cb + lane + 64 * k;
-    if (u[k] != 0xFFFFFFFFu) x = op.combine(x, op.value(u[k], a.w ? stream_load(&a.w[e]) : 0.f));
-  }
+  for (uint32_t c = heavy_first[h] + lane; c < heavy_first[h + 1]; c += 64)
+    x = op.combine(x, *reinterpret_cast<const T *>(reinterpret_cast<const uint64_t *>(p) + c));
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) x = op.combine(x, Shfl<T>::xr(x, m));
-  if (lane == 0) {
-    if (op.commit(v, a.chunk_h[c], x)) op.after(activate(a.out, v), v);
-    count_examined(a.exam, ce - cb);
-  }
+  if (lane == 0 && op.commit(v, h, x)) op.after(activate(out, v), v);
 }
 
 }  // namespace gg

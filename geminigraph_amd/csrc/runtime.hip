@@ -199,6 +199,8 @@ static int64_t *option_slot(const char *name) {
   if (!strcmp(name, "count_examined")) return &o.count_examined;
   if (!strcmp(name, "level_masks")) return &o.level_masks;
   if (!strcmp(name, "level_mask_bytes")) return &o.level_mask_bytes;
+  if (!strcmp(name, "pr_hot_sources")) return &o.pr_hot_sources;
+  if (!strcmp(name, "heavy_waves")) return &o.heavy_waves;
   if (!strcmp(name, "bfs_pull_waves")) return &o.bfs_pull_waves;
   return nullptr;
 }
@@ -212,7 +214,10 @@ int gg_set_option(const char *name, int64_t value) {
   if (flag && (value < 0 || value > 1)) return fail(GG_EINVAL, std::string(name) + " must be 0 or 1");
   if (!strcmp(name, "bfs_pull_waves") && (value < 0 || value > (1 << 24)))
     return fail(GG_EINVAL, "bfs_pull_waves must be in [0, 2^24]");
-  if (strcmp(name, "force_mode") && strcmp(name, "bfs_pull_waves") && !flag && value <= 0)
+  const bool zero_ok = !strcmp(name, "bfs_pull_waves") || !strcmp(name, "pr_hot_sources") ||
+                       !strcmp(name, "heavy_waves");
+  if (zero_ok && value < 0) return fail(GG_EINVAL, std::string(name) + " must be >= 0");
+  if (strcmp(name, "force_mode") && !zero_ok && !flag && value <= 0)
     return fail(GG_EINVAL, std::string(name) + " must be positive");
   if (!strcmp(name, "push_small_max") && value > 64 * 64)
     return fail(GG_EINVAL, "push_small_max must be <= 4096");

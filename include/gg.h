@@ -69,10 +69,16 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *   "pull_serial_max"  BFS pull: largest in-degree scanned by a single lane
  *   "huge_grid_waves"  waves sharing the adjacency of the largest-degree bin
  *   "count_examined"   1 = count adjacency entries read (gg_run_stats)
+ *   "pr_hot_sources"   PageRank on a relabelled graph: > 0 streams the
+ *                      arcs (hot sources below the value, cold arcs binned
+ *                      by destination); 0 = pull (default)
+ *   "heavy_waves"      waves of the dense heavy-chunk grids (0 = one per chunk)
+ *   "bfs_pull_waves"   waves of the BFS pull grids (0 = one per group / chunk)
  *   "level_masks"      batched BFS/BC: 1 = keep per-level root masks
  *                      (default; depth rows are built once at the end and
  *                      BC selects backward arcs exactly), 0 = write depth
- *                      rows level by level */
+ *                      rows level by level
+ *   "level_mask_bytes" size limit of that mask store (default 4 GiB) */
 int gg_set_option(const char *name, int64_t value);
 int gg_get_option(const char *name, int64_t *value);
 

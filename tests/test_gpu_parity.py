@@ -158,6 +158,39 @@ def test_pagerank_heavy_vertices(gg):
     assert_rel(pr, ref)
 
 
+@pytest.mark.parametrize("hot", [0, 1, 64, 1000, 30000])
+def test_pagerank_hot_cold_plan(gg, hot):
+    # relabelled graph: in-edges from the `hot` highest out-degree sources are
+    # gathered by the pull, the others streamed through the cold bins
+    scale = 15
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, 16, seed=61)
+    g = gg.Graph.build(e, V, reorder=True)
+    say("pagerank plan", hot)
+    with gg.options(pr_hot_sources=hot):
+        pr, deltas, _ = g.pagerank(20)
+        pr2, _, _ = g.pagerank(20)  # the plan is reused
+    ref, ref_d = oracle.Graph(e, V).pagerank(20)
+    assert_rel(pr, ref)
+    assert_rel(pr2, ref)
+    assert_rel(deltas[:20], ref_d[:20], rel=1e-8)
+
+
+@pytest.mark.parametrize("hot", [2, 2500])
+def test_pagerank_plan_heavy_prefixes(gg, hot):
+    # a hub whose hot prefix alone exceeds the chunk limit, another whose
+    # in-list is mostly cold, bins spanning several of the 8192-vertex ranges
+    n = 20000
+    pairs = [(i, 0) for i in range(1, 5001)] + [(0, i) for i in range(1, 3000)] + \
+            [(i, 19999) for i in range(3000, 12000)] + [(i, i + 1) for i in range(1, n - 1)]
+    e = oracle.edges_from_pairs(pairs)
+    g = gg.Graph.build(e, n, reorder=True)
+    with gg.options(pr_hot_sources=hot):
+        pr, _, _ = g.pagerank(10)
+    ref, _ = oracle.Graph(e, n).pagerank(10)
+    assert_rel(pr, ref)
+
+
 # ---------------------------------------------------------------------- BFS
 
 @pytest.mark.parametrize("mode", ["auto", "push", "pull"])
