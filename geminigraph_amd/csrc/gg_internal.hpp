@@ -118,15 +118,18 @@ struct PrPlan {
   uint64_t n_cold = 0;         // cold arcs
   uint64_t n_hot = 0;          // hot arcs; the packed arrays are padded to whole chunks
   uint32_t n_hchunks = 0;      // hot chunks of kPrHotChunk arcs
-  uint32_t *slot = nullptr;       // per cold out-arc: position in its bin's list
+  uint32_t *src_list = nullptr;   // per bin-list position: its source
   uint16_t *dst_local = nullptr;  // per bin-list position: destination - bin start
+  uint32_t nranges = 0;           // cold source ranges of kPrRange sources
+  uint64_t *run_pos = nullptr;    // (nranges + 1) x nbins: first position of each range in each bin
+  uint32_t ncopy = 0;             // copy workgroups
+  uint32_t *copy_blk = nullptr;   // per copy workgroup: range << 16 | part, and parts of the range
   uint64_t *seg = nullptr;        // per segment: bin << 40 | first position (+ end in seg_end)
   uint64_t *seg_end = nullptr;
   uint8_t *bin_multi = nullptr;   // per bin: more than one segment
   uint32_t *hot_src = nullptr;    // packed hot arcs (source), dst order
   uint16_t *hot_dst = nullptr;    // destination - first destination of the chunk
   uint32_t *hchunk_first = nullptr;  // first destination of each hot chunk
-  uint32_t xcd_g[9] = {};            // cold source groups of each XCD (balanced by arcs)
   std::vector<void *> allocations;
 };
 

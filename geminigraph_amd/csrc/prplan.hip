@@ -12,13 +12,15 @@
 //     contiguous arcs per lane, sixteen independent gathers in flight that
 //     hit L2, segment sums within the lane and one segmented scan across the
 //     wave;
-//   cold: each iteration writes curr[u] into the slot of every cold out-arc
-//     in its destination's bin (bins of kPrBinDst destinations; within a bin
-//     slots follow the sources, and each XCD walks its own contiguous eighth
-//     of the sources, so the writes fill whole lines), then workgroups add
-//     each bin's (destination, value) list into LDS;
+//   cold: the cold arcs are kept as per-bin lists (bins of kPrBinDst
+//     destinations), sorted by source.  Each iteration a workgroup loads the
+//     values of one range of kPrRange cold sources into LDS (a coalesced
+//     read) and fills that range's run of every bin list from LDS (coalesced
+//     writes); then workgroups add each bin's (destination, value) list into
+//     LDS;
 //   finish: one pass over V applies the PageRank update to hot + cold.
-// Streamed bytes per arc: hot 6 (+ an L2 hit), cold 4 + 8 + 8 + 2 = 22.
+// Streamed bytes per arc: hot 6 (+ an L2 or LDS hit), cold 4 + 8 + 8 + 2 = 22
+// (+ 8 per cold source).
 #include <string.h>
 
 #include <algorithm>
@@ -28,12 +30,18 @@
 
 namespace gg {
 
-constexpr uint32_t kPrBinDst = 8192;      // destinations per bin: 64 KiB of LDS doubles
-constexpr uint64_t kPrSegArcs = 1 << 15;  // arcs per accumulate workgroup
+constexpr uint32_t kPrBinDst = 16384;     // destinations per bin: 128 KiB of LDS doubles
+constexpr uint64_t kPrSegArcs = 1 << 17;  // arcs per accumulate workgroup
+constexpr int kAccBlock = 1024;           // accumulate workgroup: one per CU (LDS)
+constexpr int kHotBlock = 1024;           // hot stream workgroup: one per CU (LDS)
+constexpr uint32_t kHub = 16384;          // hub values cached in LDS: 128 KiB
 constexpr uint32_t kPrPer = 16;           // hot arcs per lane
 constexpr uint32_t kPrHotChunk = 64 * kPrPer;
 constexpr uint16_t kNoKey = 0xFFFF;       // padding arcs of the last hot chunk
-constexpr int kXcds = 8;                  // MI355X: workgroup i runs on XCD i mod 8
+constexpr uint32_t kPrRange = 16384;      // cold sources per copy workgroup: 128 KiB of LDS
+constexpr int kCopyBlock = 1024;
+constexpr uint32_t kCopyBins = 1024;      // bins per copy workgroup (LDS run table)
+constexpr uint64_t kCopyArcs = 1 << 17;   // arcs per copy workgroup (ranges are split)
 
 // ------------------------------------------------------------------ build
 // hot_cnt[v] = in-neighbours of v below H (lower bound in the sorted list).
@@ -103,87 +111,151 @@ __global__ void pr_bin_keys_kernel(const uint32_t *nbr, uint64_t cold_base, uint
   }
 }
 
-// slot[a] = position of cold arc a in bin order; dst_local from its target.
-__global__ void pr_slots_kernel(const uint32_t *sorted_a, const uint32_t *nbr, uint64_t cold_base,
-                                uint64_t n_cold, uint32_t *slot, uint16_t *dst_local) {
+// Source of every cold out-arc (one wave per cold source).
+__global__ void pr_arc_src_kernel(const uint64_t *out_index, uint32_t H, uint32_t V,
+                                  uint64_t cold_base, uint32_t *arc_src) {
+  const uint32_t W = gridDim.x * (blockDim.x / 64);
+  const int lane = lane_id();
+  for (uint32_t u = H + blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); u < V; u += W) {
+    const uint64_t b = out_index[u], e = out_index[u + 1];
+    for (uint64_t a = b + lane; a < e; a += 64) arc_src[a - cold_base] = u;
+  }
+}
+
+// Bin lists in bin order: source and local destination of each position.
+__global__ void pr_lists_kernel(const uint32_t *sorted_a, const uint32_t *arc_src,
+                                const uint32_t *nbr, uint64_t cold_base, uint64_t n_cold,
+                                uint32_t *src_list, uint16_t *dst_local) {
   const uint64_t s = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n_cold; p += s) {
     const uint32_t a = sorted_a[p];
-    slot[a] = (uint32_t)p;
+    src_list[p] = arc_src[a];
     dst_local[p] = (uint16_t)(nbr[cold_base + a] % kPrBinDst);
   }
 }
 
-// -------------------------------------------------------------- iteration
-// Cold scatter: XCD x walks its contiguous share of the cold sources (an
-// eighth of the cold arcs), 64
-// consecutive sources per wave step; their out-arcs are contiguous, so slot
-// reads are coalesced, and lane j writes the value of the source of arc j.
-struct XcdSplit {
-  uint32_t g[kXcds + 1];  // source groups [g[x], g[x+1]) go to XCD x
-};
-__global__ __launch_bounds__(kBlock) void pr_cold_scatter_kernel(const uint64_t *out_index,
-                                                                 const double *curr,
-                                                                 const uint32_t *slot,
-                                                                 uint64_t cold_base, uint32_t H,
-                                                                 uint32_t V, XcdSplit split,
-                                                                 double *vals) {
-  const int lane = lane_id();
-  const uint32_t x = blockIdx.x % kXcds, blocks_x = gridDim.x / kXcds;
-  const uint32_t g0 = split.g[x], g1 = split.g[x + 1];
-  const uint32_t wx = (blockIdx.x / kXcds) * kWavesPerBlock + wave_in_block();
-  for (uint32_t g = g0 + wx; g < g1; g += blocks_x * kWavesPerBlock) {
-    const uint32_t u = H + g * 64 + lane;
-    const bool valid = u < V;
-    const uint64_t b = valid ? out_index[u] : 0;
-    const uint32_t d = valid ? (uint32_t)(out_index[u + 1] - b) : 0;
-    const double c = valid && d ? curr[u] : 0.0;
-    uint32_t total;
-    const uint32_t excl = wave_excl_scan(d, &total);
-    for (uint32_t j0 = 0; j0 < total; j0 += 64) {
-      const uint32_t j = j0 + lane;
-      const int k = wave_owner(excl, j);
-      const uint64_t bb = shfl_u64(b, k);
-      const uint32_t ex = __shfl(excl, k);
-      const double ck = shfl_f64(c, k);
-      if (j < total) vals[slot[bb + (j - ex) - cold_base]] = ck;
+// run_pos[r * nbins + b] = first position in bin b's list whose source is
+// at least H + r * kPrRange (lists are sorted by source).
+__global__ void pr_run_pos_kernel(const uint64_t *bin_off, const uint32_t *src_list, uint32_t H,
+                                  uint32_t nbins, uint32_t nranges, uint64_t *run_pos) {
+  const uint64_t total = (uint64_t)(nranges + 1) * nbins;
+  const uint64_t s = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += s) {
+    const uint32_t r = (uint32_t)(i / nbins), b = (uint32_t)(i % nbins);
+    const uint64_t key = (uint64_t)H + (uint64_t)r * kPrRange;
+    uint64_t lo = bin_off[b], hi = bin_off[b + 1];
+    while (lo < hi) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if (src_list[mid] < key)
+        lo = mid + 1;
+      else
+        hi = mid;
     }
+    run_pos[i] = lo;
+  }
+}
+
+// -------------------------------------------------------------- iteration
+// Cold copy: a workgroup takes one range of kPrRange cold sources and a
+// strided subset of the bins (part q of Q: bins q, q + Q, ...), loads the
+// range's values into LDS, gathers its runs of those bins' lists into an
+// LDS table (start, exclusive-scanned length), and then copies the
+// concatenated runs item by item: reads of the static source list and
+// writes of the values run along each list.
+__global__ __launch_bounds__(kCopyBlock) void pr_cold_copy_kernel(
+    const double *curr, const uint32_t *src_list, const uint64_t *run_pos,
+    const uint32_t *copy_blk, uint32_t H, uint32_t V, uint32_t nbins, double *vals) {
+  __shared__ double val[kPrRange];
+  __shared__ uint64_t rstart[kCopyBins];
+  __shared__ uint32_t rscan[kCopyBins + 1];
+  __shared__ uint32_t wsum[kCopyBlock / 64];
+  const uint32_t info = copy_blk[2 * blockIdx.x], parts = copy_blk[2 * blockIdx.x + 1];
+  const uint32_t r = info >> 16, q = info & 0xFFFF;
+  const uint32_t u0 = H + r * kPrRange;
+  const uint32_t n = V - u0 < kPrRange ? V - u0 : kPrRange;
+  for (uint32_t i = threadIdx.x; i < n; i += kCopyBlock) val[i] = curr[u0 + i];
+  // this workgroup's bins: q, q + parts, ... (at most kCopyBins of them)
+  const uint32_t nb = (nbins - q + parts - 1) / parts;
+  const uint64_t *pos = run_pos + (uint64_t)r * nbins;
+  uint32_t len = 0;
+  if (threadIdx.x < nb) {
+    const uint32_t b = q + threadIdx.x * parts;
+    rstart[threadIdx.x] = pos[b];
+    len = (uint32_t)(pos[nbins + b] - pos[b]);
+  }
+  // block-wide exclusive scan of the run lengths
+  uint32_t wtot;
+  const uint32_t wex = wave_excl_scan(len, &wtot);
+  if (lane_id() == 0) wsum[threadIdx.x >> 6] = wtot;
+  __syncthreads();
+  uint32_t base = 0;
+  for (uint32_t i = 0; i < (threadIdx.x >> 6); i++) base += wsum[i];
+  if (threadIdx.x < nb) rscan[threadIdx.x] = base + wex;
+  uint32_t total = 0;
+  for (uint32_t i = 0; i < kCopyBlock / 64; i++) total += wsum[i];
+  if (threadIdx.x == 0) rscan[nb] = total;
+  __syncthreads();
+  constexpr int kU = 4;
+  for (uint32_t j0 = threadIdx.x; j0 < total; j0 += kCopyBlock * kU) {
+    uint64_t pp[kU];
+    uint32_t su[kU];
+#pragma unroll
+    for (int t = 0; t < kU; t++) {
+      const uint32_t j = j0 + t * kCopyBlock;
+      pp[t] = ~0ULL;
+      if (j < total) {
+        uint32_t lo = 0, hi = nb;  // last run with rscan <= j
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (rscan[mid] <= j)
+            lo = mid;
+          else
+            hi = mid;
+        }
+        pp[t] = rstart[lo] + (j - rscan[lo]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kU; t++) su[t] = pp[t] != ~0ULL ? stream_load(&src_list[pp[t]]) : u0;
+#pragma unroll
+    for (int t = 0; t < kU; t++)
+      if (pp[t] != ~0ULL) vals[pp[t]] = val[su[t] - u0];
   }
 }
 
 // Cold accumulate: one workgroup per segment (at most kPrSegArcs arcs of one
 // bin).  A bin with one segment stores its sums; segments of a larger bin
 // add theirs (the sums start at zero every iteration).
-__global__ __launch_bounds__(kBlock) void pr_cold_accumulate_kernel(
+__global__ __launch_bounds__(kAccBlock) void pr_cold_accumulate_kernel(
     const uint64_t *seg, const uint64_t *seg_end, const uint8_t *bin_multi,
     const uint16_t *dst_local, const double *vals, uint32_t V, double *cold) {
   __shared__ double sum[kPrBinDst];
   const uint64_t packed = seg[blockIdx.x];
   const uint32_t bin = (uint32_t)(packed >> 40);
   const uint64_t p0 = packed & ((1ULL << 40) - 1), p1 = seg_end[blockIdx.x];
-  for (uint32_t i = threadIdx.x; i < kPrBinDst; i += kBlock) sum[i] = 0.0;
+  for (uint32_t i = threadIdx.x; i < kPrBinDst; i += kAccBlock) sum[i] = 0.0;
   __syncthreads();
   constexpr int kU = 8;  // loads in flight per thread
-  for (uint64_t p = p0 + threadIdx.x; p < p1; p += kBlock * kU) {
+  for (uint64_t p = p0 + threadIdx.x; p < p1; p += kAccBlock * kU) {
     uint16_t d[kU];
     double x[kU];
 #pragma unroll
     for (int t = 0; t < kU; t++) {
-      const uint64_t q = p + (uint64_t)t * kBlock;
+      const uint64_t q = p + (uint64_t)t * kAccBlock;
       d[t] = q < p1 ? stream_load(&dst_local[q]) : (uint16_t)0;
       x[t] = q < p1 ? stream_load(&vals[q]) : 0.0;
     }
 #pragma unroll
     for (int t = 0; t < kU; t++)
-      if (p + (uint64_t)t * kBlock < p1) atomicAdd(&sum[d[t]], x[t]);
+      if (p + (uint64_t)t * kAccBlock < p1) atomicAdd(&sum[d[t]], x[t]);
   }
   __syncthreads();
   const uint32_t base = bin * kPrBinDst;
   const uint32_t n = V - base < kPrBinDst ? V - base : kPrBinDst;
   if (!bin_multi[bin]) {
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock) cold[base + i] = sum[i];
+    for (uint32_t i = threadIdx.x; i < n; i += kAccBlock) cold[base + i] = sum[i];
   } else {
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock)
+    for (uint32_t i = threadIdx.x; i < n; i += kAccBlock)
       if (sum[i] != 0.0) atomicAdd(&cold[base + i], sum[i]);
   }
 }
@@ -195,14 +267,21 @@ __global__ __launch_bounds__(kBlock) void pr_cold_accumulate_kernel(
 // lanes by a segmented scan.  Only a segment containing the chunk's first or
 // last arc can continue into a neighbouring chunk: those are added with an
 // atomic, every other segment is stored (the sums start at zero).
-__global__ __launch_bounds__(kBlock) void pr_hot_stream_kernel(const uint32_t *hot_src,
-                                                               const uint16_t *hot_dst,
-                                                               const uint32_t *first,
-                                                               uint32_t nchunks,
-                                                               const double *curr, double *hot) {
+__global__ __launch_bounds__(kHotBlock) void pr_hot_stream_kernel(const uint32_t *hot_src,
+                                                                  const uint16_t *hot_dst,
+                                                                  const uint32_t *first,
+                                                                  uint32_t nchunks,
+                                                                  const double *curr, double *hot,
+                                                                  uint32_t n_hub) {
+  // the values of the n_hub hottest sources (about 30% of all arcs on RMAT),
+  // kept in LDS: their gathers cost no L2 request
+  __shared__ double hub[kHub];
+  for (uint32_t i = threadIdx.x; i < n_hub; i += kHotBlock) hub[i] = curr[i];
+  __syncthreads();
   const int lane = lane_id();
-  const uint32_t W = gridDim.x * kWavesPerBlock;
-  for (uint32_t c = global_wave(); c < nchunks; c += W) {
+  constexpr uint32_t kWaves = kHotBlock / 64;
+  const uint32_t W = gridDim.x * kWaves;
+  for (uint32_t c = blockIdx.x * kWaves + (threadIdx.x >> 6); c < nchunks; c += W) {
     const uint64_t base = (uint64_t)c * kPrHotChunk + (uint64_t)lane * kPrPer;
     uint32_t s[kPrPer];
     uint16_t d[kPrPer];
@@ -230,7 +309,8 @@ __global__ __launch_bounds__(kBlock) void pr_hot_stream_kernel(const uint32_t *h
     }
     double x[kPrPer];
 #pragma unroll
-    for (int i = 0; i < (int)kPrPer; i++) x[i] = d[i] != kNoKey ? curr[s[i]] : 0.0;
+    for (int i = 0; i < (int)kPrPer; i++)
+      x[i] = d[i] == kNoKey ? 0.0 : s[i] < n_hub ? hub[s[i]] : curr[s[i]];
     const uint32_t fdst = first[c];
     const uint32_t first_key = (uint32_t)__builtin_amdgcn_readlane((int)d[0], 0);
     const uint32_t last_key = (uint32_t)__builtin_amdgcn_readlane((int)d[kPrPer - 1], 63);
@@ -369,7 +449,7 @@ static int plan_cold(const gg_graph *g, PrPlan *p) {
   const uint32_t V = g->vertices;
   const uint64_t n = p->n_cold;
   p->nbins = (V + kPrBinDst - 1) / kPrBinDst;
-  GG_TRY(plan_alloc(p, (void **)&p->slot, std::max<uint64_t>(n, 1) * 4));
+  GG_TRY(plan_alloc(p, (void **)&p->src_list, std::max<uint64_t>(n, 1) * 4));
   GG_TRY(plan_alloc(p, (void **)&p->dst_local, std::max<uint64_t>(n, 1) * 2));
   Scratch t;
   uint32_t *keys = nullptr, *keys2 = nullptr, *vals = nullptr, *vals2 = nullptr, *cnt = nullptr;
@@ -391,8 +471,12 @@ static int plan_cold(const gg_graph *g, PrPlan *p) {
     GG_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, kb, vb, (size_t)n, 0, bits, stream()));
     GG_TRY(t.get(&tmp, tmp_bytes));
     GG_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, kb, vb, (size_t)n, 0, bits, stream()));
-    hipLaunchKernelGGL(pr_slots_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(), vb.current(),
-                       out.nbr, p->cold_base, n, p->slot, p->dst_local);
+    // keys2 is free again: the source of every cold arc
+    uint32_t *arc_src = kb.current() == keys ? keys2 : keys;
+    hipLaunchKernelGGL(pr_arc_src_kernel, dim3(grid1((uint64_t)(V - p->H) * 64)), dim3(kBlock), 0,
+                       stream(), out.index, p->H, V, p->cold_base, arc_src);
+    hipLaunchKernelGGL(pr_lists_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(), vb.current(),
+                       arc_src, out.nbr, p->cold_base, n, p->src_list, p->dst_local);
   }
   std::vector<uint32_t> hc(p->nbins);
   GG_HIP(hipMemcpyAsync(hc.data(), cnt, (size_t)p->nbins * 4, hipMemcpyDeviceToHost, stream()));
@@ -420,28 +504,42 @@ static int plan_cold(const gg_graph *g, PrPlan *p) {
                           stream()));
   }
   GG_HIP(hipMemcpyAsync(p->bin_multi, multi.data(), p->nbins, hipMemcpyHostToDevice, stream()));
+  // copy ranges over the cold sources that have out-arcs
+  uint64_t *boff = nullptr;
+  GG_TRY(t.get((void **)&boff, ((size_t)p->nbins + 1) * 8));
+  std::vector<uint64_t> off((size_t)p->nbins + 1, 0);
+  for (uint32_t b = 0; b < p->nbins; b++) off[b + 1] = off[b] + hc[b];
+  GG_HIP(hipMemcpyAsync(boff, off.data(), off.size() * 8, hipMemcpyHostToDevice, stream()));
+  p->nranges = (uint32_t)(((uint64_t)V - p->H + kPrRange - 1) / kPrRange);
+  const uint64_t npos = (uint64_t)(p->nranges + 1) * p->nbins;
+  GG_TRY(plan_alloc(p, (void **)&p->run_pos, npos * 8));
+  hipLaunchKernelGGL(pr_run_pos_kernel, dim3(grid1(npos)), dim3(kBlock), 0, stream(), boff,
+                     p->src_list, p->H, p->nbins, p->nranges, p->run_pos);
+  // copy workgroups: a range is split into parts of about kCopyArcs arcs and
+  // of at most kCopyBins bins
+  std::vector<uint64_t> rb((size_t)p->nranges + 1);
+  GG_HIP(hipMemcpy2DAsync(rb.data(), 8, out.index + p->H, (size_t)kPrRange * 8, 8, p->nranges,
+                          hipMemcpyDeviceToHost, stream()));
   GG_HIP(hipStreamSynchronize(stream()));
-  return GG_OK;
-}
-
-// Source-group boundaries that give every XCD an eighth of the cold arcs.
-static int plan_xcd_split(const gg_graph *g, PrPlan *p) {
-  const Dir &out = out_dir(g);
-  const uint32_t V = g->vertices;
-  const uint32_t groups = (V - p->H + 63) / 64;
-  std::vector<uint64_t> bound(groups + 1);
-  // group starts: out.index[H + 64 j], read with a strided copy
-  GG_HIP(hipMemcpy2DAsync(bound.data(), 8, out.index + p->H, 64 * 8, 8, groups, hipMemcpyDeviceToHost,
-                          stream()));
-  GG_HIP(hipStreamSynchronize(stream()));
-  bound[groups] = g->arcs;
-  p->xcd_g[0] = 0;
-  for (int x = 1; x < kXcds; x++) {
-    const uint64_t target = p->cold_base + p->n_cold * (uint64_t)x / kXcds;
-    p->xcd_g[x] = (uint32_t)(std::lower_bound(bound.begin(), bound.end(), target) - bound.begin());
-    p->xcd_g[x] = std::max(p->xcd_g[x], p->xcd_g[x - 1]);
+  rb[p->nranges] = g->arcs;
+  std::vector<uint32_t> blk;
+  for (uint32_t r = 0; r < p->nranges; r++) {
+    const uint64_t arcs = rb[r + 1] - rb[r];
+    uint32_t parts = (uint32_t)std::max<uint64_t>(1, (arcs + kCopyArcs - 1) / kCopyArcs);
+    parts = std::max(parts, (p->nbins + kCopyBins - 1) / kCopyBins);
+    parts = std::min(parts, std::min(p->nbins, 0xFFFFu));
+    if (!arcs) continue;
+    for (uint32_t q = 0; q < parts; q++) {
+      blk.push_back(r << 16 | q);
+      blk.push_back(parts);
+    }
   }
-  p->xcd_g[kXcds] = groups;
+  if (p->nranges >= (1u << 16)) return fail(GG_EINVAL, "PageRank plan: too many source ranges");
+  p->ncopy = (uint32_t)(blk.size() / 2);
+  GG_TRY(plan_alloc(p, (void **)&p->copy_blk, std::max<size_t>(blk.size(), 2) * 4));
+  if (!blk.empty())
+    GG_HIP(hipMemcpyAsync(p->copy_blk, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, stream()));
+  GG_HIP(hipStreamSynchronize(stream()));
   return GG_OK;
 }
 
@@ -454,7 +552,6 @@ static int plan_build(const gg_graph *g, uint32_t H, PrPlan *p, bool *ok) {
   p->n_cold = g->arcs - p->cold_base;
   GG_TRY(plan_hot(g, p, ok));
   if (!*ok) return GG_OK;
-  GG_TRY(plan_xcd_split(g, p));
   return plan_cold(g, p);
 }
 
@@ -508,23 +605,25 @@ int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull 
   }
   {
     ProfScope ps("pr_cold_scatter");
-    XcdSplit split;
-    for (int x = 0; x <= kXcds; x++) split.g[x] = p->xcd_g[x];
-    hipLaunchKernelGGL(pr_cold_scatter_kernel, dim3(kXcds * 256), dim3(kBlock), 0, stream(),
-                       out_dir(g).index, op.curr, p->slot, p->cold_base, p->H, V, split,
-                       w->pr_vals);
+    if (p->ncopy)
+      hipLaunchKernelGGL(pr_cold_copy_kernel, dim3(p->ncopy), dim3(kCopyBlock), 0, stream(),
+                         op.curr, p->src_list, p->run_pos, p->copy_blk, p->H, V, p->nbins,
+                         w->pr_vals);
   }
   if (p->nseg) {
     ProfScope ps("pr_cold_accumulate");
-    hipLaunchKernelGGL(pr_cold_accumulate_kernel, dim3(p->nseg), dim3(kBlock), 0, stream(),
+    hipLaunchKernelGGL(pr_cold_accumulate_kernel, dim3(p->nseg), dim3(kAccBlock), 0, stream(),
                        p->seg, p->seg_end, p->bin_multi, p->dst_local, w->pr_vals, V, w->pr_cold);
   }
   {
     ProfScope ps("pr_hot_stream");
-    hipLaunchKernelGGL(pr_hot_stream_kernel,
-                       dim3(grid_waves(std::min<uint32_t>(p->n_hchunks, 16384))), dim3(kBlock),
-                       0, stream(), p->hot_src, p->hot_dst, p->hchunk_first, p->n_hchunks,
-                       op.curr, w->pr_hot);
+    int cus = 0;
+    GG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g->device));
+    const uint32_t blocks = std::min<uint32_t>((uint32_t)std::max(cus, 1),
+                                               (p->n_hchunks + kHotBlock / 64 - 1) / (kHotBlock / 64));
+    hipLaunchKernelGGL(pr_hot_stream_kernel, dim3(std::max(blocks, 1u)), dim3(kHotBlock), 0,
+                       stream(), p->hot_src, p->hot_dst, p->hchunk_first, p->n_hchunks, op.curr,
+                       w->pr_hot, std::min(p->H, kHub));
   }
   {
     ProfScope ps("pr_finish");
