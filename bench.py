@@ -163,6 +163,40 @@ def serve_extra(gg, seed, reorder, seconds, rate=2000.0, workers=4, budget=200e9
     return out
 
 
+def pagerank_extra(gg, g, V, E, iterations=20, runs=3):
+    """PageRank (FP64, d = 0.85) on the bench graph: wall time of `runs`
+    calls, then one call with per-launch events for the kernel time."""
+    pr = gg.DeviceArray(V, np.float64)
+    g.pagerank_into(pr, iterations)
+    gg.synchronize()
+    t = time.perf_counter()
+    for _ in range(runs):
+        g.pagerank_into(pr, iterations)
+    gg.synchronize()
+    pr_s = (time.perf_counter() - t) / runs
+    names = ("pr_pull", "pr_heavy", "pr_cold_scatter", "pr_cold_accumulate", "pr_hot_stream",
+             "pr_finish")
+    gg.prof_reset()
+    gg.prof_enable(True)
+    g.pagerank_into(pr, iterations)
+    gg.prof_enable(False)
+    prk = {k: gg.prof_read(k) for k in names}
+    pr_kernel_s = sum(v[0] for v in prk.values()) / 1e3
+    pr.free()
+    b_iter = 12 * E + 28 * V
+    return {
+        "iterations": iterations,
+        "ms": round(pr_s * 1e3, 3),
+        "gteps": round(iterations * E / pr_s / 1e9, 2),
+        "ms_per_iter": round(pr_s * 1e3 / iterations, 4),
+        "model_bytes_per_iter": b_iter,
+        "achieved_GBs": round(iterations * b_iter / pr_kernel_s / 1e9, 1),
+        "frac": round(iterations * b_iter / pr_kernel_s / 1e9 / HBM_PEAK_GBS, 4),
+        "kernel_ms_per_iter": {k: round(v[0] / iterations, 4) for k, v in prk.items() if v[1]},
+        "hot_sources": gg.get_option("pr_hot_sources"),
+    }
+
+
 def cpu_baseline(scale, ef, seed, roots, gpu_depths, sample_roots):
     import oracle
     threads = os.environ.get("OMP_NUM_THREADS")
@@ -198,6 +232,79 @@ def cpu_baseline(scale, ef, seed, roots, gpu_depths, sample_roots):
     }, parity
 
 
+PULL_KERNELS = ("gg::bfs_pull_kernel", "gg::bfs_pull_heavy_kernel")
+COPY_KERNEL = "gg::gg_copy_kernel"
+COPY_BYTES = 256 << 20
+
+
+def pmc_child(args):
+    """Workload of the counter passes: one step of the bench (the same graph
+    and roots), then one calibration copy of COPY_BYTES."""
+    import geminigraph_amd as gg
+    gg.init(0)
+    g = gg.Graph.rmat(args.scale, args.edge_factor, seed=args.seed, permute=True,
+                      reorder=not args.no_reorder)
+    V = 1 << args.scale
+    roots = pick_roots(g.out_degree(), args.roots, args.seed + 1)
+    parent, depth = gg.DeviceArray(V, np.uint32), gg.DeviceArray(V, np.uint32)
+    for r in roots:
+        g.bfs_into(r, parent, depth)
+    src, dst = gg.DeviceArray(COPY_BYTES, np.uint8), gg.DeviceArray(COPY_BYTES, np.uint8)
+    gg.device_copy(dst, src, COPY_BYTES)
+    gg.synchronize()
+
+
+def measure_traffic(args):
+    """HBM bytes per launch of the dense BFS kernels, from two rocprofv3
+    counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each
+    scaled by what the same counter reads for the plain copy kernel moving
+    COPY_BYTES each way.  Returns (bytes per launch or None, detail)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None, {"error": "rocprofv3 not found"}
+    tmp = tempfile.mkdtemp(prefix="ggpmc", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    child = [sys.executable, os.path.abspath(__file__), "--pmc-child", "--scale", str(args.scale),
+             "--edge-factor", str(args.edge_factor), "--roots", str(args.roots),
+             "--seed", str(args.seed)] + (["--no-reorder"] if args.no_reorder else [])
+    per = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = os.path.join(tmp, counter)
+        cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "-d", out, "-o", "run",
+               "-f", "csv", "--"] + child
+        log(f"counter pass: {counter}")
+        r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL)
+        path = None
+        for root, _, files in os.walk(out):
+            for f in files:
+                if f.endswith("counter_collection.csv"):
+                    path = os.path.join(root, f)
+        if path is None:
+            return None, {"error": f"{counter} pass produced no counters (exit {r.returncode})"}
+        pull, pull_n, copy = 0.0, 0, None
+        for row in csv.DictReader(open(path)):
+            name = row["Kernel_Name"].split("(")[0].replace("void ", "")
+            v = float(row["Counter_Value"])
+            if name in PULL_KERNELS:
+                pull += v
+                pull_n += name == PULL_KERNELS[0]
+            elif name == COPY_KERNEL:
+                copy = v
+        if not copy or not pull_n:
+            return None, {"error": f"{counter}: kernels not found"}
+        factor = COPY_BYTES / (copy * 1024.0)
+        per[counter] = {"kib_per_launch": pull / pull_n, "factor": factor,
+                        "bytes_per_launch": pull / pull_n * 1024.0 * factor}
+    traffic = per["FETCH_SIZE"]["bytes_per_launch"] + per["WRITE_SIZE"]["bytes_per_launch"]
+    return traffic, {"fetch": per["FETCH_SIZE"], "write": per["WRITE_SIZE"],
+                     "launches": pull_n, "workload": "one bench step (same graph and roots)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,11 +321,31 @@ def main():
                     help="keep file vertex ids (no degree-sorted relabeling)")
     ap.add_argument("--serve-seconds", type=float, default=5.0,
                     help="length of the serving trace (0 skips the serving leg)")
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="skip the rocprofv3 counter passes behind roofline.traffic")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--pagerank-only", action="store_true",
+                    help="run only the PageRank leg (for profiling) and print its figures")
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args)
+        return
+    if args.pagerank_only:
+        import geminigraph_amd as gg
+        gg.init(0)
+        g = gg.Graph.rmat(args.scale, args.edge_factor, seed=args.seed, permute=True,
+                          reorder=not args.no_reorder)
+        V, E = 1 << args.scale, (1 << args.scale) * args.edge_factor
+        log(f"PageRank: {pagerank_extra(gg, g, V, E)}")
+        # the calibration copy, so counter passes over this run can be scaled
+        src, dst = gg.DeviceArray(COPY_BYTES, np.uint8), gg.DeviceArray(COPY_BYTES, np.uint8)
+        gg.device_copy(dst, src, COPY_BYTES)
+        gg.synchronize()
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(one partition per GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
+              "(one partition on one GPU)", file=sys.stderr)
         sys.exit(2)
 
     import geminigraph_amd as gg
@@ -290,6 +417,14 @@ def main():
         "avg_launch_ms": round(kernels["bfs_pull"][0] / max(1, pull_launches), 4),
     }
 
+    if not args.no_pmc:
+        traffic, detail = measure_traffic(args)
+        roofline["traffic"] = round(traffic, 1) if traffic else None
+        roofline["traffic_detail"] = detail
+        if traffic:
+            roofline["traffic_over_algorithmic"] = round(traffic / roofline["bytes_per_launch"], 3)
+        log(f"traffic: {roofline['traffic']} B/launch {detail}")
+
     extras = {"build_s": round(build_s, 2)}
     # measured stream-copy bandwidth (plain copy kernel, 1 GiB read + 1 GiB write)
     nb = 1 << 30
@@ -306,31 +441,8 @@ def main():
     src.free()
     dst.free()
     if not args.no_extras:
-        pr = gg.DeviceArray(V, np.float64)
-        g.pagerank_into(pr, 20)
-        gg.prof_reset()
-        gg.prof_enable(True)
-        runs = 3
-        t = time.perf_counter()
-        for _ in range(runs):
-            g.pagerank_into(pr, 20)
-        gg.synchronize()
-        pr_s = (time.perf_counter() - t) / runs
-        gg.prof_enable(False)
-        prk = {k: gg.prof_read(k) for k in ("pr_pull", "pr_heavy")}
-        pr_kernel_s = (prk["pr_pull"][0] + prk["pr_heavy"][0]) / 1e3 / runs
-        b_iter = 12 * E + 28 * V
-        extras["pagerank"] = {
-            "iterations": 20,
-            "ms": round(pr_s * 1e3, 3),
-            "gteps": round(20 * E / pr_s / 1e9, 2),
-            "ms_per_iter": round(pr_s * 1e3 / 20, 4),
-            "model_bytes_per_iter": b_iter,
-            "achieved_GBs": round(20 * b_iter / pr_kernel_s / 1e9, 1),
-            "frac": round(20 * b_iter / pr_kernel_s / 1e9 / HBM_PEAK_GBS, 4),
-        }
+        extras["pagerank"] = pagerank_extra(gg, g, V, E)
         log(f"PageRank: {extras['pagerank']}")
-        pr.free()
         del g
         parent.free()
         depth.free()
