@@ -159,8 +159,7 @@ void work_free(Work *w) {
   device_free(w->ms_vis);
   device_free(w->ms_lmask);
   device_free(w->pr_vals);
-  device_free(w->pr_cold);
-  device_free(w->pr_hot);
+  device_free(w->pr_gsum);
   device_free(w->hpart);
   bfs_state_free(w->bfs);
   device_free(w->ms_f[0]);
@@ -1335,7 +1334,7 @@ int gg_pagerank(gg_graph *g, int iterations, double *d_curr_out, double *h_delta
     w->dsum = s;
     w->dacc_cap = cap;
   }
-  // relabelled graphs: hot arcs and cold arcs both streamed (prplan.hip)
+  // relabelled graphs: destination bins summed in LDS (prplan.hip)
   const PrPlan *plan = nullptr;
   GG_TRY(pr_plan_get(g, &plan));
   hipLaunchKernelGGL(pr_init_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), curr, out.degree, V);
@@ -1344,7 +1343,7 @@ int gg_pagerank(gg_graph *g, int iterations, double *d_curr_out, double *h_delta
     // active = every vertex, so active_edges = |E| and the step is dense
     // (core/graph.hpp:1509).
     double *acc = w->dacc + (size_t)(it + 1) * kScalarStride;
-    PrPull op{{}, curr, next, out.degree, it == iterations - 1, nullptr};
+    PrPull op{{}, curr, next, out.degree, it == iterations - 1};
     if (plan) {
       GG_TRY(pr_plan_iteration(g, plan, w, op, acc));
     } else {

@@ -60,7 +60,9 @@ struct Options {
   int64_t count_examined = 0;    // count adjacency entries read (roofline model)
   int64_t level_masks = 1;       // batched BFS/BC: per-level root masks (else depth rows)
   int64_t level_mask_bytes = int64_t(4) << 30;  // size limit of that store
-  int64_t pr_hot_sources = 0;    // PageRank streamed plan: hot sources (0 = plain pull)
+  int64_t pr_hot_sources = 1 << 19;  // PageRank bin plan: hot sources (0 = plain pull)
+  int64_t pr_bin_bits = 13;      // PageRank bin plan: 2^bits destinations per bin
+  int64_t pr_range_bits = 14;    // PageRank bin plan: 2^bits sources per cold range
   int64_t heavy_waves = 8192;    // dense heavy-chunk grids: waves (0 = one per chunk)
   int64_t bfs_pull_waves = 8192; // BFS pull grids: waves (0 = one per group / chunk)
 };
@@ -109,27 +111,28 @@ struct Dir {
 // fit in L2.  Hot arcs are packed into one array streamed in fixed chunks;
 // the out-arcs of the other sources are cold: each iteration streams their
 // contributions into per-destination-range bins, which workgroups sum in LDS.
+struct PrSeg;
 struct PrPlan {
   int transposed = 0;          // orientation it was built for
-  uint32_t H = 0;              // hot sources: internal id < H
-  uint32_t nbins = 0;          // bins of kPrBinDst destinations
-  uint32_t nseg = 0;           // accumulate segments (<= kPrSegArcs arcs, one bin each)
+  uint32_t H = 0;              // hot sources: internal id < H (at most 2^(32 - bin_bits))
+  int bin_bits = 14;           // destination bins of 2^bin_bits vertices
+  int range_bits = 14;         // cold source ranges of 2^range_bits sources
+  uint32_t nbins = 0;
   uint64_t cold_base = 0;      // first cold out-arc (out.index[H])
   uint64_t n_cold = 0;         // cold arcs
-  uint64_t n_hot = 0;          // hot arcs; the packed arrays are padded to whole chunks
-  uint32_t n_hchunks = 0;      // hot chunks of kPrHotChunk arcs
-  uint32_t *src_list = nullptr;   // per bin-list position: its source
-  uint16_t *dst_local = nullptr;  // per bin-list position: destination - bin start
-  uint32_t nranges = 0;           // cold source ranges of kPrRange sources
-  uint64_t *run_pos = nullptr;    // (nranges + 1) x nbins: first position of each range in each bin
-  uint32_t ncopy = 0;             // copy workgroups
-  uint32_t *copy_blk = nullptr;   // per copy workgroup: range << 16 | part, and parts of the range
-  uint64_t *seg = nullptr;        // per segment: bin << 40 | first position (+ end in seg_end)
-  uint64_t *seg_end = nullptr;
-  uint8_t *bin_multi = nullptr;   // per bin: more than one segment
-  uint32_t *hot_src = nullptr;    // packed hot arcs (source), dst order
-  uint16_t *hot_dst = nullptr;    // destination - first destination of the chunk
-  uint32_t *hchunk_first = nullptr;  // first destination of each hot chunk
+  uint64_t n_hot = 0;          // hot arcs
+  uint32_t *hot = nullptr;     // per bin, sorted: source << bin_bits | destination - bin start
+  uint32_t nranges = 0;
+  uint16_t *src16 = nullptr;   // cold arcs by (range, bin, source): source - range start
+  uint16_t *dst16 = nullptr;   //   and destination - bin start
+  uint32_t *run_beg = nullptr; // nbins x nranges: run of range r in bin b (bin-major)
+  uint32_t *run_end = nullptr;
+  uint32_t nparts = 0;         // emit workgroups: (range, first arc, end arc)
+  uint32_t *parts = nullptr;
+  uint32_t nseg = 0;           // bin kernel workgroups
+  PrSeg *segs = nullptr;
+  uint32_t nsplit = 0;         // bins split over several segments
+  uint32_t *split_bin = nullptr;
   std::vector<void *> allocations;
 };
 
@@ -208,10 +211,10 @@ struct Work {
   size_t ms_lmask_cap = 0;
   size_t bytes = 0;                    // device bytes held (fixed part)
   void *bfs = nullptr;         // device-driven BFS loop state (bfs.hip)
-  double *pr_vals = nullptr;   // PageRank cold contributions, bin order
-  double *pr_cold = nullptr;   // PageRank cold sums per destination (kept at 0 between)
-  double *pr_hot = nullptr;    // PageRank hot sums per destination (kept at 0 between)
+  double *pr_vals = nullptr;   // PageRank cold contributions, in cold-arc storage order
+  double *pr_gsum = nullptr;   // PageRank sums of split bins (kept at 0 between iterations)
   uint64_t pr_vals_cap = 0;
+  uint64_t pr_gsum_cap = 0;
   Counters *h_cnt = nullptr;   // pinned host copy
   unsigned long long *h_exam = nullptr;
   std::vector<void *> allocations;
@@ -221,11 +224,11 @@ int work_get(gg_graph *g, Work **w);
 void work_free(Work *w);
 void bfs_state_free(void *state);
 // PageRank plan (prplan.hip): *out = null when the graph gets the plain pull
-// (not relabelled, option pr_hot_sources = 0, or no cold arcs).
+// (not relabelled, option pr_hot_sources = 0, or 2^32 arcs or more).
 int pr_plan_get(gg_graph *g, const PrPlan **out);
 void pr_plan_free(PrPlan *p);
-// One PageRank iteration on the plan: hot and cold sums, then
-// finish(v, hot + cold) of the PageRank op for every vertex.
+// One PageRank iteration on the plan: per destination bin the sums of its
+// hot and cold arcs, then finish(v, sum) of the PageRank op for every vertex.
 struct PrPull;
 int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull &op,
                       double *dacc);

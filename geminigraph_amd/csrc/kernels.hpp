@@ -355,7 +355,6 @@ struct PrPull : NoAfter {
   double *next;
   const uint32_t *outdeg;
   int last;
-  const double *cold;  // sums over the cold arcs (hot/cold plan), or null
   __device__ double identity() const { return 0.0; }
   __device__ bool dst_active(uint32_t) const { return true; }
   __device__ double value(uint32_t src, float) const { return curr[src]; }
@@ -364,7 +363,6 @@ struct PrPull : NoAfter {
     // next = 1 - d + d * next, divided by the out-degree except in the last
     // iteration (pagerank.cpp:77-97)
     const double d = 0.85;
-    if (cold) sum += cold[v];
     double nx = 1 - d + d * sum;
     if (!last) {
       uint32_t od = outdeg[v];

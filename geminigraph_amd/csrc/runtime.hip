@@ -200,6 +200,8 @@ static int64_t *option_slot(const char *name) {
   if (!strcmp(name, "level_masks")) return &o.level_masks;
   if (!strcmp(name, "level_mask_bytes")) return &o.level_mask_bytes;
   if (!strcmp(name, "pr_hot_sources")) return &o.pr_hot_sources;
+  if (!strcmp(name, "pr_bin_bits")) return &o.pr_bin_bits;
+  if (!strcmp(name, "pr_range_bits")) return &o.pr_range_bits;
   if (!strcmp(name, "heavy_waves")) return &o.heavy_waves;
   if (!strcmp(name, "bfs_pull_waves")) return &o.bfs_pull_waves;
   return nullptr;
@@ -221,6 +223,9 @@ int gg_set_option(const char *name, int64_t value) {
     return fail(GG_EINVAL, std::string(name) + " must be positive");
   if (!strcmp(name, "push_small_max") && value > 64 * 64)
     return fail(GG_EINVAL, "push_small_max must be <= 4096");
+  if ((!strcmp(name, "pr_bin_bits") || !strcmp(name, "pr_range_bits")) &&
+      (value < 8 || value > 14))
+    return fail(GG_EINVAL, std::string(name) + " must be in [8, 14]");
   if (!strcmp(name, "pull_light_max") && value > (1 << 20))
     return fail(GG_EINVAL, "pull_light_max must be <= 2^20");
   *s = value;

@@ -69,9 +69,16 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *   "pull_serial_max"  BFS pull: largest in-degree scanned by a single lane
  *   "huge_grid_waves"  waves sharing the adjacency of the largest-degree bin
  *   "count_examined"   1 = count adjacency entries read (gg_run_stats)
- *   "pr_hot_sources"   PageRank on a relabelled graph: > 0 streams the
- *                      arcs (hot sources below the value, cold arcs binned
- *                      by destination); 0 = pull (default)
+ *   "pr_hot_sources"   PageRank on a relabelled graph: H > 0 sums per
+ *                      destination bin in LDS (arcs from the H hottest
+ *                      sources gathered in source order, H capped at
+ *                      2^(32 - pr_bin_bits); the other arcs streamed by
+ *                      propagation blocking), built on the first call and
+ *                      kept with the graph (default 2^19); 0 = plain pull
+ *   "pr_bin_bits"      that plan: 2^bits destinations per bin (8..14,
+ *                      default 13)
+ *   "pr_range_bits"    that plan: 2^bits sources per cold range (8..14,
+ *                      default 14)
  *   "heavy_waves"      waves of the dense heavy-chunk grids (0 = one per chunk)
  *   "bfs_pull_waves"   waves of the BFS pull grids (0 = one per group / chunk)
  *   "level_masks"      batched BFS/BC: 1 = keep per-level root masks

@@ -158,10 +158,11 @@ def test_pagerank_heavy_vertices(gg):
     assert_rel(pr, ref)
 
 
-@pytest.mark.parametrize("hot", [0, 1, 64, 1000, 30000])
+@pytest.mark.parametrize("hot", [0, 1, 64, 1000, 30000, 1 << 20])
 def test_pagerank_hot_cold_plan(gg, hot):
-    # relabelled graph: in-edges from the `hot` highest out-degree sources are
-    # gathered by the pull, the others streamed through the cold bins
+    # relabelled graph: in-arcs from the `hot` highest out-degree sources are
+    # gathered in source order per destination bin, the others streamed by
+    # propagation blocking (1 << 20: every source is hot, no cold arcs)
     scale = 15
     V = 1 << scale
     e = oracle.rmat_edges(scale, 16, seed=61)
@@ -174,6 +175,36 @@ def test_pagerank_hot_cold_plan(gg, hot):
     assert_rel(pr, ref)
     assert_rel(pr2, ref)
     assert_rel(deltas[:20], ref_d[:20], rel=1e-8)
+
+
+@pytest.mark.parametrize("hot", [4096, 1 << 18])
+def test_pagerank_plan_many_ranges(gg, hot):
+    # 8 destination bins, several split bins, cold source ranges whose arcs
+    # span several emit parts
+    scale = 17
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, 16, seed=67)
+    g = gg.Graph.build(e, V, reorder=True)
+    say("pagerank plan many ranges", hot)
+    with gg.options(pr_hot_sources=hot):
+        pr, deltas, _ = g.pagerank(8)
+    ref, ref_d = oracle.Graph(e, V).pagerank(8)
+    assert_rel(pr, ref)
+    assert_rel(deltas[:8], ref_d[:8], rel=1e-8)
+
+
+@pytest.mark.parametrize("bins,ranges,hot", [(10, 9, 3000), (13, 14, 1 << 19), (14, 8, 700)])
+def test_pagerank_plan_sizes(gg, bins, ranges, hot):
+    # other bin and range sizes: many bins and ranges, hot ids above 2^18
+    scale = 16
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, 16, seed=71)
+    g = gg.Graph.build(e, V, reorder=True)
+    say("pagerank plan sizes", bins, ranges, hot)
+    with gg.options(pr_hot_sources=hot, pr_bin_bits=bins, pr_range_bits=ranges):
+        pr, _, _ = g.pagerank(6)
+    ref, _ = oracle.Graph(e, V).pagerank(6)
+    assert_rel(pr, ref)
 
 
 @pytest.mark.parametrize("hot", [2, 2500])
