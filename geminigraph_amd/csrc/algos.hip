@@ -971,12 +971,16 @@ __global__ __launch_bounds__(kBlock) void mss_push_kernel(const uint64_t *index,
     uint32_t u = gw < n0 ? q0[gw] : q1[gw - n0];
     relax_range(u, index[u], index[u + 1]);
   } else {
+    // the grid is rounded up to whole blocks: waves past waves2 have no share
     uint32_t hw = gw - n0 - n1;
+    if (hw >= waves2) return;
     for (uint32_t h = 0; h < n2; h++) {
       uint32_t u = q2[h];
       uint64_t b = index[u], e = index[u + 1];
       uint64_t nch = (e - b + kChunkEdges - 1) / kChunkEdges;
-      for (uint64_t c = hw; c < nch; c += waves2) {
+      // chunk c goes to wave (c + rot) % waves2, rotated per vertex
+      const uint32_t rot = (uint32_t)(((uint64_t)h * 2654435761u) % waves2);
+      for (uint64_t c = (hw + waves2 - rot) % waves2; c < nch; c += waves2) {
         uint64_t cb = b + c * kChunkEdges;
         relax_range(u, cb, cb + kChunkEdges < e ? cb + kChunkEdges : e);
       }

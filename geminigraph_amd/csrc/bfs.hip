@@ -15,11 +15,10 @@
 // for an iteration before issuing the next one: it stays one iteration ahead
 // and reads each iteration's done flag through an event-ordered copy.
 //
-// Claims: a push edge claims its target by being the first to set the
-// target's bit in the output frontier; chunks of one heavy in-list race with a
-// compare-and-swap on the vertex's depth entry, which sits at its original
-// id (relabelling scatters those, so polling them does not pile onto a few
-// cache lines).  The reference claims with a compare-and-swap on parent
+// Claims: a push edge, and each chunk of a heavy in-list, claims its vertex
+// with a compare-and-swap on the vertex's depth entry, which sits at its
+// original id (relabelling scatters those, so the atomics do not pile onto a
+// few cache lines).  The reference claims with a compare-and-swap on parent
 // (bfs.cpp:53, 71); each way keeps exactly one valid parent.  Results are
 // written in original vertex ids as vertices are found (parent[iperm[v]] =
 // iperm[u]), so a relabelled graph needs no pass over V at the end.  A
@@ -45,6 +44,8 @@ struct BfsCtl {
   int32_t force;
   uint32_t lv_cap;
   uint64_t reached, m_reach, examined, threshold;
+  uint64_t m_unvisited;  // in-arcs of the vertices not visited yet (pull's scan bound)
+  uint64_t alpha;        // 0, or: pull only if frontier out-arcs * alpha > m_unvisited
   uint32_t modes[64];
   uint64_t active[64], fresh[64], exam[64], unvisited[64];
 };
@@ -54,6 +55,7 @@ struct BfsOut {
   uint32_t *parent, *depth;  // indexed by original ids
   const uint32_t *iperm;     // internal -> original (null: not relabelled)
   const uint32_t *deg;       // out-degree by internal id
+  const uint32_t *indeg;     // in-degree by internal id
   uint64_t *visited, *fout;
 };
 
@@ -61,39 +63,47 @@ __device__ __forceinline__ uint32_t orig_id(const uint32_t *iperm, uint32_t v) {
   return iperm ? iperm[v] : v;
 }
 
-// Lane 0 of a wave adds its counts to one of kScalarSlots spread slots.
-__device__ __forceinline__ void bfs_count(uint64_t *set, uint32_t slot, uint64_t n, uint64_t e) {
+// Lane 0 of a wave adds its counts (vertices found, their out-arcs and their
+// in-arcs) to one of kScalarSlots spread slots.
+__device__ __forceinline__ void bfs_count(uint64_t *set, uint32_t slot, uint64_t n, uint64_t e,
+                                          uint64_t ie) {
   if (!n) return;
   uint64_t *p = set + (size_t)(slot % kScalarSlots) * kCountStride;
   atomicAdd((unsigned long long *)p, (unsigned long long)n);
   atomicAdd((unsigned long long *)(p + 1), (unsigned long long)e);
+  atomicAdd((unsigned long long *)(p + 2), (unsigned long long)ie);
 }
 
 __device__ __forceinline__ void bfs_wave_flush(uint64_t *set, uint32_t gw, uint64_t cnt,
-                                               uint64_t edg, unsigned long long *exam,
-                                               uint64_t ex) {
+                                               uint64_t edg, uint64_t ind,
+                                               unsigned long long *exam, uint64_t ex) {
   cnt = wave_sum_u64(cnt);
   edg = wave_sum_u64(edg);
+  ind = wave_sum_u64(ind);
   ex = wave_sum_u64(ex);
   if (lane_id() == 0) {
-    bfs_count(set, gw, cnt, edg);
+    bfs_count(set, gw, cnt, edg, ind);
     count_examined(exam, ex);
   }
 }
 
 // Push claim: `visited` holds every vertex of the earlier levels and of the
-// frontier (the bin kernel adds the frontier before the push), so the first
-// edge to set dst's bit in the output frontier claims it; only the winner
-// touches the scattered outputs.
+// frontier (the bin kernel adds the frontier before the push).  An edge whose
+// target is neither visited nor already in the output frontier races a
+// compare-and-swap on the target's depth entry, at its original id: with
+// relabelled hubs packed into a few bitmap words, claiming on those words
+// instead would pile returning atomics onto a few cache lines.  The winner
+// writes the parent and sets the frontier bit (an atomic whose result is
+// not needed).
 __device__ __forceinline__ bool bfs_claim(const BfsOut &o, uint32_t src, uint32_t dst,
                                           uint32_t level) {
   const uint64_t bit = 1ULL << (dst & 63);
   if ((o.visited[dst >> 6] | o.fout[dst >> 6]) & bit) return false;  // plain reads first
-  if (atomicOr((unsigned long long *)&o.fout[dst >> 6], (unsigned long long)bit) & bit)
-    return false;
   const uint32_t od = orig_id(o.iperm, dst);
+  if (atomicCAS(&o.depth[od], kUnset, level) != kUnset) return false;
   o.parent[od] = orig_id(o.iperm, src);
-  o.depth[od] = level;
+  __hip_atomic_fetch_or((unsigned long long *)&o.fout[dst >> 6], (unsigned long long)bit,
+                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return true;
 }
 
@@ -110,7 +120,7 @@ struct BfsHostFlag {
 // traversal.  Called by all kBlock threads of one block.
 __device__ void bfs_decide(BfsCtl *c, uint64_t *set, Counters *qc, unsigned long long *exam,
                            uint32_t *lv, BfsHostFlag *hf) {
-  __shared__ uint64_t sn[kBlock], se[kBlock], sx[kBlock];
+  __shared__ uint64_t sn[kBlock], se[kBlock], si[kBlock], sx[kBlock];
   __shared__ uint32_t s_done;
   const int t = threadIdx.x;  // kBlock == kScalarSlots: one slot per thread
   if (t == 0) s_done = c->done;
@@ -119,8 +129,10 @@ __device__ void bfs_decide(BfsCtl *c, uint64_t *set, Counters *qc, unsigned long
   uint64_t *p = set + (size_t)t * kCountStride;
   sn[t] = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   se[t] = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  si[t] = __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_store(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_store(p + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 2, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   sx[t] = 0;
   if (exam && t < 64) {
     sx[t] = __hip_atomic_load(&exam[t * 16], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -131,6 +143,7 @@ __device__ void bfs_decide(BfsCtl *c, uint64_t *set, Counters *qc, unsigned long
     if (t < s) {
       sn[t] += sn[t + s];
       se[t] += se[t + s];
+      si[t] += si[t + s];
       sx[t] += sx[t + s];
     }
     __syncthreads();
@@ -152,12 +165,17 @@ __device__ void bfs_decide(BfsCtl *c, uint64_t *set, Counters *qc, unsigned long
   }
   c->reached += total;
   c->m_reach += edges;
+  c->m_unvisited -= si[0] < c->m_unvisited ? si[0] : c->m_unvisited;
   if (i < c->lv_cap) lv[i] = (uint32_t)total;
-  // core/graph.hpp:1509: sparse iff active out-edges < |E| / divisor
+  // core/graph.hpp:1509: sparse iff active out-edges < |E| / divisor; with
+  // alpha, also sparse while the pull would have to scan more than alpha
+  // times the frontier's out-arcs (the unvisited vertices' in-arcs bound it)
+  const bool sparse =
+      edges < c->threshold || (c->alpha && edges * c->alpha <= c->m_unvisited);
   uint32_t mode = c->force == 1   ? kBfsPush
                   : c->force == 2 ? kBfsPull
-                  : edges < c->threshold ? kBfsPush
-                                         : kBfsPull;
+                  : sparse        ? kBfsPush
+                                  : kBfsPull;
   if (i < 64) {
     c->modes[i] = mode == kBfsPush ? 0u : 1u;
     c->active[i] = total;
@@ -174,10 +192,12 @@ __device__ void bfs_decide(BfsCtl *c, uint64_t *set, Counters *qc, unsigned long
   __hip_atomic_store(&hf->decided, i + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// Output arrays to "unreached" (bfs.cpp:35-36), bitmaps and count slots to 0.
+// Output arrays to "unreached" (bfs.cpp:35-36), frontiers and count slots to
+// 0.  Vertices without in-arcs start in `visited`: no step can find them (the
+// root is seeded separately), so the bottom-up step skips their groups.
 __global__ void bfs_init_kernel(uint32_t *parent, uint32_t *depth, uint32_t V, uint64_t *visited,
-                                uint64_t *f0, uint64_t *f1, size_t words, uint64_t *sets,
-                                size_t set_words, unsigned long long *exam) {
+                                const uint64_t *no_in, uint64_t *f0, uint64_t *f1, size_t words,
+                                uint64_t *sets, size_t set_words, unsigned long long *exam) {
   const size_t s = (size_t)gridDim.x * blockDim.x;
   const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (size_t i = i0; i < V; i += s) {
@@ -185,7 +205,7 @@ __global__ void bfs_init_kernel(uint32_t *parent, uint32_t *depth, uint32_t V, u
     depth[i] = kUnset;
   }
   for (size_t i = i0; i < words; i += s) {
-    visited[i] = 0;
+    visited[i] = no_in[i];
     f0[i] = 0;
     f1[i] = 0;
   }
@@ -199,8 +219,8 @@ __global__ void bfs_init_kernel(uint32_t *parent, uint32_t *depth, uint32_t V, u
 __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
     BfsCtl *c, uint64_t *set0, Counters *qc, unsigned long long *exam, uint32_t *lv,
     BfsHostFlag *hf, uint64_t *visited, uint64_t *fin, uint32_t root, uint32_t root_orig,
-    uint32_t *parent, uint32_t *depth, const uint32_t *deg, uint32_t V, int32_t force,
-    uint64_t threshold, uint32_t lv_cap) {
+    uint32_t *parent, uint32_t *depth, const uint32_t *deg, const uint32_t *indeg, uint32_t V,
+    int32_t force, uint64_t threshold, uint64_t arcs, uint64_t alpha, uint32_t lv_cap) {
   if (threadIdx.x == 0) {
     c->done = 0;
     c->iter = 0;
@@ -214,6 +234,8 @@ __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
     c->m_reach = 0;
     c->examined = 0;
     c->threshold = threshold;
+    c->m_unvisited = arcs;
+    c->alpha = alpha;
     const uint64_t bit = 1ULL << (root & 63);
     visited[root >> 6] |= bit;
     fin[root >> 6] |= bit;
@@ -221,6 +243,7 @@ __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
     depth[root_orig] = 0;
     set0[0] = 1;
     set0[1] = deg[root];
+    set0[2] = indeg[root];
     __threadfence();
   }
   __syncthreads();
@@ -264,11 +287,12 @@ __global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOu
   const int lane = lane_id();
   const uint32_t n0 = a.qc->n[0], n1 = a.qc->n[1], n2 = a.qc->n[2];
   const uint32_t waves0 = (n0 + 63) / 64;
-  uint64_t cnt = 0, edg = 0, ex = 0;
+  uint64_t cnt = 0, edg = 0, ind = 0, ex = 0;
   auto visit = [&](uint32_t src, uint32_t dst) {
     if (bfs_claim(o, src, dst, level)) {
       cnt++;
       edg += o.deg[dst];
+      ind += o.indeg[dst];
     }
   };
   for (uint32_t vw = gw; vw < waves0 + n1; vw += W) {
@@ -300,14 +324,17 @@ __global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOu
     const uint32_t v = a.q2[h];
     const uint64_t b = a.index[v], e1 = a.index[v + 1];
     const uint64_t nch = (e1 - b + kChunkEdges - 1) / kChunkEdges;
-    for (uint64_t ch = gw; ch < nch; ch += W) {
+    // chunk ch of vertex h goes to wave (ch + rot) % W: a vertex with fewer
+    // chunks than waves lands on a different set of waves than the last one
+    const uint32_t rot = (uint32_t)(((uint64_t)h * 2654435761u) % W);
+    for (uint64_t ch = (gw + W - rot) % W; ch < nch; ch += W) {
       const uint64_t cb = b + ch * kChunkEdges;
       const uint64_t ce = cb + kChunkEdges < e1 ? cb + kChunkEdges : e1;
       for (uint64_t e = cb + lane; e < ce; e += 64) visit(v, stream_load(&a.nbr[e]));
       if (lane == 0) ex += ce - cb;
     }
   }
-  bfs_wave_flush(set_out, gw, cnt, edg, exam, ex);
+  bfs_wave_flush(set_out, gw, cnt, edg, ind, exam, ex);
 }
 
 struct BfsPullArgs {
@@ -330,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOu
   const uint32_t level = c->level;
   const uint32_t W = gridDim.x * kWavesPerBlock;
   const int lane = lane_id();
-  uint64_t cnt = 0, edg = 0, examined = 0;
+  uint64_t cnt = 0, edg = 0, ind = 0, examined = 0;
   for (uint32_t g = global_wave(); g < a.ngroups; g += W) {
     const uint64_t visw = o.visited[g] | a.fin[g];
     if (visw == ~0ULL) {
@@ -394,6 +421,7 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOu
       o.parent[ov] = orig_id(o.iperm, found);
       o.depth[ov] = level;
       edg += o.deg[v];
+      ind += deg;
     }
     if (lane == 0) {
       o.fout[g] = m;
@@ -401,7 +429,7 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOu
       cnt += (uint64_t)__popcll(m);
     }
   }
-  bfs_wave_flush(set_out, global_wave(), cnt, edg, exam, examined);
+  bfs_wave_flush(set_out, global_wave(), cnt, edg, ind, exam, examined);
 }
 
 struct BfsHeavyArgs {
@@ -422,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_heavy_kernel(const BfsCtl *c,
   const uint32_t level = c->level;
   const uint32_t W = gridDim.x * kWavesPerBlock;
   const int lane = lane_id();
-  uint64_t cnt = 0, edg = 0, examined = 0;
+  uint64_t cnt = 0, edg = 0, ind = 0, examined = 0;
   for (uint32_t ch = global_wave(); ch < a.n_chunks; ch += W) {
     const uint32_t v = a.chunk_v[ch];
     const uint64_t bit = 1ULL << (v & 63);
@@ -444,13 +472,14 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_heavy_kernel(const BfsCtl *c,
           atomicOr((unsigned long long *)&o.fout[v >> 6], (unsigned long long)bit);
           cnt++;
           edg += o.deg[v];
+          ind += o.indeg[v];
         }
         break;
       }
     }
   }
   if (lane != 0) examined = 0;  // counted once per wave
-  bfs_wave_flush(set_out, global_wave(), cnt, edg, exam, examined);
+  bfs_wave_flush(set_out, global_wave(), cnt, edg, ind, exam, examined);
 }
 
 // Device state of the BFS loop, kept in the traversal context.
@@ -527,10 +556,12 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
   hf->done = 0;
   DeviceTimer tm;
   hipLaunchKernelGGL(bfs_init_kernel, dim3(grid1(V)), dim3(kBlock), 0, st, d_parent, d_depth, V,
-                     w->visited, w->f[0].bits, w->f[1].bits, words, s->sets, 2 * set_words, exam);
+                     w->visited, in.empty, w->f[0].bits, w->f[1].bits, words, s->sets,
+                     2 * set_words, exam);
   hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(kBlock), 0, st, s->ctl, s->sets, s->qc, exam,
                      s->lv, s->h_flag, w->visited, w->f[0].bits, r_int, root, d_parent, d_depth,
-                     out.degree, V, (int32_t)opt.force_mode, threshold, s->lv_cap);
+                     out.degree, in.degree, V, (int32_t)opt.force_mode, threshold, g->arcs,
+                     (uint64_t)opt.bfs_alpha, s->lv_cap);
   // fixed launch shapes: the frontier sizes live on the device
   const unsigned bin_grid = grid1(words, 2048);
   const unsigned push_grid = grid_waves((uint64_t)opt.huge_grid_waves);
@@ -554,7 +585,7 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
     if (n > V + 2) return fail(GG_EHIP, "gg_bfs: traversal did not terminate");
     uint64_t *fin = w->f[n & 1].bits, *fout = w->f[(n + 1) & 1].bits;
     uint64_t *set_out = s->sets + (size_t)((n + 1) & 1) * set_words;
-    BfsOut o{d_parent, d_depth, iperm, out.degree, w->visited, fout};
+    BfsOut o{d_parent, d_depth, iperm, out.degree, in.degree, w->visited, fout};
     {
       ProfScope p("compact");
       hipLaunchKernelGGL(bfs_bin_kernel, dim3(bin_grid), dim3(kBlock), 0, st, s->ctl, fin, fout,

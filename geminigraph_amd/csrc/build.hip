@@ -270,6 +270,12 @@ static int build_chunks(gg_graph *g, Dir &d) {
       ce.push_back(std::min<uint64_t>(b + kChunkEdges, idx[v + 1]));
     }
   }
+  // vertices without arcs in this direction (a bottom-up BFS never finds them)
+  std::vector<uint64_t> eb((size_t)V / 64 + 1, 0);
+  for (uint32_t v = 0; v < V; v++)
+    if (!deg[v]) eb[v >> 6] |= 1ULL << (v & 63);
+  GG_TRY(graph_alloc(g, (void **)&d.empty, eb.size() * 8));
+  GG_HIP(hipMemcpyAsync(d.empty, eb.data(), eb.size() * 8, hipMemcpyHostToDevice, stream()));
   d.n_heavy = (uint32_t)hv.size();
   d.n_chunks = (uint32_t)cv.size();
   hf.push_back((uint32_t)cv.size());

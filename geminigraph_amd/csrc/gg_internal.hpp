@@ -53,13 +53,14 @@ struct Options {
   int64_t switch_divisor = 200;
   int64_t force_mode = 0;        // 0 auto, 1 push, 2 pull
   int64_t push_small_max = 64;   // in-wave load-balanced bin
-  int64_t push_big_max = 4096;   // one wave per vertex; above: shared by the grid
+  int64_t push_big_max = 1024;   // one wave per vertex; above: shared by the grid
   int64_t pull_light_max = 1024; // in-group; above: heavy chunks (fixed at build)
   int64_t pull_serial_max = 128; // BFS pull: lane-serial scan (4 probes in flight)
   int64_t huge_grid_waves = 4096;
   int64_t count_examined = 0;    // count adjacency entries read (roofline model)
   int64_t level_masks = 1;       // batched BFS/BC: per-level root masks (else depth rows)
   int64_t level_mask_bytes = int64_t(4) << 30;  // size limit of that store
+  int64_t bfs_alpha = 24;        // BFS: push while frontier out-arcs * alpha <= unvisited in-arcs
   int64_t pr_hot_sources = 1 << 19;  // PageRank bin plan: hot sources (0 = plain pull)
   int64_t pr_bin_bits = 13;      // PageRank bin plan: 2^bits destinations per bin
   int64_t pr_range_bits = 14;    // PageRank bin plan: 2^bits sources per cold range
@@ -94,6 +95,7 @@ struct Dir {
   uint32_t *nbr = nullptr;    // arcs
   float *w = nullptr;         // arcs or null
   uint32_t *degree = nullptr; // V
+  uint64_t *empty = nullptr;  // V/64 + 1 words: bit v set iff degree[v] == 0
   // heavy chunks (degree > light limit at build time)
   uint32_t n_chunks = 0;
   uint32_t *chunk_v = nullptr;
@@ -106,11 +108,9 @@ struct Dir {
   uint32_t light_max = 0;
 };
 
-// PageRank hot/cold plan of a relabelled graph (built on first use, see
-// prplan.hip).  Sources with internal id < H are hot: their contributions
-// fit in L2.  Hot arcs are packed into one array streamed in fixed chunks;
-// the out-arcs of the other sources are cold: each iteration streams their
-// contributions into per-destination-range bins, which workgroups sum in LDS.
+// PageRank bin plan of a relabelled graph (built on first use, see
+// prplan.hip): per destination bin the in-arcs from the H hottest sources,
+// sorted by source; the other (cold) arcs stored for propagation blocking.
 struct PrSeg;
 struct PrPlan {
   int transposed = 0;          // orientation it was built for

@@ -321,15 +321,20 @@ __global__ __launch_bounds__(kBlock) void push_kernel(PushArgs a, Op op) {
     }
     if (lane == 0) count_examined(a.exam, e1 - b);
   } else {
-    // huge bin: every remaining wave takes chunks of every huge vertex
+    // huge bin: every remaining wave takes chunks of every huge vertex (the
+    // grid is rounded up to whole blocks: waves past waves2 have no share)
     uint32_t hw = gw - a.waves0 - a.waves1;
+    if (hw >= a.waves2) return;
     uint64_t done = 0;
     for (uint32_t h = 0; h < a.n2; h++) {
       uint32_t v = a.q2[h];
       uint64_t b = a.index[v], e1 = a.index[v + 1];
       Msg m = op.msg(v, a.s2 ? a.s2[h] : 0u);
       uint64_t nch = (e1 - b + kChunkEdges - 1) / kChunkEdges;
-      for (uint64_t c = hw; c < nch; c += a.waves2) {
+      // chunk c goes to huge-bin wave (c + rot) % waves2, rotated per vertex
+      // so vertices with few chunks spread over all the waves
+      const uint32_t rot = (uint32_t)(((uint64_t)h * 2654435761u) % a.waves2);
+      for (uint64_t c = (hw + a.waves2 - rot) % a.waves2; c < nch; c += a.waves2) {
         uint64_t cb = b + c * kChunkEdges;
         uint64_t ce = cb + kChunkEdges < e1 ? cb + kChunkEdges : e1;
         for (uint64_t e = cb + lane; e < ce; e += 64) {

@@ -201,6 +201,7 @@ static int64_t *option_slot(const char *name) {
   if (!strcmp(name, "level_mask_bytes")) return &o.level_mask_bytes;
   if (!strcmp(name, "pr_hot_sources")) return &o.pr_hot_sources;
   if (!strcmp(name, "pr_bin_bits")) return &o.pr_bin_bits;
+  if (!strcmp(name, "bfs_alpha")) return &o.bfs_alpha;
   if (!strcmp(name, "pr_range_bits")) return &o.pr_range_bits;
   if (!strcmp(name, "heavy_waves")) return &o.heavy_waves;
   if (!strcmp(name, "bfs_pull_waves")) return &o.bfs_pull_waves;
@@ -216,7 +217,10 @@ int gg_set_option(const char *name, int64_t value) {
   if (flag && (value < 0 || value > 1)) return fail(GG_EINVAL, std::string(name) + " must be 0 or 1");
   if (!strcmp(name, "bfs_pull_waves") && (value < 0 || value > (1 << 24)))
     return fail(GG_EINVAL, "bfs_pull_waves must be in [0, 2^24]");
+  if (!strcmp(name, "bfs_alpha") && value > (1 << 20))
+    return fail(GG_EINVAL, "bfs_alpha must be in [0, 2^20]");
   const bool zero_ok = !strcmp(name, "bfs_pull_waves") || !strcmp(name, "pr_hot_sources") ||
+                       !strcmp(name, "bfs_alpha") ||
                        !strcmp(name, "heavy_waves");
   if (zero_ok && value < 0) return fail(GG_EINVAL, std::string(name) + " must be >= 0");
   if (strcmp(name, "force_mode") && !zero_ok && !flag && value <= 0)

@@ -69,6 +69,10 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *   "pull_serial_max"  BFS pull: largest in-degree scanned by a single lane
  *   "huge_grid_waves"  waves sharing the adjacency of the largest-degree bin
  *   "count_examined"   1 = count adjacency entries read (gg_run_stats)
+ *   "bfs_alpha"        BFS: also push while the frontier's out-arcs times
+ *                      alpha are at most the in-arcs of the unvisited
+ *                      vertices (what a pull would scan); 0 = the divisor
+ *                      rule alone (default 24)
  *   "pr_hot_sources"   PageRank on a relabelled graph: H > 0 sums per
  *                      destination bin in LDS (arcs from the H hottest
  *                      sources gathered in source order, H capped at

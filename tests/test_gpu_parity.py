@@ -351,6 +351,36 @@ def test_bc_hub_graph(gg):
         assert_rel(dep, rdep)
 
 
+@pytest.mark.parametrize("huge_waves", [1, 3, 6])
+def test_push_huge_bin_small_grid(gg, huge_waves):
+    # huge frontier vertices (12000 and 9000 out-arcs: 12 and 9 chunks) spread
+    # over fewer huge-bin waves than chunks, in a grid rounded up to whole
+    # blocks: every chunk must be expanded exactly once (BC adds per arc)
+    n = 30000
+    pairs = [(0, i) for i in range(1, 12001)] + [(i, 12001 + (i % 50)) for i in range(1, 12001)] + \
+            [(12001 + j, 20000) for j in range(50)] + [(20000, 21000 + i) for i in range(9000)] + \
+            [(i, 20000) for i in range(1, 3000)]
+    w = (np.arange(len(pairs)) % 7 + 1) * np.float32(0.375)
+    e = oracle.edges_from_pairs(pairs, weights=w)
+    g = gg.Graph.build(e, n)
+    o = oracle.Graph(e, n)
+    say("push huge bin", huge_waves)
+    with gg.options(force_mode=MODES["push"], huge_grid_waves=huge_waves, push_big_max=1024):
+        dep, inv, fr, _ = g.bc(0)
+        dist, sfr, _ = g.sssp(0)
+        _, depth, bfr, _ = g.bfs(0)
+    rdep, rinv, rfr = o.bc(0)
+    assert np.array_equal(fr, rfr)
+    assert_rel(inv, rinv)
+    assert_rel(dep, rdep)
+    rdist, rsfr = o.sssp(0)
+    assert np.array_equal(dist.view(np.uint32), rdist.view(np.uint32))
+    assert np.array_equal(sfr, rsfr)
+    _, rdepth, rbfr = o.bfs(0)
+    assert np.array_equal(depth, rdepth)
+    assert np.array_equal(bfr, rbfr)
+
+
 def test_repeat_runs_identical(gg):
     e = oracle.rmat_edges(14, seed=31)
     V = 1 << 14
