@@ -34,6 +34,8 @@
 namespace gg {
 
 enum : uint32_t { kBfsPush = 0, kBfsPull = 1, kBfsDone = 2 };
+constexpr uint32_t kHubBatch = 512;  // huge frontier vertices staged per push round
+static_assert(kHubBatch % kBlock == 0, "whole rows of the staged batch per thread");
 constexpr int kCountStride = 16;  // u64 per count slot: one 128-byte line
 
 // Device control block of one traversal (done and iter first: the host
@@ -320,19 +322,74 @@ __global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOu
       if (lane == 0) ex += e1 - b;
     }
   }
-  for (uint32_t h = 0; h < n2; h++) {
-    const uint32_t v = a.q2[h];
-    const uint64_t b = a.index[v], e1 = a.index[v + 1];
-    const uint64_t nch = (e1 - b + kChunkEdges - 1) / kChunkEdges;
-    // chunk ch of vertex h goes to wave (ch + rot) % W: a vertex with fewer
-    // chunks than waves lands on a different set of waves than the last one
-    const uint32_t rot = (uint32_t)(((uint64_t)h * 2654435761u) % W);
-    for (uint64_t ch = (gw + W - rot) % W; ch < nch; ch += W) {
-      const uint64_t cb = b + ch * kChunkEdges;
+  // Huge vertices: their chunks form one global sequence, chunk g going to
+  // wave g % W.  Each block stages kHubBatch of them at a time in LDS (list
+  // bounds and the prefix of chunk counts), so a wave finds the vertex of a
+  // chunk by a binary search in LDS instead of walking the list.
+  __shared__ uint32_t s_v[kHubBatch];
+  __shared__ uint64_t s_b[kHubBatch], s_e[kHubBatch];
+  __shared__ uint64_t s_pre[kHubBatch + 1];
+  __shared__ uint64_t s_wsum[kWavesPerBlock];
+  uint64_t coff = 0;  // chunks of the earlier batches
+  for (uint32_t base = 0; base < n2; base += kHubBatch) {
+    const uint32_t nb = n2 - base < kHubBatch ? n2 - base : kHubBatch;
+    __syncthreads();  // the previous batch is done with the LDS arrays
+    uint64_t mine[kHubBatch / kBlock];
+    uint64_t tsum = 0;
+#pragma unroll
+    for (int r = 0; r < (int)(kHubBatch / kBlock); r++) {
+      const uint32_t i = threadIdx.x * (kHubBatch / kBlock) + r;
+      mine[r] = 0;
+      if (i < nb) {
+        const uint32_t v = a.q2[base + i];
+        const uint64_t b = a.index[v], e1 = a.index[v + 1];
+        s_v[i] = v;
+        s_b[i] = b;
+        s_e[i] = e1;
+        mine[r] = (e1 - b + kChunkEdges - 1) / kChunkEdges;
+      }
+      tsum += mine[r];
+    }
+    // exclusive scan of the per-thread sums over the block
+    uint64_t inc = tsum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint64_t y = shfl_u64(inc, lane >= d ? lane - d : lane);
+      if (lane >= d) inc += y;
+    }
+    if (lane == 63) s_wsum[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    uint64_t wbase = 0, total = 0;
+    for (int i = 0; i < kWavesPerBlock; i++) {
+      if (i < (int)(threadIdx.x >> 6)) wbase += s_wsum[i];
+      total += s_wsum[i];
+    }
+    uint64_t run = wbase + inc - tsum;
+#pragma unroll
+    for (int r = 0; r < (int)(kHubBatch / kBlock); r++) {
+      const uint32_t i = threadIdx.x * (kHubBatch / kBlock) + r;
+      if (i < nb) s_pre[i] = run;
+      run += mine[r];
+    }
+    if (threadIdx.x == 0) s_pre[nb] = total;
+    __syncthreads();
+    for (uint64_t j = (gw + W - coff % W) % W; j < total; j += W) {
+      uint32_t lo = 0, hi = nb;  // last vertex with s_pre <= j
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_pre[mid] <= j)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      const uint32_t v = s_v[lo];
+      const uint64_t e1 = s_e[lo];
+      const uint64_t cb = s_b[lo] + (j - s_pre[lo]) * kChunkEdges;
       const uint64_t ce = cb + kChunkEdges < e1 ? cb + kChunkEdges : e1;
       for (uint64_t e = cb + lane; e < ce; e += 64) visit(v, stream_load(&a.nbr[e]));
       if (lane == 0) ex += ce - cb;
     }
+    coff += total;
   }
   bfs_wave_flush(set_out, gw, cnt, edg, ind, exam, ex);
 }
