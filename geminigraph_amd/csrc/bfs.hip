@@ -397,6 +397,8 @@ __global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOu
 struct BfsPullArgs {
   const uint64_t *index;
   const uint32_t *nbr;
+  const uint32_t *first;  // first in-neighbour of each vertex
+  const uint32_t *indeg;
   const uint64_t *fin;
   uint32_t V, light_max, serial_max, ngroups;
 };
@@ -426,15 +428,24 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOu
     }
     const uint32_t v = g * 64 + lane;
     bool cand = v < a.V && !((visw >> lane) & 1ULL);
+    const uint32_t deg = cand ? a.indeg[v] : 0;
+    cand = cand && deg > 0 && deg <= a.light_max;
+    // the first in-neighbour (the highest-degree one on a relabelled graph)
+    // decides most vertices without touching the list
+    uint32_t found = kUnset;
+    if (cand) {
+      const uint32_t f = a.first[v];
+      if (test_bit(a.fin, f)) found = f;
+      examined += 1;
+    }
+    cand = cand && found == kUnset && deg > 1;
     uint64_t b = 0, e1 = 0;
     if (cand) {
       b = a.index[v];
-      e1 = a.index[v + 1];
+      e1 = b + deg;
+      b += 1;
     }
-    const uint64_t deg = e1 - b;
-    cand = cand && deg > 0 && deg <= a.light_max;
     const bool serial = cand && deg <= a.serial_max;
-    uint32_t found = kUnset;
     if (serial) {
       // four neighbours and their frontier words in flight per step
       for (uint64_t e = b; e < e1 && found == kUnset; e += 4) {
@@ -658,7 +669,8 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
     }
     {
       ProfScope p("bfs_pull");
-      BfsPullArgs a{in.index, in.nbr, fin, V, in.light_max, (uint32_t)opt.pull_serial_max,
+      BfsPullArgs a{in.index, in.nbr, in.first, in.degree, fin, V, in.light_max,
+                    (uint32_t)opt.pull_serial_max,
                     ngroups};
       hipLaunchKernelGGL(bfs_pull_kernel, dim3(pull_grid), dim3(kBlock), 0, st, s->ctl, o, a,
                          set_out, exam);

@@ -244,6 +244,15 @@ static int graph_alloc(gg_graph *g, void **p, size_t bytes) {
   return GG_OK;
 }
 
+__global__ void first_nbr_kernel(const uint64_t *index, const uint32_t *nbr, uint32_t V,
+                                 uint32_t *first) {
+  const uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s) {
+    const uint64_t b = index[v];
+    first[v] = b < index[v + 1] ? nbr[b] : kUnset;
+  }
+}
+
 // Heavy-vertex chunk list of one direction (degree > light_max), computed on
 // the host from the degree array: the dense-mode analogue of tune_chunks
 // (core/graph.hpp:1341-1404), which balances threads by in-edges.
@@ -276,6 +285,13 @@ static int build_chunks(gg_graph *g, Dir &d) {
     if (!deg[v]) eb[v >> 6] |= 1ULL << (v & 63);
   GG_TRY(graph_alloc(g, (void **)&d.empty, eb.size() * 8));
   GG_HIP(hipMemcpyAsync(d.empty, eb.data(), eb.size() * 8, hipMemcpyHostToDevice, stream()));
+  // first neighbour of every list: a bottom-up BFS step tests it before
+  // reading the list (lists are sorted, so on a relabelled graph it is the
+  // highest-degree neighbour, the likeliest to be in the frontier)
+  GG_TRY(graph_alloc(g, (void **)&d.first, (size_t)std::max(V, 1u) * 4));
+  const unsigned fgrid = (unsigned)std::min<uint64_t>(((uint64_t)V + kBlock - 1) / kBlock, 65536);
+  hipLaunchKernelGGL(first_nbr_kernel, dim3(std::max(fgrid, 1u)), dim3(kBlock), 0, stream(),
+                     d.index, d.nbr, V, d.first);
   d.n_heavy = (uint32_t)hv.size();
   d.n_chunks = (uint32_t)cv.size();
   hf.push_back((uint32_t)cv.size());

@@ -96,6 +96,7 @@ struct Dir {
   float *w = nullptr;         // arcs or null
   uint32_t *degree = nullptr; // V
   uint64_t *empty = nullptr;  // V/64 + 1 words: bit v set iff degree[v] == 0
+  uint32_t *first = nullptr;  // V: first (smallest) neighbour of each list, kUnset if empty
   // heavy chunks (degree > light limit at build time)
   uint32_t n_chunks = 0;
   uint32_t *chunk_v = nullptr;
