@@ -35,6 +35,7 @@ namespace gg {
 
 enum : uint32_t { kBfsPush = 0, kBfsPull = 1, kBfsDone = 2 };
 constexpr uint32_t kHubBatch = 512;  // huge frontier vertices staged per push round
+constexpr uint32_t kPushChunk = 256;  // arcs of a huge frontier vertex per wave step
 static_assert(kHubBatch % kBlock == 0, "whole rows of the staged batch per thread");
 constexpr int kCountStride = 16;  // u64 per count slot: one 128-byte line
 
@@ -55,15 +56,9 @@ struct BfsCtl {
 // Where found vertices go.
 struct BfsOut {
   uint32_t *parent, *depth;  // indexed by original ids
-  const uint32_t *iperm;     // internal -> original (null: not relabelled)
-  const uint32_t *deg;       // out-degree by internal id
-  const uint32_t *indeg;     // in-degree by internal id
+  const uint4 *vinfo;        // by internal id: {original id, out-degree, in-degree, first in-nbr}
   uint64_t *visited, *fout;
 };
-
-__device__ __forceinline__ uint32_t orig_id(const uint32_t *iperm, uint32_t v) {
-  return iperm ? iperm[v] : v;
-}
 
 // Lane 0 of a wave adds its counts (vertices found, their out-arcs and their
 // in-arcs) to one of kScalarSlots spread slots.
@@ -97,13 +92,13 @@ __device__ __forceinline__ void bfs_wave_flush(uint64_t *set, uint32_t gw, uint6
 // instead would pile returning atomics onto a few cache lines.  The winner
 // writes the parent and sets the frontier bit (an atomic whose result is
 // not needed).
-__device__ __forceinline__ bool bfs_claim(const BfsOut &o, uint32_t src, uint32_t dst,
-                                          uint32_t level) {
+__device__ __forceinline__ bool bfs_claim(const BfsOut &o, uint32_t src_orig, uint32_t dst,
+                                          uint32_t level, uint4 &vi) {
   const uint64_t bit = 1ULL << (dst & 63);
   if ((o.visited[dst >> 6] | o.fout[dst >> 6]) & bit) return false;  // plain reads first
-  const uint32_t od = orig_id(o.iperm, dst);
-  if (atomicCAS(&o.depth[od], kUnset, level) != kUnset) return false;
-  o.parent[od] = orig_id(o.iperm, src);
+  vi = o.vinfo[dst];
+  if (atomicCAS(&o.depth[vi.x], kUnset, level) != kUnset) return false;
+  o.parent[vi.x] = src_orig;
   __hip_atomic_fetch_or((unsigned long long *)&o.fout[dst >> 6], (unsigned long long)bit,
                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return true;
@@ -221,8 +216,8 @@ __global__ void bfs_init_kernel(uint32_t *parent, uint32_t *depth, uint32_t V, u
 __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
     BfsCtl *c, uint64_t *set0, Counters *qc, unsigned long long *exam, uint32_t *lv,
     BfsHostFlag *hf, uint64_t *visited, uint64_t *fin, uint32_t root, uint32_t root_orig,
-    uint32_t *parent, uint32_t *depth, const uint32_t *deg, const uint32_t *indeg, uint32_t V,
-    int32_t force, uint64_t threshold, uint64_t arcs, uint64_t alpha, uint32_t lv_cap) {
+    uint32_t *parent, uint32_t *depth, const uint4 *vinfo, uint32_t V, int32_t force,
+    uint64_t threshold, uint64_t arcs, uint64_t alpha, uint32_t lv_cap) {
   if (threadIdx.x == 0) {
     c->done = 0;
     c->iter = 0;
@@ -244,8 +239,8 @@ __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
     parent[root_orig] = root_orig;
     depth[root_orig] = 0;
     set0[0] = 1;
-    set0[1] = deg[root];
-    set0[2] = indeg[root];
+    set0[1] = vinfo[root].y;
+    set0[2] = vinfo[root].z;
     __threadfence();
   }
   __syncthreads();
@@ -290,11 +285,12 @@ __global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOu
   const uint32_t n0 = a.qc->n[0], n1 = a.qc->n[1], n2 = a.qc->n[2];
   const uint32_t waves0 = (n0 + 63) / 64;
   uint64_t cnt = 0, edg = 0, ind = 0, ex = 0;
-  auto visit = [&](uint32_t src, uint32_t dst) {
-    if (bfs_claim(o, src, dst, level)) {
+  auto visit = [&](uint32_t src_orig, uint32_t dst) {
+    uint4 vi;
+    if (bfs_claim(o, src_orig, dst, level, vi)) {
       cnt++;
-      edg += o.deg[dst];
-      ind += o.indeg[dst];
+      edg += vi.y;
+      ind += vi.z;
     }
   };
   for (uint32_t vw = gw; vw < waves0 + n1; vw += W) {
@@ -302,6 +298,7 @@ __global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOu
       const uint32_t i = vw * 64 + lane;
       const bool valid = i < n0;
       const uint32_t v = valid ? a.q0[i] : 0;
+      const uint32_t vo = valid ? o.vinfo[v].x : 0;
       const uint64_t b = valid ? a.index[v] : 0;
       const uint32_t d = valid ? (uint32_t)(a.index[v + 1] - b) : 0;
       uint32_t total;
@@ -309,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOu
       for (uint32_t j0 = 0; j0 < total; j0 += 64) {
         const uint32_t j = j0 + lane;
         const int k = wave_owner(excl, j);
-        const uint32_t src = __shfl(v, k);
+        const uint32_t src = __shfl(vo, k);
         const uint64_t bb = shfl_u64(b, k);
         const uint32_t exk = __shfl(excl, k);
         if (j < total) visit(src, stream_load(&a.nbr[bb + (j - exk)]));
@@ -317,8 +314,9 @@ __global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOu
       if (lane == 0) ex += total;
     } else {
       const uint32_t v = a.q1[vw - waves0];
+      const uint32_t vo = o.vinfo[v].x;
       const uint64_t b = a.index[v], e1 = a.index[v + 1];
-      for (uint64_t e = b + lane; e < e1; e += 64) visit(v, stream_load(&a.nbr[e]));
+      for (uint64_t e = b + lane; e < e1; e += 64) visit(vo, stream_load(&a.nbr[e]));
       if (lane == 0) ex += e1 - b;
     }
   }
@@ -326,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOu
   // wave g % W.  Each block stages kHubBatch of them at a time in LDS (list
   // bounds and the prefix of chunk counts), so a wave finds the vertex of a
   // chunk by a binary search in LDS instead of walking the list.
-  __shared__ uint32_t s_v[kHubBatch];
+  __shared__ uint32_t s_v[kHubBatch];  // original ids (the parent written for claims)
   __shared__ uint64_t s_b[kHubBatch], s_e[kHubBatch];
   __shared__ uint64_t s_pre[kHubBatch + 1];
   __shared__ uint64_t s_wsum[kWavesPerBlock];
@@ -343,10 +341,10 @@ __global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOu
       if (i < nb) {
         const uint32_t v = a.q2[base + i];
         const uint64_t b = a.index[v], e1 = a.index[v + 1];
-        s_v[i] = v;
+        s_v[i] = o.vinfo[v].x;
         s_b[i] = b;
         s_e[i] = e1;
-        mine[r] = (e1 - b + kChunkEdges - 1) / kChunkEdges;
+        mine[r] = (e1 - b + kPushChunk - 1) / kPushChunk;
       }
       tsum += mine[r];
     }
@@ -384,8 +382,8 @@ __global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOu
       }
       const uint32_t v = s_v[lo];
       const uint64_t e1 = s_e[lo];
-      const uint64_t cb = s_b[lo] + (j - s_pre[lo]) * kChunkEdges;
-      const uint64_t ce = cb + kChunkEdges < e1 ? cb + kChunkEdges : e1;
+      const uint64_t cb = s_b[lo] + (j - s_pre[lo]) * kPushChunk;
+      const uint64_t ce = cb + kPushChunk < e1 ? cb + kPushChunk : e1;
       for (uint64_t e = cb + lane; e < ce; e += 64) visit(v, stream_load(&a.nbr[e]));
       if (lane == 0) ex += ce - cb;
     }
@@ -397,8 +395,6 @@ __global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOu
 struct BfsPullArgs {
   const uint64_t *index;
   const uint32_t *nbr;
-  const uint32_t *first;  // first in-neighbour of each vertex
-  const uint32_t *indeg;
   const uint64_t *fin;
   uint32_t V, light_max, serial_max, ngroups;
 };
@@ -428,14 +424,14 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOu
     }
     const uint32_t v = g * 64 + lane;
     bool cand = v < a.V && !((visw >> lane) & 1ULL);
-    const uint32_t deg = cand ? a.indeg[v] : 0;
+    const uint4 vi = cand ? o.vinfo[v] : make_uint4(0, 0, 0, 0);
+    const uint32_t deg = vi.z;
     cand = cand && deg > 0 && deg <= a.light_max;
     // the first in-neighbour (the highest-degree one on a relabelled graph)
     // decides most vertices without touching the list
     uint32_t found = kUnset;
     if (cand) {
-      const uint32_t f = a.first[v];
-      if (test_bit(a.fin, f)) found = f;
+      if (test_bit(a.fin, vi.w)) found = vi.w;
       examined += 1;
     }
     cand = cand && found == kUnset && deg > 1;
@@ -485,10 +481,9 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOu
     const bool got = found != kUnset;
     const uint64_t m = __ballot(got);
     if (got) {
-      const uint32_t ov = orig_id(o.iperm, v);
-      o.parent[ov] = orig_id(o.iperm, found);
-      o.depth[ov] = level;
-      edg += o.deg[v];
+      o.parent[vi.x] = o.vinfo[found].x;
+      o.depth[vi.x] = level;
+      edg += vi.y;
       ind += deg;
     }
     if (lane == 0) {
@@ -523,8 +518,8 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_heavy_kernel(const BfsCtl *c,
     const uint32_t v = a.chunk_v[ch];
     const uint64_t bit = 1ULL << (v & 63);
     if ((o.visited[v >> 6] | a.fin[v >> 6]) & bit) continue;
-    const uint32_t ov = orig_id(o.iperm, v);
-    uint32_t *dv = &o.depth[ov];
+    const uint4 vi = o.vinfo[v];
+    uint32_t *dv = &o.depth[vi.x];
     const uint64_t cb = a.chunk_beg[ch], ce = a.chunk_end[ch];
     for (uint64_t e0 = cb; e0 < ce; e0 += 64) {
       if (__hip_atomic_load(dv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kUnset) break;
@@ -536,11 +531,11 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_heavy_kernel(const BfsCtl *c,
       if (hm) {
         const uint32_t fu = __shfl(u, __builtin_ctzll(hm));
         if (lane == 0 && atomicCAS(dv, kUnset, level) == kUnset) {
-          o.parent[ov] = orig_id(o.iperm, fu);
+          o.parent[vi.x] = o.vinfo[fu].x;
           atomicOr((unsigned long long *)&o.fout[v >> 6], (unsigned long long)bit);
           cnt++;
-          edg += o.deg[v];
-          ind += o.indeg[v];
+          edg += vi.y;
+          ind += vi.z;
         }
         break;
       }
@@ -613,7 +608,6 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
   GG_TRY(bfs_state(w, lv_need, &s));
   const Options &opt = options();
   unsigned long long *exam = opt.count_examined ? w->exam : nullptr;
-  const uint32_t *iperm = g->reordered ? g->iperm : nullptr;
   const uint32_t r_int = g->to_internal(root);
   const size_t words = w->words;
   const uint64_t threshold = g->edges / (uint64_t)opt.switch_divisor;
@@ -628,7 +622,7 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
                      2 * set_words, exam);
   hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(kBlock), 0, st, s->ctl, s->sets, s->qc, exam,
                      s->lv, s->h_flag, w->visited, w->f[0].bits, r_int, root, d_parent, d_depth,
-                     out.degree, in.degree, V, (int32_t)opt.force_mode, threshold, g->arcs,
+                     in.vinfo, V, (int32_t)opt.force_mode, threshold, g->arcs,
                      (uint64_t)opt.bfs_alpha, s->lv_cap);
   // fixed launch shapes: the frontier sizes live on the device
   const unsigned bin_grid = grid1(words, 2048);
@@ -653,7 +647,7 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
     if (n > V + 2) return fail(GG_EHIP, "gg_bfs: traversal did not terminate");
     uint64_t *fin = w->f[n & 1].bits, *fout = w->f[(n + 1) & 1].bits;
     uint64_t *set_out = s->sets + (size_t)((n + 1) & 1) * set_words;
-    BfsOut o{d_parent, d_depth, iperm, out.degree, in.degree, w->visited, fout};
+    BfsOut o{d_parent, d_depth, in.vinfo, w->visited, fout};
     {
       ProfScope p("compact");
       hipLaunchKernelGGL(bfs_bin_kernel, dim3(bin_grid), dim3(kBlock), 0, st, s->ctl, fin, fout,
@@ -669,8 +663,7 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
     }
     {
       ProfScope p("bfs_pull");
-      BfsPullArgs a{in.index, in.nbr, in.first, in.degree, fin, V, in.light_max,
-                    (uint32_t)opt.pull_serial_max,
+      BfsPullArgs a{in.index, in.nbr, fin, V, in.light_max, (uint32_t)opt.pull_serial_max,
                     ngroups};
       hipLaunchKernelGGL(bfs_pull_kernel, dim3(pull_grid), dim3(kBlock), 0, st, s->ctl, o, a,
                          set_out, exam);

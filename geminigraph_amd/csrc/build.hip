@@ -244,13 +244,26 @@ static int graph_alloc(gg_graph *g, void **p, size_t bytes) {
   return GG_OK;
 }
 
-__global__ void first_nbr_kernel(const uint64_t *index, const uint32_t *nbr, uint32_t V,
-                                 uint32_t *first) {
+__global__ void vinfo_kernel(const uint64_t *index, const uint32_t *nbr, const uint32_t *other_deg,
+                             const uint32_t *iperm, uint32_t V, uint4 *vinfo) {
   const uint32_t s = gridDim.x * blockDim.x;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s) {
-    const uint64_t b = index[v];
-    first[v] = b < index[v + 1] ? nbr[b] : kUnset;
+    const uint64_t b = index[v], e = index[v + 1];
+    vinfo[v] = make_uint4(iperm ? iperm[v] : v, other_deg[v], (uint32_t)(e - b),
+                          b < e ? nbr[b] : kUnset);
   }
+}
+
+// Per-vertex records of one direction (after both directions are built).
+static int build_vinfo(gg_graph *g, Dir &d, const Dir &other) {
+  const uint32_t V = g->vertices;
+  GG_TRY(graph_alloc(g, (void **)&d.vinfo, (size_t)std::max(V, 1u) * sizeof(uint4)));
+  const unsigned grid = (unsigned)std::min<uint64_t>(((uint64_t)V + kBlock - 1) / kBlock, 65536);
+  hipLaunchKernelGGL(vinfo_kernel, dim3(std::max(grid, 1u)), dim3(kBlock), 0, stream(), d.index,
+                     d.nbr, other.degree, g->reordered ? g->iperm : nullptr, V, d.vinfo);
+  GG_HIP(hipGetLastError());
+  GG_HIP(hipStreamSynchronize(stream()));  // the graph may be used from other streams
+  return GG_OK;
 }
 
 // Heavy-vertex chunk list of one direction (degree > light_max), computed on
@@ -285,13 +298,6 @@ static int build_chunks(gg_graph *g, Dir &d) {
     if (!deg[v]) eb[v >> 6] |= 1ULL << (v & 63);
   GG_TRY(graph_alloc(g, (void **)&d.empty, eb.size() * 8));
   GG_HIP(hipMemcpyAsync(d.empty, eb.data(), eb.size() * 8, hipMemcpyHostToDevice, stream()));
-  // first neighbour of every list: a bottom-up BFS step tests it before
-  // reading the list (lists are sorted, so on a relabelled graph it is the
-  // highest-degree neighbour, the likeliest to be in the frontier)
-  GG_TRY(graph_alloc(g, (void **)&d.first, (size_t)std::max(V, 1u) * 4));
-  const unsigned fgrid = (unsigned)std::min<uint64_t>(((uint64_t)V + kBlock - 1) / kBlock, 65536);
-  hipLaunchKernelGGL(first_nbr_kernel, dim3(std::max(fgrid, 1u)), dim3(kBlock), 0, stream(),
-                     d.index, d.nbr, V, d.first);
   d.n_heavy = (uint32_t)hv.size();
   d.n_chunks = (uint32_t)cv.size();
   hf.push_back((uint32_t)cv.size());
@@ -461,10 +467,17 @@ static int graph_build_device(const uint32_t *d_edges, uint64_t ne, uint32_t V, 
   }
   if (r == GG_OK) r = build_chunks(g, g->out);
   if (r == GG_OK) {
-    if (undirected)
+    if (undirected) {
+      r = build_vinfo(g, g->out, g->out);
       g->in = g->out;
-    else
+    } else {
       r = build_chunks(g, g->in);
+      // the first in-neighbour: a bottom-up BFS step tests it before reading
+      // the list (lists are sorted, so on a relabelled graph it is the
+      // highest-degree in-neighbour, the likeliest to be in the frontier)
+      if (r == GG_OK) r = build_vinfo(g, g->out, g->in);
+      if (r == GG_OK) r = build_vinfo(g, g->in, g->out);
+    }
   }
   if (r != GG_OK) {
     std::string msg = gg_last_error();

@@ -96,7 +96,10 @@ struct Dir {
   float *w = nullptr;         // arcs or null
   uint32_t *degree = nullptr; // V
   uint64_t *empty = nullptr;  // V/64 + 1 words: bit v set iff degree[v] == 0
-  uint32_t *first = nullptr;  // V: first (smallest) neighbour of each list, kUnset if empty
+  // V: per vertex {original id, degree in the other direction, degree here,
+  // first (smallest) neighbour here or kUnset}: what a BFS step needs of a
+  // vertex it finds, in one 16-byte read
+  uint4 *vinfo = nullptr;
   // heavy chunks (degree > light limit at build time)
   uint32_t n_chunks = 0;
   uint32_t *chunk_v = nullptr;
