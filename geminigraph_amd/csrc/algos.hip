@@ -1197,6 +1197,8 @@ static int ms_bfs_run(gg_graph *g, Work *w, Engine &en, const uint32_t *roots, i
                           stream()));
   if (store_levels) GG_TRY(hook(0, nullptr, w->ms_f[0]));
   const uint64_t full = n_roots == 64 ? ~0ULL : ((1ULL << n_roots) - 1ULL);
+  const PrPlan *plan = nullptr;
+  GG_TRY(pr_plan_get(g, &plan));
   int fc = 0;
   uint32_t level = 1;
   for (; en.in.total > 0; level++) {
@@ -1209,6 +1211,10 @@ static int ms_bfs_run(gg_graph *g, Work *w, Engine &en, const uint32_t *roots, i
       op.vis = w->ms_vis;
       op.fnext = fnext;
       GG_TRY(en.launch_push(out, op, false));
+    } else if (plan) {
+      // relabelled graph: the dense step is an OR over every in-arc, done on
+      // the destination-bin plan (LDS masks per bin, prplan.hip)
+      GG_TRY(ms_plan_iteration(g, plan, w, fcur, w->ms_vis, full, fnext, en.fout().bits));
     } else {
       MsPull op;
       op.fcur = fcur;

@@ -236,6 +236,12 @@ void pr_plan_free(PrPlan *p);
 struct PrPull;
 int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull &op,
                       double *dacc);
+// One dense step of the batched BFS on the plan: for every vertex the OR of
+// its in-neighbours' 64-root frontier masks (fcur); the bits not in vis (and
+// within full) go to fnext[v] and v's bit to the bitmap `out` (every word is
+// written).
+int ms_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *fcur,
+                      const uint64_t *vis, uint64_t full, uint64_t *fnext, uint64_t *out);
 // Device bytes of one traversal context that runs batched queries of `kind`
 // (GG_QUERY_*) on g, counting the first 64 stored BFS levels.
 size_t work_footprint(const gg_graph *g, int kind);

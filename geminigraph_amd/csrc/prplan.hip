@@ -132,13 +132,51 @@ __global__ void pr_cold_split_kernel(const uint64_t *keys, uint64_t n, int bb, i
 }
 
 // -------------------------------------------------------------- iteration
+// The per-iteration kernels combine 8-byte values of the in-neighbours of
+// every vertex under a BinOp: PageRank sums FP64 contributions; the batched
+// BFS ORs 64-root frontier masks.  A BinOp gives the value type T, the
+// source value, the LDS and global combining atomics, and the per-vertex
+// finish (which reports whether the vertex joins the output frontier).
+struct PrSumOp {
+  using T = double;
+  PrPull op;
+  __device__ T value(uint32_t u) const { return op.curr[u]; }
+  __device__ static void lds_add(T *acc, uint32_t i, T x) { atomicAdd(&acc[i], x); }
+  __device__ static void global_add(T *p, T x) {
+    if (x != 0.0) unsafeAtomicAdd(p, x);
+  }
+  __device__ bool finish(uint32_t v, T sum, double &dsum) const {
+    op.finish(v, sum, 0, dsum);
+    return false;
+  }
+};
+struct MsOrOp {
+  using T = uint64_t;
+  const uint64_t *fcur, *vis;
+  uint64_t *fnext;
+  uint64_t full;
+  __device__ T value(uint32_t u) const { return fcur[u]; }
+  __device__ static void lds_add(T *acc, uint32_t i, T x) {
+    if (x) atomicOr((unsigned long long *)&acc[i], (unsigned long long)x);
+  }
+  __device__ static void global_add(T *p, T x) {
+    if (x) atomicOr((unsigned long long *)p, (unsigned long long)x);
+  }
+  __device__ bool finish(uint32_t v, T acc, double &) const {
+    const uint64_t nw = acc & full & ~vis[v];
+    if (nw) fnext[v] = nw;
+    return nw != 0;
+  }
+};
+
 // Emit: part p = arcs [lo, hi) of range r.  The range's values go to LDS
 // (one coalesced read), then every arc's value is written in storage order.
+// Values are copied as 8-byte words.
 __global__ __launch_bounds__(kPrBlock) void pr_emit_kernel(const uint32_t *parts,
-                                                           const double *curr,
+                                                           const uint64_t *curr,
                                                            const uint16_t *src16, uint32_t H,
-                                                           uint32_t V, int rb, double *vals) {
-  extern __shared__ double val[];  // 2^rb values
+                                                           uint32_t V, int rb, uint64_t *vals) {
+  extern __shared__ uint64_t val[];  // 2^rb values
   const uint32_t r = parts[3 * blockIdx.x], lo = parts[3 * blockIdx.x + 1],
                  hi = parts[3 * blockIdx.x + 2];
   const uint32_t u0 = H + (r << rb);
@@ -161,16 +199,21 @@ __global__ __launch_bounds__(kPrBlock) void pr_emit_kernel(const uint32_t *parts
   }
 }
 
-// Bin segment: hot arcs, then cold runs, summed in LDS; then the update of
-// the bin's vertices (whole bin) or an add into the split bin's row.
+// Bin segment: hot arcs, then cold runs, combined in LDS; then the finish of
+// the bin's vertices (whole bin) or a combine into the split bin's row.
+// `out`, when given, receives the bitmap words of the vertices whose finish
+// reports true (a whole bin owns its words).
+template <class Op>
 __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
     const PrSeg *segs, const uint32_t *hot, const uint32_t *run_beg, const uint32_t *run_end,
-    uint32_t nranges, const uint16_t *dst16, const double *vals, uint32_t V, int bb, PrPull op,
-    double *gsum, double *dacc) {
-  extern __shared__ double acc[];  // 2^bb sums
+    uint32_t nranges, const uint16_t *dst16, const typename Op::T *vals, uint32_t V, int bb,
+    Op op, typename Op::T *gsum, uint64_t *out, double *dacc) {
+  using T = typename Op::T;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *acc = reinterpret_cast<T *>(lds_raw);  // 2^bb values
   const uint32_t nb = 1u << bb, dmask = nb - 1;
   const PrSeg &sg = segs[blockIdx.x];  // read in place: cut[] is indexed by wave
-  for (uint32_t i = threadIdx.x; i < nb; i += kPrBlock) acc[i] = 0.0;
+  for (uint32_t i = threadIdx.x; i < nb; i += kPrBlock) acc[i] = T(0);
   __syncthreads();
   {  // hot arcs: sorted by source, so lanes share gathered lines
     constexpr int kU = 8;
@@ -181,12 +224,12 @@ __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
         const uint32_t q = a + t * kPrBlock;
         k[t] = q < sg.hot_hi ? stream_load(&hot[q]) : 0u;
       }
-      double x[kU];
+      T x[kU];
 #pragma unroll
-      for (int t = 0; t < kU; t++) x[t] = op.curr[k[t] >> bb];
+      for (int t = 0; t < kU; t++) x[t] = op.value(k[t] >> bb);
 #pragma unroll
       for (int t = 0; t < kU; t++)
-        if (a + t * kPrBlock < sg.hot_hi) atomicAdd(&acc[k[t] & dmask], x[t]);
+        if (a + t * kPrBlock < sg.hot_hi) Op::lds_add(acc, k[t] & dmask, x[t]);
     }
   }
   {  // cold runs: this wave's share, 64 runs per step, flattened by an in-wave scan
@@ -219,7 +262,7 @@ __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
           pos[t] = __shfl(beg, o) + (jj - __shfl(excl, o));
         }
         uint16_t d[kU];
-        double x[kU];
+        T x[kU];
 #pragma unroll
         for (int t = 0; t < kU; t++) {
           d[t] = stream_load(&dst16[pos[t]]);
@@ -227,7 +270,7 @@ __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
         }
 #pragma unroll
         for (int t = 0; t < kU; t++)
-          if (j0 + t * 64 + lane < total) atomicAdd(&acc[d[t]], x[t]);
+          if (j0 + t * 64 + lane < total) Op::lds_add(acc, d[t], x[t]);
       }
       if (end_r < ri + 64 || (end_r == ri + 64 && !end_off)) break;
       ri += 64;
@@ -238,38 +281,57 @@ __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
   const uint32_t base = sg.bin << bb;
   const uint32_t n = V - base < nb ? V - base : nb;
   if (sg.slot) {
-    double *gs = gsum + ((uint64_t)(sg.slot - 1) << bb);
-    for (uint32_t i = threadIdx.x; i < n; i += kPrBlock)
-      if (acc[i] != 0.0) unsafeAtomicAdd(&gs[i], acc[i]);
+    T *gs = gsum + ((uint64_t)(sg.slot - 1) << bb);
+    for (uint32_t i = threadIdx.x; i < n; i += kPrBlock) Op::global_add(&gs[i], acc[i]);
     return;
   }
   double dsum = 0;
-  for (uint32_t i = threadIdx.x; i < n; i += kPrBlock) op.finish(base + i, acc[i], 0, dsum);
-  const double w = wave_sum_f64(dsum);
-  if (lane_id() == 0 && w != 0.0)
-    unsafeAtomicAdd(
-        &dacc[((blockIdx.x * (kPrBlock / 64) + (threadIdx.x >> 6)) % kScalarSlots) * 16], w);
+  // the block's waves take 64 consecutive vertices per step (one bitmap word)
+  for (uint32_t i0 = threadIdx.x - lane_id(); i0 < n; i0 += kPrBlock) {
+    const uint32_t i = i0 + lane_id();
+    const bool on = i < n && op.finish(base + i, acc[i], dsum);
+    const uint64_t m = __ballot(on);
+    if (out && lane_id() == 0) out[(base + i0) >> 6] = m;
+  }
+  if (dacc) {
+    const double w = wave_sum_f64(dsum);
+    if (lane_id() == 0 && w != 0.0)
+      unsafeAtomicAdd(
+          &dacc[((blockIdx.x * (kPrBlock / 64) + (threadIdx.x >> 6)) % kScalarSlots) * 16], w);
+  }
 }
 
-// The update of the split bins' vertices, clearing their rows for the next
+// The finish of the split bins' vertices, clearing their rows for the next
 // iteration.
+template <class Op>
 __global__ __launch_bounds__(kBlock) void pr_split_finish_kernel(const uint32_t *split_bin,
                                                                  uint32_t nsplit, uint32_t V,
-                                                                 int bb, PrPull op, double *gsum,
-                                                                 double *dacc) {
+                                                                 int bb, Op op,
+                                                                 typename Op::T *gsum,
+                                                                 uint64_t *out, double *dacc) {
+  using T = typename Op::T;
   const uint64_t total = (uint64_t)nsplit << bb;
   const uint64_t s = (uint64_t)gridDim.x * blockDim.x;
   double dsum = 0;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += s) {
-    const uint32_t v = (split_bin[i >> bb] << bb) + (uint32_t)(i & ((1u << bb) - 1));
-    if (v >= V) continue;
-    const double t = gsum[i];
-    gsum[i] = 0.0;
-    op.finish(v, t, 0, dsum);
+  // a wave covers 64 consecutive vertices of one bin (bins are whole words)
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x - lane_id(); i0 < total;
+       i0 += s) {
+    const uint64_t i = i0 + lane_id();
+    const uint32_t v = (split_bin[i0 >> bb] << bb) + (uint32_t)(i & ((1u << bb) - 1));
+    bool on = false;
+    if (i < total && v < V) {
+      const T t = gsum[i];
+      gsum[i] = T(0);
+      on = op.finish(v, t, dsum);
+    }
+    const uint64_t m = __ballot(on);
+    if (out && lane_id() == 0 && (v - lane_id()) < V) out[(v - lane_id()) >> 6] = m;
   }
-  const double w = wave_sum_f64(dsum);
-  if (lane_id() == 0 && w != 0.0)
-    unsafeAtomicAdd(&dacc[(global_wave() % kScalarSlots) * 16], w);
+  if (dacc) {
+    const double w = wave_sum_f64(dsum);
+    if (lane_id() == 0 && w != 0.0)
+      unsafeAtomicAdd(&dacc[(global_wave() % kScalarSlots) * 16], w);
+  }
 }
 
 // ------------------------------------------------------------------ plan
@@ -533,9 +595,8 @@ int pr_plan_get(gg_graph *g, const PrPlan **out) {
   return GG_OK;
 }
 
-int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull &op,
-                      double *dacc) {
-  const uint32_t V = g->vertices;
+// Buffers of the per-iteration kernels, kept in the traversal context.
+static int plan_buffers(const PrPlan *p, Work *w) {
   const uint64_t gs_n = (uint64_t)p->nsplit << p->bin_bits;
   if (w->pr_vals_cap < p->n_cold || w->pr_gsum_cap < gs_n || !w->pr_vals) {
     device_free(w->pr_vals);
@@ -548,24 +609,52 @@ int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull 
     w->pr_vals_cap = p->n_cold;
     w->pr_gsum_cap = gs_n;
   }
+  return GG_OK;
+}
+
+// One pass of the plan: emit the cold values of `src` (8-byte values per
+// vertex), then the bins, then the split bins.
+template <class Op>
+static int plan_pass(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *src, const Op &op,
+                     uint64_t *out, double *dacc, const char *names[3]) {
+  using T = typename Op::T;
+  static_assert(sizeof(T) == 8, "plan values are 8 bytes");
+  const uint32_t V = g->vertices;
+  GG_TRY(plan_buffers(p, w));
+  T *vals = reinterpret_cast<T *>(w->pr_vals), *gsum = reinterpret_cast<T *>(w->pr_gsum);
   if (p->nparts) {
-    ProfScope ps("pr_emit");
+    ProfScope ps(names[0]);
     hipLaunchKernelGGL(pr_emit_kernel, dim3(p->nparts), dim3(kPrBlock), (size_t)8 << p->range_bits,
-                       stream(), p->parts, op.curr, p->src16, p->H, V, p->range_bits, w->pr_vals);
+                       stream(), p->parts, src, p->src16, p->H, V, p->range_bits,
+                       reinterpret_cast<uint64_t *>(vals));
   }
   if (p->nseg) {
-    ProfScope ps("pr_bins");
-    hipLaunchKernelGGL(pr_bin_kernel, dim3(p->nseg), dim3(kPrBlock), (size_t)8 << p->bin_bits,
+    ProfScope ps(names[1]);
+    hipLaunchKernelGGL(pr_bin_kernel<Op>, dim3(p->nseg), dim3(kPrBlock), (size_t)8 << p->bin_bits,
                        stream(), p->segs, p->hot, p->run_beg, p->run_end, p->nranges, p->dst16,
-                       w->pr_vals, V, p->bin_bits, op, w->pr_gsum, dacc);
+                       vals, V, p->bin_bits, op, gsum, out, dacc);
   }
   if (p->nsplit) {
-    ProfScope ps("pr_split_finish");
-    hipLaunchKernelGGL(pr_split_finish_kernel, dim3(grid1(gs_n, 4096)), dim3(kBlock), 0, stream(),
-                       p->split_bin, p->nsplit, V, p->bin_bits, op, w->pr_gsum, dacc);
+    ProfScope ps(names[2]);
+    const uint64_t gs_n = (uint64_t)p->nsplit << p->bin_bits;
+    hipLaunchKernelGGL(pr_split_finish_kernel<Op>, dim3(grid1(gs_n, 4096)), dim3(kBlock), 0,
+                       stream(), p->split_bin, p->nsplit, V, p->bin_bits, op, gsum, out, dacc);
   }
   GG_HIP(hipGetLastError());
   return GG_OK;
+}
+
+int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull &op,
+                      double *dacc) {
+  static const char *names[3] = {"pr_emit", "pr_bins", "pr_split_finish"};
+  return plan_pass(g, p, w, reinterpret_cast<const uint64_t *>(op.curr), PrSumOp{op}, nullptr,
+                   dacc, names);
+}
+
+int ms_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *fcur,
+                      const uint64_t *vis, uint64_t full, uint64_t *fnext, uint64_t *out) {
+  static const char *names[3] = {"ms_emit", "ms_bins", "ms_split_finish"};
+  return plan_pass(g, p, w, fcur, MsOrOp{fcur, vis, fnext, full}, out, nullptr, names);
 }
 
 }  // namespace gg

@@ -485,6 +485,24 @@ def test_bfs_batch_matches_plain(gg, reorder, mode):
     assert st["m_reach"] == m
 
 
+@pytest.mark.parametrize("bins,ranges,hot", [(9, 9, 500), (8, 12, 1 << 14), (13, 14, 1 << 19)])
+def test_bfs_batch_bin_plan(gg, bins, ranges, hot):
+    # dense steps of the batched BFS on the destination-bin plan (relabelled
+    # graph): many bins and cold ranges, split bins, 40 roots (a partial mask)
+    scale = 14
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, 16, seed=73)
+    g = gg.Graph.build(e, V, reorder=True)
+    o = oracle.Graph(e, V)
+    roots = roots_of(o.out_degree(), 40, 73)
+    say("bfs_batch plan", bins, ranges, hot)
+    with gg.options(force_mode=MODES["pull"], pr_bin_bits=bins, pr_range_bits=ranges,
+                    pr_hot_sources=hot):
+        depths, _ = gg.bfs_batch(g, roots)
+    for i, r in enumerate(roots):
+        assert np.array_equal(depths[i], o.bfs(r)[1]), (i, r)
+
+
 def test_bfs_batch_partial_and_repeated_roots(gg):
     e = oracle.rmat_edges(11, seed=72)
     V = 1 << 11
