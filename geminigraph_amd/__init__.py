@@ -95,6 +95,22 @@ class GraphInfo(ctypes.Structure):
 BUILD_REORDER = 1
 
 
+class TuneResult(ctypes.Structure):
+    """gg_tune_result of include/gg.h."""
+    _fields_ = [("kind", ctypes.c_int), ("n_options", ctypes.c_int),
+                ("names", (ctypes.c_char * 24) * 8), ("values", ctypes.c_int64 * 8),
+                ("baseline_ms", ctypes.c_double), ("tuned_ms", ctypes.c_double),
+                ("trials", ctypes.c_int)]
+
+    def as_dict(self):
+        n = self.n_options
+        return {"options": {bytes(self.names[i].value).decode(): int(self.values[i]) for i in range(n)},
+                "baseline_ms": self.baseline_ms, "tuned_ms": self.tuned_ms, "trials": self.trials}
+
+
+TUNE_BFS, TUNE_PAGERANK = 0, 1
+
+
 class ServerInfo(ctypes.Structure):
     """gg_server_info of include/gg.h."""
     _fields_ = [("budget_bytes", ctypes.c_size_t), ("used_bytes", ctypes.c_size_t),
@@ -160,6 +176,7 @@ SIGNATURES = {
     "gg_server_wait": (_i32, [_vp, _u64, _vp, _sz, ctypes.POINTER(_dbl)]),
     "gg_server_replay": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_dbl)]),
     "gg_server_get_info": (_i32, [_vp, ctypes.POINTER(ServerInfo)]),
+    "gg_tune": (_i32, [_vp, _i32, _i32, _i32, ctypes.POINTER(TuneResult)]),
     "gg_prof_enable": (_i32, [_i32]),
     "gg_prof_reset": (_i32, []),
     "gg_prof_read": (_i32, [_cp, ctypes.POINTER(_dbl), ctypes.POINTER(_u64)]),
@@ -495,6 +512,18 @@ class Graph:
         st = self.bc_into(root, dep, inv, fr)
         n = int(np.max(np.nonzero(fr)[0]) + 1) if fr.any() else 0
         return dep.numpy(), inv.numpy(), fr[:n].copy(), st.as_dict()
+
+
+def tune(graph, kind="bfs", samples=8, apply=True):
+    """Times candidate option values on a sample of this graph's work (BFS
+    from `samples` roots, or `samples` PageRank iterations) and keeps the
+    fastest per option; with apply=True the winners stay set (options are
+    process-wide).  Returns {"options": {...}, "baseline_ms", "tuned_ms",
+    "trials"}."""
+    k = {"bfs": TUNE_BFS, "pagerank": TUNE_PAGERANK}.get(kind, kind)
+    r = TuneResult()
+    _check(lib().gg_tune(graph.handle, int(k), int(samples), int(bool(apply)), ctypes.byref(r)))
+    return r.as_dict()
 
 
 def _batch_roots(roots):

@@ -279,6 +279,31 @@ int gg_server_replay(gg_server *s, int n, const double *arrival_s, const int *gr
                      const int *kind, const uint32_t *root, double *latency_ms, double *wall_s);
 int gg_server_get_info(gg_server *s, gg_server_info *info);
 
+/* ------------------------------------------------------------ autotuner */
+/* Times candidate values of the engine's options on a sample of this graph's
+ * own work and keeps the fastest, one option at a time (no counterpart in
+ * the reference, whose switch is fixed at |E| / 20, core/graph.hpp:1509).
+ * GG_TUNE_BFS: `samples` BFS roots (spread over the ids, with out-arcs);
+ * tunes switch_divisor, bfs_alpha, push_big_max, pull_serial_max,
+ * bfs_pull_waves.  GG_TUNE_PAGERANK: `samples` iterations (the plan build
+ * is not timed); tunes pr_hot_sources, pr_bin_bits, pr_range_bits.
+ * Each candidate is timed twice (best of two) with device events.  apply != 0
+ * keeps the chosen values (options are process-wide), else the previous
+ * values are restored.  No result depends on these options. */
+#define GG_TUNE_BFS 0
+#define GG_TUNE_PAGERANK 1
+#define GG_TUNE_MAX_OPTIONS 8
+typedef struct gg_tune_result {
+  int kind;
+  int n_options;                        /* entries used below */
+  char names[GG_TUNE_MAX_OPTIONS][24];  /* option names, in tuning order */
+  int64_t values[GG_TUNE_MAX_OPTIONS];  /* chosen values */
+  double baseline_ms;                   /* the sample with the options in force before */
+  double tuned_ms;                      /* the sample with the chosen values */
+  int trials;                           /* configurations timed */
+} gg_tune_result;
+int gg_tune(gg_graph *g, int kind, int samples, int apply, gg_tune_result *out);
+
 /* ----------------------------------------------------------- profiling */
 
 /* Per-kernel device time accumulated with HIP events on the launching
