@@ -177,6 +177,11 @@ SIGNATURES = {
     "gg_server_replay": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_dbl)]),
     "gg_server_get_info": (_i32, [_vp, ctypes.POINTER(ServerInfo)]),
     "gg_tune": (_i32, [_vp, _i32, _i32, _i32, ctypes.POINTER(TuneResult)]),
+    "gg_dump_vertex_array": (_i32, [_vp, _vp, _sz, _cp]),
+    "gg_restore_vertex_array": (_i32, [_vp, _vp, _sz, _cp]),
+    "gg_gather_vertex_array": (_i32, [_vp, _vp, _sz, _vp]),
+    "gg_graph_save": (_i32, [_vp, _cp]),
+    "gg_graph_open": (_i32, [_cp, _pp]),
     "gg_prof_enable": (_i32, [_i32]),
     "gg_prof_reset": (_i32, []),
     "gg_prof_read": (_i32, [_cp, ctypes.POINTER(_dbl), ctypes.POINTER(_u64)]),
@@ -359,6 +364,42 @@ class Graph:
         _check(lib().gg_graph_load_ex(str(path).encode(), int(vertices), int(weighted), 1,
                                       BUILD_REORDER if reorder else 0, ctypes.byref(h)))
         return cls(h.value)
+
+    @classmethod
+    def open(cls, path):
+        """A graph written by save(): no edge sort on load."""
+        h = ctypes.c_void_p()
+        _check(lib().gg_graph_open(str(path).encode(), ctypes.byref(h)))
+        return cls(h.value)
+
+    def save(self, path):
+        """Both directions, weights and the relabeling, to a binary file."""
+        _check(lib().gg_graph_save(self.handle, str(path).encode()))
+
+    # -- vertex arrays (core/graph.hpp:253-324, one partition) -------------
+    def dump_vertex_array(self, array, path):
+        """A DeviceArray of `vertices` elements to a file of its bytes."""
+        self._check_vertex_array(array)
+        _check(lib().gg_dump_vertex_array(self.handle, ctypes.c_void_p(array.ptr),
+                                          array.dtype.itemsize, str(path).encode()))
+
+    def restore_vertex_array(self, array, path):
+        """A file written by dump_vertex_array back into a DeviceArray."""
+        self._check_vertex_array(array)
+        _check(lib().gg_restore_vertex_array(self.handle, ctypes.c_void_p(array.ptr),
+                                             array.dtype.itemsize, str(path).encode()))
+
+    def gather_vertex_array(self, array):
+        """The whole array on the host (one partition holds every vertex)."""
+        self._check_vertex_array(array)
+        out = np.empty(self.vertices, dtype=array.dtype)
+        _check(lib().gg_gather_vertex_array(self.handle, ctypes.c_void_p(array.ptr),
+                                            array.dtype.itemsize, out.ctypes.data))
+        return out
+
+    def _check_vertex_array(self, array):
+        if array.n < self.vertices:
+            raise ValueError("vertex array shorter than the graph's vertex count")
 
     def close(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -237,7 +237,7 @@ static int sort_keys(uint64_t *&keys, uint64_t *&keys_alt, float *&vals, float *
   return GG_OK;
 }
 
-static int graph_alloc(gg_graph *g, void **p, size_t bytes) {
+int graph_alloc(gg_graph *g, void **p, size_t bytes) {
   GG_TRY(device_alloc(p, bytes));
   g->allocations.push_back(*p);
   g->device_bytes += bytes;
@@ -318,6 +318,24 @@ static int build_chunks(gg_graph *g, Dir &d) {
   }
   GG_HIP(hipStreamSynchronize(stream()));
   return GG_OK;
+}
+
+// What every graph derives from its CSR/CSC: heavy chunk lists, the
+// empty-list bitmaps and the per-vertex records (symmetric graphs alias the
+// in direction to the out direction).
+int graph_finish(gg_graph *g) {
+  GG_TRY(build_chunks(g, g->out));
+  if (g->symmetric) {
+    GG_TRY(build_vinfo(g, g->out, g->out));
+    g->in = g->out;
+    return GG_OK;
+  }
+  GG_TRY(build_chunks(g, g->in));
+  // the first in-neighbour: a bottom-up BFS step tests it before reading the
+  // list (lists are sorted, so on a relabelled graph it is the highest-degree
+  // in-neighbour, the likeliest to be in the frontier)
+  GG_TRY(build_vinfo(g, g->out, g->in));
+  return build_vinfo(g, g->in, g->out);
 }
 
 // One direction from the device edge list.
@@ -465,20 +483,7 @@ static int graph_build_device(const uint32_t *d_edges, uint64_t ne, uint32_t V, 
       if (r == GG_OK) r = build_dir(g, g->in, d_edges, ne, rec_words, 1);
     }
   }
-  if (r == GG_OK) r = build_chunks(g, g->out);
-  if (r == GG_OK) {
-    if (undirected) {
-      r = build_vinfo(g, g->out, g->out);
-      g->in = g->out;
-    } else {
-      r = build_chunks(g, g->in);
-      // the first in-neighbour: a bottom-up BFS step tests it before reading
-      // the list (lists are sorted, so on a relabelled graph it is the
-      // highest-degree in-neighbour, the likeliest to be in the frontier)
-      if (r == GG_OK) r = build_vinfo(g, g->out, g->in);
-      if (r == GG_OK) r = build_vinfo(g, g->in, g->out);
-    }
-  }
+  if (r == GG_OK) r = graph_finish(g);
   if (r != GG_OK) {
     std::string msg = gg_last_error();
     gg_graph_destroy(g);

@@ -224,6 +224,10 @@ struct Work {
   std::vector<void *> allocations;
 };
 
+// Graph construction (build.hip): device memory owned by the graph, and the
+// derived structures every graph needs once its CSR/CSC are in place.
+int graph_alloc(gg_graph *g, void **p, size_t bytes);
+int graph_finish(gg_graph *g);
 int work_get(gg_graph *g, Work **w);
 void work_free(Work *w);
 void bfs_state_free(void *state);

@@ -167,7 +167,7 @@ struct DeviceTimer {
   }
 };
 
-inline int check_graph(gg_graph *g) {
+inline int check_graph(const gg_graph *g) {
   if (!g) return fail(GG_EINVAL, "null graph");
   GG_TRY(ensure_init());
   return GG_OK;

@@ -279,6 +279,27 @@ int gg_server_replay(gg_server *s, int n, const double *arrival_s, const int *gr
                      const int *kind, const uint32_t *root, double *latency_ms, double *wall_s);
 int gg_server_get_info(gg_server *s, gg_server_info *info);
 
+/* ------------------------------------------------- vertex arrays, files */
+/* dump_vertex_array / restore_vertex_array (core/graph.hpp:253-307) with one
+ * partition: a file of elem_bytes x vertices bytes holding the whole array
+ * (indexed by original vertex ids, as every entry point returns them).
+ * Restore refuses a file of another size (GG_EIO; the reference asserts).
+ * d_array holds `vertices` elements of elem_bytes bytes each. */
+int gg_dump_vertex_array(const gg_graph *g, const void *d_array, size_t elem_bytes,
+                         const char *path);
+int gg_restore_vertex_array(const gg_graph *g, void *d_array, size_t elem_bytes,
+                            const char *path);
+/* gather_vertex_array (core/graph.hpp:310-324): with one partition the whole
+ * array is local; gathering means the device-to-host copy into h_out. */
+int gg_gather_vertex_array(const gg_graph *g, const void *d_array, size_t elem_bytes,
+                           void *h_out);
+/* A built graph to a binary file (both directions, weights, the relabeling)
+ * and back without the edge sort: the reference rebuilds its CSR/CSC from
+ * the edge list on every launch (core/graph.hpp:772-1339).  Heavy chunks and
+ * per-vertex records are rebuilt on open with the options in force then. */
+int gg_graph_save(const gg_graph *g, const char *path);
+int gg_graph_open(const char *path, gg_graph **out);
+
 /* ------------------------------------------------------------ autotuner */
 /* Times candidate values of the engine's options on a sample of this graph's
  * own work and keeps the fastest, one option at a time (no counterpart in
