@@ -33,8 +33,10 @@ class GGError(RuntimeError):
 
 
 def build():
-    """Compile libgg.so for gfx950 (hipcc; runs without a GPU)."""
+    """Compile libgg.so and the facade toolkits for gfx950 (hipcc; runs
+    without a GPU)."""
     subprocess.run(["make", "-s", "-B", "-j4", "-C", _HERE], check=True)
+    subprocess.run(["make", "-s", "-B", "-j4", "-C", _HERE, "toolkits"], check=True)
 
 
 class RunStats(ctypes.Structure):
