@@ -38,15 +38,18 @@ def pick_roots(out_degree, k, seed):
     return [int(x) for x in rng.choice(cand, size=min(k, len(cand)), replace=False)]
 
 
-def bfs_model_bytes(st, V):
+def bfs_model_bytes(st, V, no_in=0):
     """Algorithmic bytes of each process_edges call of one BFS:
     push: 20F + 8X + 8N  (queue entry, offset pair, col, state probe, writes)
     pull: 16U + 4X + 8N + 3V/8  (offsets of unvisited, col until exit, writes,
-          visited / frontier-in / frontier-out bitmaps)."""
+          visited / frontier-in / frontier-out bitmaps).
+    U counts the unvisited vertices that have in-arcs: the `no_in` vertices
+    without any (other than a root among them) can never be found, and the
+    bottom-up step does not read their lists."""
     push = pull = 0
     for i, mode in enumerate(st["modes"]):
         F, N, X = st["it_active"][i], st["it_new"][i], st["it_examined"][i]
-        U = st["it_unvisited"][i]
+        U = max(0, st["it_unvisited"][i] - no_in)
         if mode == 0:
             push += 20 * F + 8 * X + 8 * N
         else:
@@ -412,10 +415,12 @@ def main():
     # one instrumented pass (edge counters on) for the algorithmic byte model
     gpu_depths = []
     push_b = pull_b = 0.0
+    indeg = g.in_degree()
+    n_no_in = int(np.count_nonzero(indeg == 0))
     with gg.options(count_examined=1):
         for r in roots:
             st = g.bfs_into(r, parent, depth).as_dict()
-            pb, lb = bfs_model_bytes(st, V)
+            pb, lb = bfs_model_bytes(st, V, n_no_in - int(indeg[r] == 0))
             push_b += pb
             pull_b += lb
             if len(gpu_depths) < args.cpu_sample_roots:
