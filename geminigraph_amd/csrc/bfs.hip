@@ -395,6 +395,7 @@ __global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOu
 struct BfsPullArgs {
   const uint64_t *index;
   const uint32_t *nbr;
+  const uint32_t *first_orig;  // original id of each vertex's first in-neighbour
   const uint64_t *fin;
   uint32_t V, light_max, serial_max, ngroups;
 };
@@ -413,8 +414,19 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOu
   const uint32_t W = gridDim.x * kWavesPerBlock;
   const int lane = lane_id();
   uint64_t cnt = 0, edg = 0, ind = 0, examined = 0;
-  for (uint32_t g = global_wave(); g < a.ngroups; g += W) {
-    const uint64_t visw = o.visited[g] | a.fin[g];
+  // the next group's bitmap words are loaded while this group is processed
+  uint32_t g = global_wave();
+  uint64_t vis_n = 0, fin_n = 0;
+  if (g < a.ngroups) {
+    vis_n = o.visited[g];
+    fin_n = a.fin[g];
+  }
+  for (; g < a.ngroups; g += W) {
+    const uint64_t visw = vis_n | fin_n;
+    if (g + W < a.ngroups) {
+      vis_n = o.visited[g + W];
+      fin_n = a.fin[g + W];
+    }
     if (visw == ~0ULL) {
       if (lane == 0) {
         o.fout[g] = 0;
@@ -428,10 +440,15 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOu
     const uint32_t deg = vi.z;
     cand = cand && deg > 0 && deg <= a.light_max;
     // the first in-neighbour (the highest-degree one on a relabelled graph)
-    // decides most vertices without touching the list
+    // decides most vertices without touching the list; its original id is
+    // read alongside, so the parent needs no further load
     uint32_t found = kUnset;
+    bool head = false;  // found at the first in-neighbour
+    uint32_t head_orig = 0;
     if (cand) {
-      if (test_bit(a.fin, vi.w)) found = vi.w;
+      head_orig = a.first_orig[v];
+      head = test_bit(a.fin, vi.w);
+      if (head) found = vi.w;
       examined += 1;
     }
     cand = cand && found == kUnset && deg > 1;
@@ -481,7 +498,7 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOu
     const bool got = found != kUnset;
     const uint64_t m = __ballot(got);
     if (got) {
-      o.parent[vi.x] = o.vinfo[found].x;
+      o.parent[vi.x] = head ? head_orig : o.vinfo[found].x;
       o.depth[vi.x] = level;
       edg += vi.y;
       ind += deg;
@@ -663,7 +680,8 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
     }
     {
       ProfScope p("bfs_pull");
-      BfsPullArgs a{in.index, in.nbr, fin, V, in.light_max, (uint32_t)opt.pull_serial_max,
+      BfsPullArgs a{in.index, in.nbr, in.first_orig, fin, V, in.light_max,
+                    (uint32_t)opt.pull_serial_max,
                     ngroups};
       hipLaunchKernelGGL(bfs_pull_kernel, dim3(pull_grid), dim3(kBlock), 0, st, s->ctl, o, a,
                          set_out, exam);

@@ -245,12 +245,14 @@ int graph_alloc(gg_graph *g, void **p, size_t bytes) {
 }
 
 __global__ void vinfo_kernel(const uint64_t *index, const uint32_t *nbr, const uint32_t *other_deg,
-                             const uint32_t *iperm, uint32_t V, uint4 *vinfo) {
+                             const uint32_t *iperm, uint32_t V, uint4 *vinfo,
+                             uint32_t *first_orig) {
   const uint32_t s = gridDim.x * blockDim.x;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s) {
     const uint64_t b = index[v], e = index[v + 1];
-    vinfo[v] = make_uint4(iperm ? iperm[v] : v, other_deg[v], (uint32_t)(e - b),
-                          b < e ? nbr[b] : kUnset);
+    const uint32_t f = b < e ? nbr[b] : kUnset;
+    vinfo[v] = make_uint4(iperm ? iperm[v] : v, other_deg[v], (uint32_t)(e - b), f);
+    first_orig[v] = f == kUnset ? V : (iperm ? iperm[f] : f);
   }
 }
 
@@ -258,9 +260,11 @@ __global__ void vinfo_kernel(const uint64_t *index, const uint32_t *nbr, const u
 static int build_vinfo(gg_graph *g, Dir &d, const Dir &other) {
   const uint32_t V = g->vertices;
   GG_TRY(graph_alloc(g, (void **)&d.vinfo, (size_t)std::max(V, 1u) * sizeof(uint4)));
+  GG_TRY(graph_alloc(g, (void **)&d.first_orig, (size_t)std::max(V, 1u) * 4));
   const unsigned grid = (unsigned)std::min<uint64_t>(((uint64_t)V + kBlock - 1) / kBlock, 65536);
   hipLaunchKernelGGL(vinfo_kernel, dim3(std::max(grid, 1u)), dim3(kBlock), 0, stream(), d.index,
-                     d.nbr, other.degree, g->reordered ? g->iperm : nullptr, V, d.vinfo);
+                     d.nbr, other.degree, g->reordered ? g->iperm : nullptr, V, d.vinfo,
+                     d.first_orig);
   GG_HIP(hipGetLastError());
   GG_HIP(hipStreamSynchronize(stream()));  // the graph may be used from other streams
   return GG_OK;

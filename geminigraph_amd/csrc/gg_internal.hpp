@@ -100,6 +100,7 @@ struct Dir {
   // first (smallest) neighbour here or kUnset}: what a BFS step needs of a
   // vertex it finds, in one 16-byte read
   uint4 *vinfo = nullptr;
+  uint32_t *first_orig = nullptr;  // V: original id of the first neighbour (V if none)
   // heavy chunks (degree > light limit at build time)
   uint32_t n_chunks = 0;
   uint32_t *chunk_v = nullptr;
