@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void bfs_decide_kernel(BfsCtl *c, uint64_t 
   bfs_decide(c, set, qc, exam, lv, hf);
 }
 
-__global__ __launch_bounds__(kBlock) void bfs_bin_kernel(const BfsCtl *c, const uint64_t *fin,
+__global__ __launch_bounds__(kCompactBlock) void bfs_bin_kernel(const BfsCtl *c, const uint64_t *fin,
                                                          uint64_t *fout, uint64_t *visited,
                                                          size_t words,
                                                          const uint32_t *deg, uint32_t small_max,
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kBlock) void bfs_bin_kernel(const BfsCtl *c, const 
                                                          uint32_t *q1, uint32_t *q2,
                                                          Counters *qc) {
   if (c->mode != kBfsPush) return;
-  compact_words(fin, words, deg, small_max, big_max, q0, q1, q2, qc, visited, fout);
+  compact_words_spread(fin, words, deg, small_max, big_max, q0, q1, q2, qc, visited, fout);
 }
 
 struct BfsPushArgs {
@@ -642,7 +642,7 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
                      in.vinfo, V, (int32_t)opt.force_mode, threshold, g->arcs,
                      (uint64_t)opt.bfs_alpha, s->lv_cap);
   // fixed launch shapes: the frontier sizes live on the device
-  const unsigned bin_grid = grid1(words, 2048);
+  const unsigned bin_grid = compact_grid(words);
   const unsigned push_grid = grid_waves((uint64_t)opt.huge_grid_waves);
   const uint32_t ngroups = (V + 63) / 64;
   const uint64_t pw = (uint64_t)opt.bfs_pull_waves;
@@ -667,7 +667,7 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
     BfsOut o{d_parent, d_depth, in.vinfo, w->visited, fout};
     {
       ProfScope p("compact");
-      hipLaunchKernelGGL(bfs_bin_kernel, dim3(bin_grid), dim3(kBlock), 0, st, s->ctl, fin, fout,
+      hipLaunchKernelGGL(bfs_bin_kernel, dim3(bin_grid), dim3(kCompactBlock), 0, st, s->ctl, fin, fout,
                          w->visited, words, out.degree, (uint32_t)opt.push_small_max,
                          (uint32_t)opt.push_big_max, w->f[0].q[0], w->f[0].q[1], w->f[0].q[2],
                          s->qc);

@@ -258,6 +258,136 @@ __device__ __forceinline__ void compact_words(const uint64_t *bits, size_t words
     __syncthreads();
   }
 }
+
+// Position of the r-th set bit of w (r < popcount(w)).
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t w, uint32_t r) {
+  uint32_t pos = 0;
+#pragma unroll
+  for (int sh = 32; sh >= 1; sh >>= 1) {
+    const uint32_t c = (uint32_t)__popcll(w & ((1ULL << sh) - 1));
+    if (r >= c) {
+      r -= c;
+      pos += sh;
+      w >>= sh;
+    }
+  }
+  return pos;
+}
+
+// The same compaction for sparse frontiers, launched with kCompactBlock
+// threads: a block takes one contiguous range of words and appends all of it
+// with one atomic per counter (the per-slice atomics above, from thousands of
+// slices, serialise in L2 when a frontier touches most of them), and a wave
+// spreads the set bits of its 64 words over its lanes (a prefix of
+// popcounts, then the owner word and the bit's position per lane), so a
+// dense word costs no lane more than its share.  Slower than the version
+// above on dense frontiers, where every word is full.
+constexpr int kCompactBlock = 1024;
+constexpr int kCompactWaves = kCompactBlock / 64;
+inline unsigned compact_grid(size_t words) {
+  size_t g = (words + kCompactBlock - 1) / kCompactBlock;
+  return (unsigned)std::max<size_t>(1, std::min<size_t>(g, 1024));
+}
+__device__ __forceinline__ void compact_words_spread(const uint64_t *bits, size_t words,
+                                                     const uint32_t *bin_deg, uint32_t small_max,
+                                                     uint32_t big_max, uint32_t *q0, uint32_t *q1,
+                                                     uint32_t *q2, Counters *cnt,
+                                                     uint64_t *visited, uint64_t *clear) {
+  __shared__ uint32_t s_tot[3][kCompactWaves];
+  __shared__ unsigned long long s_edges[kCompactWaves];
+  __shared__ uint32_t s_base[3];
+  const int lane = lane_id(), wv = (int)(threadIdx.x >> 6);
+  const uint64_t below = lanemask_lt();
+  // this block's words: [beg, end), whole multiples of the block size
+  const size_t per = (words + (size_t)gridDim.x * kCompactBlock - 1) /
+                     ((size_t)gridDim.x * kCompactBlock) * kCompactBlock;
+  const size_t beg = (size_t)blockIdx.x * per;
+  const size_t end = beg + per < words ? beg + per : words;
+  // the j-th set bit of a wave's 64 words (vertex and bin degree)
+  auto bit_at = [&](uint64_t word, uint32_t excl, uint32_t total, size_t wbase, uint32_t j,
+                    uint32_t &v, uint32_t &d) {
+    const bool ok = j < total;
+    const uint32_t jj = ok ? j : total - 1;
+    const int k = wave_owner(excl, jj);
+    const uint64_t wk = shfl_u64(word, k);
+    v = (uint32_t)((wbase + (size_t)k) * 64 + nth_set_bit(wk, jj - (uint32_t)__shfl(excl, k)));
+    d = ok ? bin_deg[v] : 0;
+  };
+  // pass 1: this wave's totals over the whole range
+  uint32_t c0 = 0, c1 = 0, c2 = 0;
+  uint64_t es = 0;
+  for (size_t base = beg; base < end; base += kCompactBlock) {
+    const size_t wi = base + threadIdx.x;
+    const uint64_t word = wi < end ? bits[wi] : 0;
+    if (visited && word) visited[wi] |= word;
+    if (clear && wi < end) clear[wi] = 0;
+    uint32_t total;
+    const uint32_t excl = wave_excl_scan((uint32_t)__popcll(word), &total);
+    for (uint32_t j0 = 0; j0 < total; j0 += 64) {
+      uint32_t v, d;
+      bit_at(word, excl, total, base + (size_t)wv * 64, j0 + lane, v, d);
+      const bool ok = j0 + lane < total;
+      es += d;
+      c0 += (uint32_t)__popcll(__ballot(ok && d <= small_max));
+      c1 += (uint32_t)__popcll(__ballot(ok && d > small_max && d <= big_max));
+      c2 += (uint32_t)__popcll(__ballot(ok && d > big_max));
+    }
+  }
+  const uint64_t ws = wave_sum_u64(es);
+  if (lane == 0) {
+    s_tot[0][wv] = c0;
+    s_tot[1][wv] = c1;
+    s_tot[2][wv] = c2;
+    s_edges[wv] = ws;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t b0 = 0, b1 = 0, b2 = 0;
+    unsigned long long be = 0;
+    for (int i = 0; i < kCompactWaves; i++) {
+      b0 += s_tot[0][i];
+      b1 += s_tot[1][i];
+      b2 += s_tot[2][i];
+      be += s_edges[i];
+    }
+    s_base[0] = b0 ? atomicAdd(&cnt->n[0], b0) : 0;
+    s_base[1] = b1 ? atomicAdd(&cnt->n[1], b1) : 0;
+    s_base[2] = b2 ? atomicAdd(&cnt->n[2], b2) : 0;
+    if (b0 + b1 + b2) {
+      atomicAdd(&cnt->total, b0 + b1 + b2);
+      atomicAdd((unsigned long long *)&cnt->edges, be);
+    }
+  }
+  __syncthreads();
+  // pass 2: each wave writes its vertices from its offsets on
+  uint32_t o0 = s_base[0], o1 = s_base[1], o2 = s_base[2];
+  for (int i = 0; i < wv; i++) {
+    o0 += s_tot[0][i];
+    o1 += s_tot[1][i];
+    o2 += s_tot[2][i];
+  }
+  if (!(c0 | c1 | c2)) return;
+  for (size_t base = beg; base < end; base += kCompactBlock) {
+    const size_t wi = base + threadIdx.x;
+    const uint64_t word = wi < end ? bits[wi] : 0;
+    uint32_t total;
+    const uint32_t excl = wave_excl_scan((uint32_t)__popcll(word), &total);
+    for (uint32_t j0 = 0; j0 < total; j0 += 64) {
+      uint32_t v, d;
+      bit_at(word, excl, total, base + (size_t)wv * 64, j0 + lane, v, d);
+      const bool ok = j0 + lane < total;
+      const bool b0 = ok && d <= small_max, b1 = ok && d > small_max && d <= big_max,
+                 b2 = ok && d > big_max;
+      const uint64_t m0 = __ballot(b0), m1 = __ballot(b1), m2 = __ballot(b2);
+      if (b0) q0[o0 + __popcll(m0 & below)] = v;
+      if (b1) q1[o1 + __popcll(m1 & below)] = v;
+      if (b2) q2[o2 + __popcll(m2 & below)] = v;
+      o0 += (uint32_t)__popcll(m0);
+      o1 += (uint32_t)__popcll(m1);
+      o2 += (uint32_t)__popcll(m2);
+    }
+  }
+}
 __global__ __launch_bounds__(kBlock) void compact_kernel(const uint64_t *bits, size_t words,
                                                          const uint32_t *bin_deg,
                                                          uint32_t small_max, uint32_t big_max,
