@@ -115,6 +115,52 @@ int read_dir(FILE *f, gg_graph *g, Dir &d, const GraphHeader &h, Staging &s, con
   return GG_OK;
 }
 
+// A saved graph is checked before any kernel walks it: offsets start at 0,
+// never decrease and end at the arc count, degrees match them, neighbours
+// are vertices, and the relabeling is a permutation with its inverse.
+__global__ void check_dir_kernel(const uint64_t *index, const uint32_t *nbr, const uint32_t *deg,
+                                 uint32_t V, uint64_t arcs, uint32_t *bad) {
+  const uint64_t s = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint64_t v = i0; v < V; v += s) {
+    const uint64_t b = index[v], e = index[v + 1];
+    if (e < b || e > arcs || deg[v] != e - b || (v == 0 && b != 0)) atomicOr(bad, 1u);
+  }
+  if (i0 == 0 && index[V] != arcs) atomicOr(bad, 1u);
+  for (uint64_t a = i0; a < arcs; a += s)
+    if (nbr[a] >= V) atomicOr(bad, 2u);
+}
+__global__ void check_perm_kernel(const uint32_t *perm, const uint32_t *iperm, uint32_t V,
+                                  uint32_t *bad) {
+  const uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += s) {
+    const uint32_t p = perm[i];
+    if (p >= V || iperm[p] != i) atomicOr(bad, 4u);
+  }
+}
+
+int check_graph_arrays(const gg_graph *g) {
+  uint32_t *bad = nullptr;
+  GG_TRY(device_alloc((void **)&bad, 4));
+  hipMemsetAsync(bad, 0, 4, stream());
+  const unsigned grid = grid1((uint64_t)g->vertices + g->arcs, 8192);
+  hipLaunchKernelGGL(check_dir_kernel, dim3(grid), dim3(kBlock), 0, stream(), g->out.index,
+                     g->out.nbr, g->out.degree, g->vertices, g->arcs, bad);
+  if (!g->symmetric)
+    hipLaunchKernelGGL(check_dir_kernel, dim3(grid), dim3(kBlock), 0, stream(), g->in.index,
+                       g->in.nbr, g->in.degree, g->vertices, g->arcs, bad);
+  if (g->reordered)
+    hipLaunchKernelGGL(check_perm_kernel, dim3(grid1(g->vertices, 8192)), dim3(kBlock), 0,
+                       stream(), g->perm, g->iperm, g->vertices, bad);
+  uint32_t h = 0;
+  hipError_t e = hipMemcpyAsync(&h, bad, 4, hipMemcpyDeviceToHost, stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(stream());
+  device_free(bad);
+  if (e != hipSuccess) return fail(GG_EHIP, hipGetErrorString(e));
+  if (h) return fail(GG_EIO, "saved graph has inconsistent arrays");
+  return GG_OK;
+}
+
 int open_graph(const char *path, gg_graph *g) {
   File fh;
   fh.f = fopen(path, "rb");
@@ -144,9 +190,12 @@ int open_graph(const char *path, gg_graph *g) {
     GG_TRY(graph_alloc(g, (void **)&g->iperm, V * 4));
     GG_TRY(read_device(fh.f, g->perm, V * 4, s, path));
     GG_TRY(read_device(fh.f, g->iperm, V * 4, s, path));
-    g->h_perm.resize(V);
-    GG_HIP(hipMemcpy(g->h_perm.data(), g->perm, V * 4, hipMemcpyDeviceToHost));
     g->reordered = 1;
+  }
+  GG_TRY(check_graph_arrays(g));
+  if (g->reordered) {
+    g->h_perm.resize(h.vertices);
+    GG_HIP(hipMemcpy(g->h_perm.data(), g->perm, (size_t)h.vertices * 4, hipMemcpyDeviceToHost));
   }
   return graph_finish(g);
 }

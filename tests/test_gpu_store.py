@@ -67,7 +67,17 @@ def test_graph_open_refuses_bad_files(gg, tmp_path):
     data = p.read_bytes()
     (tmp_path / "short").write_bytes(data[:-4])
     (tmp_path / "magic").write_bytes(b"XXXXXXXX" + data[8:])
-    for name in ("short", "magic"):
+    # a neighbour id past the vertex count at the end of the file (the last
+    # array is the in-direction's neighbour list of this unweighted graph)
+    bad = bytearray(data)
+    bad[-4:] = (5000).to_bytes(4, "little")
+    (tmp_path / "nbr").write_bytes(bytes(bad))
+    # offsets that decrease (the out-direction index follows the header and degrees)
+    off = 48 + 4 * (1 << 10) + 8 * 5
+    bad = bytearray(data)
+    bad[off:off + 8] = (1 << 40).to_bytes(8, "little")
+    (tmp_path / "index").write_bytes(bytes(bad))
+    for name in ("short", "magic", "nbr", "index"):
         with pytest.raises(gg.GGError):
             gg.Graph.open(tmp_path / name)
     with pytest.raises(gg.GGError):
