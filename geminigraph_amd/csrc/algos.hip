@@ -169,6 +169,7 @@ void work_free(Work *w) {
   device_free(w->mb_a);
   device_free(w->mb_b);
   device_free(w->ms_levels);
+  msd_state_free(w->msd);
   if (w->h_cnt) hipHostFree(w->h_cnt);
   if (w->h_exam) hipHostFree(w->h_exam);
   delete w;
@@ -669,11 +670,13 @@ struct MsPullArgs {
   uint64_t *fnext, *out;
   uint64_t full;
   uint32_t V, light_max, serial_max, ngroups;
+  const uint32_t *gate;  // null, or: run only when *gate == kGateDense
 };
 __global__ __launch_bounds__(kBlock) void ms_pull_group_kernel(MsPullArgs a) {
   const uint32_t g = global_wave();
   const int lane = lane_id();
   if (g >= a.ngroups) return;
+  if (a.gate && *a.gate != kGateDense) return;
   uint32_t v = g * 64 + lane;
   uint64_t need = 0, b = 0, e1 = 0;
   if (v < a.V) {
@@ -1222,7 +1225,7 @@ static int ms_bfs_run(gg_graph *g, Work *w, Engine &en, const uint32_t *roots, i
       op.fnext = fnext;
       op.full = full;
       MsPullArgs pa{in.index, in.nbr, fcur, w->ms_vis, fnext, en.fout().bits, full, V,
-                    in.light_max, (uint32_t)options().pull_serial_max, (V + 63) / 64};
+                    in.light_max, (uint32_t)options().pull_serial_max, (V + 63) / 64, nullptr};
       {
         ProfScope p("ms_pull");
         hipLaunchKernelGGL(ms_pull_group_kernel, dim3(grid_waves(pa.ngroups)), dim3(kBlock), 0,
@@ -1252,6 +1255,461 @@ static int ms_bfs_run(gg_graph *g, Work *w, Engine &en, const uint32_t *roots, i
     fc ^= 1;
   }
   res->levels = level;  // levels 0 .. level-1 hold vertices (the last one may be empty)
+  res->masks = masks;
+  return GG_OK;
+}
+
+// ------------------------------------------- batched BFS, device-driven loop
+// The level loop of the batched BFS without a host round trip per level.
+// Every level is the same fixed sequence of launches, issued ahead on one
+// stream; each kernel reads the level's mode from the control block and
+// returns at once when the level is not its kind or the traversal is over:
+//   bin    -- push only: frontier bitmap -> three degree-binned queues;
+//   push   -- push only: a persistent grid over the queues (MsPush);
+//   dense  -- pull only: the destination-bin plan (relabelled graphs), or
+//             the 64-vertex group pull plus the heavy in-list chunks;
+//   apply  -- every level: the new masks join visited, the level's root
+//             masks (or depth rows) are stored, the new frontier is counted,
+//             and the spent frontier's bitmap and masks are cleared, so each
+//             buffer of the two pairs is clean when it is next written;
+//   decide -- one block: the push/pull switch on the new frontier's
+//             out-arcs (core/graph.hpp:1509), the level's records, the end.
+// The host stays one level ahead and polls the decisions in coherent pinned
+// memory, as the single-root BFS loop does.
+enum : uint32_t { kMsPush = 0, kMsPull = kGateDense, kMsDone = 2 };
+static_assert(kMsPull != kMsPush && kMsPull != kMsDone, "distinct modes");
+static_assert(kBlock == (int)kScalarSlots, "one count slot per decide thread");
+
+struct MsCtl {
+  uint32_t done, iter;   // iter = iterations decided
+  uint32_t mode, level;  // level = the one the current iteration produces
+  uint32_t push_iters;
+  int32_t force;
+  uint64_t threshold;
+  uint32_t modes[64];
+  uint64_t active[64], fresh[64];
+};
+struct MsHostFlag {
+  uint32_t decided;  // iterations decided so far
+  uint32_t done;
+};
+struct MsdState {
+  MsCtl *ctl = nullptr;
+  uint64_t *set = nullptr;  // kScalarSlots count slots of 16 u64 (one 128-byte line)
+  Counters *qc = nullptr;
+  MsCtl *h_ctl = nullptr;         // pinned
+  MsHostFlag *h_flag = nullptr;   // pinned, coherent: written by the device
+};
+
+void msd_state_free(void *p) {
+  MsdState *s = (MsdState *)p;
+  if (!s) return;
+  device_free(s->ctl);
+  device_free(s->set);
+  device_free(s->qc);
+  if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+  if (s->h_flag) (void)hipHostFree(s->h_flag);
+  delete s;
+}
+
+static int msd_state(Work *w, MsdState **out) {
+  MsdState *s = (MsdState *)w->msd;
+  if (!s) {
+    s = new MsdState;
+    w->msd = s;
+    GG_TRY(device_alloc((void **)&s->ctl, sizeof(MsCtl)));
+    GG_TRY(device_alloc((void **)&s->set, (size_t)kScalarSlots * 16 * 8));
+    GG_TRY(device_alloc((void **)&s->qc, sizeof(Counters)));
+    GG_HIP(hipHostMalloc((void **)&s->h_ctl, sizeof(MsCtl)));
+    GG_HIP(hipHostMalloc((void **)&s->h_flag, sizeof(MsHostFlag), hipHostMallocCoherent));
+  }
+  *out = s;
+  return GG_OK;
+}
+
+// Reduces the new frontier's counts (the spread slots are cleared for the
+// next level) and decides the next iteration.  All kBlock threads of one
+// block.
+__device__ void msd_decide(MsCtl *c, uint64_t *set, Counters *qc, MsHostFlag *hf) {
+  __shared__ uint64_t sn[kBlock], se[kBlock];
+  __shared__ uint32_t s_done;
+  const int t = threadIdx.x;
+  if (t == 0) s_done = c->done;
+  __syncthreads();
+  if (s_done) return;
+  uint64_t *p = set + (size_t)t * 16;
+  sn[t] = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  se[t] = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      sn[t] += sn[t + s];
+      se[t] += se[t + s];
+    }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  const uint64_t total = sn[0], edges = se[0];
+  const uint32_t i = c->iter;
+  if (i > 0 && i - 1 < 64) c->fresh[i - 1] = total;
+  if (total == 0) {
+    c->done = 1;
+    c->mode = kMsDone;
+    __threadfence_system();
+    __hip_atomic_store(&hf->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    return;
+  }
+  // core/graph.hpp:1509: sparse iff the frontier's out-arcs < |E| / divisor
+  const uint32_t mode = c->force == 1                ? kMsPush
+                        : c->force == 2              ? kMsPull
+                        : edges < c->threshold       ? kMsPush
+                                                     : kMsPull;
+  if (i < 64) {
+    c->modes[i] = mode == kMsPush ? 0u : 1u;
+    c->active[i] = total;
+  }
+  c->mode = mode;
+  c->level = i + 1;
+  c->iter = i + 1;
+  if (mode == kMsPush) c->push_iters++;
+  qc->n[0] = qc->n[1] = qc->n[2] = 0;
+  qc->total = 0;
+  qc->edges = 0;
+  __threadfence_system();
+  __hip_atomic_store(&hf->decided, i + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// One block: the roots (root i gets bit i of its vertex's masks; roots may
+// repeat), the control block, the count of distinct roots and their
+// out-arcs, and the decision for the first iteration.
+__global__ __launch_bounds__(kBlock) void msd_seed_kernel(MsCtl *c, uint64_t *set, Counters *qc,
+                                                          MsHostFlag *hf, const uint32_t *roots,
+                                                          int n, uint64_t *vis, uint64_t *f,
+                                                          uint64_t *bits, uint32_t *depth,
+                                                          const uint32_t *outdeg, int32_t force,
+                                                          uint64_t threshold) {
+  const int t = threadIdx.x;
+  if (t < n) {
+    const uint32_t r = roots[t];
+    atomicOr((unsigned long long *)&vis[r], 1ULL << t);
+    atomicOr((unsigned long long *)&f[r], 1ULL << t);
+    atomicOr((unsigned long long *)&bits[r >> 6], 1ULL << (r & 63));
+    if (depth) depth[(size_t)r * 64 + t] = 0;
+    bool first = true;
+    for (int j = 0; j < t; j++) first = first && roots[j] != r;
+    if (first) {
+      atomicAdd((unsigned long long *)&set[0], 1ull);
+      atomicAdd((unsigned long long *)&set[1], (unsigned long long)outdeg[r]);
+    }
+  }
+  if (t == 0) {
+    c->done = 0;
+    c->iter = 0;
+    c->mode = kMsDone;
+    c->level = 0;
+    c->push_iters = 0;
+    c->force = force;
+    c->threshold = threshold;
+  }
+  __threadfence();
+  __syncthreads();
+  msd_decide(c, set, qc, hf);
+}
+
+__global__ __launch_bounds__(kBlock) void msd_decide_kernel(MsCtl *c, uint64_t *set, Counters *qc,
+                                                            MsHostFlag *hf) {
+  msd_decide(c, set, qc, hf);
+}
+
+__global__ __launch_bounds__(kCompactBlock) void msd_bin_kernel(const MsCtl *c, const uint64_t *fin,
+                                                                size_t words, const uint32_t *deg,
+                                                                uint32_t small_max,
+                                                                uint32_t big_max, uint32_t *q0,
+                                                                uint32_t *q1, uint32_t *q2,
+                                                                Counters *qc) {
+  if (c->mode != kMsPush) return;
+  compact_words_spread(fin, words, deg, small_max, big_max, q0, q1, q2, qc, nullptr, nullptr);
+}
+
+struct MsdPushArgs {
+  const uint64_t *index;
+  const uint32_t *nbr;
+  const uint32_t *q0, *q1, *q2;
+  const Counters *qc;
+  uint64_t *fout;
+};
+
+// Persistent grid over the three bins (sizes on the device): 64 small
+// vertices per wave step (in-wave load balancing), one mid vertex per wave
+// step, then chunks of every huge vertex shared by all waves, rotated per
+// vertex so a vertex with few chunks does not load the first waves only.
+__global__ __launch_bounds__(kBlock) void msd_push_kernel(const MsCtl *c, MsdPushArgs a,
+                                                          MsPush op) {
+  if (c->mode != kMsPush) return;
+  const uint32_t W = gridDim.x * kWavesPerBlock, gw = global_wave();
+  const int lane = lane_id();
+  const uint32_t n0 = a.qc->n[0], n1 = a.qc->n[1], n2 = a.qc->n[2];
+  const uint32_t waves0 = (n0 + 63) / 64;
+  for (uint32_t vw = gw; vw < waves0 + n1; vw += W) {
+    if (vw < waves0) {
+      const uint32_t i = vw * 64 + lane;
+      const bool valid = i < n0;
+      const uint32_t v = valid ? a.q0[i] : 0;
+      const uint64_t b = valid ? a.index[v] : 0;
+      const uint32_t deg = valid ? (uint32_t)(a.index[v + 1] - b) : 0;
+      const uint64_t m = valid ? op.fcur[v] : 0;
+      uint32_t total;
+      const uint32_t excl = wave_excl_scan(deg, &total);
+      for (uint32_t j0 = 0; j0 < total; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        const int k = wave_owner(excl, j);
+        const uint64_t bk = shfl_u64(b, k), mk = shfl_u64(m, k);
+        const uint32_t ek = __shfl(excl, k);
+        if (j < total) {
+          const uint32_t dst = a.nbr[bk + (j - ek)];
+          if (op.edge(0, mk, dst, 0.f)) activate(a.fout, dst);
+        }
+      }
+    } else {
+      const uint32_t v = a.q1[vw - waves0];
+      const uint64_t b = a.index[v], e1 = a.index[v + 1];
+      const uint64_t m = op.fcur[v];
+      for (uint64_t e = b + lane; e < e1; e += 64) {
+        const uint32_t dst = a.nbr[e];
+        if (op.edge(0, m, dst, 0.f)) activate(a.fout, dst);
+      }
+    }
+  }
+  for (uint32_t h = 0; h < n2; h++) {
+    const uint32_t v = a.q2[h];
+    const uint64_t b = a.index[v], e1 = a.index[v + 1];
+    const uint64_t m = op.fcur[v];
+    const uint64_t nch = (e1 - b + kChunkEdges - 1) / kChunkEdges;
+    const uint32_t rot = (uint32_t)(((uint64_t)h * 2654435761u) % W);
+    for (uint64_t ch = (gw + W - rot) % W; ch < nch; ch += W) {
+      const uint64_t cb = b + ch * kChunkEdges;
+      const uint64_t ce = cb + kChunkEdges < e1 ? cb + kChunkEdges : e1;
+      for (uint64_t e = cb + lane; e < ce; e += 64) {
+        const uint32_t dst = a.nbr[e];
+        if (op.edge(0, m, dst, 0.f)) activate(a.fout, dst);
+      }
+    }
+  }
+}
+
+struct MsdHeavyArgs {
+  const uint32_t *nbr;
+  const uint32_t *chunk_v;
+  const uint64_t *chunk_beg, *chunk_end;
+  uint32_t n_chunks;
+  const uint64_t *fcur, *vis;
+  uint64_t *fnext, *fout;
+  uint64_t full;
+};
+
+// The heavy in-lists of the dense step (graphs without the bin plan): one
+// wave per chunk ORs the in-neighbours' masks, 64 entries per step, until
+// every root still missing from the vertex is found; the new bits go to the
+// vertex's mask and bitmap word with atomics (chunks of a vertex combine).
+__global__ __launch_bounds__(kBlock) void msd_heavy_kernel(const MsCtl *c, MsdHeavyArgs a) {
+  if (c->mode != kMsPull) return;
+  const uint32_t W = gridDim.x * kWavesPerBlock;
+  const int lane = lane_id();
+  for (uint32_t ch = global_wave(); ch < a.n_chunks; ch += W) {
+    const uint32_t v = a.chunk_v[ch];
+    const uint64_t need = a.full & ~a.vis[v];
+    if (!need) continue;
+    const uint64_t e1 = a.chunk_end[ch];
+    uint64_t acc = 0;
+    for (uint64_t e0 = a.chunk_beg[ch]; e0 < e1; e0 += 64) {
+      const uint64_t e = e0 + lane;
+      uint64_t x = e < e1 ? a.fcur[stream_load(&a.nbr[e])] : 0ULL;
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) x |= shfl_u64(x, lane ^ m);
+      acc |= x;
+      if ((acc & need) == need) break;
+    }
+    const uint64_t nw = acc & need;
+    if (lane == 0 && nw) {
+      atomicOr((unsigned long long *)&a.fnext[v], (unsigned long long)nw);
+      atomicOr((unsigned long long *)&a.fout[v >> 6], 1ULL << (v & 63));
+    }
+  }
+}
+
+struct MsdApplyArgs {
+  uint64_t *fin, *fout;    // spent and new frontier bitmaps
+  uint64_t *fcur, *fnext;  // spent and new masks
+  uint64_t *vis;
+  uint64_t *lmask;         // per-level root masks (V per level), or null
+  uint32_t *depth;         // depth rows when lmask is null (internal order), or null
+  const uint32_t *outdeg;
+  uint64_t *set;
+  size_t words;
+  uint32_t V;
+};
+
+// One wave per 64-vertex bitmap word (lane = vertex): see the loop above.
+__global__ __launch_bounds__(kBlock) void msd_apply_kernel(const MsCtl *c, MsdApplyArgs a) {
+  if (c->mode == kMsDone) return;
+  const uint32_t level = c->level;
+  const int lane = lane_id();
+  const size_t W = (size_t)gridDim.x * kWavesPerBlock;
+  uint64_t cnt = 0, edg = 0;
+  for (size_t wi = global_wave(); wi < a.words; wi += W) {
+    const uint64_t act = a.fout[wi], old = a.fin[wi];
+    const uint32_t v = (uint32_t)(wi * 64 + lane);
+    uint64_t nw = 0;
+    if ((act >> lane) & 1ULL) {
+      // every step writes only roots not yet in vis, so fnext[v] stays as it is
+      nw = a.fnext[v];
+      a.vis[v] |= nw;
+      cnt++;
+      edg += a.outdeg[v];
+    }
+    if ((old >> lane) & 1ULL) a.fcur[v] = 0;
+    if (lane == 0 && old) a.fin[wi] = 0;
+    if (a.lmask && v < a.V) a.lmask[(size_t)level * a.V + v] = nw;
+    if (a.depth) {
+      for (uint64_t x = act; x; x &= x - 1) {
+        const int k = __builtin_ctzll(x);
+        const uint64_t m = shfl_u64(nw, k);
+        if ((m >> lane) & 1ULL) a.depth[(wi * 64 + (size_t)k) * 64 + lane] = level;
+      }
+    }
+  }
+  cnt = wave_sum_u64(cnt);
+  edg = wave_sum_u64(edg);
+  if (lane == 0 && cnt) {
+    uint64_t *p = a.set + (size_t)(global_wave() % kScalarSlots) * 16;
+    atomicAdd((unsigned long long *)p, (unsigned long long)cnt);
+    atomicAdd((unsigned long long *)(p + 1), (unsigned long long)edg);
+  }
+}
+
+// Multi-root BFS levels on the device-driven loop.  Same results and the
+// same level records as ms_bfs_run without a hook: with level masks on, each
+// level's root masks go to the store (grown, or given up for depth rows,
+// before the level is issued); `depth` holds rows in internal order.
+static int ms_bfs_device(gg_graph *g, Work *w, const uint32_t *roots, int n_roots,
+                         uint32_t *depth, gg_run_stats *st, MsLevels *res) {
+  const Dir &out = out_dir(g), &in = in_dir(g);
+  const uint32_t V = g->vertices;
+  const size_t words = w->words;
+  const Options &opt = options();
+  MsdState *s;
+  GG_TRY(msd_state(w, &s));
+  hipStream_t sm = stream();
+  bool masks = opt.level_masks != 0;
+  if (masks) GG_TRY(ms_mask_room(w, 0, &masks));
+  if (!masks) GG_HIP(hipMemsetAsync(depth, 0xFF, (size_t)V * 64 * 4, sm));
+  GG_HIP(hipMemsetAsync(w->ms_vis, 0, (size_t)V * 8, sm));
+  GG_HIP(hipMemsetAsync(w->ms_f[0], 0, (size_t)V * 8, sm));
+  GG_HIP(hipMemsetAsync(w->ms_f[1], 0, (size_t)V * 8, sm));
+  GG_HIP(hipMemsetAsync(w->f[0].bits, 0, words * 8, sm));
+  GG_HIP(hipMemsetAsync(w->f[1].bits, 0, words * 8, sm));
+  GG_HIP(hipMemsetAsync(s->set, 0, (size_t)kScalarSlots * 16 * 8, sm));
+  GG_HIP(hipMemcpyAsync(w->ms_roots, roots, 4 * (size_t)n_roots, hipMemcpyHostToDevice, sm));
+  volatile MsHostFlag *hf = s->h_flag;
+  hf->decided = 0;
+  hf->done = 0;
+  const uint64_t threshold = g->edges / (uint64_t)opt.switch_divisor;
+  hipLaunchKernelGGL(msd_seed_kernel, dim3(1), dim3(kBlock), 0, sm, s->ctl, s->set, s->qc,
+                     s->h_flag, w->ms_roots, n_roots, w->ms_vis, w->ms_f[0], w->f[0].bits,
+                     masks ? nullptr : depth, out.degree, (int32_t)opt.force_mode, threshold);
+  if (masks)
+    GG_HIP(hipMemcpyAsync(w->ms_lmask, w->ms_f[0], (size_t)V * 8, hipMemcpyDeviceToDevice, sm));
+  const uint64_t full = n_roots == 64 ? ~0ULL : ((1ULL << n_roots) - 1ULL);
+  const PrPlan *plan = nullptr;
+  GG_TRY(pr_plan_get(g, &plan));
+  // fixed launch shapes: the frontier sizes live on the device
+  const unsigned bin_grid = compact_grid(words);
+  const unsigned push_grid = grid_waves((uint64_t)opt.huge_grid_waves);
+  const uint32_t ngroups = (V + 63) / 64;
+  const unsigned heavy_grid =
+      grid_waves(std::min<uint64_t>(std::max(in.n_chunks, 1u), (uint64_t)opt.huge_grid_waves));
+  // several words per wave: one word each left the sparse levels launch-bound
+  const unsigned apply_grid = grid_waves(std::min<size_t>(words, 1u << 13));
+  for (uint32_t n = 0;; n++) {
+    // keep one iteration queued: before issuing iteration n, wait until
+    // iteration n-1 has been decided (so iteration n-2 has finished)
+    if (n >= 1) {
+      uint64_t spins = 0;
+      while (!hf->done && hf->decided < n) {
+        if ((++spins & 0xFFF) == 0 && hipStreamQuery(sm) == hipSuccess && !hf->done &&
+            hf->decided < n)
+          return fail(GG_EHIP, "batched BFS: device loop stopped without a decision");
+      }
+      if (hf->done) break;
+    }
+    if (n > V + 2) return fail(GG_EHIP, "batched BFS: traversal did not terminate");
+    const uint32_t level = n + 1;
+    if (masks) {
+      bool room;
+      GG_TRY(ms_mask_room(w, level, &room));
+      if (!room) {  // rows of levels [0, level) from the store, then rows directly
+        GG_TRY(ms_rows_from_masks(w, level, nullptr, depth));
+        masks = false;
+      }
+    }
+    uint64_t *fin = w->f[n & 1].bits, *fout = w->f[(n + 1) & 1].bits;
+    uint64_t *fcur = w->ms_f[n & 1], *fnext = w->ms_f[(n + 1) & 1];
+    uint32_t **q = w->f[0].q;
+    {
+      ProfScope p("ms_bin");
+      hipLaunchKernelGGL(msd_bin_kernel, dim3(bin_grid), dim3(kCompactBlock), 0, sm, s->ctl, fin,
+                         words, out.degree, (uint32_t)opt.push_small_max,
+                         (uint32_t)opt.push_big_max, q[0], q[1], q[2], s->qc);
+    }
+    {
+      ProfScope p("ms_push");
+      MsPush op;
+      op.fcur = fcur;
+      op.vis = w->ms_vis;
+      op.fnext = fnext;
+      MsdPushArgs a{out.index, out.nbr, q[0], q[1], q[2], s->qc, fout};
+      hipLaunchKernelGGL(msd_push_kernel, dim3(push_grid), dim3(kBlock), 0, sm, s->ctl, a, op);
+    }
+    if (plan) {
+      GG_TRY(ms_plan_iteration(g, plan, w, fcur, w->ms_vis, full, fnext, fout, &s->ctl->mode));
+    } else {
+      MsPullArgs pa{in.index, in.nbr, fcur, w->ms_vis, fnext, fout, full, V,
+                    in.light_max, (uint32_t)opt.pull_serial_max, ngroups, &s->ctl->mode};
+      {
+        ProfScope p("ms_pull");
+        hipLaunchKernelGGL(ms_pull_group_kernel, dim3(grid_waves(ngroups)), dim3(kBlock), 0, sm,
+                           pa);
+      }
+      if (in.n_chunks) {
+        ProfScope p("ms_pull_heavy");
+        MsdHeavyArgs h{in.nbr, in.chunk_v, in.chunk_beg, in.chunk_end, in.n_chunks,
+                       fcur, w->ms_vis, fnext, fout, full};
+        hipLaunchKernelGGL(msd_heavy_kernel, dim3(heavy_grid), dim3(kBlock), 0, sm, s->ctl, h);
+      }
+    }
+    {
+      ProfScope p("ms_apply");
+      MsdApplyArgs a{fin, fout, fcur, fnext, w->ms_vis, masks ? w->ms_lmask : nullptr,
+                     masks ? nullptr : depth, out.degree, s->set, words, V};
+      hipLaunchKernelGGL(msd_apply_kernel, dim3(apply_grid), dim3(kBlock), 0, sm, s->ctl, a);
+    }
+    hipLaunchKernelGGL(msd_decide_kernel, dim3(1), dim3(kBlock), 0, sm, s->ctl, s->set, s->qc,
+                       s->h_flag);
+    GG_HIP(hipGetLastError());
+  }
+  GG_HIP(hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(MsCtl), hipMemcpyDeviceToHost, sm));
+  GG_HIP(hipStreamSynchronize(sm));
+  const MsCtl &c = *s->h_ctl;
+  st->iterations = c.iter;
+  st->push_iterations = c.push_iters;
+  for (uint32_t i = 0; i < std::min<uint32_t>(c.iter, 64); i++) {
+    st->modes[i] = c.modes[i];
+    st->it_active[i] = c.active[i];
+    st->it_new[i] = c.fresh[i];
+  }
+  res->levels = c.level + 1;  // levels 0 .. c.level hold vertices (the last one is empty)
   res->masks = masks;
   return GG_OK;
 }
@@ -1599,9 +2057,7 @@ int bfs_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, uint32_t *d
   DeviceTimer tm;
   uint32_t *depth = g->reordered || keep ? (uint32_t *)w->ms_tmp : d_depth;
   MsLevels lv;
-  GG_TRY(ms_bfs_run(
-      g, w, en, roots, n_roots, depth,
-      [](uint32_t, const uint64_t *, const uint64_t *) { return GG_OK; }, false, &lv));
+  GG_TRY(ms_bfs_device(g, w, roots, n_roots, depth, &en.st, &lv));
   uint64_t m = 0;
   GG_TRY(ms_reach(g, w, &m));
   if (rows) *rows = depth;

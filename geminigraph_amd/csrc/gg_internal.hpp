@@ -216,6 +216,7 @@ struct Work {
   size_t ms_lmask_cap = 0;
   size_t bytes = 0;                    // device bytes held (fixed part)
   void *bfs = nullptr;         // device-driven BFS loop state (bfs.hip)
+  void *msd = nullptr;         // device-driven batched BFS loop state (algos.hip)
   double *pr_vals = nullptr;   // PageRank cold contributions, in cold-arc storage order
   double *pr_gsum = nullptr;   // PageRank sums of split bins (kept at 0 between iterations)
   uint64_t pr_vals_cap = 0;
@@ -232,6 +233,7 @@ int graph_finish(gg_graph *g);
 int work_get(gg_graph *g, Work **w);
 void work_free(Work *w);
 void bfs_state_free(void *state);
+void msd_state_free(void *state);
 // PageRank plan (prplan.hip): *out = null when the graph gets the plain pull
 // (not relabelled, option pr_hot_sources = 0, or 2^32 arcs or more).
 int pr_plan_get(gg_graph *g, const PrPlan **out);
@@ -244,9 +246,12 @@ int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull 
 // One dense step of the batched BFS on the plan: for every vertex the OR of
 // its in-neighbours' 64-root frontier masks (fcur); the bits not in vis (and
 // within full) go to fnext[v] and v's bit to the bitmap `out` (every word is
-// written).
+// written).  With a gate, the kernels return at once unless *gate is
+// kGateDense when they run (a device-driven loop's mode word).
+constexpr uint32_t kGateDense = 1;
 int ms_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *fcur,
-                      const uint64_t *vis, uint64_t full, uint64_t *fnext, uint64_t *out);
+                      const uint64_t *vis, uint64_t full, uint64_t *fnext, uint64_t *out,
+                      const uint32_t *gate = nullptr);
 // Device bytes of one traversal context that runs batched queries of `kind`
 // (GG_QUERY_*) on g, counting the first 64 stored BFS levels.
 size_t work_footprint(const gg_graph *g, int kind);

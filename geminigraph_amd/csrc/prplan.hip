@@ -136,10 +136,14 @@ __global__ void pr_cold_split_kernel(const uint64_t *keys, uint64_t n, int bb, i
 // every vertex under a BinOp: PageRank sums FP64 contributions; the batched
 // BFS ORs 64-root frontier masks.  A BinOp gives the value type T, the
 // source value, the LDS and global combining atomics, and the per-vertex
-// finish (which reports whether the vertex joins the output frontier).
+// finish (which reports whether the vertex joins the output frontier).  With
+// kSkipDone, done(v) marks destinations whose result is already settled: the
+// bin kernel then reads no source value for their arcs.
 struct PrSumOp {
   using T = double;
+  static constexpr bool kSkipDone = false;
   PrPull op;
+  __device__ bool done(uint32_t) const { return false; }
   __device__ T value(uint32_t u) const { return op.curr[u]; }
   __device__ static void lds_add(T *acc, uint32_t i, T x) { atomicAdd(&acc[i], x); }
   __device__ static void global_add(T *p, T x) {
@@ -152,9 +156,11 @@ struct PrSumOp {
 };
 struct MsOrOp {
   using T = uint64_t;
+  static constexpr bool kSkipDone = true;  // every root has reached the vertex
   const uint64_t *fcur, *vis;
   uint64_t *fnext;
   uint64_t full;
+  __device__ bool done(uint32_t v) const { return (vis[v] & full) == full; }
   __device__ T value(uint32_t u) const { return fcur[u]; }
   __device__ static void lds_add(T *acc, uint32_t i, T x) {
     if (x) atomicOr((unsigned long long *)&acc[i], (unsigned long long)x);
@@ -172,10 +178,12 @@ struct MsOrOp {
 // Emit: part p = arcs [lo, hi) of range r.  The range's values go to LDS
 // (one coalesced read), then every arc's value is written in storage order.
 // Values are copied as 8-byte words.
-__global__ __launch_bounds__(kPrBlock) void pr_emit_kernel(const uint32_t *parts,
+__global__ __launch_bounds__(kPrBlock) void pr_emit_kernel(const uint32_t *gate,
+                                                           const uint32_t *parts,
                                                            const uint64_t *curr,
                                                            const uint16_t *src16, uint32_t H,
                                                            uint32_t V, int rb, uint64_t *vals) {
+  if (gate && *gate != kGateDense) return;
   extern __shared__ uint64_t val[];  // 2^rb values
   const uint32_t r = parts[3 * blockIdx.x], lo = parts[3 * blockIdx.x + 1],
                  hi = parts[3 * blockIdx.x + 2];
@@ -205,16 +213,51 @@ __global__ __launch_bounds__(kPrBlock) void pr_emit_kernel(const uint32_t *parts
 // reports true (a whole bin owns its words).
 template <class Op>
 __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
-    const PrSeg *segs, const uint32_t *hot, const uint32_t *run_beg, const uint32_t *run_end,
+    const uint32_t *gate, const PrSeg *segs, const uint32_t *hot, const uint32_t *run_beg, const uint32_t *run_end,
     uint32_t nranges, const uint16_t *dst16, const typename Op::T *vals, uint32_t V, int bb,
     Op op, typename Op::T *gsum, uint64_t *out, double *dacc) {
   using T = typename Op::T;
+  if (gate && *gate != kGateDense) return;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *acc = reinterpret_cast<T *>(lds_raw);  // 2^bb values
   const uint32_t nb = 1u << bb, dmask = nb - 1;
   const PrSeg &sg = segs[blockIdx.x];  // read in place: cut[] is indexed by wave
+  const uint32_t base = sg.bin << bb;
+  const uint32_t n = V - base < nb ? V - base : nb;
   for (uint32_t i = threadIdx.x; i < nb; i += kPrBlock) acc[i] = T(0);
-  __syncthreads();
+  // settled destinations of this bin, one bit each (bins hold at most 2^14)
+  __shared__ uint32_t sdone[Op::kSkipDone ? (1u << 14) / 32 : 1];
+  __shared__ uint32_t s_ndone;
+  bool skip = false;  // this segment tests sdone before reading a source value
+  if constexpr (Op::kSkipDone) {
+    if (threadIdx.x == 0) s_ndone = 0;
+    __syncthreads();
+    for (uint32_t i0 = threadIdx.x - lane_id(); i0 < nb; i0 += kPrBlock) {
+      const uint32_t i = i0 + lane_id();
+      const uint64_t m = __ballot(i >= n || op.done(base + i));
+      if (lane_id() == 0) {
+        sdone[i0 >> 5] = (uint32_t)m;
+        sdone[(i0 >> 5) + 1] = (uint32_t)(m >> 32);
+        atomicAdd(&s_ndone, (uint32_t)__popcll(m));
+      }
+    }
+    __syncthreads();
+    const uint32_t nd = s_ndone;
+    if (nd == nb) {  // nothing left to find in this bin
+      if (sg.slot) return;
+      for (uint32_t i0 = threadIdx.x - lane_id(); i0 < n; i0 += kPrBlock)
+        if (out && lane_id() == 0) out[(base + i0) >> 6] = 0;
+      return;
+    }
+    // testing costs an LDS read per arc: worth it once a quarter is settled
+    skip = 4 * nd >= nb;
+  } else {
+    __syncthreads();
+  }
+  auto settled = [&](uint32_t d) -> bool {
+    if constexpr (Op::kSkipDone) return skip && ((sdone[d >> 5] >> (d & 31)) & 1u);
+    return false;
+  };
   {  // hot arcs: sorted by source, so lanes share gathered lines
     constexpr int kU = 8;
     for (uint32_t a = sg.hot_lo + threadIdx.x; a < sg.hot_hi; a += kPrBlock * kU) {
@@ -226,7 +269,7 @@ __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
       }
       T x[kU];
 #pragma unroll
-      for (int t = 0; t < kU; t++) x[t] = op.value(k[t] >> bb);
+      for (int t = 0; t < kU; t++) x[t] = settled(k[t] & dmask) ? T(0) : op.value(k[t] >> bb);
 #pragma unroll
       for (int t = 0; t < kU; t++)
         if (a + t * kPrBlock < sg.hot_hi) Op::lds_add(acc, k[t] & dmask, x[t]);
@@ -263,10 +306,17 @@ __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
         }
         uint16_t d[kU];
         T x[kU];
+        if (skip) {  // destinations first, then only the values that count
 #pragma unroll
-        for (int t = 0; t < kU; t++) {
-          d[t] = stream_load(&dst16[pos[t]]);
-          x[t] = stream_load(&vals[pos[t]]);
+          for (int t = 0; t < kU; t++) d[t] = stream_load(&dst16[pos[t]]);
+#pragma unroll
+          for (int t = 0; t < kU; t++) x[t] = settled(d[t]) ? T(0) : stream_load(&vals[pos[t]]);
+        } else {
+#pragma unroll
+          for (int t = 0; t < kU; t++) {
+            d[t] = stream_load(&dst16[pos[t]]);
+            x[t] = stream_load(&vals[pos[t]]);
+          }
         }
 #pragma unroll
         for (int t = 0; t < kU; t++)
@@ -278,8 +328,6 @@ __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
     }
   }
   __syncthreads();
-  const uint32_t base = sg.bin << bb;
-  const uint32_t n = V - base < nb ? V - base : nb;
   if (sg.slot) {
     T *gs = gsum + ((uint64_t)(sg.slot - 1) << bb);
     for (uint32_t i = threadIdx.x; i < n; i += kPrBlock) Op::global_add(&gs[i], acc[i]);
@@ -304,12 +352,14 @@ __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
 // The finish of the split bins' vertices, clearing their rows for the next
 // iteration.
 template <class Op>
-__global__ __launch_bounds__(kBlock) void pr_split_finish_kernel(const uint32_t *split_bin,
+__global__ __launch_bounds__(kBlock) void pr_split_finish_kernel(const uint32_t *gate,
+                                                                 const uint32_t *split_bin,
                                                                  uint32_t nsplit, uint32_t V,
                                                                  int bb, Op op,
                                                                  typename Op::T *gsum,
                                                                  uint64_t *out, double *dacc) {
   using T = typename Op::T;
+  if (gate && *gate != kGateDense) return;
   const uint64_t total = (uint64_t)nsplit << bb;
   const uint64_t s = (uint64_t)gridDim.x * blockDim.x;
   double dsum = 0;
@@ -616,7 +666,7 @@ static int plan_buffers(const PrPlan *p, Work *w) {
 // vertex), then the bins, then the split bins.
 template <class Op>
 static int plan_pass(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *src, const Op &op,
-                     uint64_t *out, double *dacc, const char *names[3]) {
+                     uint64_t *out, double *dacc, const uint32_t *gate, const char *names[3]) {
   using T = typename Op::T;
   static_assert(sizeof(T) == 8, "plan values are 8 bytes");
   const uint32_t V = g->vertices;
@@ -625,20 +675,20 @@ static int plan_pass(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t
   if (p->nparts) {
     ProfScope ps(names[0]);
     hipLaunchKernelGGL(pr_emit_kernel, dim3(p->nparts), dim3(kPrBlock), (size_t)8 << p->range_bits,
-                       stream(), p->parts, src, p->src16, p->H, V, p->range_bits,
+                       stream(), gate, p->parts, src, p->src16, p->H, V, p->range_bits,
                        reinterpret_cast<uint64_t *>(vals));
   }
   if (p->nseg) {
     ProfScope ps(names[1]);
     hipLaunchKernelGGL(pr_bin_kernel<Op>, dim3(p->nseg), dim3(kPrBlock), (size_t)8 << p->bin_bits,
-                       stream(), p->segs, p->hot, p->run_beg, p->run_end, p->nranges, p->dst16,
+                       stream(), gate, p->segs, p->hot, p->run_beg, p->run_end, p->nranges, p->dst16,
                        vals, V, p->bin_bits, op, gsum, out, dacc);
   }
   if (p->nsplit) {
     ProfScope ps(names[2]);
     const uint64_t gs_n = (uint64_t)p->nsplit << p->bin_bits;
     hipLaunchKernelGGL(pr_split_finish_kernel<Op>, dim3(grid1(gs_n, 4096)), dim3(kBlock), 0,
-                       stream(), p->split_bin, p->nsplit, V, p->bin_bits, op, gsum, out, dacc);
+                       stream(), gate, p->split_bin, p->nsplit, V, p->bin_bits, op, gsum, out, dacc);
   }
   GG_HIP(hipGetLastError());
   return GG_OK;
@@ -648,13 +698,14 @@ int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull 
                       double *dacc) {
   static const char *names[3] = {"pr_emit", "pr_bins", "pr_split_finish"};
   return plan_pass(g, p, w, reinterpret_cast<const uint64_t *>(op.curr), PrSumOp{op}, nullptr,
-                   dacc, names);
+                   dacc, nullptr, names);
 }
 
 int ms_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *fcur,
-                      const uint64_t *vis, uint64_t full, uint64_t *fnext, uint64_t *out) {
+                      const uint64_t *vis, uint64_t full, uint64_t *fnext, uint64_t *out,
+                      const uint32_t *gate) {
   static const char *names[3] = {"ms_emit", "ms_bins", "ms_split_finish"};
-  return plan_pass(g, p, w, fcur, MsOrOp{fcur, vis, fnext, full}, out, nullptr, names);
+  return plan_pass(g, p, w, fcur, MsOrOp{fcur, vis, fnext, full}, out, nullptr, gate, names);
 }
 
 }  // namespace gg

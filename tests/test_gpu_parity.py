@@ -503,6 +503,43 @@ def test_bfs_batch_bin_plan(gg, bins, ranges, hot):
         assert np.array_equal(depths[i], o.bfs(r)[1]), (i, r)
 
 
+@pytest.mark.parametrize("reorder,mode", [(False, "auto"), (False, "pull"), (True, "auto"),
+                                          (True, "push")])
+def test_bfs_batch_level_records(gg, reorder, mode):
+    # the device-driven batched loop: per-level frontier sizes equal what the
+    # oracle's depths give (vertices some root first reaches at that level),
+    # the level count is the deepest level + 1, push iterations are the
+    # levels run in push mode; without relabelling every in-list longer than
+    # 32 entries is split into chunks (heavy kernel of the dense step)
+    scale = 12
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, 16, seed=75)
+    with gg.options(pull_light_max=32):
+        g = gg.Graph.build(e, V, reorder=reorder)
+    o = oracle.Graph(e, V)
+    roots = roots_of(o.out_degree(), 50, 75)
+    roots = roots + roots[:3]  # repeated roots count once in the first frontier
+    say("bfs_batch records", reorder, mode)
+    with gg.options(force_mode=MODES[mode]):
+        depths, st = gg.bfs_batch(g, roots)
+    ref = np.stack([o.bfs(r)[1] for r in roots])
+    assert np.array_equal(depths, ref)
+    reached = ref != 0xFFFFFFFF
+    levels = int(ref[reached].max()) + 1
+    assert st["iterations"] == levels
+    for i in range(levels):
+        fresh = int(((ref == i + 1) & reached).any(axis=0).sum())
+        active = int(((ref == i) & reached).any(axis=0).sum())
+        assert st["it_active"][i] == active, i
+        assert st["it_new"][i] == fresh, i
+    modes = st["modes"][:levels]
+    assert st["push_iterations"] == sum(1 for m in modes if m == 0)
+    if mode == "push":
+        assert all(m == 0 for m in modes)
+    if mode == "pull":
+        assert all(m == 1 for m in modes)
+
+
 def test_bfs_batch_partial_and_repeated_roots(gg):
     e = oracle.rmat_edges(11, seed=72)
     V = 1 << 11
