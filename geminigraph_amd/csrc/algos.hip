@@ -1293,10 +1293,18 @@ struct MsHostFlag {
   uint32_t decided;  // iterations decided so far
   uint32_t done;
 };
+// One iteration's launches captured as a graph, with everything the
+// launches bake in (buffers, shapes, options): replayed while that stays.
+struct MsdGraph {
+  hipGraphExec_t exec = nullptr;
+  uint64_t key[16] = {};
+};
 struct MsdState {
+  MsdGraph graphs[2];  // by iteration parity (the buffer pairs alternate)
   MsCtl *ctl = nullptr;
   uint64_t *set = nullptr;  // kScalarSlots count slots of 16 u64 (one 128-byte line)
   Counters *qc = nullptr;
+  uint64_t *settled = nullptr;  // bitmap: every root has reached the vertex
   MsCtl *h_ctl = nullptr;         // pinned
   MsHostFlag *h_flag = nullptr;   // pinned, coherent: written by the device
 };
@@ -1307,6 +1315,9 @@ void msd_state_free(void *p) {
   device_free(s->ctl);
   device_free(s->set);
   device_free(s->qc);
+  device_free(s->settled);
+  for (MsdGraph &gr : s->graphs)
+    if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
   if (s->h_ctl) (void)hipHostFree(s->h_ctl);
   if (s->h_flag) (void)hipHostFree(s->h_flag);
   delete s;
@@ -1320,6 +1331,7 @@ static int msd_state(Work *w, MsdState **out) {
     GG_TRY(device_alloc((void **)&s->ctl, sizeof(MsCtl)));
     GG_TRY(device_alloc((void **)&s->set, (size_t)kScalarSlots * 16 * 8));
     GG_TRY(device_alloc((void **)&s->qc, sizeof(Counters)));
+    GG_TRY(device_alloc((void **)&s->settled, w->words * 8));
     GG_HIP(hipHostMalloc((void **)&s->h_ctl, sizeof(MsCtl)));
     GG_HIP(hipHostMalloc((void **)&s->h_flag, sizeof(MsHostFlag), hipHostMallocCoherent));
   }
@@ -1543,6 +1555,8 @@ struct MsdApplyArgs {
   uint64_t *fin, *fout;    // spent and new frontier bitmaps
   uint64_t *fcur, *fnext;  // spent and new masks
   uint64_t *vis;
+  uint64_t *settled;       // bit v set once vis[v] covers full
+  uint64_t full;
   uint64_t *lmask;         // per-level root masks (V per level), or null
   uint32_t *depth;         // depth rows when lmask is null (internal order), or null
   const uint32_t *outdeg;
@@ -1562,13 +1576,18 @@ __global__ __launch_bounds__(kBlock) void msd_apply_kernel(const MsCtl *c, MsdAp
     const uint64_t act = a.fout[wi], old = a.fin[wi];
     const uint32_t v = (uint32_t)(wi * 64 + lane);
     uint64_t nw = 0;
+    bool now_settled = false;
     if ((act >> lane) & 1ULL) {
       // every step writes only roots not yet in vis, so fnext[v] stays as it is
       nw = a.fnext[v];
-      a.vis[v] |= nw;
+      const uint64_t vs = a.vis[v] | nw;
+      a.vis[v] = vs;
+      now_settled = (vs & a.full) == a.full;
       cnt++;
       edg += a.outdeg[v];
     }
+    const uint64_t sm = __ballot(now_settled);
+    if (lane == 0 && sm) a.settled[wi] |= sm;  // this wave owns word wi
     if ((old >> lane) & 1ULL) a.fcur[v] = 0;
     if (lane == 0 && old) a.fin[wi] = 0;
     if (a.lmask && v < a.V) a.lmask[(size_t)level * a.V + v] = nw;
@@ -1592,7 +1611,10 @@ __global__ __launch_bounds__(kBlock) void msd_apply_kernel(const MsCtl *c, MsdAp
 // Multi-root BFS levels on the device-driven loop.  Same results and the
 // same level records as ms_bfs_run without a hook: with level masks on, each
 // level's root masks go to the store (grown, or given up for depth rows,
-// before the level is issued); `depth` holds rows in internal order.
+// before the level is issued); `depth` holds rows in internal order.  With
+// the batch_graphs option (and no per-kernel profiling), iterations after
+// the first replay one captured graph per parity instead of their launches;
+// the first one also sizes the plan's buffers, so no capture allocates.
 static int ms_bfs_device(gg_graph *g, Work *w, const uint32_t *roots, int n_roots,
                          uint32_t *depth, gg_run_stats *st, MsLevels *res) {
   const Dir &out = out_dir(g), &in = in_dir(g);
@@ -1610,6 +1632,7 @@ static int ms_bfs_device(gg_graph *g, Work *w, const uint32_t *roots, int n_root
   GG_HIP(hipMemsetAsync(w->ms_f[1], 0, (size_t)V * 8, sm));
   GG_HIP(hipMemsetAsync(w->f[0].bits, 0, words * 8, sm));
   GG_HIP(hipMemsetAsync(w->f[1].bits, 0, words * 8, sm));
+  GG_HIP(hipMemsetAsync(s->settled, 0, words * 8, sm));
   GG_HIP(hipMemsetAsync(s->set, 0, (size_t)kScalarSlots * 16 * 8, sm));
   GG_HIP(hipMemcpyAsync(w->ms_roots, roots, 4 * (size_t)n_roots, hipMemcpyHostToDevice, sm));
   volatile MsHostFlag *hf = s->h_flag;
@@ -1632,6 +1655,7 @@ static int ms_bfs_device(gg_graph *g, Work *w, const uint32_t *roots, int n_root
       grid_waves(std::min<uint64_t>(std::max(in.n_chunks, 1u), (uint64_t)opt.huge_grid_waves));
   // several words per wave: one word each left the sparse levels launch-bound
   const unsigned apply_grid = grid_waves(std::min<size_t>(words, 1u << 13));
+  const bool use_graphs = opt.batch_graphs && !prof_on();
   for (uint32_t n = 0;; n++) {
     // keep one iteration queued: before issuing iteration n, wait until
     // iteration n-1 has been decided (so iteration n-2 has finished)
@@ -1657,6 +1681,7 @@ static int ms_bfs_device(gg_graph *g, Work *w, const uint32_t *roots, int n_root
     uint64_t *fin = w->f[n & 1].bits, *fout = w->f[(n + 1) & 1].bits;
     uint64_t *fcur = w->ms_f[n & 1], *fnext = w->ms_f[(n + 1) & 1];
     uint32_t **q = w->f[0].q;
+    auto issue = [&]() -> int {
     {
       ProfScope p("ms_bin");
       hipLaunchKernelGGL(msd_bin_kernel, dim3(bin_grid), dim3(kCompactBlock), 0, sm, s->ctl, fin,
@@ -1673,7 +1698,8 @@ static int ms_bfs_device(gg_graph *g, Work *w, const uint32_t *roots, int n_root
       hipLaunchKernelGGL(msd_push_kernel, dim3(push_grid), dim3(kBlock), 0, sm, s->ctl, a, op);
     }
     if (plan) {
-      GG_TRY(ms_plan_iteration(g, plan, w, fcur, w->ms_vis, full, fnext, fout, &s->ctl->mode));
+      GG_TRY(ms_plan_iteration(g, plan, w, fcur, w->ms_vis, full, fnext, fout, &s->ctl->mode,
+                               s->settled));
     } else {
       MsPullArgs pa{in.index, in.nbr, fcur, w->ms_vis, fnext, fout, full, V,
                     in.light_max, (uint32_t)opt.pull_serial_max, ngroups, &s->ctl->mode};
@@ -1691,13 +1717,48 @@ static int ms_bfs_device(gg_graph *g, Work *w, const uint32_t *roots, int n_root
     }
     {
       ProfScope p("ms_apply");
-      MsdApplyArgs a{fin, fout, fcur, fnext, w->ms_vis, masks ? w->ms_lmask : nullptr,
-                     masks ? nullptr : depth, out.degree, s->set, words, V};
+      MsdApplyArgs a{fin,     fout, fcur, fnext, w->ms_vis, s->settled, full,
+                     masks ? w->ms_lmask : nullptr, masks ? nullptr : depth, out.degree, s->set,
+                     words, V};
       hipLaunchKernelGGL(msd_apply_kernel, dim3(apply_grid), dim3(kBlock), 0, sm, s->ctl, a);
     }
     hipLaunchKernelGGL(msd_decide_kernel, dim3(1), dim3(kBlock), 0, sm, s->ctl, s->set, s->qc,
                        s->h_flag);
     GG_HIP(hipGetLastError());
+    return GG_OK;
+    };
+    if (!use_graphs || n == 0) {
+      GG_TRY(issue());
+      continue;
+    }
+    MsdGraph &gr = s->graphs[n & 1];
+    const uint64_t key[16] = {(uint64_t)fin, (uint64_t)fcur, (uint64_t)(masks ? w->ms_lmask : nullptr),
+                              (uint64_t)(masks ? nullptr : depth), full,
+                              plan ? plan->serial : 0,
+                              (uint64_t)s->ctl, (uint64_t)w->ms_vis, (uint64_t)q[0],
+                              (uint64_t)opt.push_small_max, (uint64_t)opt.push_big_max,
+                              (uint64_t)opt.huge_grid_waves, (uint64_t)opt.pull_serial_max,
+                              (uint64_t)g, (uint64_t)s->settled, (uint64_t)w->pr_vals};
+    if (!gr.exec || memcmp(gr.key, key, sizeof(key))) {
+      if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
+      gr.exec = nullptr;
+      hipGraph_t graph = nullptr;
+      GG_HIP(hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
+      const int rc = issue();
+      const hipError_t e = hipStreamEndCapture(sm, &graph);
+      if (rc != GG_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc != GG_OK ? rc : fail(GG_EHIP, "batched BFS: graph capture failed");
+      }
+      const hipError_t ie = hipGraphInstantiate(&gr.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        gr.exec = nullptr;
+        return fail(GG_EHIP, "batched BFS: graph instantiation failed");
+      }
+      memcpy(gr.key, key, sizeof(key));
+    }
+    GG_HIP(hipGraphLaunch(gr.exec, sm));
   }
   GG_HIP(hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(MsCtl), hipMemcpyDeviceToHost, sm));
   GG_HIP(hipStreamSynchronize(sm));

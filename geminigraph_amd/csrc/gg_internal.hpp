@@ -66,8 +66,10 @@ struct Options {
   int64_t pr_range_bits = 14;    // PageRank bin plan: 2^bits sources per cold range
   int64_t heavy_waves = 8192;    // dense heavy-chunk grids: waves (0 = one per chunk)
   int64_t bfs_pull_waves = 8192; // BFS pull grids: waves (0 = one per group / chunk)
+  int64_t batch_graphs = 0;      // batched BFS levels replayed from captured hipGraphs
 };
 Options &options();
+bool prof_on();  // per-kernel event timing (gg_prof_enable) is on
 
 // ----------------------------------------------------------------- runtime
 int ensure_init();
@@ -118,6 +120,7 @@ struct Dir {
 // sorted by source; the other (cold) arcs stored for propagation blocking.
 struct PrSeg;
 struct PrPlan {
+  uint64_t serial = 0;         // distinct for every plan built in the process
   int transposed = 0;          // orientation it was built for
   uint32_t H = 0;              // hot sources: internal id < H (at most 2^(32 - bin_bits))
   int bin_bits = 14;           // destination bins of 2^bin_bits vertices
@@ -247,11 +250,13 @@ int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull 
 // its in-neighbours' 64-root frontier masks (fcur); the bits not in vis (and
 // within full) go to fnext[v] and v's bit to the bitmap `out` (every word is
 // written).  With a gate, the kernels return at once unless *gate is
-// kGateDense when they run (a device-driven loop's mode word).
+// kGateDense when they run (a device-driven loop's mode word).  Arcs into
+// vertices whose vis covers full read no source value; `settled`, if given,
+// is a bitmap of those vertices (else vis is tested per bin).
 constexpr uint32_t kGateDense = 1;
 int ms_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *fcur,
                       const uint64_t *vis, uint64_t full, uint64_t *fnext, uint64_t *out,
-                      const uint32_t *gate = nullptr);
+                      const uint32_t *gate = nullptr, const uint64_t *settled = nullptr);
 // Device bytes of one traversal context that runs batched queries of `kind`
 // (GG_QUERY_*) on g, counting the first 64 stored BFS levels.
 size_t work_footprint(const gg_graph *g, int kind);

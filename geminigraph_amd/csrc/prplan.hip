@@ -24,6 +24,7 @@
 // those bins.
 // Streamed bytes per arc: hot 4 (+ an L2 hit shared by several lanes), cold
 // 2 + 8 (emit) + 8 + 2 (bin) (+ 8 per cold source).
+#include <atomic>
 #include <string.h>
 
 #include <algorithm>
@@ -137,13 +138,15 @@ __global__ void pr_cold_split_kernel(const uint64_t *keys, uint64_t n, int bb, i
 // BFS ORs 64-root frontier masks.  A BinOp gives the value type T, the
 // source value, the LDS and global combining atomics, and the per-vertex
 // finish (which reports whether the vertex joins the output frontier).  With
-// kSkipDone, done(v) marks destinations whose result is already settled: the
-// bin kernel then reads no source value for their arcs.
+// kSkipDone, done_mask(v0, valid) gives the wave's 64 destinations v0 ..
+// v0+63 (v0 a multiple of 64; lanes with valid false are not read) whose
+// result is already settled: the bin kernel then reads no source value for
+// their arcs.
 struct PrSumOp {
   using T = double;
   static constexpr bool kSkipDone = false;
   PrPull op;
-  __device__ bool done(uint32_t) const { return false; }
+  __device__ uint64_t done_mask(uint32_t, bool) const { return 0; }
   __device__ T value(uint32_t u) const { return op.curr[u]; }
   __device__ static void lds_add(T *acc, uint32_t i, T x) { atomicAdd(&acc[i], x); }
   __device__ static void global_add(T *p, T x) {
@@ -160,7 +163,11 @@ struct MsOrOp {
   const uint64_t *fcur, *vis;
   uint64_t *fnext;
   uint64_t full;
-  __device__ bool done(uint32_t v) const { return (vis[v] & full) == full; }
+  const uint64_t *settled;  // bit v set once vis[v] covers full, or null (test vis)
+  __device__ uint64_t done_mask(uint32_t v0, bool valid) const {
+    if (settled) return settled[v0 >> 6];
+    return __ballot(valid && (vis[v0 + lane_id()] & full) == full);
+  }
   __device__ T value(uint32_t u) const { return fcur[u]; }
   __device__ static void lds_add(T *acc, uint32_t i, T x) {
     if (x) atomicOr((unsigned long long *)&acc[i], (unsigned long long)x);
@@ -233,8 +240,9 @@ __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
     if (threadIdx.x == 0) s_ndone = 0;
     __syncthreads();
     for (uint32_t i0 = threadIdx.x - lane_id(); i0 < nb; i0 += kPrBlock) {
-      const uint32_t i = i0 + lane_id();
-      const uint64_t m = __ballot(i >= n || op.done(base + i));
+      // lanes past the last vertex count as settled
+      const uint64_t past = i0 >= n ? ~0ULL : n - i0 >= 64 ? 0ULL : ~0ULL << (n - i0);
+      const uint64_t m = past | (i0 < n ? op.done_mask(base + i0, i0 + lane_id() < n) : 0ULL);
       if (lane_id() == 0) {
         sdone[i0 >> 5] = (uint32_t)m;
         sdone[(i0 >> 5) + 1] = (uint32_t)(m >> 32);
@@ -633,6 +641,8 @@ int pr_plan_get(gg_graph *g, const PrPlan **out) {
   }
   if (!p) {
     p = new PrPlan;
+    static std::atomic<uint64_t> serials{0};
+    p->serial = ++serials;
     int r = plan_build(g, H, bb, rb, p);
     if (r != GG_OK) {
       std::string msg = gg_last_error();
@@ -703,9 +713,10 @@ int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull 
 
 int ms_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *fcur,
                       const uint64_t *vis, uint64_t full, uint64_t *fnext, uint64_t *out,
-                      const uint32_t *gate) {
+                      const uint32_t *gate, const uint64_t *settled) {
   static const char *names[3] = {"ms_emit", "ms_bins", "ms_split_finish"};
-  return plan_pass(g, p, w, fcur, MsOrOp{fcur, vis, fnext, full}, out, nullptr, gate, names);
+  return plan_pass(g, p, w, fcur, MsOrOp{fcur, vis, fnext, full, settled}, out, nullptr, gate,
+                   names);
 }
 
 }  // namespace gg

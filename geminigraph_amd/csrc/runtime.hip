@@ -90,6 +90,7 @@ struct ProfEntry {
 static bool g_prof_on = false;
 static std::map<std::string, ProfEntry> g_prof;
 static std::mutex g_prof_mu;
+bool prof_on() { return g_prof_on; }
 
 ProfScope::ProfScope(const char *n) : name(n) {
   if (!g_prof_on) return;
@@ -205,6 +206,7 @@ static int64_t *option_slot(const char *name) {
   if (!strcmp(name, "pr_range_bits")) return &o.pr_range_bits;
   if (!strcmp(name, "heavy_waves")) return &o.heavy_waves;
   if (!strcmp(name, "bfs_pull_waves")) return &o.bfs_pull_waves;
+  if (!strcmp(name, "batch_graphs")) return &o.batch_graphs;
   return nullptr;
 }
 
@@ -213,7 +215,8 @@ int gg_set_option(const char *name, int64_t value) {
   if (!s) return fail(GG_EINVAL, std::string("unknown option: ") + (name ? name : "(null)"));
   if (!strcmp(name, "force_mode") && (value < 0 || value > 2))
     return fail(GG_EINVAL, "force_mode must be 0, 1 or 2");
-  const bool flag = !strcmp(name, "count_examined") || !strcmp(name, "level_masks");
+  const bool flag = !strcmp(name, "count_examined") || !strcmp(name, "level_masks") ||
+                    !strcmp(name, "batch_graphs");
   if (flag && (value < 0 || value > 1)) return fail(GG_EINVAL, std::string(name) + " must be 0 or 1");
   if (!strcmp(name, "bfs_pull_waves") && (value < 0 || value > (1 << 24)))
     return fail(GG_EINVAL, "bfs_pull_waves must be in [0, 2^24]");

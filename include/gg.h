@@ -89,7 +89,12 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *                      (default; depth rows are built once at the end and
  *                      BC selects backward arcs exactly), 0 = write depth
  *                      rows level by level
- *   "level_mask_bytes" size limit of that mask store (default 4 GiB) */
+ *   "level_mask_bytes" size limit of that mask store (default 4 GiB)
+ *   "batch_graphs"     batched BFS: 1 = replay each level (after the first)
+ *                      from a hipGraph captured once per buffer parity;
+ *                      0 = launch the level's kernels (default: measured
+ *                      faster on MI355X, the host already runs a level
+ *                      ahead of the device) */
 int gg_set_option(const char *name, int64_t value);
 int gg_get_option(const char *name, int64_t *value);
 

@@ -540,6 +540,31 @@ def test_bfs_batch_level_records(gg, reorder, mode):
         assert all(m == 1 for m in modes)
 
 
+def test_bfs_batch_graph_replays(gg):
+    # levels after the first replay captured graphs (one per parity): calls
+    # in a row with other root counts, with plain BFS work and syncs between
+    # them, after a plan rebuild (other bin sizes) and with the graphs off,
+    # each checked against the oracle
+    scale = 13
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, 16, seed=76)
+    g = gg.Graph.build(e, V, reorder=True)
+    o = oracle.Graph(e, V)
+    cand = roots_of(o.out_degree(), 64, 76)
+    ref = {r: o.bfs(r)[1] for r in cand}
+    cases = [(64, {}), (64, {}), (17, {}), (64, {"pr_bin_bits": 12}), (5, {"batch_graphs": 0}),
+             (64, {}), (33, {"level_masks": 0}), (64, {})]
+    for k, (n, opts) in enumerate(cases):
+        roots = cand[:n]
+        say("bfs_batch replay", k, n, opts)
+        with gg.options(**{"batch_graphs": 1, **opts}):
+            depths, _ = gg.bfs_batch(g, roots)
+        for i, r in enumerate(roots):
+            assert np.array_equal(depths[i], ref[r]), (k, i)
+        g.bfs(cand[k])  # eager work between the replays
+        gg.synchronize()
+
+
 def test_bfs_batch_partial_and_repeated_roots(gg):
     e = oracle.rmat_edges(11, seed=72)
     V = 1 << 11
