@@ -220,6 +220,65 @@ def pagerank_extra(gg, g, V, E, iterations=20, runs=3):
     }
 
 
+def rmat26_extra(gg, seed, scale=26, ef=16, n_roots=8, iterations=20):
+    """The scale-26 case: SSSP (plain method) from 8 seeded non-isolated
+    roots on the weighted graph (float32 weights in (0,1]), then PageRank
+    (FP64, 20 iterations) on the same topology, relabelled, with the plain
+    pull beside it.  SSSP GTEPS = sum of m_reach / sum of times."""
+    V = 1 << scale
+    t = time.perf_counter()
+    g = gg.Graph.rmat(scale, ef, seed=seed, permute=True, weighted=True)
+    gg.synchronize()
+    build_s = time.perf_counter() - t
+    roots = pick_roots(g.out_degree(), n_roots, seed)
+    dist = gg.DeviceArray(V, np.float32)
+    g.sssp_into(roots[0], dist)
+    times, iters, m = [], [], 0
+    for r in roots:
+        t = time.perf_counter()
+        st = g.sssp_into(r, dist)
+        gg.synchronize()
+        times.append(time.perf_counter() - t)
+        iters.append(st.iterations)
+        m += st.m_reach
+    dist.free()
+    E = g.edges
+    g.close()
+    out = {"workload": f"RMAT scale {scale} edge factor {ef}, weighted (SSSP) and relabelled "
+                       f"unweighted (PageRank), seed {seed}",
+           "vertices": V, "edges": E, "build_s": round(build_s, 2),
+           "sssp_roots": len(roots), "sssp_ms_mean": round(1e3 * float(np.mean(times)), 1),
+           "sssp_ms_max": round(1e3 * float(np.max(times)), 1),
+           "sssp_iterations": iters, "sssp_gteps": round(m / sum(times) / 1e9, 2)}
+    h = gg.Graph.rmat(scale, ef, seed=seed, permute=True, reorder=True)
+    pr = gg.DeviceArray(V, np.float64)
+    t = time.perf_counter()
+    h.pagerank_into(pr, iterations)
+    gg.synchronize()
+    first_s = time.perf_counter() - t
+    t = time.perf_counter()
+    h.pagerank_into(pr, iterations)
+    gg.synchronize()
+    pr_s = time.perf_counter() - t
+    plan = pr.numpy()
+    with gg.options(pr_hot_sources=0):
+        t = time.perf_counter()
+        h.pagerank_into(pr, iterations)
+        gg.synchronize()
+        pull_s = time.perf_counter() - t
+    pulled = pr.numpy()
+    pr.free()
+    h.close()
+    b_iter = 12 * E + 28 * V
+    out.update({"pagerank_ms_per_iter": round(pr_s * 1e3 / iterations, 3),
+                "pagerank_first_call_ms": round(first_s * 1e3, 1),
+                "pagerank_gteps": round(iterations * E / pr_s / 1e9, 2),
+                "pagerank_frac": round(iterations * b_iter / pr_s / 1e9 / HBM_PEAK_GBS, 4),
+                "pagerank_pull_ms_per_iter": round(pull_s * 1e3 / iterations, 3),
+                "pagerank_max_rel_vs_pull": float(np.max(np.abs(plan - pulled) / np.abs(pulled)))})
+    return out
+
+
 def cpu_baseline(scale, ef, seed, roots, gpu_depths, sample_roots):
     import oracle
     threads = os.environ.get("OMP_NUM_THREADS")
@@ -487,6 +546,8 @@ def main():
         lab.free()
         extras["batched"] = batch_extra(gg, args.seed + 7, reorder)
         log(f"batched: {extras['batched']}")
+        extras["rmat26"] = rmat26_extra(gg, args.seed + 2)
+        log(f"rmat26: {extras['rmat26']}")
         if args.serve_seconds > 0:
             extras["serving"] = serve_extra(gg, args.seed + 11, reorder, args.serve_seconds)
             log(f"serving: {extras['serving']}")

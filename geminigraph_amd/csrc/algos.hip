@@ -792,6 +792,16 @@ __global__ __launch_bounds__(kBlock) void ms_rows_from_masks_kernel(const uint64
     }
   }
 }
+// out-degrees of the vertices an SSSP reached (distance below the 1e9 sentinel)
+__global__ void sssp_reach_kernel(const float *dist, const uint32_t *outdeg, uint32_t V,
+                                  unsigned long long *acc) {
+  uint32_t s = gridDim.x * blockDim.x;
+  uint64_t x = 0;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s)
+    if (dist[v] < 1e9f) x += outdeg[v];
+  x = wave_sum_u64(x);
+  if (lane_id() == 0 && x) atomicAdd(&acc[(global_wave() & 63) * 16], (unsigned long long)x);
+}
 // sum over roots of the out-degrees of reached vertices
 __global__ void ms_reach_kernel(const uint64_t *vis, const uint32_t *outdeg, uint32_t V,
                                 unsigned long long *acc) {
@@ -1988,7 +1998,6 @@ int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int
   fill<uint32_t>(msg, kInfBits, V);
   GG_TRY(en.seed(root, out.degree, nullptr));
   int lv = 0;
-  uint64_t reached_edges = en.in.edges;
   while (en.in.total > 0) {
     if (h_frontier && lv < cap) h_frontier[lv] = en.in.total;
     lv++;
@@ -2008,12 +2017,21 @@ int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int
                          d_dist, msg, V, en.fout().bits);
     }
     GG_TRY(en.finish_iteration(sp, out.degree, nullptr));
-    reached_edges += en.in.edges;
   }
   if (g->reordered)
     hipLaunchKernelGGL(unpermute_kernel<float>, dim3(grid1(V)), dim3(kBlock), 0, stream(), d_dist,
                        out_dist, g->iperm, V);
   en.st.device_ms = tm.stop();
+  // out-degree sum of the reached vertices (outside the timed compute, as
+  // the reference's SSSP has no such count)
+  GG_HIP(hipMemsetAsync(w->exam, 0, 1024 * 8, stream()));
+  hipLaunchKernelGGL(sssp_reach_kernel, dim3(grid1(V, 4096)), dim3(kBlock), 0, stream(), d_dist,
+                     out.degree, V, w->exam);
+  GG_HIP(hipMemcpyAsync(w->h_exam, w->exam, 1024 * 8, hipMemcpyDeviceToHost, stream()));
+  GG_HIP(hipMemsetAsync(w->exam, 0, 1024 * 8, stream()));
+  GG_HIP(hipStreamSynchronize(stream()));
+  uint64_t reached_edges = 0;
+  for (int k = 0; k < 64; k++) reached_edges += w->h_exam[k * 16];
   en.st.m_reach = reached_edges;
   put_stats(stats, en.st);
   return GG_OK;

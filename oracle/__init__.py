@@ -90,6 +90,8 @@ def lib():
         L.og_cc.restype = i32
         L.og_sssp.argtypes = [G, u32, vp, vp, i32, T]
         L.og_sssp.restype = i32
+        L.og_sssp_fixed_point_violations.argtypes = [vp, vp, vp, u32, vp, u32]
+        L.og_sssp_fixed_point_violations.restype = u64
         L.og_bc.argtypes = [G, u32, vp, vp, vp, i32, T]
         L.og_bc.restype = i32
         L.og_time.restype = ctypes.c_double
@@ -103,6 +105,20 @@ def _ptr(a):
 
 EDGE_DTYPE = np.dtype([("src", "<u4"), ("dst", "<u4")])
 WEDGE_DTYPE = np.dtype([("src", "<u4"), ("dst", "<u4"), ("w", "<f4")])
+
+
+def sssp_fixed_point_violations(index, src, w, dist, root):
+    """Vertices where `dist` is not a fixed point of the reference's f32
+    relaxation over the in-arcs (CSC arrays index/src/w); 0 for a correct
+    SSSP result (og_sssp_fixed_point_violations)."""
+    index = np.ascontiguousarray(index, dtype=np.uint64)
+    src = np.ascontiguousarray(src, dtype=np.uint32)
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    dist = np.ascontiguousarray(dist, dtype=np.float32)
+    V = len(index) - 1
+    assert len(dist) == V and len(src) == len(w) == int(index[-1])
+    return int(lib().og_sssp_fixed_point_violations(_ptr(index), _ptr(src), _ptr(w), V,
+                                                    _ptr(dist), int(root)))
 
 
 def rmat_edges(scale, edge_factor=16, seed=1, a=RMAT_A, b=RMAT_B, c=RMAT_C,

@@ -661,6 +661,26 @@ int og_cc(og_graph *g, uint32_t *label_out, uint32_t *frontier_out, int frontier
 /* ----------------------------------------------------------------- sssp */
 /* toolkits/sssp.cpp:22-105 */
 
+uint64_t og_sssp_fixed_point_violations(const uint64_t *index, const uint32_t *src,
+                                        const float *w, uint32_t vertices,
+                                        const float *dist, uint32_t root) {
+  uint64_t bad = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : bad) num_threads(og_get_threads())
+  for (int64_t v = 0; v < (int64_t)vertices; v++) {
+    float m = INFINITY;
+    for (uint64_t e = index[v]; e < index[v + 1]; e++) {
+      const float c = dist[src[e]] + w[e]; /* sssp.cpp:50, :65 */
+      if (c < m) m = c;
+    }
+    const float d = dist[v];
+    if (v == (int64_t)root)
+      bad += d != 0.0f || m < d;
+    else
+      bad += m < d || (d < 1e9f && m != d);
+  }
+  return bad;
+}
+
 typedef struct sssp_ctx {
   og_graph *g;
   float *dist;

@@ -99,6 +99,16 @@ int og_cc(og_graph *g, uint32_t *label_out, uint32_t *frontier_out, int frontier
 /* toolkits/sssp.cpp:22-105 (weighted graph).  Returns iterations. */
 int og_sssp(og_graph *g, uint32_t root, float *dist_out, uint32_t *frontier_out,
             int frontier_cap, og_trace *tr);
+/* Checker for SSSP results too large for og_sssp: counts the vertices at
+ * which `dist` is not a fixed point of the relaxation the reference iterates
+ * (sssp.cpp:44-71, f32 adds fl(dist[u] + w)).  With m(v) = the minimum of
+ * fl(dist[u] + w) over v's in-arcs (CSC: index[V+1], src, w; +inf if none),
+ * v violates when m(v) < dist[v] (an arc still relaxes), or when v != root,
+ * dist[v] < 1e9 and m(v) != dist[v] (no arc gives the distance), or when
+ * v == root and dist[v] != 0. */
+uint64_t og_sssp_fixed_point_violations(const uint64_t *index, const uint32_t *src,
+                                        const float *w, uint32_t vertices,
+                                        const float *dist, uint32_t root);
 /* toolkits/bc.cpp:24-196.  dep_out[V], inv_paths_out[V] (=1/sigma), levels sizes. */
 int og_bc(og_graph *g, uint32_t root, double *dep_out, double *inv_paths_out,
           uint32_t *frontier_out, int frontier_cap, og_trace *tr);

@@ -314,6 +314,8 @@ def test_sssp_matches_oracle(gg, scale, seed, mode):
         ref, ref_fr = o.sssp(root)
         assert np.array_equal(dist.view(np.uint32), ref.view(np.uint32))
         assert np.array_equal(fr, ref_fr)
+        reached = ref < 1e9
+        assert st["m_reach"] == int(o.out_degree()[reached].astype(np.int64).sum())
 
 
 # ----------------------------------------------------------------------- BC

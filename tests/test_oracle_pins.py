@@ -348,6 +348,35 @@ def test_sssp_bitexact_vs_f32_dijkstra(scale, seed):
         assert np.array_equal(frontier, sizes)
 
 
+def test_sssp_fixed_point_checker():
+    # the full-size checker: zero on Dijkstra's distances, and each planted
+    # mistake (a distance one ulp too high or too low, a reached vertex
+    # marked unreached, an unreached one given a distance, the root off 0)
+    # is caught
+    scale, seed = 10, 4
+    e = oracle.rmat_edges(scale, seed=seed, weighted=True)
+    V = 1 << scale
+    idx, src, w = oracle.Graph(e, V).csc()
+    root = int(_roots(oracle.Graph(e, V), 1, seed)[0])
+    ref = f32_dijkstra(e, V, root)
+    assert oracle.sssp_fixed_point_violations(idx, src, w, ref, root) == 0
+    reached = np.nonzero((ref < 1e9) & (np.arange(V) != root))[0]
+    unreached = np.nonzero(ref >= 1e9)[0]
+    assert len(reached) > 10 and len(unreached) > 0
+    v = int(reached[len(reached) // 2])
+    for bad_value in (np.nextafter(ref[v], np.float32(np.inf)),
+                      np.nextafter(ref[v], np.float32(0)), np.float32(1e9)):
+        d = ref.copy()
+        d[v] = bad_value
+        assert oracle.sssp_fixed_point_violations(idx, src, w, d, root) >= 1, bad_value
+    d = ref.copy()
+    d[int(unreached[0])] = np.float32(0.5)
+    assert oracle.sssp_fixed_point_violations(idx, src, w, d, root) >= 1
+    d = ref.copy()
+    d[root] = np.float32(2 ** -24)
+    assert oracle.sssp_fixed_point_violations(idx, src, w, d, root) >= 1
+
+
 # ----------------------------------------------------------------------- bc
 
 def test_bc_matches_networkx_brandes():
