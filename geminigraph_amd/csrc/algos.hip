@@ -879,37 +879,42 @@ __device__ __forceinline__ void lane_range(const Op &op, const uint32_t *nbr, co
   }
 }
 
-// One wave per listed vertex.  Op adds: bool active(v) (wave-uniform);
-// T init(v, lane); bool on(v, lane, T) (root takes part at v);
-// void finish(v, lane, T).
+// One wave per listed vertex.  Op adds: bool enabled() (the whole launch
+// does nothing when false); bool active(v) (wave-uniform); T init(v, lane);
+// bool on(v, lane, T) (root takes part at v); void finish(v, lane, T).  A
+// grid of fewer than n waves loops.
 template <class Op>
 __global__ __launch_bounds__(kBlock) void lane_vertex_kernel(LaneArgs a, Op op) {
-  const uint32_t gw = global_wave();
+  const uint32_t W = gridDim.x * kWavesPerBlock;
   const int lane = lane_id();
-  if (gw >= a.n) return;
-  uint32_t v = a.list ? a.list[gw] : gw;
-  if (!op.active(v)) return;
-  uint64_t b = a.index[v], e = a.index[v + 1];
-  if (e - b > a.light_max || e == b) return;
-  typename Op::T acc = op.init(v, lane);
-  const uint64_t vm = __ballot(op.on(v, lane, acc));
-  if (!vm) return;
-  lane_range(op, a.nbr, a.w, b, e, vm, lane, acc);
-  op.finish(v, lane, acc);
+  if (!op.enabled()) return;
+  for (uint32_t gw = global_wave(); gw < a.n; gw += W) {
+    uint32_t v = a.list ? a.list[gw] : gw;
+    if (!op.active(v)) continue;
+    uint64_t b = a.index[v], e = a.index[v + 1];
+    if (e - b > a.light_max || e == b) continue;
+    typename Op::T acc = op.init(v, lane);
+    const uint64_t vm = __ballot(op.on(v, lane, acc));
+    if (!vm) continue;
+    lane_range(op, a.nbr, a.w, b, e, vm, lane, acc);
+    op.finish(v, lane, acc);
+  }
 }
 // Heavy lists: one wave per chunk; partials through op.commit (atomics).
 template <class Op>
 __global__ __launch_bounds__(kBlock) void lane_heavy_kernel(HeavyArgs a, Op op) {
-  const uint32_t c = global_wave();
+  const uint32_t W = gridDim.x * kWavesPerBlock;
   const int lane = lane_id();
-  if (c >= a.n_chunks) return;
-  uint32_t v = a.chunk_v[c];
-  if (!op.active(v)) return;
-  typename Op::T acc = op.init(v, lane);
-  const uint64_t vm = __ballot(op.on(v, lane, acc));
-  if (!vm) return;
-  lane_range(op, a.nbr, a.w, a.chunk_beg[c], a.chunk_end[c], vm, lane, acc);
-  op.commit(v, lane, acc);
+  if (!op.enabled()) return;
+  for (uint32_t c = global_wave(); c < a.n_chunks; c += W) {
+    uint32_t v = a.chunk_v[c];
+    if (!op.active(v)) continue;
+    typename Op::T acc = op.init(v, lane);
+    const uint64_t vm = __ballot(op.on(v, lane, acc));
+    if (!vm) continue;
+    lane_range(op, a.nbr, a.w, a.chunk_beg[c], a.chunk_end[c], vm, lane, acc);
+    op.commit(v, lane, acc);
+  }
 }
 
 // Batched SSSP, dense step: dist row of v = min over in-edges of (row of u + w).
@@ -919,6 +924,8 @@ struct MssPull {
   float *dist;
   const uint64_t *cm_cur;  // roots whose distance at a vertex changed last step
   uint64_t *cm_next, *out_bits;
+  const uint32_t *gate;    // null, or: run only when *gate == kGateDense
+  __device__ bool enabled() const { return !gate || *gate == kGateDense; }
   __device__ bool active(uint32_t) const { return true; }
   __device__ float init(uint32_t, int) const { return 1e9f; }
   __device__ bool on(uint32_t, int, float) const { return true; }
@@ -952,6 +959,32 @@ struct MssPull {
   }
 };
 // Batched SSSP, sparse step: each changed (vertex, root) relaxes its out-edges.
+// Arcs [b, e) of a changed vertex u, lane = root: roots whose distance at u
+// changed last step relax each arc (sssp.cpp:44-56); improvements go to
+// the target's mask of changed roots and its frontier bit.
+__device__ __forceinline__ void mss_relax_range(const uint32_t *nbr, const float *wt, float *dist,
+                                                const uint64_t *cm_cur, uint64_t *cm_next,
+                                                uint64_t *out_bits, uint32_t u, uint64_t b,
+                                                uint64_t e) {
+  const int lane = lane_id();
+  uint64_t m = cm_cur[u];
+  bool act = (m >> lane) & 1ULL;
+  float du = dist[(size_t)u * 64 + lane];
+  for (uint64_t k = b; k < e; k++) {
+    uint32_t v = nbr[k];
+    bool imp = false;
+    if (act) {
+      float c = du + wt[k];
+      float *p = &dist[(size_t)v * 64 + lane];
+      if (c < *p) imp = c < __uint_as_float(atomicMin((uint32_t *)p, __float_as_uint(c)));
+    }
+    uint64_t im = __ballot(imp);
+    if (im && lane == 0) {
+      atomicOr((unsigned long long *)&cm_next[v], (unsigned long long)im);
+      activate(out_bits, v);
+    }
+  }
+}
 __global__ __launch_bounds__(kBlock) void mss_push_kernel(const uint64_t *index, const uint32_t *nbr,
                                                            const float *wt, const uint32_t *q0,
                                                            uint32_t n0, const uint32_t *q1,
@@ -960,25 +993,8 @@ __global__ __launch_bounds__(kBlock) void mss_push_kernel(const uint64_t *index,
                                                            float *dist, const uint64_t *cm_cur,
                                                            uint64_t *cm_next, uint64_t *out_bits) {
   const uint32_t gw = global_wave();
-  const int lane = lane_id();
   auto relax_range = [&](uint32_t u, uint64_t b, uint64_t e) {
-    uint64_t m = cm_cur[u];
-    bool act = (m >> lane) & 1ULL;
-    float du = dist[(size_t)u * 64 + lane];
-    for (uint64_t k = b; k < e; k++) {
-      uint32_t v = nbr[k];
-      bool imp = false;
-      if (act) {
-        float c = du + wt[k];
-        float *p = &dist[(size_t)v * 64 + lane];
-        if (c < *p) imp = c < __uint_as_float(atomicMin((uint32_t *)p, __float_as_uint(c)));
-      }
-      uint64_t im = __ballot(imp);
-      if (im && lane == 0) {
-        atomicOr((unsigned long long *)&cm_next[v], (unsigned long long)im);
-        activate(out_bits, v);
-      }
-    }
+    mss_relax_range(nbr, wt, dist, cm_cur, cm_next, out_bits, u, b, e);
   };
   if (gw < n0 + n1) {
     uint32_t u = gw < n0 ? q0[gw] : q1[gw - n0];
@@ -1015,6 +1031,7 @@ __global__ void mss_seed_kernel(const uint32_t *roots, int n, float *dist, uint6
 struct MsbcSigma {
   using T = double;
   using L = double;
+  __device__ bool enabled() const { return true; }
   double *sigma;
   const uint64_t *fprev, *fnew;
   __device__ bool active(uint32_t v) const { return fnew[v] != 0; }
@@ -1034,6 +1051,7 @@ struct MsbcSigma {
 // D(v) += sum of D(w) over out-edges v->w with w at depth L (bc.cpp:125-167).
 struct MsbcBack {
   using T = double;
+  __device__ bool enabled() const { return true; }
   struct L {
     uint32_t d;
     double dd;
@@ -1785,6 +1803,205 @@ static int ms_bfs_device(gg_graph *g, Work *w, const uint32_t *roots, int n_root
   return GG_OK;
 }
 
+// ------------------------------------------ batched SSSP, device-driven loop
+// The batched SSSP's iterations on the same control block and decide as the
+// batched BFS: bin (push only), a persistent push over the queues, the pull
+// (lane kernels gated through MssPull), then an apply that counts the new
+// frontier and clears the spent frontier's bitmap and changed-root masks,
+// and the decide.  The host stays one iteration ahead.
+
+// One block: root i's distance 0 and its bit in the root vertex's changed
+// mask (mss_seed_kernel), the control block, the count of distinct roots
+// and their out-arcs, and the first decision.
+__global__ __launch_bounds__(kBlock) void mssd_seed_kernel(MsCtl *c, uint64_t *set, Counters *qc,
+                                                           MsHostFlag *hf, const uint32_t *roots,
+                                                           int n, float *dist, uint64_t *cm,
+                                                           uint64_t *bits, const uint32_t *outdeg,
+                                                           int32_t force, uint64_t threshold) {
+  const int t = threadIdx.x;
+  if (t < n) {
+    const uint32_t r = roots[t];
+    dist[(size_t)r * 64 + t] = 0.f;
+    atomicOr((unsigned long long *)&cm[r], 1ULL << t);
+    atomicOr((unsigned long long *)&bits[r >> 6], 1ULL << (r & 63));
+    bool first = true;
+    for (int j = 0; j < t; j++) first = first && roots[j] != r;
+    if (first) {
+      atomicAdd((unsigned long long *)&set[0], 1ull);
+      atomicAdd((unsigned long long *)&set[1], (unsigned long long)outdeg[r]);
+    }
+  }
+  if (t == 0) {
+    c->done = 0;
+    c->iter = 0;
+    c->mode = kMsDone;
+    c->level = 0;
+    c->push_iters = 0;
+    c->force = force;
+    c->threshold = threshold;
+  }
+  __threadfence();
+  __syncthreads();
+  msd_decide(c, set, qc, hf);
+}
+
+struct MssdPushArgs {
+  const uint64_t *index;
+  const uint32_t *nbr;
+  const float *wt;
+  const uint32_t *q0, *q1, *q2;
+  const Counters *qc;
+  float *dist;
+  const uint64_t *cm_cur;
+  uint64_t *cm_next, *out_bits;
+};
+
+// Persistent grid over the binned queues (sizes on the device): one wave
+// per small or mid vertex, then chunks of every huge vertex shared by all
+// waves, rotated per vertex.
+__global__ __launch_bounds__(kBlock) void mssd_push_kernel(const MsCtl *c, MssdPushArgs a) {
+  if (c->mode != kMsPush) return;
+  const uint32_t W = gridDim.x * kWavesPerBlock, gw = global_wave();
+  const uint32_t n0 = a.qc->n[0], n1 = a.qc->n[1], n2 = a.qc->n[2];
+  for (uint32_t i = gw; i < n0 + n1; i += W) {
+    const uint32_t u = i < n0 ? a.q0[i] : a.q1[i - n0];
+    mss_relax_range(a.nbr, a.wt, a.dist, a.cm_cur, a.cm_next, a.out_bits, u, a.index[u],
+                    a.index[u + 1]);
+  }
+  for (uint32_t h = 0; h < n2; h++) {
+    const uint32_t u = a.q2[h];
+    const uint64_t b = a.index[u], e = a.index[u + 1];
+    const uint64_t nch = (e - b + kChunkEdges - 1) / kChunkEdges;
+    const uint32_t rot = (uint32_t)(((uint64_t)h * 2654435761u) % W);
+    for (uint64_t ch = (gw + W - rot) % W; ch < nch; ch += W) {
+      const uint64_t cb = b + ch * kChunkEdges;
+      mss_relax_range(a.nbr, a.wt, a.dist, a.cm_cur, a.cm_next, a.out_bits, u, cb,
+                      cb + kChunkEdges < e ? cb + kChunkEdges : e);
+    }
+  }
+}
+
+// One wave per 64-vertex bitmap word: counts the new frontier (vertices and
+// out-arcs, for the decide) and clears the spent frontier's word and masks.
+__global__ __launch_bounds__(kBlock) void mssd_apply_kernel(const MsCtl *c, uint64_t *fin,
+                                                            const uint64_t *fout,
+                                                            uint64_t *cm_spent,
+                                                            const uint32_t *outdeg, uint64_t *set,
+                                                            size_t words) {
+  if (c->mode == kMsDone) return;
+  const int lane = lane_id();
+  const size_t W = (size_t)gridDim.x * kWavesPerBlock;
+  uint64_t cnt = 0, edg = 0;
+  for (size_t wi = global_wave(); wi < words; wi += W) {
+    const uint64_t act = fout[wi], old = fin[wi];
+    const uint32_t v = (uint32_t)(wi * 64 + lane);
+    if ((act >> lane) & 1ULL) {
+      cnt++;
+      edg += outdeg[v];
+    }
+    if ((old >> lane) & 1ULL) cm_spent[v] = 0;
+    if (lane == 0 && old) fin[wi] = 0;
+  }
+  cnt = wave_sum_u64(cnt);
+  edg = wave_sum_u64(edg);
+  if (lane == 0 && cnt) {
+    uint64_t *p = set + (size_t)(global_wave() % kScalarSlots) * 16;
+    atomicAdd((unsigned long long *)p, (unsigned long long)cnt);
+    atomicAdd((unsigned long long *)(p + 1), (unsigned long long)edg);
+  }
+}
+
+// Batched SSSP iterations on the device-driven loop: `dist` (rows in
+// internal order, 1e9 everywhere) gets the distances; the same iterations,
+// modes and frontier sizes as the host-driven loop.
+static int mss_device(gg_graph *g, Work *w, const uint32_t *roots, int n_roots, float *dist,
+                      gg_run_stats *st) {
+  const Dir &out = out_dir(g), &in = in_dir(g);
+  const uint32_t V = g->vertices;
+  const size_t words = w->words;
+  const Options &opt = options();
+  MsdState *s;
+  GG_TRY(msd_state(w, &s));
+  hipStream_t sm = stream();
+  GG_HIP(hipMemsetAsync(w->ms_f[0], 0, (size_t)V * 8, sm));
+  GG_HIP(hipMemsetAsync(w->ms_f[1], 0, (size_t)V * 8, sm));
+  GG_HIP(hipMemsetAsync(w->f[0].bits, 0, words * 8, sm));
+  GG_HIP(hipMemsetAsync(w->f[1].bits, 0, words * 8, sm));
+  GG_HIP(hipMemsetAsync(s->set, 0, (size_t)kScalarSlots * 16 * 8, sm));
+  GG_HIP(hipMemcpyAsync(w->ms_roots, roots, 4 * (size_t)n_roots, hipMemcpyHostToDevice, sm));
+  volatile MsHostFlag *hf = s->h_flag;
+  hf->decided = 0;
+  hf->done = 0;
+  const uint64_t threshold = g->edges / (uint64_t)opt.switch_divisor;
+  hipLaunchKernelGGL(mssd_seed_kernel, dim3(1), dim3(kBlock), 0, sm, s->ctl, s->set, s->qc,
+                     s->h_flag, w->ms_roots, n_roots, dist, w->ms_f[0], w->f[0].bits, out.degree,
+                     (int32_t)opt.force_mode, threshold);
+  const unsigned bin_grid = compact_grid(words);
+  const unsigned push_grid = grid_waves((uint64_t)opt.huge_grid_waves);
+  // looping grids: a gated launch of one wave per vertex would cost its
+  // dispatch on every push iteration
+  const unsigned pull_grid = grid_waves(std::min<uint64_t>(V, 1u << 16));
+  const uint64_t hw = std::max<uint64_t>((uint64_t)opt.heavy_waves, 1);
+  const unsigned heavy_grid = grid_waves(std::min<uint64_t>(std::max(in.n_chunks, 1u), hw));
+  const unsigned apply_grid = grid_waves(std::min<size_t>(words, 1u << 13));
+  for (uint32_t n = 0;; n++) {
+    // one iteration queued ahead, as in the batched BFS loop
+    if (n >= 1) {
+      uint64_t spins = 0;
+      while (!hf->done && hf->decided < n) {
+        if ((++spins & 0xFFF) == 0 && hipStreamQuery(sm) == hipSuccess && !hf->done &&
+            hf->decided < n)
+          return fail(GG_EHIP, "batched SSSP: device loop stopped without a decision");
+      }
+      if (hf->done) break;
+    }
+    if ((uint64_t)n > 64ull * V + 2) return fail(GG_EHIP, "batched SSSP: did not converge");
+    uint64_t *fin = w->f[n & 1].bits, *fout = w->f[(n + 1) & 1].bits;
+    uint64_t *cm_cur = w->ms_f[n & 1], *cm_next = w->ms_f[(n + 1) & 1];
+    uint32_t **q = w->f[0].q;
+    {
+      ProfScope p("mss_bin");
+      hipLaunchKernelGGL(msd_bin_kernel, dim3(bin_grid), dim3(kCompactBlock), 0, sm, s->ctl, fin,
+                         words, out.degree, (uint32_t)opt.push_small_max,
+                         (uint32_t)opt.push_big_max, q[0], q[1], q[2], s->qc);
+    }
+    {
+      ProfScope p("mss_push");
+      MssdPushArgs a{out.index, out.nbr, out.w, q[0], q[1], q[2], s->qc, dist, cm_cur, cm_next,
+                     fout};
+      hipLaunchKernelGGL(mssd_push_kernel, dim3(push_grid), dim3(kBlock), 0, sm, s->ctl, a);
+    }
+    {
+      ProfScope p("mss_pull");
+      MssPull op{dist, cm_cur, cm_next, fout, &s->ctl->mode};
+      LaneArgs a{in.index, in.nbr, in.w, nullptr, V, in.light_max};
+      hipLaunchKernelGGL(lane_vertex_kernel<MssPull>, dim3(pull_grid), dim3(kBlock), 0, sm, a, op);
+      if (in.n_chunks) {
+        HeavyArgs h{in.nbr, in.w, in.chunk_v, in.chunk_h, in.chunk_beg, in.chunk_end, in.n_chunks,
+                    fout, nullptr, nullptr};
+        hipLaunchKernelGGL(lane_heavy_kernel<MssPull>, dim3(heavy_grid), dim3(kBlock), 0, sm, h,
+                           op);
+      }
+    }
+    hipLaunchKernelGGL(mssd_apply_kernel, dim3(apply_grid), dim3(kBlock), 0, sm, s->ctl, fin, fout,
+                       cm_cur, out.degree, s->set, words);
+    hipLaunchKernelGGL(msd_decide_kernel, dim3(1), dim3(kBlock), 0, sm, s->ctl, s->set, s->qc,
+                       s->h_flag);
+    GG_HIP(hipGetLastError());
+  }
+  GG_HIP(hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(MsCtl), hipMemcpyDeviceToHost, sm));
+  GG_HIP(hipStreamSynchronize(sm));
+  const MsCtl &c = *s->h_ctl;
+  st->iterations = c.iter;
+  st->push_iterations = c.push_iters;
+  for (uint32_t i = 0; i < std::min<uint32_t>(c.iter, 64); i++) {
+    st->modes[i] = c.modes[i];
+    st->it_active[i] = c.active[i];
+    st->it_new[i] = c.fresh[i];
+  }
+  return GG_OK;
+}
+
 // Sum over roots of the out-degrees of the vertices each reached.
 static int ms_reach(gg_graph *g, Work *w, uint64_t *m) {
   GG_HIP(hipMemsetAsync(w->exam, 0, 1024 * 8, stream()));
@@ -2136,7 +2353,12 @@ int bfs_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, uint32_t *d
   DeviceTimer tm;
   uint32_t *depth = g->reordered || keep ? (uint32_t *)w->ms_tmp : d_depth;
   MsLevels lv;
-  GG_TRY(ms_bfs_device(g, w, roots, n_roots, depth, &en.st, &lv));
+  if (options().batch_device_loop)
+    GG_TRY(ms_bfs_device(g, w, roots, n_roots, depth, &en.st, &lv));
+  else
+    GG_TRY(ms_bfs_run(
+        g, w, en, roots, n_roots, depth,
+        [](uint32_t, const uint64_t *, const uint64_t *) { return GG_OK; }, false, &lv));
   uint64_t m = 0;
   GG_TRY(ms_reach(g, w, &m));
   if (rows) *rows = depth;
@@ -2171,40 +2393,45 @@ int sssp_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, float *d_d
   float *dist = g->reordered || keep ? (float *)w->ms_tmp : d_dist;
   if (rows) *rows = dist;
   fill<uint32_t>((uint32_t *)dist, kInfBits, (uint64_t)V * 64);  // 1e9 (sssp.cpp:163)
-  GG_HIP(hipMemsetAsync(w->ms_f[0], 0, (size_t)V * 8, stream()));
-  GG_HIP(hipMemcpyAsync(w->ms_roots, roots, 4 * (size_t)n_roots, hipMemcpyHostToDevice, stream()));
-  GG_TRY(en.clear_bits(en.fin()));
-  hipLaunchKernelGGL(mss_seed_kernel, dim3(1), dim3(1), 0, stream(), w->ms_roots, n_roots, dist,
-                     w->ms_f[0], en.fin().bits);
-  GG_TRY(en.compact_in(out.degree));
-  int fc = 0;
-  while (en.in.total > 0) {
-    uint64_t *cm_cur = w->ms_f[fc], *cm_next = w->ms_f[fc ^ 1];
-    GG_HIP(hipMemsetAsync(cm_next, 0, (size_t)V * 8, stream()));
-    bool sp = en.sparse(en.in.edges);
-    if (sp) {
-      uint32_t waves2 = en.in.n[2] ? (uint32_t)options().huge_grid_waves : 0;
-      uint64_t waves = (uint64_t)en.in.n[0] + en.in.n[1] + waves2;
-      ProfScope p("mss_push");
-      hipLaunchKernelGGL(mss_push_kernel, dim3(grid_waves(waves)), dim3(kBlock), 0, stream(),
-                         out.index, out.nbr, out.w, en.fin().q[0], en.in.n[0], en.fin().q[1],
-                         en.in.n[1], en.fin().q[2], en.in.n[2], waves2, dist, cm_cur, cm_next,
-                         en.fout().bits);
-    } else {
-      MssPull op{dist, cm_cur, cm_next, en.fout().bits};
-      ProfScope p("mss_pull");
-      LaneArgs a{in.index, in.nbr, in.w, nullptr, V, in.light_max};
-      hipLaunchKernelGGL(lane_vertex_kernel<MssPull>, dim3(grid_waves(V)), dim3(kBlock), 0,
-                         stream(), a, op);
-      if (in.n_chunks) {
-        HeavyArgs h{in.nbr, in.w, in.chunk_v, in.chunk_h, in.chunk_beg, in.chunk_end, in.n_chunks,
-                    en.fout().bits, nullptr};
-        hipLaunchKernelGGL(lane_heavy_kernel<MssPull>, dim3(grid_waves(in.n_chunks)), dim3(kBlock),
-                           0, stream(), h, op);
+  if (options().batch_device_loop) {
+    GG_TRY(mss_device(g, w, roots, n_roots, dist, &en.st));
+  } else {
+    GG_HIP(hipMemsetAsync(w->ms_f[0], 0, (size_t)V * 8, stream()));
+    GG_HIP(hipMemcpyAsync(w->ms_roots, roots, 4 * (size_t)n_roots, hipMemcpyHostToDevice,
+                          stream()));
+    GG_TRY(en.clear_bits(en.fin()));
+    hipLaunchKernelGGL(mss_seed_kernel, dim3(1), dim3(1), 0, stream(), w->ms_roots, n_roots, dist,
+                       w->ms_f[0], en.fin().bits);
+    GG_TRY(en.compact_in(out.degree));
+    int fc = 0;
+    while (en.in.total > 0) {
+      uint64_t *cm_cur = w->ms_f[fc], *cm_next = w->ms_f[fc ^ 1];
+      GG_HIP(hipMemsetAsync(cm_next, 0, (size_t)V * 8, stream()));
+      bool sp = en.sparse(en.in.edges);
+      if (sp) {
+        uint32_t waves2 = en.in.n[2] ? (uint32_t)options().huge_grid_waves : 0;
+        uint64_t waves = (uint64_t)en.in.n[0] + en.in.n[1] + waves2;
+        ProfScope p("mss_push");
+        hipLaunchKernelGGL(mss_push_kernel, dim3(grid_waves(waves)), dim3(kBlock), 0, stream(),
+                           out.index, out.nbr, out.w, en.fin().q[0], en.in.n[0], en.fin().q[1],
+                           en.in.n[1], en.fin().q[2], en.in.n[2], waves2, dist, cm_cur, cm_next,
+                           en.fout().bits);
+      } else {
+        MssPull op{dist, cm_cur, cm_next, en.fout().bits, nullptr};
+        ProfScope p("mss_pull");
+        LaneArgs a{in.index, in.nbr, in.w, nullptr, V, in.light_max};
+        hipLaunchKernelGGL(lane_vertex_kernel<MssPull>, dim3(grid_waves(V)), dim3(kBlock), 0,
+                           stream(), a, op);
+        if (in.n_chunks) {
+          HeavyArgs h{in.nbr,       in.w,           in.chunk_v, in.chunk_h, in.chunk_beg,
+                      in.chunk_end, in.n_chunks, en.fout().bits, nullptr};
+          hipLaunchKernelGGL(lane_heavy_kernel<MssPull>, dim3(grid_waves(in.n_chunks)),
+                             dim3(kBlock), 0, stream(), h, op);
+        }
       }
+      GG_TRY(en.finish_iteration(sp, out.degree, nullptr));
+      fc ^= 1;
     }
-    GG_TRY(en.finish_iteration(sp, out.degree, nullptr));
-    fc ^= 1;
   }
   if (g->reordered && !keep)
     hipLaunchKernelGGL(unpermute_rows_kernel<float>, dim3(grid1((uint64_t)V * 64)), dim3(kBlock), 0,

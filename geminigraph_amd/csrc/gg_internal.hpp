@@ -67,6 +67,7 @@ struct Options {
   int64_t heavy_waves = 8192;    // dense heavy-chunk grids: waves (0 = one per chunk)
   int64_t bfs_pull_waves = 8192; // BFS pull grids: waves (0 = one per group / chunk)
   int64_t batch_graphs = 0;      // batched BFS levels replayed from captured hipGraphs
+  int64_t batch_device_loop = 1; // batched BFS/SSSP iterations decided on the device
 };
 Options &options();
 bool prof_on();  // per-kernel event timing (gg_prof_enable) is on

@@ -90,6 +90,10 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *                      BC selects backward arcs exactly), 0 = write depth
  *                      rows level by level
  *   "level_mask_bytes" size limit of that mask store (default 4 GiB)
+ *   "batch_device_loop" batched BFS and SSSP: 1 = each iteration's mode is
+ *                      decided on the device and the host issues iterations
+ *                      ahead without waiting (default); 0 = the host reads
+ *                      the frontier counts back after every iteration
  *   "batch_graphs"     batched BFS: 1 = replay each level (after the first)
  *                      from a hipGraph captured once per buffer parity;
  *                      0 = launch the level's kernels (default: measured

@@ -27,6 +27,37 @@ def test_sssp_batch_matches_oracle(gg, reorder, mode):
         assert np.array_equal(dist[i].view(np.uint32), ref.view(np.uint32)), (i, r)
 
 
+@pytest.mark.parametrize("mode", ["auto", "push", "pull"])
+def test_batch_device_loop_matches_host_loop(gg, mode):
+    # the device-driven loops (mode decided on the device, no per-iteration
+    # host sync) against the host-driven ones: identical distances, depths
+    # and BFS per-level records (the batched SSSP relaxes asynchronously
+    # within an iteration, so only its fixed point is fixed, not which
+    # vertices change in which iteration); in-lists over 32 entries go to
+    # the chunked (gated) heavy kernels
+    scale = 12
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, 16, seed=83, weighted=True)
+    with gg.options(pull_light_max=32):
+        g = gg.Graph.build(e, V)
+    o = oracle.Graph(e, V)
+    roots = roots_of(o.out_degree(), 48, 83)
+    keys = ("iterations", "push_iterations", "modes", "it_active", "it_new")
+    say("batch device vs host loop", mode)
+    out = {}
+    for loop in (1, 0):
+        with gg.options(force_mode=MODES[mode], batch_device_loop=loop):
+            dist, sst = gg.sssp_batch(g, roots)
+            depth, bst = gg.bfs_batch(g, roots)
+        out[loop] = (dist, depth, {k: sst[k] for k in keys}, {k: bst[k] for k in keys})
+    assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][3] == out[1][3]
+    for i in (0, 17, 47):
+        assert np.array_equal(out[1][0][i].view(np.uint32), o.sssp(roots[i])[0].view(np.uint32))
+        assert np.array_equal(out[1][1][i], o.bfs(roots[i])[1])
+
+
 def test_sssp_batch_hub_graph(gg):
     # out-degree 20000 root (grid-shared push chunks), heavy in-degree sink
     n = 30000
