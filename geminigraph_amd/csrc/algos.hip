@@ -2487,6 +2487,8 @@ int bc_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_de
     }
     return GG_OK;
   };
+  // (host-driven: sigma runs on the compacted queues of each level; on the
+  // device-driven loop, launches over every vertex were measured slower)
   MsLevels lv;
   GG_TRY(ms_bfs_run(g, w, en, roots, n_roots, depth, hook, true, &lv));
   const bool masks = lv.masks;
