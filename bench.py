@@ -408,6 +408,8 @@ def main():
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--pagerank-only", action="store_true",
                     help="run only the PageRank leg (for profiling) and print its figures")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched BFS/BC leg (for profiling) and print its figures")
     args = ap.parse_args()
     if args.pmc_child:
         pmc_child(args)
@@ -423,6 +425,11 @@ def main():
         src, dst = gg.DeviceArray(COPY_BYTES, np.uint8), gg.DeviceArray(COPY_BYTES, np.uint8)
         gg.device_copy(dst, src, COPY_BYTES)
         gg.synchronize()
+        return
+    if args.batched_only:
+        import geminigraph_amd as gg
+        gg.init(0)
+        log(f"batched: {batch_extra(gg, args.seed + 7, not args.no_reorder)}")
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
