@@ -58,6 +58,34 @@ def test_batch_device_loop_matches_host_loop(gg, mode):
         assert np.array_equal(out[1][1][i], o.bfs(roots[i])[1])
 
 
+@pytest.mark.parametrize("case", ["one_vertex", "no_edges", "v65", "loops_dups"])
+@pytest.mark.parametrize("reorder", [False, True])
+def test_batch_device_loop_small_graphs(gg, case, reorder):
+    # degenerate inputs on the device-driven loops: a single vertex, no edges,
+    # V = 65 (one vertex past a bitmap word), self-loops and duplicate arcs;
+    # one root, the same root 64 times, and mixed roots
+    if case == "one_vertex":
+        V, pairs = 1, [(0, 0)]
+    elif case == "no_edges":
+        V, pairs = 5, []
+    elif case == "v65":
+        V, pairs = 65, [(i, i + 1) for i in range(64)] + [(64, 0), (3, 64)]
+    else:
+        V, pairs = 40, [(i, (i * 7) % 40) for i in range(40)] * 3 + [(5, 5), (6, 6), (0, 39)]
+    w = (np.arange(max(len(pairs), 1))[:len(pairs)] % 13 + 1).astype(np.float32) / 13
+    e = oracle.edges_from_pairs(pairs, w) if pairs else oracle.edges_from_pairs(
+        np.zeros((0, 2), np.uint32), np.zeros(0, np.float32))
+    g = gg.Graph.build(e, V, reorder=reorder)
+    o = oracle.Graph(e, V)
+    say("batch small graph", case, reorder)
+    for roots in ([V - 1], [0] * 64, [i % V for i in range(0, 3 * V, 3)][:64]):
+        depth, _ = gg.bfs_batch(g, roots)
+        dist, _ = gg.sssp_batch(g, roots)
+        for i, r in enumerate(roots):
+            assert np.array_equal(depth[i], o.bfs(r)[1]), (case, roots, i)
+            assert np.array_equal(dist[i].view(np.uint32), o.sssp(r)[0].view(np.uint32)), (case, i)
+
+
 def test_sssp_batch_hub_graph(gg):
     # out-degree 20000 root (grid-shared push chunks), heavy in-degree sink
     n = 30000
