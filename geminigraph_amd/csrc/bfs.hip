@@ -438,10 +438,12 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOu
     bool cand = v < a.V && !((visw >> lane) & 1ULL);
     const uint4 vi = cand ? o.vinfo[v] : make_uint4(0, 0, 0, 0);
     const uint32_t deg = vi.z;
-    cand = cand && deg > 0 && deg <= a.light_max;
+    cand = cand && deg > 0;
     // the first in-neighbour (the highest-degree one on a relabelled graph)
     // decides most vertices without touching the list; its original id is
-    // read alongside, so the parent needs no further load
+    // read alongside, so the parent needs no further load.  Heavy vertices
+    // (in-lists above light_max) get this test too; the heavy-chunk kernel,
+    // which runs after this one, scans the lists of those it leaves.
     uint32_t found = kUnset;
     bool head = false;  // found at the first in-neighbour
     uint32_t head_orig = 0;
@@ -451,7 +453,7 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOu
       if (head) found = vi.w;
       examined += 1;
     }
-    cand = cand && found == kUnset && deg > 1;
+    cand = cand && found == kUnset && deg > 1 && deg <= a.light_max;
     uint64_t b = 0, e1 = 0;
     if (cand) {
       b = a.index[v];
@@ -534,7 +536,9 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_heavy_kernel(const BfsCtl *c,
   for (uint32_t ch = global_wave(); ch < a.n_chunks; ch += W) {
     const uint32_t v = a.chunk_v[ch];
     const uint64_t bit = 1ULL << (v & 63);
-    if ((o.visited[v >> 6] | a.fin[v >> 6]) & bit) continue;
+    // found earlier, in this level's frontier, or at its head by the pull
+    // kernel (or by another chunk) this level
+    if ((o.visited[v >> 6] | a.fin[v >> 6] | o.fout[v >> 6]) & bit) continue;
     const uint4 vi = o.vinfo[v];
     uint32_t *dv = &o.depth[vi.x];
     const uint64_t cb = a.chunk_beg[ch], ce = a.chunk_end[ch];
