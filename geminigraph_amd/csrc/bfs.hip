@@ -533,32 +533,44 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_heavy_kernel(const BfsCtl *c,
   const uint32_t W = gridDim.x * kWavesPerBlock;
   const int lane = lane_id();
   uint64_t cnt = 0, edg = 0, ind = 0, examined = 0;
-  for (uint32_t ch = global_wave(); ch < a.n_chunks; ch += W) {
-    const uint32_t v = a.chunk_v[ch];
-    const uint64_t bit = 1ULL << (v & 63);
-    // found earlier, in this level's frontier, or at its head by the pull
-    // kernel (or by another chunk) this level
-    if ((o.visited[v >> 6] | a.fin[v >> 6] | o.fout[v >> 6]) & bit) continue;
-    const uint4 vi = o.vinfo[v];
-    uint32_t *dv = &o.depth[vi.x];
-    const uint64_t cb = a.chunk_beg[ch], ce = a.chunk_end[ch];
-    for (uint64_t e0 = cb; e0 < ce; e0 += 64) {
-      if (__hip_atomic_load(dv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kUnset) break;
-      const uint64_t e = e0 + lane;
-      const uint32_t u = e < ce ? stream_load(&a.nbr[e]) : 0;
-      const bool hit = e < ce && test_bit(a.fin, u);
-      const uint64_t hm = __ballot(hit);
-      examined += hm ? (uint64_t)(__builtin_ctzll(hm) + 1) : (ce - e0 < 64 ? ce - e0 : 64);
-      if (hm) {
-        const uint32_t fu = __shfl(u, __builtin_ctzll(hm));
-        if (lane == 0 && atomicCAS(dv, kUnset, level) == kUnset) {
-          o.parent[vi.x] = o.vinfo[fu].x;
-          atomicOr((unsigned long long *)&o.fout[v >> 6], (unsigned long long)bit);
-          cnt++;
-          edg += vi.y;
-          ind += vi.z;
+  // lane j tests chunk base + j*W (the chunks of one vertex stay spread over
+  // the waves), then the wave scans the chunks whose vertex is still open
+  for (uint64_t base = global_wave(); base < a.n_chunks; base += (uint64_t)W * 64) {
+    const uint64_t mine = base + (uint64_t)lane * W;
+    uint32_t vl = 0;
+    bool open = false;
+    if (mine < a.n_chunks) {
+      vl = a.chunk_v[mine];
+      // found earlier, in this level's frontier, or at its head by the pull
+      // kernel (or by another chunk) this level
+      open = !(((o.visited[vl >> 6] | a.fin[vl >> 6] | o.fout[vl >> 6]) >> (vl & 63)) & 1ULL);
+    }
+    for (uint64_t todo = __ballot(open); todo; todo &= todo - 1) {
+      const int j = __builtin_ctzll(todo);
+      const uint64_t ch = base + (uint64_t)j * W;
+      const uint32_t v = (uint32_t)__shfl((int)vl, j);
+      const uint64_t bit = 1ULL << (v & 63);
+      const uint4 vi = o.vinfo[v];
+      uint32_t *dv = &o.depth[vi.x];
+      const uint64_t cb = a.chunk_beg[ch], ce = a.chunk_end[ch];
+      for (uint64_t e0 = cb; e0 < ce; e0 += 64) {
+        if (__hip_atomic_load(dv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kUnset) break;
+        const uint64_t e = e0 + lane;
+        const uint32_t u = e < ce ? stream_load(&a.nbr[e]) : 0;
+        const bool hit = e < ce && test_bit(a.fin, u);
+        const uint64_t hm = __ballot(hit);
+        examined += hm ? (uint64_t)(__builtin_ctzll(hm) + 1) : (ce - e0 < 64 ? ce - e0 : 64);
+        if (hm) {
+          const uint32_t fu = __shfl(u, __builtin_ctzll(hm));
+          if (lane == 0 && atomicCAS(dv, kUnset, level) == kUnset) {
+            o.parent[vi.x] = o.vinfo[fu].x;
+            atomicOr((unsigned long long *)&o.fout[v >> 6], (unsigned long long)bit);
+            cnt++;
+            edg += vi.y;
+            ind += vi.z;
+          }
+          break;
         }
-        break;
       }
     }
   }
