@@ -416,6 +416,13 @@ struct CcPull : NoAfter {  // phase 1 of the dense step: gather the minimum in-l
     return false;
   }
 };
+// The labels at the start of a dense step on the plan, complemented and
+// widened to 64 bits (the plan's value type).
+__global__ void cc_snapshot_kernel(const uint32_t *label, uint64_t *nlab, uint32_t V) {
+  const uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s)
+    nlab[v] = ~(uint64_t)label[v];
+}
 // phase 2: dense_slot (cc.cpp:72-79), one lane per vertex.
 __global__ void cc_apply_kernel(uint32_t *label, uint32_t *msg, uint32_t V, uint64_t *out) {
   uint32_t n = (V + 255) & ~255u;
@@ -2162,6 +2169,9 @@ int gg_cc(gg_graph *g, uint32_t *d_label, uint32_t *h_frontier, int cap, gg_run_
   hipLaunchKernelGGL(bitmap_fill_kernel, dim3(grid1(w->words)), dim3(kBlock), 0, stream(),
                      en.fin().bits, w->words, V);
   GG_TRY(en.compact_in(out.degree));
+  // relabelled graph: dense steps on the destination-bin plan (min in LDS)
+  const PrPlan *plan = nullptr;
+  GG_TRY(pr_plan_get(g, &plan));
   int lv = 0;
   while (en.in.total > 0) {
     if (h_frontier && lv < cap) h_frontier[lv] = en.in.total;
@@ -2174,6 +2184,11 @@ int gg_cc(gg_graph *g, uint32_t *d_label, uint32_t *h_frontier, int cap, gg_run_
       CcPush op;
       op.label = d_label;
       GG_TRY(en.launch_push(out, op, true));
+    } else if (plan) {
+      uint64_t *nlab = (uint64_t *)w->msg;  // V x 8 bytes
+      hipLaunchKernelGGL(cc_snapshot_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), d_label,
+                         nlab, V);
+      GG_TRY(cc_plan_iteration(g, plan, w, nlab, d_label, en.fout().bits));
     } else {
       CcPull op{{}, d_label, msg};
       GG_TRY(en.launch_pull_group(in, op, nullptr));

@@ -255,6 +255,11 @@ int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull 
 // vertices whose vis covers full read no source value; `settled`, if given,
 // is a bitmap of those vertices (else vis is tested per bin).
 constexpr uint32_t kGateDense = 1;
+// One dense CC step on the plan: label[v] = min(label[v], min over v's
+// in-neighbours u of ~nlab[u]) with nlab = ~label (64-bit) copied at the
+// start of the step; lowered vertices go to `out` (every word is written).
+int cc_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *nlab,
+                      uint32_t *label, uint64_t *out);
 int ms_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *fcur,
                       const uint64_t *vis, uint64_t full, uint64_t *fnext, uint64_t *out,
                       const uint32_t *gate = nullptr, const uint64_t *settled = nullptr);

@@ -182,6 +182,34 @@ struct MsOrOp {
   }
 };
 
+// CC (cc.cpp:57-79): the minimum label over each vertex's in-neighbours,
+// taken from the labels at the start of the step (BSP: the finish writes the
+// label array, the gathers read the copy).  Values are stored complemented,
+// ~label as 64 bits, and combined by max, so 0 (the plan's initial
+// accumulator and split-bin row) is the identity; the finish lowers a label
+// and reports the vertex for the output frontier.
+struct CcMinOp {
+  using T = uint64_t;
+  static constexpr bool kSkipDone = false;
+  const uint64_t *nlab;  // ~label at the start of the step, per vertex
+  uint32_t *label;
+  __device__ uint64_t done_mask(uint32_t, bool) const { return 0; }
+  __device__ T value(uint32_t u) const { return nlab[u]; }
+  __device__ static void lds_add(T *acc, uint32_t i, T x) {
+    atomicMax((unsigned long long *)&acc[i], (unsigned long long)x);
+  }
+  __device__ static void global_add(T *p, T x) {
+    if (x) atomicMax((unsigned long long *)p, (unsigned long long)x);
+  }
+  __device__ bool finish(uint32_t v, T acc, double &) const {
+    if (!acc) return false;  // no in-arcs
+    const uint32_t m = (uint32_t)~acc;
+    if (m >= label[v]) return false;
+    label[v] = m;
+    return true;
+  }
+};
+
 // Emit: part p = arcs [lo, hi) of range r.  The range's values go to LDS
 // (one coalesced read), then every arc's value is written in storage order.
 // Values are copied as 8-byte words.
@@ -709,6 +737,12 @@ int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull 
   static const char *names[3] = {"pr_emit", "pr_bins", "pr_split_finish"};
   return plan_pass(g, p, w, reinterpret_cast<const uint64_t *>(op.curr), PrSumOp{op}, nullptr,
                    dacc, nullptr, names);
+}
+
+int cc_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *nlab,
+                      uint32_t *label, uint64_t *out) {
+  static const char *names[3] = {"cc_emit", "cc_bins", "cc_split_finish"};
+  return plan_pass(g, p, w, nlab, CcMinOp{nlab, label}, out, nullptr, nullptr, names);
 }
 
 int ms_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *fcur,

@@ -284,15 +284,34 @@ def test_bfs_hub_bins(gg):
 
 # ----------------------------------------------------------------------- CC
 
+@pytest.mark.parametrize("reorder", [False, True])
 @pytest.mark.parametrize("mode", ["auto", "push", "pull"])
 @pytest.mark.parametrize("scale,seed", [(12, 5), (16, 6)])
-def test_cc_matches_oracle(gg, scale, seed, mode):
+def test_cc_matches_oracle(gg, scale, seed, mode, reorder):
+    # relabelled: dense steps on the destination-bin plan (min over in-labels)
     e = oracle.rmat_edges(scale, seed=seed)
     V = 1 << scale
-    say("cc", scale, mode)
-    g = gg.Graph.build(e, V, undirected=True)
+    say("cc", scale, mode, reorder)
+    g = gg.Graph.build(e, V, undirected=True, reorder=reorder)
     with gg.options(force_mode=MODES[mode]):
         label, fr, st = g.cc()
+    ref, ref_fr = oracle.Graph(e, V, undirected=True).cc()
+    assert np.array_equal(label, ref)
+    assert np.array_equal(fr, ref_fr)
+
+
+@pytest.mark.parametrize("bins,ranges,hot", [(9, 9, 500), (8, 12, 1 << 14), (13, 14, 0)])
+def test_cc_plan_sizes(gg, bins, ranges, hot):
+    # CC's dense steps on the bin plan with many bins, split bins, cold ranges
+    # (hot = 0: the plain pull); labels and frontier sizes bit-exact
+    scale = 15
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, seed=7)
+    g = gg.Graph.build(e, V, undirected=True, reorder=True)
+    say("cc plan", bins, ranges, hot)
+    with gg.options(force_mode=MODES["pull"], pr_bin_bits=bins, pr_range_bits=ranges,
+                    pr_hot_sources=hot):
+        label, fr, _ = g.cc()
     ref, ref_fr = oracle.Graph(e, V, undirected=True).cc()
     assert np.array_equal(label, ref)
     assert np.array_equal(fr, ref_fr)
