@@ -416,6 +416,13 @@ struct CcPull : NoAfter {  // phase 1 of the dense step: gather the minimum in-l
     return false;
   }
 };
+// The distances at the start of a dense SSSP step on the plan, as f32 bits
+// in 64-bit values (the plan's value type).
+__global__ void sssp_snapshot_kernel(const float *dist, uint64_t *snap, uint32_t V) {
+  const uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s)
+    snap[v] = (uint64_t)__float_as_uint(dist[v]);
+}
 // The labels at the start of a dense step on the plan, complemented and
 // widened to 64 bits (the plan's value type).
 __global__ void cc_snapshot_kernel(const uint32_t *label, uint64_t *nlab, uint32_t V) {
@@ -2229,6 +2236,11 @@ int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int
   uint32_t *msg = (uint32_t *)w->msg;
   fill<uint32_t>(msg, kInfBits, V);
   GG_TRY(en.seed(root, out.degree, nullptr));
+  // relabelled graph: dense steps on the destination-bin plan (its arcs carry
+  // their weights)
+  const PrPlan *plan = nullptr;
+  GG_TRY(pr_plan_get(g, &plan));
+  if (plan && !plan->hot_w) plan = nullptr;
   int lv = 0;
   while (en.in.total > 0) {
     if (h_frontier && lv < cap) h_frontier[lv] = en.in.total;
@@ -2241,6 +2253,11 @@ int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int
       SsspPush op;
       op.dist = d_dist;
       GG_TRY(en.launch_push(out, op, true));
+    } else if (plan) {
+      uint64_t *snap = (uint64_t *)w->msg;  // V x 8 bytes
+      hipLaunchKernelGGL(sssp_snapshot_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), d_dist,
+                         snap, V);
+      GG_TRY(sssp_plan_iteration(g, plan, w, snap, d_dist, en.fout().bits));
     } else {
       SsspPull op{{}, d_dist, msg};
       GG_TRY(en.launch_pull_group(in, op, nullptr));

@@ -131,6 +131,8 @@ struct PrPlan {
   uint64_t n_cold = 0;         // cold arcs
   uint64_t n_hot = 0;          // hot arcs
   uint32_t *hot = nullptr;     // per bin, sorted: source << bin_bits | destination - bin start
+  float *hot_w = nullptr;      // weighted graphs: each hot arc's weight
+  float *cold_w = nullptr;     // weighted graphs: each cold arc's weight (cold order)
   uint32_t nranges = 0;
   uint16_t *src16 = nullptr;   // cold arcs by (range, bin, source): source - range start
   uint16_t *dst16 = nullptr;   //   and destination - bin start
@@ -260,6 +262,12 @@ constexpr uint32_t kGateDense = 1;
 // start of the step; lowered vertices go to `out` (every word is written).
 int cc_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *nlab,
                       uint32_t *label, uint64_t *out);
+// One dense SSSP step on the plan of a weighted graph: dist[v] is lowered to
+// the minimum over v's in-arcs of fl(snap[u] + w) when that is below 1e9,
+// where snap holds the distances' f32 bits (low 32 bits) at the step's start;
+// lowered vertices go to `out` (every word is written).
+int sssp_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *snap,
+                        float *dist, uint64_t *out);
 int ms_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *fcur,
                       const uint64_t *vis, uint64_t full, uint64_t *fnext, uint64_t *out,
                       const uint32_t *gate = nullptr, const uint64_t *settled = nullptr);

@@ -73,9 +73,10 @@ __global__ void pr_hot_count_kernel(const uint64_t *index, const uint32_t *nbr, 
 
 // One wave per destination v: key (bin, source, local destination) of each
 // hot in-arc, at hot_off[v] onwards.
-__global__ void pr_hot_keys_kernel(const uint64_t *index, const uint32_t *nbr, uint32_t V,
-                                   const uint32_t *hot_cnt, const uint64_t *hot_off, int bb,
-                                   uint64_t *keys) {
+// (With weights, wbits[] receives each arc's weight bits beside its key.)
+__global__ void pr_hot_keys_kernel(const uint64_t *index, const uint32_t *nbr, const float *w,
+                                   uint32_t V, const uint32_t *hot_cnt, const uint64_t *hot_off,
+                                   int bb, uint64_t *keys, uint32_t *wbits) {
   const uint32_t W = gridDim.x * (blockDim.x / 64);
   const int lane = lane_id();
   for (uint32_t v = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); v < V; v += W) {
@@ -83,7 +84,10 @@ __global__ void pr_hot_keys_kernel(const uint64_t *index, const uint32_t *nbr, u
     const uint32_t n = hot_cnt[v];
     const uint64_t hi = (uint64_t)(v >> bb) << 32;
     const uint32_t dl = v & ((1u << bb) - 1);
-    for (uint32_t i = lane; i < n; i += 64) keys[o + i] = hi | (uint64_t)nbr[b + i] << bb | dl;
+    for (uint32_t i = lane; i < n; i += 64) {
+      keys[o + i] = hi | (uint64_t)nbr[b + i] << bb | dl;
+      if (w) wbits[o + i] = __float_as_uint(w[b + i]);
+    }
   }
 }
 
@@ -95,9 +99,9 @@ __global__ void pr_hot_pack_kernel(const uint64_t *keys, uint64_t n, uint32_t *a
 
 // One wave per cold source u: key (range, bin, local source, local
 // destination) of each out-arc, at its CSR position - cold_base.
-__global__ void pr_cold_keys_kernel(const uint64_t *index, const uint32_t *nbr, uint32_t H,
-                                    uint32_t V, uint64_t cold_base, int bb, int rb, int bbits,
-                                    uint64_t *keys) {
+__global__ void pr_cold_keys_kernel(const uint64_t *index, const uint32_t *nbr, const float *w,
+                                    uint32_t H, uint32_t V, uint64_t cold_base, int bb, int rb,
+                                    int bbits, uint64_t *keys, uint32_t *wbits) {
   const uint32_t W = gridDim.x * (blockDim.x / 64);
   const int lane = lane_id();
   for (uint32_t u = H + blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); u < V; u += W) {
@@ -108,6 +112,7 @@ __global__ void pr_cold_keys_kernel(const uint64_t *index, const uint32_t *nbr, 
     for (uint64_t a = b + lane; a < e; a += 64) {
       const uint32_t d = nbr[a];
       keys[a - cold_base] = hi | ((uint64_t)(d >> bb) << (rb + bb)) | (d & dmask);
+      if (w) wbits[a - cold_base] = __float_as_uint(w[a]);
     }
   }
 }
@@ -145,6 +150,7 @@ __global__ void pr_cold_split_kernel(const uint64_t *keys, uint64_t n, int bb, i
 struct PrSumOp {
   using T = double;
   static constexpr bool kSkipDone = false;
+  static constexpr bool kWeighted = false;  // true: relax(value, weight) per arc
   PrPull op;
   __device__ uint64_t done_mask(uint32_t, bool) const { return 0; }
   __device__ T value(uint32_t u) const { return op.curr[u]; }
@@ -160,6 +166,7 @@ struct PrSumOp {
 struct MsOrOp {
   using T = uint64_t;
   static constexpr bool kSkipDone = true;  // every root has reached the vertex
+  static constexpr bool kWeighted = false;
   const uint64_t *fcur, *vis;
   uint64_t *fnext;
   uint64_t full;
@@ -191,6 +198,7 @@ struct MsOrOp {
 struct CcMinOp {
   using T = uint64_t;
   static constexpr bool kSkipDone = false;
+  static constexpr bool kWeighted = false;
   const uint64_t *nlab;  // ~label at the start of the step, per vertex
   uint32_t *label;
   __device__ uint64_t done_mask(uint32_t, bool) const { return 0; }
@@ -206,6 +214,38 @@ struct CcMinOp {
     const uint32_t m = (uint32_t)~acc;
     if (m >= label[v]) return false;
     label[v] = m;
+    return true;
+  }
+};
+
+// SSSP (sssp.cpp:58-79): per vertex the minimum over its in-arcs of
+// fl(dist[u] + w) with dist at the start of the step (the snapshot holds the
+// f32 bits); like CC, the arc values are complemented 64-bit keys combined
+// by max (distances are >= 0, so the bit order is the numeric order).  The
+// finish applies the slot: lowered below 1e9 and below dist[v].
+struct SsspMinOp {
+  using T = uint64_t;
+  static constexpr bool kSkipDone = false;
+  static constexpr bool kWeighted = true;
+  const uint64_t *snap;
+  float *dist;
+  __device__ uint64_t done_mask(uint32_t, bool) const { return 0; }
+  __device__ T value(uint32_t u) const { return snap[u]; }
+  __device__ T relax(T x, float w) const {
+    const float r = __uint_as_float((uint32_t)x) + w;  // one f32 add, round to nearest
+    return ~(uint64_t)__float_as_uint(r);
+  }
+  __device__ static void lds_add(T *acc, uint32_t i, T x) {
+    atomicMax((unsigned long long *)&acc[i], (unsigned long long)x);
+  }
+  __device__ static void global_add(T *p, T x) {
+    if (x) atomicMax((unsigned long long *)p, (unsigned long long)x);
+  }
+  __device__ bool finish(uint32_t v, T acc, double &) const {
+    if (!acc) return false;  // no in-arcs
+    const float m = __uint_as_float((uint32_t)~acc);
+    if (!(m < 1e9f) || !(m < dist[v])) return false;
+    dist[v] = m;
     return true;
   }
 };
@@ -248,7 +288,8 @@ __global__ __launch_bounds__(kPrBlock) void pr_emit_kernel(const uint32_t *gate,
 // reports true (a whole bin owns its words).
 template <class Op>
 __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
-    const uint32_t *gate, const PrSeg *segs, const uint32_t *hot, const uint32_t *run_beg, const uint32_t *run_end,
+    const uint32_t *gate, const float *hot_w, const float *cold_w, const PrSeg *segs,
+    const uint32_t *hot, const uint32_t *run_beg, const uint32_t *run_end,
     uint32_t nranges, const uint16_t *dst16, const typename Op::T *vals, uint32_t V, int bb,
     Op op, typename Op::T *gsum, uint64_t *out, double *dacc) {
   using T = typename Op::T;
@@ -306,6 +347,13 @@ __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
       T x[kU];
 #pragma unroll
       for (int t = 0; t < kU; t++) x[t] = settled(k[t] & dmask) ? T(0) : op.value(k[t] >> bb);
+      if constexpr (Op::kWeighted) {
+#pragma unroll
+        for (int t = 0; t < kU; t++) {
+          const uint32_t q = a + t * kPrBlock;
+          x[t] = op.relax(x[t], q < sg.hot_hi ? stream_load(&hot_w[q]) : 0.f);
+        }
+      }
 #pragma unroll
       for (int t = 0; t < kU; t++)
         if (a + t * kPrBlock < sg.hot_hi) Op::lds_add(acc, k[t] & dmask, x[t]);
@@ -353,6 +401,10 @@ __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
             d[t] = stream_load(&dst16[pos[t]]);
             x[t] = stream_load(&vals[pos[t]]);
           }
+        }
+        if constexpr (Op::kWeighted) {
+#pragma unroll
+          for (int t = 0; t < kU; t++) x[t] = op.relax(x[t], stream_load(&cold_w[pos[t]]));
         }
 #pragma unroll
         for (int t = 0; t < kU; t++)
@@ -452,15 +504,29 @@ static int bits_for(uint64_t n) {  // bits that hold 0 .. n-1 (at least 1)
   return b;
 }
 
-static int sort_keys(Scratch &t, uint64_t *&keys, uint64_t n, int end_bit) {
+// Sorts keys[0, n) (and, when vals is not null, the u32 values beside them);
+// keys and vals then point at the sorted copies.
+static int sort_keys(Scratch &t, uint64_t *&keys, uint64_t n, int end_bit,
+                     uint32_t **vals = nullptr) {
   uint64_t *alt = nullptr;
   void *tmp = nullptr;
   GG_TRY(t.get((void **)&alt, n * 8));
   rocprim::double_buffer<uint64_t> kb(keys, alt);
   size_t tmp_bytes = 0;
-  GG_HIP(rocprim::radix_sort_keys(nullptr, tmp_bytes, kb, (size_t)n, 0, end_bit, stream()));
-  GG_TRY(t.get(&tmp, tmp_bytes));
-  GG_HIP(rocprim::radix_sort_keys(tmp, tmp_bytes, kb, (size_t)n, 0, end_bit, stream()));
+  if (vals && *vals) {
+    uint32_t *valt = nullptr;
+    GG_TRY(t.get((void **)&valt, n * 4));
+    rocprim::double_buffer<uint32_t> vb(*vals, valt);
+    GG_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, kb, vb, (size_t)n, 0, end_bit,
+                                     stream()));
+    GG_TRY(t.get(&tmp, tmp_bytes));
+    GG_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, kb, vb, (size_t)n, 0, end_bit, stream()));
+    *vals = vb.current();
+  } else {
+    GG_HIP(rocprim::radix_sort_keys(nullptr, tmp_bytes, kb, (size_t)n, 0, end_bit, stream()));
+    GG_TRY(t.get(&tmp, tmp_bytes));
+    GG_HIP(rocprim::radix_sort_keys(tmp, tmp_bytes, kb, (size_t)n, 0, end_bit, stream()));
+  }
   keys = kb.current();
   return GG_OK;
 }
@@ -495,13 +561,18 @@ static int plan_hot(const gg_graph *g, PrPlan *p, std::vector<uint64_t> &hot_bin
   p->n_hot = last_off + last_cnt;
   hot_bin_off[p->nbins] = p->n_hot;
   GG_TRY(plan_alloc(p, (void **)&p->hot, std::max<uint64_t>(p->n_hot, 1) * 4));
+  if (in.w) GG_TRY(plan_alloc(p, (void **)&p->hot_w, std::max<uint64_t>(p->n_hot, 1) * 4));
   if (!p->n_hot) return GG_OK;
   GG_TRY(t.get((void **)&keys, p->n_hot * 8));
+  uint32_t *wbits = nullptr;
+  if (in.w) GG_TRY(t.get((void **)&wbits, p->n_hot * 4));
   hipLaunchKernelGGL(pr_hot_keys_kernel, dim3(grid1((uint64_t)V * 64)), dim3(kBlock), 0, stream(),
-                     in.index, in.nbr, V, cnt, off, p->bin_bits, keys);
-  GG_TRY(sort_keys(t, keys, p->n_hot, 32 + bits_for(p->nbins)));
+                     in.index, in.nbr, in.w, V, cnt, off, p->bin_bits, keys, wbits);
+  GG_TRY(sort_keys(t, keys, p->n_hot, 32 + bits_for(p->nbins), &wbits));
   hipLaunchKernelGGL(pr_hot_pack_kernel, dim3(grid1(p->n_hot)), dim3(kBlock), 0, stream(), keys,
                      p->n_hot, p->hot);
+  if (in.w)
+    GG_HIP(hipMemcpyAsync(p->hot_w, wbits, p->n_hot * 4, hipMemcpyDeviceToDevice, stream()));
   GG_HIP(hipStreamSynchronize(stream()));
   return GG_OK;
 }
@@ -518,6 +589,7 @@ static int plan_cold(const gg_graph *g, PrPlan *p, std::vector<uint32_t> &hb,
   const uint64_t nruns = (uint64_t)p->nranges * p->nbins;
   GG_TRY(plan_alloc(p, (void **)&p->src16, std::max<uint64_t>(n, 1) * 2));
   GG_TRY(plan_alloc(p, (void **)&p->dst16, std::max<uint64_t>(n, 1) * 2));
+  if (out.w) GG_TRY(plan_alloc(p, (void **)&p->cold_w, std::max<uint64_t>(n, 1) * 4));
   GG_TRY(plan_alloc(p, (void **)&p->run_beg, std::max<uint64_t>(nruns, 1) * 4));
   GG_TRY(plan_alloc(p, (void **)&p->run_end, std::max<uint64_t>(nruns, 1) * 4));
   GG_HIP(hipMemsetAsync(p->run_beg, 0, std::max<uint64_t>(nruns, 1) * 4, stream()));
@@ -531,10 +603,14 @@ static int plan_cold(const gg_graph *g, PrPlan *p, std::vector<uint32_t> &hb,
   Scratch t;
   uint64_t *keys = nullptr;
   GG_TRY(t.get((void **)&keys, n * 8));
+  uint32_t *wbits = nullptr;
+  if (out.w) GG_TRY(t.get((void **)&wbits, n * 4));
   hipLaunchKernelGGL(pr_cold_keys_kernel, dim3(grid1((uint64_t)(V - p->H) * 64)), dim3(kBlock), 0,
-                     stream(), out.index, out.nbr, p->H, V, p->cold_base, p->bin_bits,
-                     p->range_bits, bbits, keys);
-  GG_TRY(sort_keys(t, keys, n, end_bit));
+                     stream(), out.index, out.nbr, out.w, p->H, V, p->cold_base, p->bin_bits,
+                     p->range_bits, bbits, keys, wbits);
+  GG_TRY(sort_keys(t, keys, n, end_bit, &wbits));
+  if (out.w)
+    GG_HIP(hipMemcpyAsync(p->cold_w, wbits, n * 4, hipMemcpyDeviceToDevice, stream()));
   hipLaunchKernelGGL(pr_cold_split_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(), keys, n,
                      p->bin_bits, p->range_bits, bbits, p->nranges, p->src16, p->dst16,
                      p->run_beg, p->run_end);
@@ -719,7 +795,7 @@ static int plan_pass(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t
   if (p->nseg) {
     ProfScope ps(names[1]);
     hipLaunchKernelGGL(pr_bin_kernel<Op>, dim3(p->nseg), dim3(kPrBlock), (size_t)8 << p->bin_bits,
-                       stream(), gate, p->segs, p->hot, p->run_beg, p->run_end, p->nranges, p->dst16,
+                       stream(), gate, p->hot_w, p->cold_w, p->segs, p->hot, p->run_beg, p->run_end, p->nranges, p->dst16,
                        vals, V, p->bin_bits, op, gsum, out, dacc);
   }
   if (p->nsplit) {
@@ -743,6 +819,14 @@ int cc_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_
                       uint32_t *label, uint64_t *out) {
   static const char *names[3] = {"cc_emit", "cc_bins", "cc_split_finish"};
   return plan_pass(g, p, w, nlab, CcMinOp{nlab, label}, out, nullptr, nullptr, names);
+}
+
+int sssp_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *snap,
+                        float *dist, uint64_t *out) {
+  if (!p->hot_w || (p->n_cold && !p->cold_w))
+    return fail(GG_EINVAL, "SSSP plan step: the plan has no arc weights");
+  static const char *names[3] = {"sssp_emit", "sssp_bins", "sssp_split_finish"};
+  return plan_pass(g, p, w, snap, SsspMinOp{snap, dist}, out, nullptr, nullptr, names);
 }
 
 int ms_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *fcur,

@@ -319,12 +319,15 @@ def test_cc_plan_sizes(gg, bins, ranges, hot):
 
 # --------------------------------------------------------------------- SSSP
 
+@pytest.mark.parametrize("reorder", [False, True])
 @pytest.mark.parametrize("mode", ["auto", "push", "pull"])
 @pytest.mark.parametrize("scale,seed", [(12, 7), (15, 8)])
-def test_sssp_matches_oracle(gg, scale, seed, mode):
+def test_sssp_matches_oracle(gg, scale, seed, mode, reorder):
+    # relabelled: dense steps on the destination-bin plan, whose arcs carry
+    # their weights
     e = oracle.rmat_edges(scale, seed=seed, weighted=True)
     V = 1 << scale
-    g = gg.Graph.build(e, V)
+    g = gg.Graph.build(e, V, reorder=reorder)
     o = oracle.Graph(e, V)
     for root in roots_of(o.out_degree(), 2, seed):
         say("sssp", scale, mode, root)
@@ -335,6 +338,25 @@ def test_sssp_matches_oracle(gg, scale, seed, mode):
         assert np.array_equal(fr, ref_fr)
         reached = ref < 1e9
         assert st["m_reach"] == int(o.out_degree()[reached].astype(np.int64).sum())
+
+
+@pytest.mark.parametrize("bins,ranges,hot", [(9, 9, 500), (8, 12, 1 << 14), (13, 14, 1 << 20)])
+def test_sssp_plan_sizes(gg, bins, ranges, hot):
+    # SSSP's dense steps on the weighted bin plan: many bins, split bins,
+    # cold ranges, every source hot; distances and frontier sizes bit-exact
+    scale = 15
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, seed=9, weighted=True)
+    g = gg.Graph.build(e, V, reorder=True)
+    o = oracle.Graph(e, V)
+    say("sssp plan", bins, ranges, hot)
+    with gg.options(force_mode=MODES["pull"], pr_bin_bits=bins, pr_range_bits=ranges,
+                    pr_hot_sources=hot):
+        for root in roots_of(o.out_degree(), 2, 9):
+            dist, fr, _ = g.sssp(root)
+            ref, ref_fr = o.sssp(root)
+            assert np.array_equal(dist.view(np.uint32), ref.view(np.uint32))
+            assert np.array_equal(fr, ref_fr)
 
 
 # ----------------------------------------------------------------------- BC
