@@ -73,12 +73,14 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *                      alpha are at most the in-arcs of the unvisited
  *                      vertices (what a pull would scan); 0 = the divisor
  *                      rule alone (default 24)
- *   "pr_hot_sources"   PageRank on a relabelled graph: H > 0 sums per
- *                      destination bin in LDS (arcs from the H hottest
- *                      sources gathered in source order, H capped at
- *                      2^(32 - pr_bin_bits); the other arcs streamed by
- *                      propagation blocking), built on the first call and
- *                      kept with the graph (default 2^19); 0 = plain pull
+ *   "pr_hot_sources"   relabelled graphs: H > 0 builds the destination-bin
+ *                      plan (arcs from the H hottest sources gathered in
+ *                      source order per bin, H capped at 2^(32 -
+ *                      pr_bin_bits); the other arcs streamed by propagation
+ *                      blocking; weighted graphs keep each arc's weight),
+ *                      built on first use and kept with the graph (default
+ *                      2^19).  PageRank, CC, SSSP and the batched BFS run
+ *                      their dense steps on it; 0 = the plain pulls
  *   "pr_bin_bits"      that plan: 2^bits destinations per bin (8..14,
  *                      default 13)
  *   "pr_range_bits"    that plan: 2^bits sources per cold range (8..14,
