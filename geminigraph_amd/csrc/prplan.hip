@@ -25,6 +25,7 @@
 // Streamed bytes per arc: hot 4 (+ an L2 hit shared by several lanes), cold
 // 2 + 8 (emit) + 8 + 2 (bin) (+ 8 per cold source).
 #include <atomic>
+#include <mutex>
 #include <string.h>
 
 #include <algorithm>
@@ -729,6 +730,12 @@ static int plan_build(const gg_graph *g, uint32_t H, int bin_bits, int range_bit
   return plan_segments(p, hot_bin_off, hb, he);
 }
 
+// Threads that traverse one graph at once (serving workers) share its plan:
+// the first to need it builds it while the others wait.  A plan is replaced
+// when the plan options change, so changing them while another thread
+// traverses the graph is not supported.
+static std::mutex g_plan_mu;
+
 int pr_plan_get(gg_graph *g, const PrPlan **out) {
   *out = nullptr;
   const int64_t hs = options().pr_hot_sources;
@@ -737,6 +744,7 @@ int pr_plan_get(gg_graph *g, const PrPlan **out) {
   const int bb = (int)options().pr_bin_bits, rb = (int)options().pr_range_bits;
   const uint32_t H =
       (uint32_t)std::min<int64_t>({hs, (int64_t)g->vertices, (int64_t)1 << (32 - bb)});
+  std::lock_guard<std::mutex> lk(g_plan_mu);
   PrPlan *p = g->pr_plan;
   if (p && (p->H != H || p->transposed != g->transposed || p->bin_bits != bb ||
             p->range_bits != rb)) {
