@@ -44,6 +44,68 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(const uint64_t *bits, s
   compact_words(bits, words, bin_deg, small_max, big_max, q0, q1, q2, cnt, visited, clear);
 }
 
+// The counts of a frontier without its queues: one lane per vertex (one
+// wave per bitmap word, degrees read coalesced), per-block sums added into
+// one of kCountSlots copies (atomics on one copy from every block would
+// serialise), summed on the host; the same side effects as the compaction
+// (visited OR, the other bitmap cleared).
+constexpr uint32_t kCountSlots = 256;
+__global__ __launch_bounds__(kBlock) void count_kernel(const uint64_t *bits, size_t words,
+                                                       uint32_t V, const uint32_t *bin_deg,
+                                                       uint32_t small_max, uint32_t big_max,
+                                                       Counters *slots, uint64_t *visited,
+                                                       uint64_t *clear) {
+  Counters *cnt = slots + blockIdx.x % kCountSlots;
+  __shared__ uint32_t s_n[3][kWavesPerBlock];
+  __shared__ unsigned long long s_e[kWavesPerBlock];
+  const int lane = lane_id(), wv = wave_in_block();
+  const size_t W = (size_t)gridDim.x * kWavesPerBlock;
+  uint32_t c0 = 0, c1 = 0, c2 = 0;
+  uint64_t es = 0;
+  for (size_t wi = global_wave(); wi < words; wi += W) {
+    const uint64_t word = bits[wi];
+    if (lane == 0) {
+      if (visited && word) visited[wi] |= word;
+      if (clear) clear[wi] = 0;
+    }
+    const uint32_t v = (uint32_t)(wi * 64 + lane);
+    if (((word >> lane) & 1ULL) && v < V) {
+      const uint32_t d = bin_deg[v];
+      es += d;
+      c0 += d <= small_max;
+      c1 += d > small_max && d <= big_max;
+      c2 += d > big_max;
+    }
+  }
+  const uint64_t w_e = wave_sum_u64(es);
+  const uint32_t w0 = (uint32_t)wave_sum_u64(c0), w1 = (uint32_t)wave_sum_u64(c1),
+                 w2 = (uint32_t)wave_sum_u64(c2);
+  if (lane == 0) {
+    s_n[0][wv] = w0;
+    s_n[1][wv] = w1;
+    s_n[2][wv] = w2;
+    s_e[wv] = w_e;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t b0 = 0, b1 = 0, b2 = 0;
+    unsigned long long be = 0;
+    for (int i = 0; i < kWavesPerBlock; i++) {
+      b0 += s_n[0][i];
+      b1 += s_n[1][i];
+      b2 += s_n[2][i];
+      be += s_e[i];
+    }
+    if (b0 | b1 | b2) {
+      if (b0) atomicAdd(&cnt->n[0], b0);
+      if (b1) atomicAdd(&cnt->n[1], b1);
+      if (b2) atomicAdd(&cnt->n[2], b2);
+      atomicAdd(&cnt->total, b0 + b1 + b2);
+      atomicAdd((unsigned long long *)&cnt->edges, be);
+    }
+  }
+}
+
 // Blocks for kernels that walk vertices in 256-aligned rounds.
 static unsigned grid_vertex_rounds(uint32_t V) { return grid1(V, 1u << 14); }
 
@@ -136,7 +198,11 @@ int work_get(gg_graph *g, Work **out) {
   size_t nh = std::max<size_t>(std::max(g->out.n_heavy, g->in.n_heavy), 1);
   if (r == GG_OK) r = al((void **)&w->hacc, nh * 8);
   if (r == GG_OK) r = al((void **)&w->exam, 1024 * 8);
+  if (r == GG_OK) r = al((void **)&w->cnt_slots, kCountSlots * sizeof(Counters));
   if (r == GG_OK && hipHostMalloc((void **)&w->h_cnt, sizeof(Counters) * 2) != hipSuccess)
+    r = fail(GG_ENOMEM, "pinned counters");
+  if (r == GG_OK &&
+      hipHostMalloc((void **)&w->h_cnt_slots, kCountSlots * sizeof(Counters)) != hipSuccess)
     r = fail(GG_ENOMEM, "pinned counters");
   if (r == GG_OK && hipHostMalloc((void **)&w->h_exam, 1024 * 8) != hipSuccess)
     r = fail(GG_ENOMEM, "pinned counters");
@@ -171,6 +237,7 @@ void work_free(Work *w) {
   device_free(w->ms_levels);
   msd_state_free(w->msd);
   if (w->h_cnt) hipHostFree(w->h_cnt);
+  if (w->h_cnt_slots) hipHostFree(w->h_cnt_slots);
   if (w->h_exam) hipHostFree(w->h_exam);
   delete w;
 }
@@ -184,6 +251,9 @@ struct Engine {
   gg_run_stats st{};
   uint32_t small_max, big_max;
   unsigned long long *exam;
+  // true: a frontier is counted first and compacted into queues only when
+  // the next iteration pushes (the loop reads the queues in push mode only)
+  bool lazy_queues = false;
 
   Engine(gg_graph *g_, Work *w_) : g(g_), w(w_) {
     small_max = (uint32_t)options().push_small_max;
@@ -200,6 +270,31 @@ struct Engine {
   // frontier's bitmap is cleared in the same pass: it is the next output.
   int compact(Frontier &f, const uint32_t *bin_deg, uint64_t *visited_or, Counters *outc) {
     Frontier &other = (&f == &w->f[0]) ? w->f[1] : w->f[0];
+    if (lazy_queues) {
+      GG_HIP(hipMemsetAsync(w->cnt_slots, 0, kCountSlots * sizeof(Counters), stream()));
+      {
+        ProfScope p("count");
+        hipLaunchKernelGGL(count_kernel, dim3(grid_waves(std::min<size_t>(w->words, 1u << 16))),
+                           dim3(kBlock), 0, stream(), f.bits, w->words, g->vertices, bin_deg,
+                           small_max, big_max, w->cnt_slots, visited_or, other.bits);
+      }
+      GG_HIP(hipMemcpyAsync(w->h_cnt_slots, w->cnt_slots, kCountSlots * sizeof(Counters),
+                            hipMemcpyDeviceToHost, stream()));
+      GG_HIP(hipStreamSynchronize(stream()));
+      GG_HIP(hipGetLastError());
+      Counters c{};
+      for (uint32_t k = 0; k < kCountSlots; k++) {
+        const Counters &x = w->h_cnt_slots[k];
+        c.n[0] += x.n[0];
+        c.n[1] += x.n[1];
+        c.n[2] += x.n[2];
+        c.total += x.total;
+        c.edges += x.edges;
+      }
+      *outc = c;
+      if (!sparse(outc->edges)) return GG_OK;  // a pull comes next: no queues
+      visited_or = nullptr;                    // (done by the count pass)
+    }
     GG_HIP(hipMemsetAsync(f.cnt, 0, sizeof(Counters), stream()));
     {
       ProfScope p("compact");
@@ -2158,6 +2253,7 @@ int gg_cc(gg_graph *g, uint32_t *d_label, uint32_t *h_frontier, int cap, gg_run_
   Work *w;
   GG_TRY(work_get(g, &w));
   Engine en(g, w);
+  en.lazy_queues = true;  // queues only before a push
   const Dir &out = out_dir(g), &in = in_dir(g);
   uint32_t V = g->vertices;
   DeviceTimer tm;
@@ -2222,6 +2318,7 @@ int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int
   Work *w;
   GG_TRY(work_get(g, &w));
   Engine en(g, w);
+  en.lazy_queues = true;  // queues only before a push
   const Dir &out = out_dir(g), &in = in_dir(g);
   uint32_t V = g->vertices;
   DeviceTimer tm;

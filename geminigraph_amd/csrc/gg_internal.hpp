@@ -228,6 +228,8 @@ struct Work {
   uint64_t pr_vals_cap = 0;
   uint64_t pr_gsum_cap = 0;
   Counters *h_cnt = nullptr;   // pinned host copy
+  Counters *cnt_slots = nullptr;    // spread partial counts of the count pass
+  Counters *h_cnt_slots = nullptr;  // pinned host copy of those
   unsigned long long *h_exam = nullptr;
   std::vector<void *> allocations;
 };
