@@ -220,6 +220,24 @@ def pagerank_extra(gg, g, V, E, iterations=20, runs=3):
     }
 
 
+def cc_extra(gg, scale, ef, seed, reorder):
+    """CC (cc.cpp) on the bench topology made undirected by reverse-edge
+    doubling: one warm call (builds the bin plan on a relabelled graph), then
+    one timed call."""
+    V, E = 1 << scale, (1 << scale) * ef
+    gu = gg.Graph.rmat(scale, ef, seed=seed, permute=True, undirected=True, reorder=reorder)
+    lab = gg.DeviceArray(V, np.uint32)
+    gu.cc_into(lab)
+    t = time.perf_counter()
+    st = gu.cc_into(lab)
+    gg.synchronize()
+    cc_s = time.perf_counter() - t
+    lab.free()
+    gu.close()
+    return {"ms": round(cc_s * 1e3, 3), "iterations": st.iterations,
+            "push_iterations": st.push_iterations, "gteps_2E": round(2 * E / cc_s / 1e9, 2)}
+
+
 def rmat26_extra(gg, seed, scale=26, ef=16, n_roots=8, iterations=20):
     """The scale-26 case: SSSP (plain method) from 8 seeded non-isolated
     roots on the weighted graph (float32 weights in (0,1]), then PageRank
@@ -411,6 +429,10 @@ def main():
                     help="run only the PageRank leg (for profiling) and print its figures")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched BFS/BC leg (for profiling) and print its figures")
+    ap.add_argument("--cc-only", action="store_true",
+                    help="run only the CC leg (for profiling) and print its figures")
+    ap.add_argument("--rmat26-only", action="store_true",
+                    help="run only the scale-26 SSSP/PageRank leg (for profiling)")
     args = ap.parse_args()
     if args.pmc_child:
         pmc_child(args)
@@ -431,6 +453,16 @@ def main():
         import geminigraph_amd as gg
         gg.init(0)
         log(f"batched: {batch_extra(gg, args.seed + 7, not args.no_reorder)}")
+        return
+    if args.cc_only:
+        import geminigraph_amd as gg
+        gg.init(0)
+        log(f"CC: {cc_extra(gg, args.scale, args.edge_factor, args.seed, not args.no_reorder)}")
+        return
+    if args.rmat26_only:
+        import geminigraph_amd as gg
+        gg.init(0)
+        log(f"rmat26: {rmat26_extra(gg, args.seed + 2)}")
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -538,20 +570,8 @@ def main():
         del g
         parent.free()
         depth.free()
-        gu = gg.Graph.rmat(scale, ef, seed=args.seed, permute=True, undirected=True,
-                           reorder=reorder)
-        lab = gg.DeviceArray(V, np.uint32)
-        gu.cc_into(lab)
-        t = time.perf_counter()
-        st = gu.cc_into(lab)
-        gg.synchronize()
-        cc_s = time.perf_counter() - t
-        extras["cc"] = {"ms": round(cc_s * 1e3, 3), "iterations": st.iterations,
-                        "push_iterations": st.push_iterations,
-                        "gteps_2E": round(2 * E / cc_s / 1e9, 2)}
+        extras["cc"] = cc_extra(gg, scale, ef, args.seed, reorder)
         log(f"CC: {extras['cc']}")
-        del gu
-        lab.free()
         extras["batched"] = batch_extra(gg, args.seed + 7, reorder)
         log(f"batched: {extras['batched']}")
         extras["rmat26"] = rmat26_extra(gg, args.seed + 2)

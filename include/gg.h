@@ -230,13 +230,20 @@ int gg_bc(gg_graph *g, uint32_t root, double *d_dep, double *d_inv_paths,
  * root per compute() (toolkits/bfs.cpp:22-109); each root's depths equal what
  * gg_bfs returns for it.  d_depth is vertex-major: d_depth[v * 64 + i] = level
  * of v from roots[i], 0xFFFFFFFF when unreached (64 entries per vertex
- * whatever n_roots is).  stats->m_reach sums over the roots. */
+ * whatever n_roots is).  stats->m_reach sums over the roots; the per-level
+ * records (modes, it_active, it_new) describe the union of the roots'
+ * frontiers.  The levels run on a device-driven loop (option
+ * "batch_device_loop"): the host never waits for a level before issuing the
+ * next. */
 int gg_bfs_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, uint32_t *d_depth,
                  gg_run_stats *stats);
 /* Up to 64 SSSP roots in one traversal (toolkits/sssp.cpp semantics per
  * root): one wave per vertex, lane i relaxes root i, so each edge moves one
  * 256-byte row of distances.  d_dist[v * 64 + i] (f32, 1e9 when unreached)
- * equals what gg_sssp returns for roots[i], bit for bit. */
+ * equals what gg_sssp returns for roots[i], bit for bit.  Relaxations within
+ * an iteration see each other, so the per-iteration records depend on
+ * timing; only the distances are fixed.  Iterations run on the device-driven
+ * loop like gg_bfs_batch's. */
 int gg_sssp_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, float *d_dist,
                   gg_run_stats *stats);
 /* Up to 64 BC roots in one traversal (toolkits/bc.cpp semantics per root):
