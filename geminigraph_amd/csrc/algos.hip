@@ -254,10 +254,12 @@ struct Engine {
   // true: a frontier is counted first and compacted into queues only when
   // the next iteration pushes (the loop reads the queues in push mode only)
   bool lazy_queues = false;
+  uint64_t divisor;  // the switch's divisor (option switch_divisor unless set)
 
   Engine(gg_graph *g_, Work *w_) : g(g_), w(w_) {
     small_max = (uint32_t)options().push_small_max;
     big_max = (uint32_t)options().push_big_max;
+    divisor = (uint64_t)options().switch_divisor;
     exam = options().count_examined ? w->exam : nullptr;
   }
   Frontier &fin() { return w->f[cur]; }
@@ -321,7 +323,7 @@ struct Engine {
     int64_t fm = options().force_mode;
     if (fm == 1) return true;
     if (fm == 2) return false;
-    return active_edges < g->edges / (uint64_t)options().switch_divisor;
+    return active_edges < g->edges / divisor;
   }
   int note_iteration(bool was_sparse) {
     int i = st.iterations;
@@ -2319,6 +2321,7 @@ int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int
   GG_TRY(work_get(g, &w));
   Engine en(g, w);
   en.lazy_queues = true;  // queues only before a push
+  en.divisor = (uint64_t)options().sssp_switch_divisor;
   const Dir &out = out_dir(g), &in = in_dir(g);
   uint32_t V = g->vertices;
   DeviceTimer tm;
