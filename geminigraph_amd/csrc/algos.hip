@@ -2273,10 +2273,13 @@ int gg_cc(gg_graph *g, uint32_t *d_label, uint32_t *h_frontier, int cap, gg_run_
   // active_in->fill() (cc.cpp:28), exact tail as in core/bitmap.hpp:42-52
   hipLaunchKernelGGL(bitmap_fill_kernel, dim3(grid1(w->words)), dim3(kBlock), 0, stream(),
                      en.fin().bits, w->words, V);
-  GG_TRY(en.compact_in(out.degree));
-  // relabelled graph: dense steps on the destination-bin plan (min in LDS)
+  // relabelled graph: dense steps on the destination-bin plan (min in LDS),
+  // which cost a few ps per arc: push longer (set before the first count,
+  // which decides whether to build queues with the loop's switch)
   const PrPlan *plan = nullptr;
   GG_TRY(pr_plan_get(g, &plan));
+  if (plan) en.divisor = (uint64_t)options().plan_switch_divisor;
+  GG_TRY(en.compact_in(out.degree));
   int lv = 0;
   while (en.in.total > 0) {
     if (h_frontier && lv < cap) h_frontier[lv] = en.in.total;
@@ -2321,7 +2324,6 @@ int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int
   GG_TRY(work_get(g, &w));
   Engine en(g, w);
   en.lazy_queues = true;  // queues only before a push
-  en.divisor = (uint64_t)options().sssp_switch_divisor;
   const Dir &out = out_dir(g), &in = in_dir(g);
   uint32_t V = g->vertices;
   DeviceTimer tm;
@@ -2335,12 +2337,15 @@ int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int
                      (float *)nullptr, 0.f, root);
   uint32_t *msg = (uint32_t *)w->msg;
   fill<uint32_t>(msg, kInfBits, V);
-  GG_TRY(en.seed(root, out.degree, nullptr));
   // relabelled graph: dense steps on the destination-bin plan (its arcs carry
-  // their weights)
+  // their weights), which cost a few ps per arc: push longer.  (Set before
+  // the first frontier is counted: the count decides whether to build queues
+  // with the same switch as the loop.)
   const PrPlan *plan = nullptr;
   GG_TRY(pr_plan_get(g, &plan));
   if (plan && !plan->hot_w) plan = nullptr;
+  if (plan) en.divisor = (uint64_t)options().plan_switch_divisor;
+  GG_TRY(en.seed(root, out.degree, nullptr));
   int lv = 0;
   while (en.in.total > 0) {
     if (h_frontier && lv < cap) h_frontier[lv] = en.in.total;

@@ -51,7 +51,7 @@ struct Options {
   // cheap relative to push atomics and the measured optimum for RMAT-24 BFS
   // lies between 200 and 2000 (results do not depend on the mode).
   int64_t switch_divisor = 200;
-  int64_t sssp_switch_divisor = 50;  // gg_sssp's own crossover (its dense steps are cheaper)
+  int64_t plan_switch_divisor = 50;  // CC / SSSP crossover when their dense steps use the plan
   int64_t force_mode = 0;        // 0 auto, 1 push, 2 pull
   int64_t push_small_max = 64;   // in-wave load-balanced bin
   int64_t push_big_max = 1024;   // one wave per vertex; above: shared by the grid

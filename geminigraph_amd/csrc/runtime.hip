@@ -191,7 +191,7 @@ static int64_t *option_slot(const char *name) {
   Options &o = options();
   if (!name) return nullptr;
   if (!strcmp(name, "switch_divisor")) return &o.switch_divisor;
-  if (!strcmp(name, "sssp_switch_divisor")) return &o.sssp_switch_divisor;
+  if (!strcmp(name, "plan_switch_divisor")) return &o.plan_switch_divisor;
   if (!strcmp(name, "force_mode")) return &o.force_mode;
   if (!strcmp(name, "push_small_max")) return &o.push_small_max;
   if (!strcmp(name, "push_big_max")) return &o.push_big_max;

@@ -62,9 +62,10 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  * default, 200, is the crossover measured on MI355X (results do not depend
  * on it).
  *   "switch_divisor"   push/pull crossover divisor (default 200)
- *   "sssp_switch_divisor" the same for gg_sssp (default 50: its dense steps
- *                      on the bin plan are cheaper per arc, so it pushes
- *                      longer); set both to 20 for the reference's switch
+ *   "plan_switch_divisor" the divisor gg_cc and gg_sssp use when their
+ *                      dense steps run on the bin plan (default 50: those
+ *                      cost a few ps per arc, so they push longer); set
+ *                      both divisors to 20 for the reference's switch
  *   "force_mode"       0 = auto, 1 = always push, 2 = always pull
  *   "push_small_max"   largest out-degree handled by in-wave load balancing
  *   "push_big_max"     largest out-degree handled by one wave per vertex

@@ -359,6 +359,31 @@ def test_sssp_plan_sizes(gg, bins, ranges, hot):
             assert np.array_equal(fr, ref_fr)
 
 
+def test_sssp_plan_hub_root(gg):
+    # a root whose out-arcs (1000) lie between E/200 and E/50: the first
+    # frontier is a push under the plan's divisor, so its queues must be built
+    # under that divisor too
+    scale = 13
+    V = 1 << scale
+    base = oracle.rmat_edges(scale, 12, seed=12, weighted=True)
+    rng = np.random.default_rng(12)
+    hub = V  # an extra vertex
+    dst = rng.choice(V, 1000, replace=False)
+    pairs = np.concatenate([np.stack([base["src"], base["dst"]], 1),
+                            np.stack([np.full(1000, hub), dst], 1)]).astype(np.uint32)
+    w = np.concatenate([base["w"], (np.arange(1000) % 7 + 1).astype(np.float32) / 8])
+    e = oracle.edges_from_pairs(pairs, w)
+    E = len(e)
+    assert E // 200 < 1000 < E // 50
+    g = gg.Graph.build(e, V + 1, reorder=True)
+    o = oracle.Graph(e, V + 1)
+    dist, fr, st = g.sssp(hub)
+    ref, ref_fr = o.sssp(hub)
+    assert np.array_equal(dist.view(np.uint32), ref.view(np.uint32))
+    assert np.array_equal(fr, ref_fr)
+    assert st["modes"][0] == 0  # pushed
+
+
 # ----------------------------------------------------------------------- BC
 
 @pytest.mark.parametrize("mode", ["auto", "push", "pull"])
