@@ -513,19 +513,17 @@ struct CcPull : NoAfter {  // phase 1 of the dense step: gather the minimum in-l
     return false;
   }
 };
-// The distances at the start of a dense SSSP step on the plan, as f32 bits
-// in 64-bit values (the plan's value type).
-__global__ void sssp_snapshot_kernel(const float *dist, uint64_t *snap, uint32_t V) {
+// The distances at the start of a dense SSSP step on the plan (f32 bits).
+__global__ void sssp_snapshot_kernel(const float *dist, uint32_t *snap, uint32_t V) {
   const uint32_t s = gridDim.x * blockDim.x;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s)
-    snap[v] = (uint64_t)__float_as_uint(dist[v]);
+    snap[v] = __float_as_uint(dist[v]);
 }
-// The labels at the start of a dense step on the plan, complemented and
-// widened to 64 bits (the plan's value type).
-__global__ void cc_snapshot_kernel(const uint32_t *label, uint64_t *nlab, uint32_t V) {
+// The labels at the start of a dense CC step on the plan, complemented.
+__global__ void cc_snapshot_kernel(const uint32_t *label, uint32_t *nlab, uint32_t V) {
   const uint32_t s = gridDim.x * blockDim.x;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s)
-    nlab[v] = ~(uint64_t)label[v];
+    nlab[v] = ~label[v];
 }
 // phase 2: dense_slot (cc.cpp:72-79), one lane per vertex.
 __global__ void cc_apply_kernel(uint32_t *label, uint32_t *msg, uint32_t V, uint64_t *out) {
@@ -2293,7 +2291,7 @@ int gg_cc(gg_graph *g, uint32_t *d_label, uint32_t *h_frontier, int cap, gg_run_
       op.label = d_label;
       GG_TRY(en.launch_push(out, op, true));
     } else if (plan) {
-      uint64_t *nlab = (uint64_t *)w->msg;  // V x 8 bytes
+      uint32_t *nlab = (uint32_t *)w->msg;
       hipLaunchKernelGGL(cc_snapshot_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), d_label,
                          nlab, V);
       GG_TRY(cc_plan_iteration(g, plan, w, nlab, d_label, en.fout().bits));
@@ -2359,7 +2357,7 @@ int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int
       op.dist = d_dist;
       GG_TRY(en.launch_push(out, op, true));
     } else if (plan) {
-      uint64_t *snap = (uint64_t *)w->msg;  // V x 8 bytes
+      uint32_t *snap = (uint32_t *)w->msg;
       hipLaunchKernelGGL(sssp_snapshot_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), d_dist,
                          snap, V);
       GG_TRY(sssp_plan_iteration(g, plan, w, snap, d_dist, en.fout().bits));

@@ -261,15 +261,15 @@ int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull 
 // is a bitmap of those vertices (else vis is tested per bin).
 constexpr uint32_t kGateDense = 1;
 // One dense CC step on the plan: label[v] = min(label[v], min over v's
-// in-neighbours u of ~nlab[u]) with nlab = ~label (64-bit) copied at the
-// start of the step; lowered vertices go to `out` (every word is written).
-int cc_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *nlab,
+// in-neighbours u of ~nlab[u]) with nlab = ~label copied at the start of the
+// step; lowered vertices go to `out` (every word is written).
+int cc_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint32_t *nlab,
                       uint32_t *label, uint64_t *out);
 // One dense SSSP step on the plan of a weighted graph: dist[v] is lowered to
 // the minimum over v's in-arcs of fl(snap[u] + w) when that is below 1e9,
-// where snap holds the distances' f32 bits (low 32 bits) at the step's start;
-// lowered vertices go to `out` (every word is written).
-int sssp_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *snap,
+// where snap holds the distances' f32 bits at the step's start; lowered
+// vertices go to `out` (every word is written).
+int sssp_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint32_t *snap,
                         float *dist, uint64_t *out);
 int ms_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *fcur,
                       const uint64_t *vis, uint64_t full, uint64_t *fnext, uint64_t *out,

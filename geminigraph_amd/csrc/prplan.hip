@@ -144,12 +144,16 @@ __global__ void pr_cold_split_kernel(const uint64_t *keys, uint64_t n, int bb, i
 // BFS ORs 64-root frontier masks.  A BinOp gives the value type T, the
 // source value, the LDS and global combining atomics, and the per-vertex
 // finish (which reports whether the vertex joins the output frontier).  With
+// VT is how a source value is stored (the per-vertex input and the emitted
+// cold values), widen(VT) gives the combining type T.  With
 // kSkipDone, done_mask(v0, valid) gives the wave's 64 destinations v0 ..
 // v0+63 (v0 a multiple of 64; lanes with valid false are not read) whose
 // result is already settled: the bin kernel then reads no source value for
 // their arcs.
 struct PrSumOp {
   using T = double;
+  using VT = double;
+  __device__ static T widen(VT v) { return v; }
   static constexpr bool kSkipDone = false;
   static constexpr bool kWeighted = false;  // true: relax(value, weight) per arc
   PrPull op;
@@ -166,6 +170,8 @@ struct PrSumOp {
 };
 struct MsOrOp {
   using T = uint64_t;
+  using VT = uint64_t;
+  __device__ static T widen(VT v) { return v; }
   static constexpr bool kSkipDone = true;  // every root has reached the vertex
   static constexpr bool kWeighted = false;
   const uint64_t *fcur, *vis;
@@ -193,14 +199,17 @@ struct MsOrOp {
 // CC (cc.cpp:57-79): the minimum label over each vertex's in-neighbours,
 // taken from the labels at the start of the step (BSP: the finish writes the
 // label array, the gathers read the copy).  Values are stored complemented,
-// ~label as 64 bits, and combined by max, so 0 (the plan's initial
-// accumulator and split-bin row) is the identity; the finish lowers a label
-// and reports the vertex for the output frontier.
+// ~label (32 bits, never 0 as labels are below 2^32 - 1), and combined by
+// max, so 0 (the plan's initial accumulator and split-bin row) is the
+// identity; the finish lowers a label and reports the vertex for the output
+// frontier.
 struct CcMinOp {
   using T = uint64_t;
+  using VT = uint32_t;
+  __device__ static T widen(VT v) { return v; }
   static constexpr bool kSkipDone = false;
   static constexpr bool kWeighted = false;
-  const uint64_t *nlab;  // ~label at the start of the step, per vertex
+  const uint32_t *nlab;  // ~label at the start of the step, per vertex
   uint32_t *label;
   __device__ uint64_t done_mask(uint32_t, bool) const { return 0; }
   __device__ T value(uint32_t u) const { return nlab[u]; }
@@ -212,7 +221,7 @@ struct CcMinOp {
   }
   __device__ bool finish(uint32_t v, T acc, double &) const {
     if (!acc) return false;  // no in-arcs
-    const uint32_t m = (uint32_t)~acc;
+    const uint32_t m = ~(uint32_t)acc;
     if (m >= label[v]) return false;
     label[v] = m;
     return true;
@@ -226,9 +235,11 @@ struct CcMinOp {
 // finish applies the slot: lowered below 1e9 and below dist[v].
 struct SsspMinOp {
   using T = uint64_t;
+  using VT = uint32_t;  // the f32 bits
+  __device__ static T widen(VT v) { return v; }
   static constexpr bool kSkipDone = false;
   static constexpr bool kWeighted = true;
-  const uint64_t *snap;
+  const uint32_t *snap;
   float *dist;
   __device__ uint64_t done_mask(uint32_t, bool) const { return 0; }
   __device__ T value(uint32_t u) const { return snap[u]; }
@@ -253,14 +264,15 @@ struct SsspMinOp {
 
 // Emit: part p = arcs [lo, hi) of range r.  The range's values go to LDS
 // (one coalesced read), then every arc's value is written in storage order.
-// Values are copied as 8-byte words.
+// Values are copied as VT words (8 or 4 bytes).
+template <class VT>
 __global__ __launch_bounds__(kPrBlock) void pr_emit_kernel(const uint32_t *gate,
-                                                           const uint32_t *parts,
-                                                           const uint64_t *curr,
+                                                           const uint32_t *parts, const VT *curr,
                                                            const uint16_t *src16, uint32_t H,
-                                                           uint32_t V, int rb, uint64_t *vals) {
+                                                           uint32_t V, int rb, VT *vals) {
   if (gate && *gate != kGateDense) return;
-  extern __shared__ uint64_t val[];  // 2^rb values
+  extern __shared__ __align__(8) unsigned char val_raw[];
+  VT *val = reinterpret_cast<VT *>(val_raw);  // 2^rb values
   const uint32_t r = parts[3 * blockIdx.x], lo = parts[3 * blockIdx.x + 1],
                  hi = parts[3 * blockIdx.x + 2];
   const uint32_t u0 = H + (r << rb);
@@ -291,7 +303,7 @@ template <class Op>
 __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
     const uint32_t *gate, const float *hot_w, const float *cold_w, const PrSeg *segs,
     const uint32_t *hot, const uint32_t *run_beg, const uint32_t *run_end,
-    uint32_t nranges, const uint16_t *dst16, const typename Op::T *vals, uint32_t V, int bb,
+    uint32_t nranges, const uint16_t *dst16, const typename Op::VT *vals, uint32_t V, int bb,
     Op op, typename Op::T *gsum, uint64_t *out, double *dacc) {
   using T = typename Op::T;
   if (gate && *gate != kGateDense) return;
@@ -395,12 +407,13 @@ __global__ __launch_bounds__(kPrBlock) void pr_bin_kernel(
 #pragma unroll
           for (int t = 0; t < kU; t++) d[t] = stream_load(&dst16[pos[t]]);
 #pragma unroll
-          for (int t = 0; t < kU; t++) x[t] = settled(d[t]) ? T(0) : stream_load(&vals[pos[t]]);
+          for (int t = 0; t < kU; t++)
+            x[t] = settled(d[t]) ? T(0) : Op::widen(stream_load(&vals[pos[t]]));
         } else {
 #pragma unroll
           for (int t = 0; t < kU; t++) {
             d[t] = stream_load(&dst16[pos[t]]);
-            x[t] = stream_load(&vals[pos[t]]);
+            x[t] = Op::widen(stream_load(&vals[pos[t]]));
           }
         }
         if constexpr (Op::kWeighted) {
@@ -784,21 +797,25 @@ static int plan_buffers(const PrPlan *p, Work *w) {
   return GG_OK;
 }
 
-// One pass of the plan: emit the cold values of `src` (8-byte values per
-// vertex), then the bins, then the split bins.
+// One pass of the plan: emit the cold values of `src` (one VT per vertex),
+// then the bins, then the split bins.
 template <class Op>
-static int plan_pass(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *src, const Op &op,
-                     uint64_t *out, double *dacc, const uint32_t *gate, const char *names[3]) {
+static int plan_pass(const gg_graph *g, const PrPlan *p, Work *w, const typename Op::VT *src,
+                     const Op &op, uint64_t *out, double *dacc, const uint32_t *gate,
+                     const char *names[3]) {
   using T = typename Op::T;
-  static_assert(sizeof(T) == 8, "plan values are 8 bytes");
+  using VT = typename Op::VT;
+  static_assert(sizeof(T) == 8, "plan accumulators are 8 bytes");
+  static_assert(sizeof(VT) <= 8, "plan values fit the 8-byte cold buffer");
   const uint32_t V = g->vertices;
   GG_TRY(plan_buffers(p, w));
-  T *vals = reinterpret_cast<T *>(w->pr_vals), *gsum = reinterpret_cast<T *>(w->pr_gsum);
+  VT *vals = reinterpret_cast<VT *>(w->pr_vals);
+  T *gsum = reinterpret_cast<T *>(w->pr_gsum);
   if (p->nparts) {
     ProfScope ps(names[0]);
-    hipLaunchKernelGGL(pr_emit_kernel, dim3(p->nparts), dim3(kPrBlock), (size_t)8 << p->range_bits,
-                       stream(), gate, p->parts, src, p->src16, p->H, V, p->range_bits,
-                       reinterpret_cast<uint64_t *>(vals));
+    hipLaunchKernelGGL(pr_emit_kernel<VT>, dim3(p->nparts), dim3(kPrBlock),
+                       sizeof(VT) << p->range_bits, stream(), gate, p->parts, src, p->src16, p->H,
+                       V, p->range_bits, vals);
   }
   if (p->nseg) {
     ProfScope ps(names[1]);
@@ -819,17 +836,16 @@ static int plan_pass(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t
 int pr_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const PrPull &op,
                       double *dacc) {
   static const char *names[3] = {"pr_emit", "pr_bins", "pr_split_finish"};
-  return plan_pass(g, p, w, reinterpret_cast<const uint64_t *>(op.curr), PrSumOp{op}, nullptr,
-                   dacc, nullptr, names);
+  return plan_pass(g, p, w, op.curr, PrSumOp{op}, nullptr, dacc, nullptr, names);
 }
 
-int cc_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *nlab,
+int cc_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint32_t *nlab,
                       uint32_t *label, uint64_t *out) {
   static const char *names[3] = {"cc_emit", "cc_bins", "cc_split_finish"};
   return plan_pass(g, p, w, nlab, CcMinOp{nlab, label}, out, nullptr, nullptr, names);
 }
 
-int sssp_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint64_t *snap,
+int sssp_plan_iteration(const gg_graph *g, const PrPlan *p, Work *w, const uint32_t *snap,
                         float *dist, uint64_t *out) {
   if (!p->hot_w || (p->n_cold && !p->cold_w))
     return fail(GG_EINVAL, "SSSP plan step: the plan has no arc weights");
