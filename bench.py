@@ -333,7 +333,9 @@ def cpu_baseline(scale, ef, seed, roots, gpu_depths, sample_roots):
     }, parity
 
 
-PULL_KERNELS = ("gg::bfs_pull_kernel", "gg::bfs_pull_heavy_kernel")
+# the edge-processing launches of every BFS iteration: the bottom-up step, and
+# the push (push iterations) or the heavy in-list chunks (pull iterations)
+STEP_KERNELS = ("gg::bfs_pull_kernel", "gg::bfs_push_heavy_kernel")
 COPY_KERNEL = "gg::gg_copy_kernel"
 COPY_BYTES = 256 << 20
 
@@ -356,7 +358,7 @@ def pmc_child(args):
 
 
 def measure_traffic(args):
-    """HBM bytes per launch of the dense BFS kernels, from two rocprofv3
+    """HBM bytes per iteration of the BFS step kernels, from two rocprofv3
     counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each
     scaled by what the same counter reads for the plain copy kernel moving
     COPY_BYTES each way.  Returns (bytes per launch or None, detail)."""
@@ -391,9 +393,9 @@ def measure_traffic(args):
         for row in csv.DictReader(open(path)):
             name = row["Kernel_Name"].split("(")[0].replace("void ", "")
             v = float(row["Counter_Value"])
-            if name in PULL_KERNELS:
+            if name in STEP_KERNELS:
                 pull += v
-                pull_n += name == PULL_KERNELS[0]
+                pull_n += name == STEP_KERNELS[0]
             elif name == COPY_KERNEL:
                 copy = v
         if not copy or not pull_n:
@@ -508,7 +510,7 @@ def main():
     gg.prof_enable(True)
     step()
     gg.prof_enable(False)
-    kernels = {k: gg.prof_read(k) for k in ("bfs_pull", "bfs_pull_heavy", "push", "compact")}
+    kernels = {k: gg.prof_read(k) for k in ("bfs_pull", "bfs_push_heavy", "compact")}
     log(f"BFS x{len(roots)}: {ms_per_step:.2f} ms/step, {gteps:.2f} GTEPS; kernels {kernels}")
 
     # one instrumented pass (edge counters on) for the algorithmic byte model
@@ -524,21 +526,26 @@ def main():
             pull_b += lb
             if len(gpu_depths) < args.cpu_sample_roots:
                 gpu_depths.append(depth.numpy())
-    # The pull kernels are launched every iteration and return at once in push
-    # iterations; per launch = per BFS-pull-kernel dispatch, as rocprof counts.
-    pull_ms = kernels["bfs_pull"][0] + kernels["bfs_pull_heavy"][0]
-    pull_launches = kernels["bfs_pull"][1]
-    achieved = pull_b / (pull_ms / 1e3) / 1e9 if pull_ms > 0 else 0.0
+    # Both step kernels are launched once per iteration: bfs_pull_kernel works
+    # in pull iterations and returns at once in push ones; bfs_push_heavy_kernel
+    # runs the push, or the heavy in-list chunks of a pull.  Per launch = per
+    # iteration (per bfs_pull_kernel dispatch, as rocprof counts); the bytes
+    # are the model's for every iteration, push and pull.
+    step_ms = kernels["bfs_pull"][0] + kernels["bfs_push_heavy"][0]
+    step_launches = kernels["bfs_pull"][1]
+    step_b = push_b + pull_b
+    achieved = step_b / (step_ms / 1e3) / 1e9 if step_ms > 0 else 0.0
     roofline = {
         "bound": "hbm",
-        "kernel": "bfs_pull_kernel + bfs_pull_heavy_kernel (dense BFS step)",
+        "kernel": "bfs_pull_kernel + bfs_push_heavy_kernel (the edge-processing step of every "
+                  "BFS iteration: bottom-up, or push; heavy in-lists)",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
-        "bytes_per_launch": round(pull_b / max(1, pull_launches), 1),
-        "avg_launch_ms": round(kernels["bfs_pull"][0] / max(1, pull_launches), 4),
+        "bytes_per_launch": round(step_b / max(1, step_launches), 1),
+        "avg_launch_ms": round(step_ms / max(1, step_launches), 4),
     }
 
     if not args.no_pmc:

@@ -1579,7 +1579,8 @@ __global__ __launch_bounds__(kCompactBlock) void msd_bin_kernel(const MsCtl *c, 
                                                                 uint32_t *q1, uint32_t *q2,
                                                                 Counters *qc) {
   if (c->mode != kMsPush) return;
-  compact_words_spread(fin, words, deg, small_max, big_max, q0, q1, q2, qc, nullptr, nullptr);
+  compact_words_spread(fin, words, deg, small_max, big_max, q0, q1, q2, qc, nullptr, nullptr,
+                       gridDim.x);
 }
 
 struct MsdPushArgs {

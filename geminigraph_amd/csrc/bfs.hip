@@ -1,19 +1,21 @@
 // BFS (toolkits/bfs.cpp) with its iteration loop driven from the device.
 //
-// Every iteration is the same fixed sequence of launches, issued ahead of
-// time on one stream:
-//   decide -- one block: reduces the size and out-degree sum of the frontier
-//             (counted by the kernels that produced it), applies the
-//             push/pull switch (core/graph.hpp:1503-1509), records the
-//             iteration's statistics, or marks the traversal done;
-//   bin    -- push only: frontier bitmap -> three degree-binned queues, and
-//             clears the output bitmap;
-//   push   -- push only: a persistent grid over the queues (bfs.cpp:49-58);
-//   pull   -- pull only: bottom-up over 64-vertex groups, then heavy chunks
-//             (bfs.cpp:60-76).
-// Launches of the mode not chosen return at once, so the host never waits
-// for an iteration before issuing the next one: it stays one iteration ahead
-// and reads each iteration's done flag through an event-ordered copy.
+// Every iteration is the same fixed sequence of four launches, issued ahead
+// of time on one stream:
+//   decide     -- one block: reduces the size and out-degree sum of the
+//                 frontier (counted by the kernels that produced it),
+//                 applies the push/pull switch (core/graph.hpp:1503-1509),
+//                 records the iteration's statistics, or marks the
+//                 traversal done;
+//   bin        -- push only: frontier bitmap -> three degree-binned queues,
+//                 and clears the output bitmap;
+//   pull       -- pull only: bottom-up over 64-vertex groups (bfs.cpp:60-76);
+//   push_heavy -- push: a persistent grid over the queues (bfs.cpp:49-58);
+//                 pull: the chunks of the heavy in-lists.
+// Each launch runs the step of the mode the decide chose or returns at once,
+// so the host never waits for an iteration before issuing the next one: it
+// stays one iteration ahead and polls the decision the device writes to
+// coherent host memory.
 //
 // Claims: a push edge, and each chunk of a heavy in-list, claims its vertex
 // with a compare-and-swap on the vertex's depth entry, which sits at its
@@ -22,7 +24,7 @@
 // (bfs.cpp:53, 71); each way keeps exactly one valid parent.  Results are
 // written in original vertex ids as vertices are found (parent[iperm[v]] =
 // iperm[u]), so a relabelled graph needs no pass over V at the end.  A
-// frontier joins `visited` one iteration later, in the kernel that reads it
+// frontier joins `visited` one iteration later, in the step that reads it
 // next (bin for push, the group pull for pull); until then the readers test
 // visited | frontier.
 #include <string.h>
@@ -254,16 +256,16 @@ __global__ __launch_bounds__(kBlock) void bfs_decide_kernel(BfsCtl *c, uint64_t 
   bfs_decide(c, set, qc, exam, lv, hf);
 }
 
-__global__ __launch_bounds__(kCompactBlock) void bfs_bin_kernel(const BfsCtl *c, const uint64_t *fin,
-                                                         uint64_t *fout, uint64_t *visited,
-                                                         size_t words,
-                                                         const uint32_t *deg, uint32_t small_max,
-                                                         uint32_t big_max, uint32_t *q0,
-                                                         uint32_t *q1, uint32_t *q2,
-                                                         Counters *qc) {
-  if (c->mode != kBfsPush) return;
-  compact_words_spread(fin, words, deg, small_max, big_max, q0, q1, q2, qc, visited, fout);
-}
+// Frontier binning of a push iteration: bitmap -> three degree-binned
+// queues, the frontier joins visited, the output bitmap is cleared.
+struct BfsBinArgs {
+  const uint64_t *fin;
+  size_t words;
+  const uint32_t *deg;
+  uint32_t small_max, big_max;
+  uint32_t *q0, *q1, *q2;
+  Counters *qc;
+};
 
 struct BfsPushArgs {
   const uint64_t *index;
@@ -275,12 +277,12 @@ struct BfsPushArgs {
 // Persistent grid over the three bins: 64 small vertices per wave step
 // (in-wave load balancing), one mid vertex per wave step, and chunks of every
 // huge vertex shared by all waves.
-__global__ __launch_bounds__(kBlock) void bfs_push_kernel(const BfsCtl *c, BfsOut o,
-                                                          BfsPushArgs a, uint64_t *set_out,
-                                                          unsigned long long *exam) {
-  if (c->mode != kBfsPush) return;
+// W waves take part (kBlock threads per block); gw is this wave's index.
+__device__ __forceinline__ void bfs_push_step(const BfsCtl *c, const BfsOut &o,
+                                              const BfsPushArgs &a, uint64_t *set_out,
+                                              unsigned long long *exam, uint32_t W,
+                                              uint32_t gw) {
   const uint32_t level = c->level;
-  const uint32_t W = gridDim.x * kWavesPerBlock, gw = global_wave();
   const int lane = lane_id();
   const uint32_t n0 = a.qc->n[0], n1 = a.qc->n[1], n2 = a.qc->n[2];
   const uint32_t waves0 = (n0 + 63) / 64;
@@ -405,17 +407,17 @@ struct BfsPullArgs {
 // Short lists are scanned by their own lane, longer ones by the whole wave
 // 64 entries at a time; lists above light_max are left to the chunk kernel.
 // The wave owns its group's words of visited and of the output frontier
-// (the chunk kernel runs after it and only ORs bits in).
-__global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOut o,
-                                                          BfsPullArgs a, uint64_t *set_out,
-                                                          unsigned long long *exam) {
-  if (c->mode != kBfsPull) return;
+// (the chunk step runs after it, in the next launch, and only ORs bits in).
+// W waves take part, of any block size; gw is this wave's index.
+__device__ __forceinline__ void bfs_pull_step(const BfsCtl *c, const BfsOut &o,
+                                              const BfsPullArgs &a, uint64_t *set_out,
+                                              unsigned long long *exam, uint32_t W,
+                                              uint32_t gw) {
   const uint32_t level = c->level;
-  const uint32_t W = gridDim.x * kWavesPerBlock;
   const int lane = lane_id();
   uint64_t cnt = 0, edg = 0, ind = 0, examined = 0;
   // the next group's bitmap words are loaded while this group is processed
-  uint32_t g = global_wave();
+  uint32_t g = gw;
   uint64_t vis_n = 0, fin_n = 0;
   if (g < a.ngroups) {
     vis_n = o.visited[g];
@@ -511,7 +513,7 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOu
       cnt += (uint64_t)__popcll(m);
     }
   }
-  bfs_wave_flush(set_out, global_wave(), cnt, edg, ind, exam, examined);
+  bfs_wave_flush(set_out, gw, cnt, edg, ind, exam, examined);
 }
 
 struct BfsHeavyArgs {
@@ -525,17 +527,16 @@ struct BfsHeavyArgs {
 // In-lists above light_max, one wave per chunk (grid-stride); chunks of one
 // vertex race and the compare-and-swap on its depth keeps one parent.  The
 // chunks poll that depth entry to stop early once another chunk has won.
-__global__ __launch_bounds__(kBlock) void bfs_pull_heavy_kernel(const BfsCtl *c, BfsOut o,
-                                                                BfsHeavyArgs a, uint64_t *set_out,
-                                                                unsigned long long *exam) {
-  if (c->mode != kBfsPull) return;
+__device__ __forceinline__ void bfs_heavy_step(const BfsCtl *c, const BfsOut &o,
+                                               const BfsHeavyArgs &a, uint64_t *set_out,
+                                               unsigned long long *exam, uint32_t W,
+                                               uint32_t gw) {
   const uint32_t level = c->level;
-  const uint32_t W = gridDim.x * kWavesPerBlock;
   const int lane = lane_id();
   uint64_t cnt = 0, edg = 0, ind = 0, examined = 0;
   // lane j tests chunk base + j*W (the chunks of one vertex stay spread over
   // the waves), then the wave scans the chunks whose vertex is still open
-  for (uint64_t base = global_wave(); base < a.n_chunks; base += (uint64_t)W * 64) {
+  for (uint64_t base = gw; base < a.n_chunks; base += (uint64_t)W * 64) {
     const uint64_t mine = base + (uint64_t)lane * W;
     uint32_t vl = 0;
     bool open = false;
@@ -575,7 +576,44 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_heavy_kernel(const BfsCtl *c,
     }
   }
   if (lane != 0) examined = 0;  // counted once per wave
-  bfs_wave_flush(set_out, global_wave(), cnt, edg, ind, exam, examined);
+  bfs_wave_flush(set_out, gw, cnt, edg, ind, exam, examined);
+}
+
+// Launches of an iteration besides the decide.  A launch of the mode not
+// chosen returns at once but still costs a kernel boundary, so the push and
+// the heavy chunks, which have the same block shape and register needs,
+// share one launch; the binning and the bottom-up step stay apart (the
+// binning needs 84 VGPRs, which would cut the step's occupancy from 8 waves
+// per SIMD to 5, measured 8% slower as one launch).
+//   bin        -- push: the frontier binning;
+//   pull       -- pull: the bottom-up step;
+//   push_heavy -- push: the persistent push over the queues (push_blocks);
+//                 pull: the heavy in-list chunks (heavy_blocks).
+__global__ __launch_bounds__(kCompactBlock) void bfs_bin_kernel(const BfsCtl *c, BfsOut o,
+                                                               BfsBinArgs b) {
+  if (c->mode != kBfsPush) return;
+  compact_words_spread(b.fin, b.words, b.deg, b.small_max, b.big_max, b.q0, b.q1, b.q2, b.qc,
+                       o.visited, o.fout, gridDim.x);
+}
+
+__global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOut o,
+                                                          BfsPullArgs a, uint64_t *set_out,
+                                                          unsigned long long *exam) {
+  if (c->mode != kBfsPull) return;
+  bfs_pull_step(c, o, a, set_out, exam, gridDim.x * kWavesPerBlock, global_wave());
+}
+
+// Blocks past the mode's count return at once.
+__global__ __launch_bounds__(kBlock) void bfs_push_heavy_kernel(
+    const BfsCtl *c, BfsOut o, BfsPushArgs p, BfsHeavyArgs h, uint64_t *set_out,
+    unsigned long long *exam, uint32_t push_blocks, uint32_t heavy_blocks) {
+  const uint32_t mode = c->mode;
+  if (mode == kBfsPush) {
+    if (blockIdx.x < push_blocks)
+      bfs_push_step(c, o, p, set_out, exam, push_blocks * kWavesPerBlock, global_wave());
+  } else if (mode == kBfsPull && h.n_chunks && blockIdx.x < heavy_blocks) {
+    bfs_heavy_step(c, o, h, set_out, exam, heavy_blocks * kWavesPerBlock, global_wave());
+  }
 }
 
 // Device state of the BFS loop, kept in the traversal context.
@@ -658,13 +696,14 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
                      in.vinfo, V, (int32_t)opt.force_mode, threshold, g->arcs,
                      (uint64_t)opt.bfs_alpha, s->lv_cap);
   // fixed launch shapes: the frontier sizes live on the device
-  const unsigned bin_grid = compact_grid(words);
-  const unsigned push_grid = grid_waves((uint64_t)opt.huge_grid_waves);
+  const unsigned bin_blocks = compact_grid(words);
   const uint32_t ngroups = (V + 63) / 64;
   const uint64_t pw = (uint64_t)opt.bfs_pull_waves;
   const unsigned pull_grid = grid_waves(pw ? std::min<uint64_t>(ngroups, pw) : ngroups);
+  const unsigned push_blocks = grid_waves((uint64_t)opt.huge_grid_waves);
   const uint64_t nch = std::max(in.n_chunks, 1u);
-  const unsigned heavy_grid = grid_waves(pw ? std::min<uint64_t>(nch, pw) : nch);
+  const unsigned heavy_blocks = grid_waves(pw ? std::min<uint64_t>(nch, pw) : nch);
+  const unsigned grid_b = std::max(push_blocks, in.n_chunks ? heavy_blocks : 1u);
   for (uint32_t n = 0;; n++) {
     // keep one iteration queued: before issuing iteration n, wait until
     // iteration n-1 has been decided (so iteration n-2 has finished)
@@ -683,30 +722,24 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
     BfsOut o{d_parent, d_depth, in.vinfo, w->visited, fout};
     {
       ProfScope p("compact");
-      hipLaunchKernelGGL(bfs_bin_kernel, dim3(bin_grid), dim3(kCompactBlock), 0, st, s->ctl, fin, fout,
-                         w->visited, words, out.degree, (uint32_t)opt.push_small_max,
-                         (uint32_t)opt.push_big_max, w->f[0].q[0], w->f[0].q[1], w->f[0].q[2],
-                         s->qc);
-    }
-    {
-      ProfScope p("push");
-      BfsPushArgs a{out.index, out.nbr, w->f[0].q[0], w->f[0].q[1], w->f[0].q[2], s->qc};
-      hipLaunchKernelGGL(bfs_push_kernel, dim3(push_grid), dim3(kBlock), 0, st, s->ctl, o, a,
-                         set_out, exam);
+      BfsBinArgs b{fin, words, out.degree, (uint32_t)opt.push_small_max,
+                   (uint32_t)opt.push_big_max, w->f[0].q[0], w->f[0].q[1], w->f[0].q[2], s->qc};
+      hipLaunchKernelGGL(bfs_bin_kernel, dim3(bin_blocks), dim3(kCompactBlock), 0, st, s->ctl, o,
+                         b);
     }
     {
       ProfScope p("bfs_pull");
       BfsPullArgs a{in.index, in.nbr, in.first_orig, fin, V, in.light_max,
-                    (uint32_t)opt.pull_serial_max,
-                    ngroups};
+                    (uint32_t)opt.pull_serial_max, ngroups};
       hipLaunchKernelGGL(bfs_pull_kernel, dim3(pull_grid), dim3(kBlock), 0, st, s->ctl, o, a,
                          set_out, exam);
     }
-    if (in.n_chunks) {
-      ProfScope p("bfs_pull_heavy");
-      BfsHeavyArgs a{in.nbr, in.chunk_v, in.chunk_beg, in.chunk_end, fin, in.n_chunks};
-      hipLaunchKernelGGL(bfs_pull_heavy_kernel, dim3(heavy_grid), dim3(kBlock), 0, st, s->ctl, o,
-                         a, set_out, exam);
+    {
+      ProfScope p("bfs_push_heavy");
+      BfsPushArgs a{out.index, out.nbr, w->f[0].q[0], w->f[0].q[1], w->f[0].q[2], s->qc};
+      BfsHeavyArgs h{in.nbr, in.chunk_v, in.chunk_beg, in.chunk_end, fin, in.n_chunks};
+      hipLaunchKernelGGL(bfs_push_heavy_kernel, dim3(grid_b), dim3(kBlock), 0, st, s->ctl, o, a,
+                         h, set_out, exam, push_blocks, heavy_blocks);
     }
     hipLaunchKernelGGL(bfs_decide_kernel, dim3(1), dim3(kBlock), 0, st, s->ctl, set_out, s->qc,
                        exam, s->lv, s->h_flag);

@@ -292,15 +292,17 @@ __device__ __forceinline__ void compact_words_spread(const uint64_t *bits, size_
                                                      const uint32_t *bin_deg, uint32_t small_max,
                                                      uint32_t big_max, uint32_t *q0, uint32_t *q1,
                                                      uint32_t *q2, Counters *cnt,
-                                                     uint64_t *visited, uint64_t *clear) {
+                                                     uint64_t *visited, uint64_t *clear,
+                                                     uint32_t nb) {
   __shared__ uint32_t s_tot[3][kCompactWaves];
   __shared__ unsigned long long s_edges[kCompactWaves];
   __shared__ uint32_t s_base[3];
   const int lane = lane_id(), wv = (int)(threadIdx.x >> 6);
   const uint64_t below = lanemask_lt();
-  // this block's words: [beg, end), whole multiples of the block size
-  const size_t per = (words + (size_t)gridDim.x * kCompactBlock - 1) /
-                     ((size_t)gridDim.x * kCompactBlock) * kCompactBlock;
+  // this block's words (one of nb blocks): [beg, end), whole multiples of
+  // the block size
+  const size_t per = (words + (size_t)nb * kCompactBlock - 1) /
+                     ((size_t)nb * kCompactBlock) * kCompactBlock;
   const size_t beg = (size_t)blockIdx.x * per;
   const size_t end = beg + per < words ? beg + per : words;
   // the j-th set bit of a wave's 64 words (vertex and bin degree)
