@@ -116,45 +116,70 @@ struct BfsHostFlag {
 // Reduces the counts of the frontier just produced (kScalarSlots spread
 // slots) and decides the next iteration: the push/pull switch
 // (core/graph.hpp:1509), the iteration's records, or the end of the
-// traversal.  Called by all kBlock threads of one block.
+// traversal.  Called by all kBlock threads of one block.  Every load is
+// issued before any result is used (the slots by all threads, the control
+// block's scalars by thread 0), so the decide waits for one round of memory
+// latency instead of three dependent ones.
 __device__ void bfs_decide(BfsCtl *c, uint64_t *set, Counters *qc, unsigned long long *exam,
                            uint32_t *lv, BfsHostFlag *hf) {
-  __shared__ uint64_t sn[kBlock], se[kBlock], si[kBlock], sx[kBlock];
-  __shared__ uint32_t s_done;
+  __shared__ uint64_t s_part[4][kWavesPerBlock];
   const int t = threadIdx.x;  // kBlock == kScalarSlots: one slot per thread
-  if (t == 0) s_done = c->done;
-  __syncthreads();
-  if (s_done) return;
+  const int lane = lane_id(), wv = t >> 6;
+  uint32_t done = 0, i = 0, lv_cap = 0, V = 0, push_iters = 0;
+  int32_t force = 0;
+  uint64_t examined = 0, reached = 0, m_reach = 0, m_unvisited = 0, threshold = 0, alpha = 0;
+  if (t == 0) {
+    done = c->done;
+    i = c->iter;
+    lv_cap = c->lv_cap;
+    V = c->V;
+    push_iters = c->push_iters;
+    force = c->force;
+    examined = c->examined;
+    reached = c->reached;
+    m_reach = c->m_reach;
+    m_unvisited = c->m_unvisited;
+    threshold = c->threshold;
+    alpha = c->alpha;
+  }
   uint64_t *p = set + (size_t)t * kCountStride;
-  sn[t] = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  se[t] = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  si[t] = __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(p + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(p + 2, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  sx[t] = 0;
-  if (exam && t < 64) {
-    sx[t] = __hip_atomic_load(&exam[t * 16], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&exam[t * 16], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t n = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t e = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t in = __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t x = 0;
+  if (exam && t < 64) x = __hip_atomic_load(&exam[t * 16], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // a slot is only ever added to with n > 0, so an empty slot is already 0
+  // (after the end every step returns at once and the slots stay 0)
+  if (n) {
+    __hip_atomic_store(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 2, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (x) __hip_atomic_store(&exam[t * 16], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  n = wave_sum_u64(n);
+  e = wave_sum_u64(e);
+  in = wave_sum_u64(in);
+  x = wave_sum_u64(x);
+  if (lane == 0) {
+    s_part[0][wv] = n;
+    s_part[1][wv] = e;
+    s_part[2][wv] = in;
+    s_part[3][wv] = x;
   }
   __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (t < s) {
-      sn[t] += sn[t + s];
-      se[t] += se[t + s];
-      si[t] += si[t + s];
-      sx[t] += sx[t + s];
-    }
-    __syncthreads();
+  if (t != 0 || done) return;
+  uint64_t total = 0, edges = 0, ind = 0, ex = 0;
+  for (int k = 0; k < kWavesPerBlock; k++) {
+    total += s_part[0][k];
+    edges += s_part[1][k];
+    ind += s_part[2][k];
+    ex += s_part[3][k];
   }
-  if (t != 0) return;
-  const uint64_t total = sn[0], edges = se[0];
-  const uint32_t i = c->iter;
   if (i > 0 && i - 1 < 64) {
     c->fresh[i - 1] = total;
-    c->exam[i - 1] = sx[0];
+    c->exam[i - 1] = ex;
   }
-  c->examined += sx[0];
+  c->examined = examined + ex;
   if (total == 0) {
     c->done = 1;
     c->mode = kBfsDone;
@@ -162,28 +187,29 @@ __device__ void bfs_decide(BfsCtl *c, uint64_t *set, Counters *qc, unsigned long
     __hip_atomic_store(&hf->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     return;
   }
-  c->reached += total;
-  c->m_reach += edges;
-  c->m_unvisited -= si[0] < c->m_unvisited ? si[0] : c->m_unvisited;
-  if (i < c->lv_cap) lv[i] = (uint32_t)total;
+  reached += total;
+  c->reached = reached;
+  c->m_reach = m_reach + edges;
+  m_unvisited -= ind < m_unvisited ? ind : m_unvisited;
+  c->m_unvisited = m_unvisited;
+  if (i < lv_cap) lv[i] = (uint32_t)total;
   // core/graph.hpp:1509: sparse iff active out-edges < |E| / divisor; with
   // alpha, also sparse while the pull would have to scan more than alpha
   // times the frontier's out-arcs (the unvisited vertices' in-arcs bound it)
-  const bool sparse =
-      edges < c->threshold || (c->alpha && edges * c->alpha <= c->m_unvisited);
-  uint32_t mode = c->force == 1   ? kBfsPush
-                  : c->force == 2 ? kBfsPull
-                  : sparse        ? kBfsPush
-                                  : kBfsPull;
+  const bool sparse = edges < threshold || (alpha && edges * alpha <= m_unvisited);
+  uint32_t mode = force == 1   ? kBfsPush
+                  : force == 2 ? kBfsPull
+                  : sparse     ? kBfsPush
+                               : kBfsPull;
   if (i < 64) {
     c->modes[i] = mode == kBfsPush ? 0u : 1u;
     c->active[i] = total;
-    c->unvisited[i] = (uint64_t)c->V - c->reached;
+    c->unvisited[i] = (uint64_t)V - reached;
   }
   c->mode = mode;
   c->level = i + 1;
   c->iter = i + 1;
-  if (mode == kBfsPush) c->push_iters++;
+  if (mode == kBfsPush) c->push_iters = push_iters + 1;
   qc->n[0] = qc->n[1] = qc->n[2] = 0;
   qc->total = 0;
   qc->edges = 0;
