@@ -38,11 +38,13 @@ def pick_roots(out_degree, k, seed):
     return [int(x) for x in rng.choice(cand, size=min(k, len(cand)), replace=False)]
 
 
-def bfs_model_bytes(st, V, no_in=0):
+def bfs_model_bytes(st, V, no_in=0, found=8):
     """Algorithmic bytes of each process_edges call of one BFS:
-    push: 20F + 8X + 8N  (queue entry, offset pair, col, state probe, writes)
-    pull: 16U + 4X + 8N + 3V/8  (offsets of unvisited, col until exit, writes,
+    push: 20F + 8X + wN  (queue entry, offset pair, col, state probe, writes)
+    pull: 16U + 4X + wN + 3V/8  (offsets of unvisited, col until exit, writes,
           visited / frontier-in / frontier-out bitmaps).
+    w = `found` bytes written per vertex found: 4 for the parent alone (the
+    reference's output), 8 with the depth array as well.
     U counts the unvisited vertices that have in-arcs: the `no_in` vertices
     without any (other than a root among them) can never be found, and the
     bottom-up step does not read their lists."""
@@ -51,10 +53,29 @@ def bfs_model_bytes(st, V, no_in=0):
         F, N, X = st["it_active"][i], st["it_new"][i], st["it_examined"][i]
         U = max(0, st["it_unvisited"][i] - no_in)
         if mode == 0:
-            push += 20 * F + 8 * X + 8 * N
+            push += 20 * F + 8 * X + found * N
         else:
-            pull += 16 * U + 4 * X + 8 * N + 3 * V / 8
+            pull += 16 * U + 4 * X + found * N + 3 * V / 8
     return push, pull
+
+
+def depths_from_parents(parent, root):
+    """Level of every vertex in the tree the parents form (0xFFFFFFFF where
+    parent == V); fails unless every reached vertex leads up to the root."""
+    V = len(parent)
+    depth = np.full(V, 0xFFFFFFFF, dtype=np.uint32)
+    node = np.nonzero(parent < V)[0]
+    cur = node.copy()
+    d = np.zeros(len(node), dtype=np.int64)
+    for _ in range(1024):
+        up = np.nonzero(cur != root)[0]
+        if not len(up):
+            break
+        cur[up] = parent[cur[up]]
+        d[up] += 1
+    assert (cur == root).all(), "parents do not form a tree rooted at the root"
+    depth[node] = d
+    return depth
 
 
 def batch_extra(gg, seed, reorder):
@@ -349,9 +370,9 @@ def pmc_child(args):
                       reorder=not args.no_reorder)
     V = 1 << args.scale
     roots = pick_roots(g.out_degree(), args.roots, args.seed + 1)
-    parent, depth = gg.DeviceArray(V, np.uint32), gg.DeviceArray(V, np.uint32)
+    parent = gg.DeviceArray(V, np.uint32)
     for r in roots:
-        g.bfs_into(r, parent, depth)
+        g.bfs_into(r, parent)
     src, dst = gg.DeviceArray(COPY_BYTES, np.uint8), gg.DeviceArray(COPY_BYTES, np.uint8)
     gg.device_copy(dst, src, COPY_BYTES)
     gg.synchronize()
@@ -488,10 +509,12 @@ def main():
     parent = gg.DeviceArray(V, np.uint32)
     depth = gg.DeviceArray(V, np.uint32)
 
-    def step():
+    # the reference's BFS computes parents only (bfs.cpp:26-94), so does the
+    # timed step; depths are checked from the tree the parents form
+    def step(with_depth=False):
         m = 0
         for r in roots:
-            m += g.bfs_into(r, parent, depth).m_reach
+            m += g.bfs_into(r, parent, depth if with_depth else None).m_reach
         return m
 
     for _ in range(args.warmup):
@@ -520,12 +543,12 @@ def main():
     n_no_in = int(np.count_nonzero(indeg == 0))
     with gg.options(count_examined=1):
         for r in roots:
-            st = g.bfs_into(r, parent, depth).as_dict()
-            pb, lb = bfs_model_bytes(st, V, n_no_in - int(indeg[r] == 0))
+            st = g.bfs_into(r, parent).as_dict()
+            pb, lb = bfs_model_bytes(st, V, n_no_in - int(indeg[r] == 0), found=4)
             push_b += pb
             pull_b += lb
             if len(gpu_depths) < args.cpu_sample_roots:
-                gpu_depths.append(depth.numpy())
+                gpu_depths.append(depths_from_parents(parent.numpy(), r))
     # Both step kernels are launched once per iteration: bfs_pull_kernel works
     # in pull iterations and returns at once in push ones; bfs_push_heavy_kernel
     # runs the push, or the heavy in-list chunks of a pull.  Per launch = per
@@ -557,6 +580,17 @@ def main():
         log(f"traffic: {roofline['traffic']} B/launch {detail}")
 
     extras = {"build_s": round(build_s, 2)}
+    # the same roots with the depth array written as well (8 bytes per vertex
+    # found instead of 4)
+    step(with_depth=True)
+    gg.synchronize()
+    t = time.perf_counter()
+    md = step(with_depth=True)
+    gg.synchronize()
+    el = time.perf_counter() - t
+    extras["bfs_with_depth"] = {"gteps": round(md / el / 1e9, 2),
+                                "ms_per_bfs": round(el * 1e3 / len(roots), 4)}
+    log(f"BFS with depths: {extras['bfs_with_depth']}")
     # measured stream-copy bandwidth (plain copy kernel, 1 GiB read + 1 GiB write)
     nb = 1 << 30
     src, dst = gg.DeviceArray(nb, np.uint8), gg.DeviceArray(nb, np.uint8)
@@ -610,6 +644,8 @@ def main():
         "config": {"workload": f"BFS from {len(roots)} seeded non-isolated roots, RMAT scale "
                                f"{scale} edge factor {ef} (a=0.57 b=0.19 c=0.19), permuted ids",
                    "vertices": V, "edges": E, "roots": len(roots), "seed": args.seed,
+                   "outputs": "parent[V] per root, as bfs.cpp:26-94 (depths checked from "
+                              "the parent tree)",
                    "reorder": reorder,
                    "switch_divisor": gg.get_option("switch_divisor")},
         "roofline": roofline,

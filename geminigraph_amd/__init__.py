@@ -492,12 +492,15 @@ class Graph:
         st = self.pagerank_into(out, iterations, deltas)
         return out.numpy(), deltas, st.as_dict()
 
-    def bfs_into(self, root, parent, depth, frontier=None):
+    def bfs_into(self, root, parent, depth=None, frontier=None):
+        """BFS into device arrays; depth None keeps parents only (the
+        reference's output, bfs.cpp:26-94)."""
         st = RunStats()
         fr_ptr = frontier.ctypes.data if frontier is not None else None
         cap = len(frontier) if frontier is not None else 0
+        dp = ctypes.c_void_p(depth.ptr) if depth is not None else None
         _check(lib().gg_bfs(self.handle, int(root), ctypes.c_void_p(parent.ptr),
-                            ctypes.c_void_p(depth.ptr), ctypes.c_void_p(fr_ptr) if fr_ptr else None,
+                            dp, ctypes.c_void_p(fr_ptr) if fr_ptr else None,
                             cap, ctypes.byref(st)))
         return st
 

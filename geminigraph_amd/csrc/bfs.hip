@@ -18,10 +18,11 @@
 // coherent host memory.
 //
 // Claims: a push edge, and each chunk of a heavy in-list, claims its vertex
-// with a compare-and-swap on the vertex's depth entry, which sits at its
-// original id (relabelling scatters those, so the atomics do not pile onto a
-// few cache lines).  The reference claims with a compare-and-swap on parent
-// (bfs.cpp:53, 71); each way keeps exactly one valid parent.  Results are
+// with a compare-and-swap on the vertex's parent entry, as the reference
+// does (bfs.cpp:53, 71); the entry sits at its original id (relabelling
+// scatters those, so the atomics do not pile onto a few cache lines).  The
+// depth array is optional: without it the BFS writes what the reference's
+// does (parents), 4 bytes per vertex found instead of 8.  Results are
 // written in original vertex ids as vertices are found (parent[iperm[v]] =
 // iperm[u]), so a relabelled graph needs no pass over V at the end.  A
 // frontier joins `visited` one iteration later, in the step that reads it
@@ -55,11 +56,14 @@ struct BfsCtl {
   uint64_t active[64], fresh[64], exam[64], unvisited[64];
 };
 
-// Where found vertices go.
+// Where found vertices go.  parent is also the claim word (none = V until
+// claimed, as bfs.cpp:35); depth is optional (null: parents only, the
+// reference's output).
 struct BfsOut {
   uint32_t *parent, *depth;  // indexed by original ids
   const uint4 *vinfo;        // by internal id: {original id, out-degree, in-degree, first in-nbr}
   uint64_t *visited, *fout;
+  uint32_t none;             // V
 };
 
 // Lane 0 of a wave adds its counts (vertices found, their out-arcs and their
@@ -89,18 +93,18 @@ __device__ __forceinline__ void bfs_wave_flush(uint64_t *set, uint32_t gw, uint6
 // Push claim: `visited` holds every vertex of the earlier levels and of the
 // frontier (the bin kernel adds the frontier before the push).  An edge whose
 // target is neither visited nor already in the output frontier races a
-// compare-and-swap on the target's depth entry, at its original id: with
-// relabelled hubs packed into a few bitmap words, claiming on those words
-// instead would pile returning atomics onto a few cache lines.  The winner
-// writes the parent and sets the frontier bit (an atomic whose result is
-// not needed).
+// compare-and-swap on the target's parent entry, at its original id, as
+// bfs.cpp:53 does: with relabelled hubs packed into a few bitmap words,
+// claiming on those words instead would pile returning atomics onto a few
+// cache lines.  The winner writes the depth (if kept) and sets the frontier
+// bit (an atomic whose result is not needed).
 __device__ __forceinline__ bool bfs_claim(const BfsOut &o, uint32_t src_orig, uint32_t dst,
                                           uint32_t level, uint4 &vi) {
   const uint64_t bit = 1ULL << (dst & 63);
   if ((o.visited[dst >> 6] | o.fout[dst >> 6]) & bit) return false;  // plain reads first
   vi = o.vinfo[dst];
-  if (atomicCAS(&o.depth[vi.x], kUnset, level) != kUnset) return false;
-  o.parent[vi.x] = src_orig;
+  if (atomicCAS(&o.parent[vi.x], o.none, src_orig) != o.none) return false;
+  if (o.depth) o.depth[vi.x] = level;
   __hip_atomic_fetch_or((unsigned long long *)&o.fout[dst >> 6], (unsigned long long)bit,
                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return true;
@@ -227,7 +231,7 @@ __global__ void bfs_init_kernel(uint32_t *parent, uint32_t *depth, uint32_t V, u
   const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (size_t i = i0; i < V; i += s) {
     parent[i] = V;
-    depth[i] = kUnset;
+    if (depth) depth[i] = kUnset;
   }
   for (size_t i = i0; i < words; i += s) {
     visited[i] = no_in[i];
@@ -265,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
     visited[root >> 6] |= bit;
     fin[root >> 6] |= bit;
     parent[root_orig] = root_orig;
-    depth[root_orig] = 0;
+    if (depth) depth[root_orig] = 0;
     set0[0] = 1;
     set0[1] = vinfo[root].y;
     set0[2] = vinfo[root].z;
@@ -529,7 +533,7 @@ __device__ __forceinline__ void bfs_pull_step(const BfsCtl *c, const BfsOut &o,
     const uint64_t m = __ballot(got);
     if (got) {
       o.parent[vi.x] = head ? head_orig : o.vinfo[found].x;
-      o.depth[vi.x] = level;
+      if (o.depth) o.depth[vi.x] = level;
       edg += vi.y;
       ind += deg;
     }
@@ -551,8 +555,8 @@ struct BfsHeavyArgs {
 };
 
 // In-lists above light_max, one wave per chunk (grid-stride); chunks of one
-// vertex race and the compare-and-swap on its depth keeps one parent.  The
-// chunks poll that depth entry to stop early once another chunk has won.
+// vertex race and the compare-and-swap on its parent entry keeps one.  The
+// chunks poll that entry to stop early once another chunk has won.
 __device__ __forceinline__ void bfs_heavy_step(const BfsCtl *c, const BfsOut &o,
                                                const BfsHeavyArgs &a, uint64_t *set_out,
                                                unsigned long long *exam, uint32_t W,
@@ -578,10 +582,10 @@ __device__ __forceinline__ void bfs_heavy_step(const BfsCtl *c, const BfsOut &o,
       const uint32_t v = (uint32_t)__shfl((int)vl, j);
       const uint64_t bit = 1ULL << (v & 63);
       const uint4 vi = o.vinfo[v];
-      uint32_t *dv = &o.depth[vi.x];
+      uint32_t *pv = &o.parent[vi.x];
       const uint64_t cb = a.chunk_beg[ch], ce = a.chunk_end[ch];
       for (uint64_t e0 = cb; e0 < ce; e0 += 64) {
-        if (__hip_atomic_load(dv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kUnset) break;
+        if (__hip_atomic_load(pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != o.none) break;
         const uint64_t e = e0 + lane;
         const uint32_t u = e < ce ? stream_load(&a.nbr[e]) : 0;
         const bool hit = e < ce && test_bit(a.fin, u);
@@ -589,8 +593,8 @@ __device__ __forceinline__ void bfs_heavy_step(const BfsCtl *c, const BfsOut &o,
         examined += hm ? (uint64_t)(__builtin_ctzll(hm) + 1) : (ce - e0 < 64 ? ce - e0 : 64);
         if (hm) {
           const uint32_t fu = __shfl(u, __builtin_ctzll(hm));
-          if (lane == 0 && atomicCAS(dv, kUnset, level) == kUnset) {
-            o.parent[vi.x] = o.vinfo[fu].x;
+          if (lane == 0 && atomicCAS(pv, o.none, o.vinfo[fu].x) == o.none) {
+            if (o.depth) o.depth[vi.x] = level;
             atomicOr((unsigned long long *)&o.fout[v >> 6], (unsigned long long)bit);
             cnt++;
             edg += vi.y;
@@ -695,7 +699,7 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
                       uint32_t *h_frontier, int cap, gg_run_stats *stats) {
   GG_TRY(check_graph(g));
   if (root >= g->vertices) return fail(GG_EINVAL, "root out of range");
-  if (!d_parent || !d_depth) return fail(GG_EINVAL, "null output");
+  if (!d_parent) return fail(GG_EINVAL, "null output");
   Work *w;
   GG_TRY(work_get(g, &w));
   const Dir &out = out_dir(g), &in = in_dir(g);
@@ -745,7 +749,7 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
     if (n > V + 2) return fail(GG_EHIP, "gg_bfs: traversal did not terminate");
     uint64_t *fin = w->f[n & 1].bits, *fout = w->f[(n + 1) & 1].bits;
     uint64_t *set_out = s->sets + (size_t)((n + 1) & 1) * set_words;
-    BfsOut o{d_parent, d_depth, in.vinfo, w->visited, fout};
+    BfsOut o{d_parent, d_depth, in.vinfo, w->visited, fout, V};
     {
       ProfScope p("compact");
       BfsBinArgs b{fin, words, out.degree, (uint32_t)opt.push_small_max,

@@ -208,8 +208,10 @@ typedef struct gg_run_stats {
  * h_deltas (iterations+1 entries, may be NULL) the printed deltas. */
 int gg_pagerank(gg_graph *g, int iterations, double *d_curr_out, double *h_deltas,
                 gg_run_stats *stats);
-/* BFS, toolkits/bfs.cpp:22-109.  d_parent[V] (sentinel V), d_depth[V]
- * (0xFFFFFFFF when unreached, else the level); h_frontier[cap] receives the
+/* BFS, toolkits/bfs.cpp:22-109.  d_parent[V] (sentinel V; the root is its
+ * own parent), d_depth[V] (0xFFFFFFFF when unreached, else the level) or
+ * NULL for parents only, the reference's output, which writes 4 bytes per
+ * vertex found instead of 8; h_frontier[cap] receives the
  * active-vertex count of every level (the reference's "active(i)" lines).
  * Returns the number of levels in stats->iterations. */
 int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *d_depth,

@@ -282,6 +282,56 @@ def test_bfs_hub_bins(gg):
         check_parents(e, n, 0, parent, depth)
 
 
+def depths_from_parents(parent, root, V, max_levels=256):
+    """Level of every vertex in the tree the parents form (0xFFFFFFFF where
+    parent == V): each reached vertex walks up to the root."""
+    depth = np.full(V, 0xFFFFFFFF, dtype=np.uint32)
+    node = np.nonzero(parent < V)[0]
+    d = np.zeros(len(node), dtype=np.int64)
+    cur = node.copy()
+    for _ in range(max_levels):
+        up = cur != root
+        if not up.any():
+            break
+        cur[up] = parent[cur[up]]
+        d[up] += 1
+    assert (cur == root).all(), "parents do not form a tree rooted at the root"
+    depth[node] = d
+    return depth
+
+
+@pytest.mark.parametrize("reorder", [False, True])
+@pytest.mark.parametrize("mode", ["auto", "push", "pull"])
+def test_bfs_parents_only(gg, reorder, mode):
+    # without a depth array (the reference's output: parents only) the claims
+    # and writes change; the tree the parents form must have the oracle's
+    # levels, its arcs must be graph arcs, and level sizes and m_reach match
+    cases = [(oracle.rmat_edges(14, seed=9), 1 << 14)]
+    n = 40000
+    pairs = [(0, i) for i in range(1, 20001)] + [(1, i) for i in range(20001, 23001)] + \
+            [(i, 39999) for i in range(2, 20002)] + [(39999, 1)]
+    cases.append((oracle.edges_from_pairs(pairs), n))
+    for e, V in cases:
+        g = gg.Graph.build(e, V, reorder=reorder)
+        o = oracle.Graph(e, V)
+        parent = gg.DeviceArray(V, np.uint32)
+        for root in roots_of(o.out_degree(), 3, 9):
+            say("bfs parents only", reorder, mode, V, root)
+            fr = np.zeros(64, dtype=np.uint32)
+            with gg.options(force_mode=MODES[mode]):
+                st = g.bfs_into(root, parent, None, fr)
+            p = parent.numpy()
+            _, ref_depth, ref_fr = o.bfs(root)
+            depth = depths_from_parents(p, root, V)
+            assert np.array_equal(depth, ref_depth)
+            assert np.array_equal(fr[:len(ref_fr)], ref_fr) and not fr[len(ref_fr):].any()
+            check_parents(e, V, root, p, depth)
+            reached = ref_depth != 0xFFFFFFFF
+            assert st.m_reach == int(o.out_degree()[reached].astype(np.int64).sum())
+        parent.free()
+        g.close()
+
+
 # ----------------------------------------------------------------------- CC
 
 @pytest.mark.parametrize("reorder", [False, True])
