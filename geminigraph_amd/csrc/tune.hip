@@ -44,13 +44,13 @@ struct Sample {
   int kind;
   std::vector<uint32_t> roots;
   int iterations = 0;
-  void *a = nullptr, *b = nullptr;
+  void *a = nullptr;  // parents (BFS) or ranks (PageRank)
   int run(double *ms) {
     DeviceTimer tm;
     gg_run_stats st;
     if (kind == GG_TUNE_BFS) {
       for (uint32_t r : roots)
-        GG_TRY(gg_bfs(g, r, (uint32_t *)a, (uint32_t *)b, nullptr, 0, &st));
+        GG_TRY(gg_bfs(g, r, (uint32_t *)a, nullptr, nullptr, 0, &st));
     } else {
       GG_TRY(gg_pagerank(g, iterations, (double *)a, nullptr, &st));
     }
@@ -103,12 +103,7 @@ extern "C" int gg_tune(gg_graph *g, int kind, int samples, int apply, gg_tune_re
   if (kind == GG_TUNE_BFS) {
     s.roots = sample_roots(g, samples);
     if (s.roots.empty()) return fail(GG_EINVAL, "gg_tune: no vertex has out-arcs");
-    GG_TRY(device_alloc(&s.a, (size_t)V * 4));
-    int r = device_alloc(&s.b, (size_t)V * 4);
-    if (r != GG_OK) {
-      device_free(s.a);
-      return r;
-    }
+    GG_TRY(device_alloc(&s.a, (size_t)V * 4));  // parents only, the reference's output
   } else {
     s.iterations = samples;
     GG_TRY(device_alloc(&s.a, (size_t)V * 8));
@@ -146,6 +141,5 @@ extern "C" int gg_tune(gg_graph *g, int kind, int samples, int apply, gg_tune_re
     gg_set_option(knobs[k].name, apply && rc == GG_OK ? chosen[k] : before[k]);
   }
   device_free(s.a);
-  device_free(s.b);
   return rc;
 }
