@@ -1480,28 +1480,42 @@ static int msd_state(Work *w, MsdState **out) {
 // next level) and decides the next iteration.  All kBlock threads of one
 // block.
 __device__ void msd_decide(MsCtl *c, uint64_t *set, Counters *qc, MsHostFlag *hf) {
-  __shared__ uint64_t sn[kBlock], se[kBlock];
-  __shared__ uint32_t s_done;
+  // every load is issued before any result is used (the slots by all
+  // threads, the control block's scalars by thread 0): one round of memory
+  // latency instead of three dependent ones
+  __shared__ uint64_t s_part[2][kWavesPerBlock];
   const int t = threadIdx.x;
-  if (t == 0) s_done = c->done;
-  __syncthreads();
-  if (s_done) return;
-  uint64_t *p = set + (size_t)t * 16;
-  sn[t] = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  se[t] = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(p + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (t < s) {
-      sn[t] += sn[t + s];
-      se[t] += se[t + s];
-    }
-    __syncthreads();
+  const int lane = lane_id(), wv = t >> 6;
+  uint32_t done = 0, i = 0, push_iters = 0;
+  int32_t force = 0;
+  uint64_t threshold = 0;
+  if (t == 0) {
+    done = c->done;
+    i = c->iter;
+    push_iters = c->push_iters;
+    force = c->force;
+    threshold = c->threshold;
   }
-  if (t != 0) return;
-  const uint64_t total = sn[0], edges = se[0];
-  const uint32_t i = c->iter;
+  uint64_t *p = set + (size_t)t * 16;
+  uint64_t n = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t e = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (n | e) {  // a slot nobody added to is already 0
+    __hip_atomic_store(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  n = wave_sum_u64(n);
+  e = wave_sum_u64(e);
+  if (lane == 0) {
+    s_part[0][wv] = n;
+    s_part[1][wv] = e;
+  }
+  __syncthreads();
+  if (t != 0 || done) return;
+  uint64_t total = 0, edges = 0;
+  for (int k = 0; k < kWavesPerBlock; k++) {
+    total += s_part[0][k];
+    edges += s_part[1][k];
+  }
   if (i > 0 && i - 1 < 64) c->fresh[i - 1] = total;
   if (total == 0) {
     c->done = 1;
@@ -1511,10 +1525,10 @@ __device__ void msd_decide(MsCtl *c, uint64_t *set, Counters *qc, MsHostFlag *hf
     return;
   }
   // core/graph.hpp:1509: sparse iff the frontier's out-arcs < |E| / divisor
-  const uint32_t mode = c->force == 1                ? kMsPush
-                        : c->force == 2              ? kMsPull
-                        : edges < c->threshold       ? kMsPush
-                                                     : kMsPull;
+  const uint32_t mode = force == 1          ? kMsPush
+                        : force == 2        ? kMsPull
+                        : edges < threshold ? kMsPush
+                                            : kMsPull;
   if (i < 64) {
     c->modes[i] = mode == kMsPush ? 0u : 1u;
     c->active[i] = total;
@@ -1522,7 +1536,7 @@ __device__ void msd_decide(MsCtl *c, uint64_t *set, Counters *qc, MsHostFlag *hf
   c->mode = mode;
   c->level = i + 1;
   c->iter = i + 1;
-  if (mode == kMsPush) c->push_iters++;
+  if (mode == kMsPush) c->push_iters = push_iters + 1;
   qc->n[0] = qc->n[1] = qc->n[2] = 0;
   qc->total = 0;
   qc->edges = 0;
