@@ -1337,8 +1337,9 @@ static int ms_bfs_run(gg_graph *g, Work *w, Engine &en, const uint32_t *roots, i
                           stream()));
   if (store_levels) GG_TRY(hook(0, nullptr, w->ms_f[0]));
   const uint64_t full = n_roots == 64 ? ~0ULL : ((1ULL << n_roots) - 1ULL);
-  const PrPlan *plan = nullptr;
-  GG_TRY(pr_plan_get(g, &plan));
+  PlanRef plan_ref;  // held until this call returns
+  GG_TRY(pr_plan_get(g, &plan_ref));
+  const PrPlan *plan = plan_ref.get();
   int fc = 0;
   uint32_t level = 1;
   for (; en.in.total > 0; level++) {
@@ -1797,8 +1798,9 @@ static int ms_bfs_device(gg_graph *g, Work *w, const uint32_t *roots, int n_root
   if (masks)
     GG_HIP(hipMemcpyAsync(w->ms_lmask, w->ms_f[0], (size_t)V * 8, hipMemcpyDeviceToDevice, sm));
   const uint64_t full = n_roots == 64 ? ~0ULL : ((1ULL << n_roots) - 1ULL);
-  const PrPlan *plan = nullptr;
-  GG_TRY(pr_plan_get(g, &plan));
+  PlanRef plan_ref;  // held until this call returns
+  GG_TRY(pr_plan_get(g, &plan_ref));
+  const PrPlan *plan = plan_ref.get();
   // fixed launch shapes: the frontier sizes live on the device
   const unsigned bin_grid = compact_grid(words);
   const unsigned push_grid = grid_waves((uint64_t)opt.huge_grid_waves);
@@ -2215,8 +2217,9 @@ int gg_pagerank(gg_graph *g, int iterations, double *d_curr_out, double *h_delta
     w->dacc_cap = cap;
   }
   // relabelled graphs: destination bins summed in LDS (prplan.hip)
-  const PrPlan *plan = nullptr;
-  GG_TRY(pr_plan_get(g, &plan));
+  PlanRef plan_ref;  // held until this call returns
+  GG_TRY(pr_plan_get(g, &plan_ref));
+  const PrPlan *plan = plan_ref.get();
   hipLaunchKernelGGL(pr_init_kernel, dim3(grid1(V)), dim3(kBlock), 0, stream(), curr, out.degree, V);
   GG_HIP(hipMemsetAsync(w->dacc, 0, sizeof(double) * kScalarStride * (iterations + 1), stream()));
   for (int it = 0; it < iterations; it++) {
@@ -2289,8 +2292,9 @@ int gg_cc(gg_graph *g, uint32_t *d_label, uint32_t *h_frontier, int cap, gg_run_
   // relabelled graph: dense steps on the destination-bin plan (min in LDS),
   // which cost a few ps per arc: push longer (set before the first count,
   // which decides whether to build queues with the loop's switch)
-  const PrPlan *plan = nullptr;
-  GG_TRY(pr_plan_get(g, &plan));
+  PlanRef plan_ref;  // held until this call returns
+  GG_TRY(pr_plan_get(g, &plan_ref));
+  const PrPlan *plan = plan_ref.get();
   if (plan) en.divisor = (uint64_t)options().plan_switch_divisor;
   GG_TRY(en.compact_in(out.degree));
   int lv = 0;
@@ -2354,8 +2358,9 @@ int gg_sssp(gg_graph *g, uint32_t root, float *d_dist, uint32_t *h_frontier, int
   // their weights), which cost a few ps per arc: push longer.  (Set before
   // the first frontier is counted: the count decides whether to build queues
   // with the same switch as the loop.)
-  const PrPlan *plan = nullptr;
-  GG_TRY(pr_plan_get(g, &plan));
+  PlanRef plan_ref;  // held until this call returns
+  GG_TRY(pr_plan_get(g, &plan_ref));
+  const PrPlan *plan = plan_ref.get();
   if (plan && !plan->hot_w) plan = nullptr;
   if (plan) en.divisor = (uint64_t)options().plan_switch_divisor;
   GG_TRY(en.seed(root, out.degree, nullptr));

@@ -111,7 +111,10 @@ __device__ __forceinline__ bool bfs_claim(const BfsOut &o, uint32_t src_orig, ui
 }
 
 // Progress the host polls, in coherent pinned memory: the deciding thread
-// publishes each decision with a system-scope store.
+// publishes each decision with a relaxed system-scope store.  The host uses
+// it only to pace its launches; the control block it reads at the end is
+// copied in stream order, so no release (an L2 write-back per level) is
+// needed.
 struct BfsHostFlag {
   uint32_t decided;  // iterations decided so far
   uint32_t done;
@@ -187,8 +190,7 @@ __device__ void bfs_decide(BfsCtl *c, uint64_t *set, Counters *qc, unsigned long
   if (total == 0) {
     c->done = 1;
     c->mode = kBfsDone;
-    __threadfence_system();
-    __hip_atomic_store(&hf->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&hf->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     return;
   }
   reached += total;
@@ -217,8 +219,7 @@ __device__ void bfs_decide(BfsCtl *c, uint64_t *set, Counters *qc, unsigned long
   qc->n[0] = qc->n[1] = qc->n[2] = 0;
   qc->total = 0;
   qc->edges = 0;
-  __threadfence_system();
-  __hip_atomic_store(&hf->decided, i + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&hf->decided, i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Output arrays to "unreached" (bfs.cpp:35-36), frontiers and count slots to
@@ -727,21 +728,31 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
                      (uint64_t)opt.bfs_alpha, s->lv_cap);
   // fixed launch shapes: the frontier sizes live on the device
   const unsigned bin_blocks = compact_grid(words);
-  const uint32_t ngroups = (V + 63) / 64;
+  // the bottom-up step walks the groups that hold a vertex with in-arcs
+  // (the ones after them can never be found) and the root's: its frontier
+  // and visited words must be carried like any other
+  const uint32_t ngroups =
+      std::min<uint32_t>((V + 63) / 64, std::max<uint32_t>(in.live_groups, r_int / 64 + 1));
   const uint64_t pw = (uint64_t)opt.bfs_pull_waves;
   const unsigned pull_grid = grid_waves(pw ? std::min<uint64_t>(ngroups, pw) : ngroups);
   const unsigned push_blocks = grid_waves((uint64_t)opt.huge_grid_waves);
   const uint64_t nch = std::max(in.n_chunks, 1u);
   const unsigned heavy_blocks = grid_waves(pw ? std::min<uint64_t>(nch, pw) : nch);
   const unsigned grid_b = std::max(push_blocks, in.n_chunks ? heavy_blocks : 1u);
+  // iterations queued ahead of the device: before issuing iteration n, wait
+  // until iteration n - ahead has been decided (iteration n - ahead - 1 has
+  // finished).  Short levels take less device time than the host needs to
+  // see a decision and issue four launches, so one queued iteration left
+  // the device idle between them; each iteration queued past the end runs
+  // as four launches that return at once.
+  const uint32_t ahead = (uint32_t)std::max<int64_t>(1, opt.bfs_ahead);
   for (uint32_t n = 0;; n++) {
-    // keep one iteration queued: before issuing iteration n, wait until
-    // iteration n-1 has been decided (so iteration n-2 has finished)
-    if (n >= 1) {
+    if (n >= ahead) {
+      const uint32_t need = n - ahead + 1;
       uint64_t spins = 0;
-      while (!hf->done && hf->decided < n) {
+      while (!hf->done && hf->decided < need) {
         if ((++spins & 0xFFF) == 0 && hipStreamQuery(st) == hipSuccess && !hf->done &&
-            hf->decided < n)
+            hf->decided < need)
           return fail(GG_EHIP, "gg_bfs: device loop stopped without a decision");
       }
       if (hf->done) break;

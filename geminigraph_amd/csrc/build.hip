@@ -121,12 +121,19 @@ __global__ void degree_kernel(const uint32_t *edges, uint64_t ne, int rec_words,
   }
 }
 
+// Endpoints must be vertices (bit 1).  Weights must be >= 0 and not NaN
+// (bit 2): the SSSP kernels keep f32 distances as unsigned bit patterns
+// (atomicMin, max of complements), an order that holds for non-negative
+// floats only, where the reference's CAS-loop write_min
+// (core/atomic.hpp:35-41) compares any floats.  -0.0 passes (its sums equal
+// +0.0's); +inf passes (it never lowers a distance).
 __global__ void check_range_kernel(const uint32_t *edges, uint64_t ne, int rec_words, uint32_t V,
                                    uint32_t *bad) {
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += stride) {
     uint32_t s = edges[i * rec_words], d = edges[i * rec_words + 1];
     if (s >= V || d >= V) atomicOr(bad, 1u);
+    if (rec_words == 3 && !weight_ok(edges[i * rec_words + 2])) atomicOr(bad, 2u);
   }
 }
 
@@ -159,12 +166,20 @@ __global__ void make_keys_kernel(const uint32_t *edges, uint64_t ne, int rec_wor
   }
 }
 
-// Relabeling helpers: sort key = complemented out-degree (descending order,
-// ties by id through the stable sort), then perm = inverse of the sorted ids.
-__global__ void relabel_keys_kernel(const uint32_t *deg, uint32_t V, uint32_t *keys, uint32_t *ids) {
+// Relabeling helpers: sort key = complemented out-degree (descending order),
+// then vertices with in-arcs before those without, then id (the stable
+// sort), and perm = inverse of the sorted ids.  Half of an RMAT graph's
+// vertices have no in-arcs; grouped at the end of their out-degree class
+// (the isolated ones, out-degree 0, form the tail of the ids) the bottom-up
+// BFS step skips their groups whole instead of reading records beside the
+// vertices it tests.  in_deg is null on a symmetric graph (no in-arcs iff
+// no arcs).
+__global__ void relabel_keys_kernel(const uint32_t *deg, const uint32_t *in_deg, uint32_t V,
+                                    uint64_t *keys, uint32_t *ids) {
   uint32_t s = gridDim.x * blockDim.x;
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s) {
-    keys[v] = ~deg[v];
+    const uint32_t noin = (in_deg ? in_deg[v] : deg[v]) == 0;
+    keys[v] = ((uint64_t)~deg[v] << 1) | noin;
     ids[v] = v;
   }
 }
@@ -284,6 +299,13 @@ static int build_chunks(gg_graph *g, Dir &d) {
   GG_HIP(hipStreamSynchronize(stream()));
   std::vector<uint32_t> cv, ch, hv, hf;
   std::vector<uint64_t> cb, ce;
+  uint32_t last = 0;  // one past the last vertex with arcs in this direction
+  for (uint32_t v = V; v > 0; v--)
+    if (deg[v - 1]) {
+      last = v;
+      break;
+    }
+  d.live_groups = (last + 63) / 64;
   for (uint32_t v = 0; v < V; v++) {
     if (deg[v] <= d.light_max) continue;
     uint32_t h = (uint32_t)hv.size();
@@ -381,40 +403,43 @@ static int build_dir(gg_graph *g, Dir &d, const uint32_t *edges, uint64_t ne, in
 }
 
 // Degree-sorted relabeling: iperm = ids sorted by descending out-degree
-// (stable, so ties keep id order), perm its inverse; degree arrays are then
-// permuted into internal order.
+// (ties: vertices with in-arcs first, then by id), perm its inverse; degree
+// arrays are then permuted into internal order.
 static int relabel(gg_graph *g) {
   uint32_t V = g->vertices;
-  uint32_t *keys = nullptr, *keys_alt = nullptr, *ids = nullptr, *ids_alt = nullptr;
-  int r = device_alloc((void **)&keys, (size_t)V * 4);
-  if (r == GG_OK) r = device_alloc((void **)&keys_alt, (size_t)V * 4);
+  uint64_t *keys = nullptr, *keys_alt = nullptr;
+  uint32_t *ids = nullptr, *ids_alt = nullptr;
+  int r = device_alloc((void **)&keys, (size_t)V * 8);
+  if (r == GG_OK) r = device_alloc((void **)&keys_alt, (size_t)V * 8);
   if (r == GG_OK) r = device_alloc((void **)&ids, (size_t)V * 4);
   if (r == GG_OK) r = device_alloc((void **)&ids_alt, (size_t)V * 4);
   if (r == GG_OK) r = graph_alloc(g, (void **)&g->perm, (size_t)V * 4);
   if (r == GG_OK) r = graph_alloc(g, (void **)&g->iperm, (size_t)V * 4);
   if (r == GG_OK) {
     hipLaunchKernelGGL(relabel_keys_kernel, dim3(grid_for(V)), dim3(256), 0, stream(),
-                       g->out.degree, V, keys, ids);
-    rocprim::double_buffer<uint32_t> kb(keys, keys_alt), vb(ids, ids_alt);
+                       g->out.degree, g->symmetric ? nullptr : g->in.degree, V, keys, ids);
+    rocprim::double_buffer<uint64_t> kb(keys, keys_alt);
+    rocprim::double_buffer<uint32_t> vb(ids, ids_alt);
     size_t tmp_bytes = 0;
-    hipError_t e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, kb, vb, (size_t)V, 0, 32, stream());
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, kb, vb, (size_t)V, 0, 33, stream());
     void *tmp = nullptr;
     if (e == hipSuccess) r = device_alloc(&tmp, tmp_bytes);
     if (e == hipSuccess && r == GG_OK)
-      e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kb, vb, (size_t)V, 0, 32, stream());
+      e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kb, vb, (size_t)V, 0, 33, stream());
     if (e == hipSuccess && r == GG_OK)
       e = hipMemcpyAsync(g->iperm, vb.current(), (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
     if (e == hipSuccess && r == GG_OK) {
       hipLaunchKernelGGL(invert_perm_kernel, dim3(grid_for(V)), dim3(256), 0, stream(), g->iperm, V,
                          g->perm);
       // degrees into internal order (keys buffer reused as scratch)
+      uint32_t *scratch = (uint32_t *)keys;
       hipLaunchKernelGGL(gather_u32_kernel, dim3(grid_for(V)), dim3(256), 0, stream(),
-                         g->out.degree, g->iperm, V, keys);
-      e = hipMemcpyAsync(g->out.degree, keys, (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
+                         g->out.degree, g->iperm, V, scratch);
+      e = hipMemcpyAsync(g->out.degree, scratch, (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
       if (e == hipSuccess && !g->symmetric) {
         hipLaunchKernelGGL(gather_u32_kernel, dim3(grid_for(V)), dim3(256), 0, stream(),
-                           g->in.degree, g->iperm, V, keys);
-        e = hipMemcpyAsync(g->in.degree, keys, (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
+                           g->in.degree, g->iperm, V, scratch);
+        e = hipMemcpyAsync(g->in.degree, scratch, (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
       }
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream());
@@ -457,7 +482,8 @@ static int graph_build_device(const uint32_t *d_edges, uint64_t ne, uint32_t V, 
       hipMemcpyAsync(&hb, bad, 4, hipMemcpyDeviceToHost, stream());
       hipStreamSynchronize(stream());
       device_free(bad);
-      if (hb) r = fail(GG_EINVAL, "edge endpoint >= vertices");
+      if (hb & 1u) r = fail(GG_EINVAL, "edge endpoint >= vertices");
+      else if (hb & 2u) r = fail(GG_EINVAL, kBadWeight);
     }
   }
   if (r == GG_OK) r = graph_alloc(g, (void **)&g->out.degree, ((size_t)V + 1) * 4);
@@ -530,6 +556,10 @@ int gg_graph_build_ex(const void *edges, int edges_on_device, uint64_t num_edges
   if (!out) return fail(GG_EINVAL, "gg_graph_build: null out");
   if (!edges && num_edges) return fail(GG_EINVAL, "gg_graph_build: null edges");
   if (flags & ~GG_BUILD_REORDER) return fail(GG_EINVAL, "gg_graph_build: unknown flags");
+  // host input is checked before any device work (the device check covers
+  // edge lists already on the device)
+  if (!edges_on_device && weighted && !host_weights_ok((const uint32_t *)edges, num_edges))
+    return fail(GG_EINVAL, kBadWeight);
   GG_TRY(ensure_init());
   size_t rec = weighted ? 12 : 8;
   if (edges_on_device || num_edges == 0)
@@ -571,7 +601,7 @@ int gg_graph_load_ex(const char *path, uint32_t vertices, int weighted, int undi
   void *d = nullptr;
   int r = device_alloc(&d, std::max<uint64_t>(ne * rec, 8));
   void *stage = nullptr;
-  const size_t chunk = (size_t)64 << 20;
+  const size_t chunk = (size_t)48 << 20;  // whole 8- and 12-byte records
   if (r == GG_OK && hipHostMalloc(&stage, chunk, hipHostMallocDefault) != hipSuccess)
     r = fail(GG_ENOMEM, "pinned staging buffer");
   uint64_t done = 0, total = ne * rec;
@@ -580,6 +610,10 @@ int gg_graph_load_ex(const char *path, uint32_t vertices, int weighted, int undi
     size_t got = fread(stage, 1, want, f);
     if (got != want) {
       r = fail(GG_EIO, std::string("short read from ") + path);
+      break;
+    }
+    if (weighted && !host_weights_ok((const uint32_t *)stage, got / rec)) {
+      r = fail(GG_EINVAL, kBadWeight);
       break;
     }
     if (hipMemcpyAsync((char *)d + done, stage, got, hipMemcpyHostToDevice, stream()) !=
@@ -600,7 +634,7 @@ int gg_graph_destroy(gg_graph *g) {
   if (!g) return GG_OK;
   for (Work *w : g->works)
     if (w) work_free(w);
-  pr_plan_free(g->pr_plan);
+  g->pr_plan.reset();
   for (void *p : g->allocations) device_free(p);
   delete g;
   return GG_OK;

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -44,6 +45,23 @@ int fail(int code, const std::string &msg);
     if (r__ != GG_OK) return r__; \
   } while (0)
 
+// ----------------------------------------------------------------- weights
+// An edge weight (f32 bits) the SSSP kernels can order: >= 0 (or -0.0) and
+// not NaN.  Those kernels compare f32 distances as unsigned bit patterns,
+// which orders non-negative floats only.
+__host__ __device__ inline bool weight_ok(uint32_t bits) {
+  return bits <= 0x7F800000u || bits == 0x80000000u;
+}
+// Weights of a host edge list of packed <u32, u32, f32> records.
+inline bool host_weights_ok(const uint32_t *rec, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++)
+    if (!weight_ok(rec[i * 3 + 2])) return false;
+  return true;
+}
+constexpr const char *kBadWeight =
+    "negative or NaN edge weight: weights must be >= 0 (SSSP orders f32 distances by their "
+    "bit patterns)";
+
 // ----------------------------------------------------------------- options
 struct Options {
   // push/pull crossover: sparse iff active out-edges < |E| / switch_divisor.
@@ -69,6 +87,7 @@ struct Options {
   int64_t bfs_pull_waves = 16384; // BFS pull grids: waves (0 = one per group / chunk)
   int64_t batch_graphs = 0;      // batched BFS levels replayed from captured hipGraphs
   int64_t batch_device_loop = 1; // batched BFS/SSSP iterations decided on the device
+  int64_t bfs_ahead = 1;         // BFS: iterations the host issues ahead of the decisions
 };
 Options &options();
 bool prof_on();  // per-kernel event timing (gg_prof_enable) is on
@@ -115,6 +134,9 @@ struct Dir {
   uint32_t *heavy_first = nullptr; // n_heavy + 1: first chunk of each heavy vertex
   uint32_t *chunk_h = nullptr;     // heavy index of each chunk
   uint32_t light_max = 0;
+  // 64-vertex groups up to the last vertex with arcs here: the groups after
+  // it hold no vertex a bottom-up step can find
+  uint32_t live_groups = 0;
 };
 
 // PageRank bin plan of a relabelled graph (built on first use, see
@@ -169,7 +191,10 @@ struct gg_graph {
   std::vector<void *> allocations;
   // traversal contexts owned by the graph, one per context slot (lazily sized)
   gg::Work *works[gg::kMaxSlots] = {};
-  gg::PrPlan *pr_plan = nullptr;
+  // the current bin plan; every call that uses it holds a reference for its
+  // duration, so a plan replaced under other threads' traversals (new plan
+  // options) is freed only when the last of them is done with it
+  std::shared_ptr<gg::PrPlan> pr_plan;
 };
 
 struct gg_subset {
@@ -245,7 +270,8 @@ void bfs_state_free(void *state);
 void msd_state_free(void *state);
 // PageRank plan (prplan.hip): *out = null when the graph gets the plain pull
 // (not relabelled, option pr_hot_sources = 0, or 2^32 arcs or more).
-int pr_plan_get(gg_graph *g, const PrPlan **out);
+using PlanRef = std::shared_ptr<const PrPlan>;
+int pr_plan_get(gg_graph *g, PlanRef *out);
 void pr_plan_free(PrPlan *p);
 // One PageRank iteration on the plan: per destination bin the sums of its
 // hot and cold arcs, then finish(v, sum) of the PageRank op for every vertex.

@@ -745,12 +745,14 @@ static int plan_build(const gg_graph *g, uint32_t H, int bin_bits, int range_bit
 
 // Threads that traverse one graph at once (serving workers) share its plan:
 // the first to need it builds it while the others wait.  A plan is replaced
-// when the plan options change, so changing them while another thread
-// traverses the graph is not supported.
+// when the plan options change; each caller holds a reference to the plan it
+// got for the length of its call, so the replaced plan is freed (its device
+// memory with hipFree, which waits for the device) when the last traversal
+// that uses it returns.
 static std::mutex g_plan_mu;
 
-int pr_plan_get(gg_graph *g, const PrPlan **out) {
-  *out = nullptr;
+int pr_plan_get(gg_graph *g, PlanRef *out) {
+  out->reset();
   const int64_t hs = options().pr_hot_sources;
   if (!g->reordered || hs <= 0 || !g->vertices) return GG_OK;
   if (g->arcs >= ((uint64_t)1 << 32)) return GG_OK;  // positions are 32-bit
@@ -758,22 +760,23 @@ int pr_plan_get(gg_graph *g, const PrPlan **out) {
   const uint32_t H =
       (uint32_t)std::min<int64_t>({hs, (int64_t)g->vertices, (int64_t)1 << (32 - bb)});
   std::lock_guard<std::mutex> lk(g_plan_mu);
-  PrPlan *p = g->pr_plan;
+  std::shared_ptr<PrPlan> p = g->pr_plan;
   if (p && (p->H != H || p->transposed != g->transposed || p->bin_bits != bb ||
             p->range_bits != rb)) {
-    pr_plan_free(p);
-    g->pr_plan = p = nullptr;
+    g->pr_plan.reset();  // freed by its last holder
+    p.reset();
   }
   if (!p) {
-    p = new PrPlan;
+    PrPlan *np = new PrPlan;
     static std::atomic<uint64_t> serials{0};
-    p->serial = ++serials;
-    int r = plan_build(g, H, bb, rb, p);
+    np->serial = ++serials;
+    int r = plan_build(g, H, bb, rb, np);
     if (r != GG_OK) {
       std::string msg = gg_last_error();
-      pr_plan_free(p);
+      pr_plan_free(np);
       return fail(r, msg);
     }
+    p = std::shared_ptr<PrPlan>(np, pr_plan_free);
     g->pr_plan = p;
   }
   *out = p;
@@ -784,6 +787,12 @@ int pr_plan_get(gg_graph *g, const PrPlan **out) {
 static int plan_buffers(const PrPlan *p, Work *w) {
   const uint64_t gs_n = (uint64_t)p->nsplit << p->bin_bits;
   if (w->pr_vals_cap < p->n_cold || w->pr_gsum_cap < gs_n || !w->pr_vals) {
+    // an allocation and a memset must not land in a captured graph: the
+    // captured paths size these buffers before capture starts
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    GG_HIP(hipStreamIsCapturing(stream(), &cs));
+    if (cs != hipStreamCaptureStatusNone)
+      return fail(GG_EINVAL, "plan buffers would be allocated inside a stream capture");
     device_free(w->pr_vals);
     device_free(w->pr_gsum);
     w->pr_vals = w->pr_gsum = nullptr;

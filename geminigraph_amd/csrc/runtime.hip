@@ -209,6 +209,7 @@ static int64_t *option_slot(const char *name) {
   if (!strcmp(name, "bfs_pull_waves")) return &o.bfs_pull_waves;
   if (!strcmp(name, "batch_graphs")) return &o.batch_graphs;
   if (!strcmp(name, "batch_device_loop")) return &o.batch_device_loop;
+  if (!strcmp(name, "bfs_ahead")) return &o.bfs_ahead;
   return nullptr;
 }
 
@@ -235,6 +236,7 @@ int gg_set_option(const char *name, int64_t value) {
   if ((!strcmp(name, "pr_bin_bits") || !strcmp(name, "pr_range_bits")) &&
       (value < 8 || value > 14))
     return fail(GG_EINVAL, std::string(name) + " must be in [8, 14]");
+  if (!strcmp(name, "bfs_ahead") && value > 8) return fail(GG_EINVAL, "bfs_ahead must be in [1, 8]");
   if (!strcmp(name, "pull_light_max") && value > (1 << 20))
     return fail(GG_EINVAL, "pull_light_max must be <= 2^20");
   *s = value;

@@ -130,6 +130,11 @@ __global__ void check_dir_kernel(const uint64_t *index, const uint32_t *nbr, con
   for (uint64_t a = i0; a < arcs; a += s)
     if (nbr[a] >= V) atomicOr(bad, 2u);
 }
+__global__ void check_weights_kernel(const float *w, uint64_t arcs, uint32_t *bad) {
+  const uint64_t s = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t a = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; a < arcs; a += s)
+    if (!weight_ok(__float_as_uint(w[a]))) atomicOr(bad, 8u);
+}
 __global__ void check_perm_kernel(const uint32_t *perm, const uint32_t *iperm, uint32_t V,
                                   uint32_t *bad) {
   const uint32_t s = gridDim.x * blockDim.x;
@@ -149,6 +154,13 @@ int check_graph_arrays(const gg_graph *g) {
   if (!g->symmetric)
     hipLaunchKernelGGL(check_dir_kernel, dim3(grid), dim3(kBlock), 0, stream(), g->in.index,
                        g->in.nbr, g->in.degree, g->vertices, g->arcs, bad);
+  if (g->weighted) {
+    hipLaunchKernelGGL(check_weights_kernel, dim3(grid), dim3(kBlock), 0, stream(), g->out.w,
+                       g->arcs, bad);
+    if (!g->symmetric)
+      hipLaunchKernelGGL(check_weights_kernel, dim3(grid), dim3(kBlock), 0, stream(), g->in.w,
+                         g->arcs, bad);
+  }
   if (g->reordered)
     hipLaunchKernelGGL(check_perm_kernel, dim3(grid1(g->vertices, 8192)), dim3(kBlock), 0,
                        stream(), g->perm, g->iperm, g->vertices, bad);
@@ -157,6 +169,7 @@ int check_graph_arrays(const gg_graph *g) {
   if (e == hipSuccess) e = hipStreamSynchronize(stream());
   device_free(bad);
   if (e != hipSuccess) return fail(GG_EHIP, hipGetErrorString(e));
+  if (h & 8u) return fail(GG_EIO, std::string("saved graph: ") + kBadWeight);
   if (h) return fail(GG_EIO, "saved graph has inconsistent arrays");
   return GG_OK;
 }

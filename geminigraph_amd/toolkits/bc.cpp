@@ -1,3 +1,27 @@
+/*
+Copyright (c) 2014-2015 Xiaowei Zhu, Tsinghua University
+
+   Licensed under the Apache License, Version 2.0 (the "License");
+   you may not use this file except in compliance with the License.
+   You may obtain a copy of the License at
+
+       http://www.apache.org/licenses/LICENSE-2.0
+
+   Unless required by applicable law or agreed to in writing, software
+   distributed under the License is distributed on an "AS IS" BASIS,
+   WITHOUT WARRANTIES OR CONDITIONS OF ANY KIND, either express or implied.
+   See the License for the specific language governing permissions and
+   limitations under the License.
+*/
+/*
+ * Modified from Gemini toolkits/bc.cpp (shugo256/GeminiGraph).  Changes:
+ * the lambdas are device callables ([=] __device__ captures) over device
+ * views of the graph, bitmaps and vertex arrays instead of [&] captures of
+ * host pointers; host-side pokes and reads of vertex arrays go through
+ * set_value() / to_host(); main() takes the same arguments and runs the same
+ * 1 + 5 timed compute() calls, with optional result dumps for tests
+ * (common.hpp); code reformatted.
+ */
 // Betweenness dependencies on the device facade: compute() of
 // toolkits/bc.cpp:24-196 (FP64; the compact variant is compiled out there too).
 #include <stdio.h>

@@ -100,6 +100,10 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *                      decided on the device and the host issues iterations
  *                      ahead without waiting (default); 0 = the host reads
  *                      the frontier counts back after every iteration
+ *   "bfs_ahead"        BFS: iterations the host issues beyond the last
+ *                      decided one (1..8, default 1); more hides the host's
+ *                      reaction time on short levels, each one past the end
+ *                      costs launches that return at once
  *   "batch_graphs"     batched BFS: 1 = replay each level (after the first)
  *                      from a hipGraph captured once per buffer parity;
  *                      0 = launch the level's kernels (default: measured
@@ -149,7 +153,12 @@ typedef struct gg_graph_info {
  * CSR, with `edges` kept at the input count.
  * Replaces Graph<E>::load_directed (core/graph.hpp:772-1339) and
  * Graph<E>::load_undirected_from_directed (core/graph.hpp:350-757) minus the
- * file read.  edges_on_device != 0: `edges` is a device pointer. */
+ * file read.  edges_on_device != 0: `edges` is a device pointer.
+ * Weighted records must carry weights >= 0 and not NaN (GG_EINVAL
+ * otherwise, from build, load and gg_graph_open alike): the SSSP kernels
+ * order f32 distances by their bit patterns, which holds for non-negative
+ * floats only.  The reference's CAS-loop write_min (core/atomic.hpp:35-41)
+ * takes any sign; its SSSP never relies on that. */
 int gg_graph_build(const void *edges, int edges_on_device, uint64_t num_edges,
                    uint32_t vertices, int weighted, int undirected, gg_graph **out);
 /* Same, reading a binary edge-list file: |E| = file size / record size

@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 import geminigraph_amd as gg
@@ -55,3 +56,33 @@ def test_last_error_and_options_without_gpu():
         gg.set_option("no_such_option", 1)
     with pytest.raises(gg.GGError):
         gg.set_option("force_mode", 7)
+
+
+@pytest.mark.parametrize("bad", [-0.5, float("nan"), -float("inf")])
+def test_negative_or_nan_weight_refused_without_gpu(bad):
+    """A weighted host edge list with a negative or NaN weight is refused
+    with GG_EINVAL before any device work: the SSSP kernels order f32
+    distances by their bit patterns, which needs weights >= 0."""
+    import oracle
+    e = np.zeros(4, dtype=oracle.WEDGE_DTYPE)
+    e["src"] = [0, 1, 2, 3]
+    e["dst"] = [1, 2, 3, 0]
+    e["w"] = [0.5, 1.0, bad, 0.25]
+    with pytest.raises(gg.GGError) as ex:
+        gg.Graph.build(e, 4)
+    assert ex.value.code == -1 and "weight" in str(ex.value)
+
+
+def test_weight_lattice_accepted_by_host_check():
+    """Zero, -0.0, denormals and +inf pass the host check (they order
+    correctly as bit patterns); the call then fails only for want of a GPU
+    here, never with the weight message."""
+    import oracle
+    e = np.zeros(4, dtype=oracle.WEDGE_DTYPE)
+    e["src"] = [0, 1, 2, 3]
+    e["dst"] = [1, 2, 3, 0]
+    e["w"] = [0.0, -0.0, 1e-45, float("inf")]
+    try:
+        gg.Graph.build(e, 4).close()
+    except gg.GGError as ex:
+        assert "weight" not in str(ex)

@@ -11,7 +11,27 @@ at this size, so parity rests on properties that hold at any size:
   bit-identical distances; m_reach is the out-degree sum of the reached;
 * PageRank (pagerank.cpp:24-129): the destination-bin plan and the plain
   pull (independent kernels) agree within 1e-9 relative.
+
+The same fixed-point checks run on the relabelled graph whose weighted bin
+plan the bench times (32-bit run tables, 16-bit cold offsets at 1.07G arcs).
+
+The timed paths at the other bench sizes are compared with the oracle
+itself, which finishes these in seconds on the host's cores:
+
+* BFS from the 16 bench roots on RMAT-24 (16,777,216 vertices, 268,435,456
+  edges, permuted ids, relabelled), parents only as bfs.cpp:26-94: the
+  depths of the tree the parents form equal the oracle's depths, every
+  parent is an in-neighbour, level sizes and m_reach exact;
+* CC on the symmetrised RMAT-24 graph (relabelled: dense steps on the bin
+  plan): labels and frontier sizes bit-exact (cc.cpp:22-109);
+* PageRank, 20 iterations, on RMAT-24 (relabelled: the bin plan): ranks and
+  deltas within 1e-9 relative;
+* batched BFS and BC on RMAT-22 edge factor 32 with the bench's 64 roots:
+  every row equals the plain call for its root (depths bit-exact, BC within
+  1e-9 with NaN positions equal).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -25,6 +45,64 @@ SEED = 26
 
 def say(*a):
     print("[gpu]", *a, flush=True)
+
+
+def pick_roots(out_degree, k, seed):
+    """The bench's root choice: a seeded sample without replacement of the
+    vertices with out-arcs."""
+    cand = np.nonzero(out_degree)[0]
+    rng = np.random.default_rng(seed)
+    return [int(x) for x in rng.choice(cand, size=min(k, len(cand)), replace=False)]
+
+
+def raw_csc(g):
+    """The device CSC exactly as stored (internal ids on a relabelled graph)."""
+    i = g.info()
+    idx = g._copy(i.d_in_index, g.vertices + 1, np.uint64)
+    src = g._copy(i.d_in_nbr, g.arcs, np.uint32)
+    w = g._copy(i.d_in_w, g.arcs, np.float32) if g.weighted else None
+    return idx, src, w
+
+
+def tree_depths(parent, root):
+    """Level of every vertex in the tree the parents form (0xFFFFFFFF where
+    parent == V); fails unless every reached vertex leads up to the root."""
+    V = len(parent)
+    depth = np.full(V, 0xFFFFFFFF, dtype=np.uint32)
+    node = np.nonzero(parent < V)[0]
+    cur = node.copy()
+    d = np.zeros(len(node), dtype=np.int64)
+    for _ in range(4096):
+        up = np.nonzero(cur != root)[0]
+        if not len(up):
+            break
+        cur[up] = parent[cur[up]]
+        d[up] += 1
+    assert (cur == root).all(), "parents do not form a tree rooted at the root"
+    depth[node] = d
+    return depth
+
+
+def assert_rel(got, ref, rel=1e-9):
+    nan_g, nan_r = np.isnan(got), np.isnan(ref)
+    assert np.array_equal(nan_g, nan_r)
+    assert np.array_equal(np.isinf(got), np.isinf(ref))
+    m = np.isfinite(ref)
+    err = np.abs(got[m] - ref[m]) / np.maximum(np.abs(ref[m]), 1e-300)
+    assert not len(err) or err.max() <= rel, err.max()
+
+
+@pytest.fixture(scope="module")
+def rmat24():
+    """The bench graph on the host: the oracle's copy (file ids) and its
+    edge list."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    oracle.set_threads(max(1, min(threads, 64)))
+    say("rmat24 oracle build")
+    e = oracle.rmat_edges(24, 16, seed=24, permute=True)
+    o = oracle.Graph(e, 1 << 24)
+    yield e, o
+    del o
 
 
 def test_sssp_rmat26_fixed_point(gg):
@@ -69,4 +147,144 @@ def test_pagerank_rmat26_plan_vs_pull(gg):
     assert np.all(np.isfinite(plan)) and np.all(plan > 0)
     assert np.max(np.abs(plan - pull) / np.abs(pull)) <= 1e-9
     pr.free()
+    g.close()
+
+
+@pytest.mark.timeout(900)
+def test_sssp_rmat26_relabelled_plan_fixed_point(gg):
+    # the bench's scale-26 path: relabelled, the weighted bin plan
+    V = 1 << SCALE
+    g = gg.Graph.rmat(SCALE, 16, seed=SEED, permute=True, weighted=True, reorder=True)
+    assert g.reordered and g.arcs == 16 * V
+    od = g.out_degree().astype(np.int64)
+    roots = pick_roots(od, 8, SEED)
+    perm = g.perm()
+    idx, src, w = raw_csc(g)
+    dist = gg.DeviceArray(V, np.float32)
+    d_int = np.empty(V, dtype=np.float32)
+    first = None
+    for r in roots:
+        say("sssp rmat26 relabelled root", r)
+        st = g.sssp_into(r, dist)
+        d = dist.numpy()
+        if first is None:
+            first = d.view(np.uint32).copy()
+        d_int[perm] = d
+        assert oracle.sssp_fixed_point_violations(idx, src, w, d_int, int(perm[r])) == 0, r
+        assert st.m_reach == int(od[d < 1e9].sum())
+        assert st.iterations > 1
+    for mode in (1, 2):
+        say("sssp rmat26 relabelled forced mode", mode)
+        with gg.options(force_mode=mode):
+            g.sssp_into(roots[0], dist)
+        assert np.array_equal(dist.numpy().view(np.uint32), first), mode
+    dist.free()
+    g.close()
+
+
+@pytest.mark.timeout(900)
+def test_bfs_rmat24_bench_roots(gg, rmat24):
+    e, o = rmat24
+    V = 1 << 24
+    g = gg.Graph.rmat(24, 16, seed=24, permute=True, reorder=True)
+    roots = pick_roots(g.out_degree(), 16, 25)
+    od = o.out_degree().astype(np.int64)
+    assert np.array_equal(g.out_degree(), o.out_degree())
+    # every parent must be an in-neighbour: sorted (dst, src) keys of the file
+    key = e["dst"].astype(np.uint64) * np.uint64(V) + e["src"].astype(np.uint64)
+    key.sort()
+    parent = gg.DeviceArray(V, np.uint32)
+    fr = np.zeros(256, dtype=np.uint32)
+    for r in roots:
+        say("bfs rmat24 root", r)
+        fr[:] = 0
+        st = g.bfs_into(r, parent, None, fr)
+        p = parent.numpy()
+        _, ref_depth, ref_fr = o.bfs(r)
+        assert np.array_equal(tree_depths(p, r), ref_depth), r
+        assert np.array_equal(fr[:len(ref_fr)], ref_fr) and not fr[len(ref_fr):].any()
+        assert st.iterations == len(ref_fr)
+        reached = ref_depth != 0xFFFFFFFF
+        assert st.m_reach == int(od[reached].sum())
+        vs = np.nonzero(reached)[0]
+        vs = vs[vs != r]
+        k = vs.astype(np.uint64) * np.uint64(V) + p[vs].astype(np.uint64)
+        pos = np.minimum(np.searchsorted(key, k), len(key) - 1)
+        assert (key[pos] == k).all(), "a parent is not an in-neighbour"
+    parent.free()
+    g.close()
+
+
+@pytest.mark.timeout(900)
+def test_cc_rmat24_relabelled_plan(gg, rmat24):
+    e, _ = rmat24
+    V = 1 << 24
+    gu = gg.Graph.rmat(24, 16, seed=24, permute=True, undirected=True, reorder=True)
+    label = gg.DeviceArray(V, np.uint32)
+    fr = np.zeros(256, dtype=np.uint32)
+    say("cc rmat24")
+    st = gu.cc_into(label, fr)
+    lab = label.numpy()
+    label.free()
+    gu.close()
+    say("cc rmat24 oracle")
+    ou = oracle.Graph(e, V, undirected=True)
+    ref, ref_fr = ou.cc()
+    del ou
+    assert np.array_equal(lab, ref)
+    assert np.array_equal(fr[:len(ref_fr)], ref_fr) and not fr[len(ref_fr):].any()
+    assert st.iterations == len(ref_fr)
+
+
+@pytest.mark.timeout(900)
+def test_pagerank_rmat24_vs_oracle(gg, rmat24):
+    _, o = rmat24
+    V = 1 << 24
+    g = gg.Graph.rmat(24, 16, seed=24, permute=True, reorder=True)
+    pr = gg.DeviceArray(V, np.float64)
+    deltas = np.zeros(21, dtype=np.float64)
+    say("pagerank rmat24")
+    g.pagerank_into(pr, 20, deltas)
+    got = pr.numpy()
+    pr.free()
+    g.close()
+    say("pagerank rmat24 oracle")
+    ref, ref_deltas = o.pagerank(20)
+    assert_rel(got, ref)
+    # the last entry is not a delta (pagerank.cpp prints 20 of them)
+    assert_rel(deltas[:20], ref_deltas[:20], rel=1e-8)
+
+
+@pytest.mark.timeout(900)
+def test_batched_rmat22_ef32_every_row(gg):
+    # the bench's batched case: seed 31, roots from seed 33
+    V = 1 << 22
+    g = gg.Graph.rmat(22, 32, seed=31, permute=True, reorder=True)
+    roots = pick_roots(g.out_degree(), 64, 33)
+    rows = gg.DeviceArray(V * 64, np.uint32)
+    say("bfs batch rmat22")
+    gg.bfs_batch_into(g, roots, rows)
+    host = rows.numpy().reshape(V, 64)
+    rows.free()
+    parent, depth = gg.DeviceArray(V, np.uint32), gg.DeviceArray(V, np.uint32)
+    for i, r in enumerate(roots):
+        g.bfs_into(r, parent, depth)
+        assert np.array_equal(host[:, i], depth.numpy()), (i, r)
+    del host
+    parent.free()
+    depth.free()
+    bdep, binv = gg.DeviceArray(V * 64, np.float64), gg.DeviceArray(V * 64, np.float64)
+    say("bc batch rmat22")
+    gg.bc_batch_into(g, roots, bdep, binv)
+    hd = bdep.numpy().reshape(V, 64)
+    hi = binv.numpy().reshape(V, 64)
+    bdep.free()
+    binv.free()
+    dep, inv = gg.DeviceArray(V, np.float64), gg.DeviceArray(V, np.float64)
+    for i, r in enumerate(roots):
+        g.bc_into(r, dep, inv)
+        assert_rel(hd[:, i].copy(), dep.numpy())
+        assert_rel(hi[:, i].copy(), inv.numpy())
+    dep.free()
+    inv.free()
     g.close()
