@@ -165,6 +165,7 @@ SIGNATURES = {
     "gg_subset_device_words": (_vp, [_vp]),
     "gg_pagerank": (_i32, [_vp, _i32, _vp, _vp, _stats]),
     "gg_bfs": (_i32, [_vp, _u32, _vp, _vp, _vp, _i32, _stats]),
+    "gg_bfs_noalloc": (_i32, [_vp, _u32, _vp, _vp, _vp, _i32, _stats]),
     "gg_cc": (_i32, [_vp, _vp, _vp, _i32, _stats]),
     "gg_sssp": (_i32, [_vp, _u32, _vp, _vp, _i32, _stats]),
     "gg_bc": (_i32, [_vp, _u32, _vp, _vp, _vp, _i32, _stats]),
@@ -502,6 +503,21 @@ class Graph:
         _check(lib().gg_bfs(self.handle, int(root), ctypes.c_void_p(parent.ptr),
                             dp, ctypes.c_void_p(fr_ptr) if fr_ptr else None,
                             cap, ctypes.byref(st)))
+        return st
+
+    def bfs_noalloc_into(self, root, parent, depth=None, frontier=None):
+        """bfs_into through gg_bfs_noalloc: no allocation inside the call
+        (graphs of at most 2^20 vertices; overlapping calls on a GPU race)."""
+        self._check_vertex_array(parent)
+        if depth is not None:
+            self._check_vertex_array(depth)
+        st = RunStats()
+        fr_ptr = frontier.ctypes.data if frontier is not None else None
+        cap = len(frontier) if frontier is not None else 0
+        dp = ctypes.c_void_p(depth.ptr) if depth is not None else None
+        _check(lib().gg_bfs_noalloc(self.handle, int(root), ctypes.c_void_p(parent.ptr),
+                                    dp, ctypes.c_void_p(fr_ptr) if fr_ptr else None,
+                                    cap, ctypes.byref(st)))
         return st
 
     def bfs(self, root):

@@ -31,6 +31,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <mutex>
 
 #include "kernels.hpp"
 
@@ -692,110 +694,104 @@ static int bfs_state(Work *w, uint32_t lv_need, BfsState **out) {
   return GG_OK;
 }
 
-}  // namespace gg
+// The device buffers one traversal uses, wherever they live (the graph's
+// traversal context, or the allocation-free variant's scratch array).
+struct BfsBufs {
+  BfsCtl *ctl;
+  uint64_t *sets;  // 2 x kScalarSlots x kCountStride
+  Counters *qc;
+  uint32_t *lv;
+  uint32_t lv_cap;
+  BfsHostFlag *hf;  // where the decide publishes its progress
+  unsigned long long *exam;  // null unless count_examined
+  uint64_t *visited, *f[2];
+  uint32_t *q[3];
+  size_t words;  // V / 64 + 1
+};
 
-using namespace gg;
+// Launch shapes and arguments fixed for one traversal.
+struct BfsPlan {
+  uint32_t V, r_int, root;
+  uint32_t *parent, *depth;
+  unsigned bin_blocks, pull_grid, push_blocks, heavy_blocks, grid_b;
+  uint32_t ngroups;
+};
 
-extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *d_depth,
-                      uint32_t *h_frontier, int cap, gg_run_stats *stats) {
-  GG_TRY(check_graph(g));
-  if (root >= g->vertices) return fail(GG_EINVAL, "root out of range");
-  if (!d_parent) return fail(GG_EINVAL, "null output");
-  Work *w;
-  GG_TRY(work_get(g, &w));
-  const Dir &out = out_dir(g), &in = in_dir(g);
-  const uint32_t V = g->vertices;
-  const uint32_t lv_need = (uint32_t)std::min<int64_t>(std::max(cap, 1), (int64_t)V + 1);
-  BfsState *s;
-  GG_TRY(bfs_state(w, lv_need, &s));
+static BfsPlan bfs_plan(const gg_graph *g, uint32_t root, uint32_t *parent, uint32_t *depth,
+                        size_t words) {
+  const Dir &in = in_dir(g);
   const Options &opt = options();
-  unsigned long long *exam = opt.count_examined ? w->exam : nullptr;
-  const uint32_t r_int = g->to_internal(root);
-  const size_t words = w->words;
-  const uint64_t threshold = g->edges / (uint64_t)opt.switch_divisor;
-  hipStream_t st = stream();
-  const size_t set_words = (size_t)kScalarSlots * kCountStride;
-  volatile BfsHostFlag *hf = s->h_flag;
-  hf->decided = 0;
-  hf->done = 0;
-  DeviceTimer tm;
-  hipLaunchKernelGGL(bfs_init_kernel, dim3(grid1(V)), dim3(kBlock), 0, st, d_parent, d_depth, V,
-                     w->visited, in.empty, w->f[0].bits, w->f[1].bits, words, s->sets,
-                     2 * set_words, exam);
-  hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(kBlock), 0, st, s->ctl, s->sets, s->qc, exam,
-                     s->lv, s->h_flag, w->visited, w->f[0].bits, r_int, root, d_parent, d_depth,
-                     in.vinfo, V, (int32_t)opt.force_mode, threshold, g->arcs,
-                     (uint64_t)opt.bfs_alpha, s->lv_cap);
-  // fixed launch shapes: the frontier sizes live on the device
-  const unsigned bin_blocks = compact_grid(words);
+  BfsPlan p;
+  p.V = g->vertices;
+  p.root = root;
+  p.r_int = g->to_internal(root);
+  p.parent = parent;
+  p.depth = depth;
+  p.bin_blocks = compact_grid(words);
   // the bottom-up step walks the groups that hold a vertex with in-arcs
   // (the ones after them can never be found) and the root's: its frontier
   // and visited words must be carried like any other
-  const uint32_t ngroups =
-      std::min<uint32_t>((V + 63) / 64, std::max<uint32_t>(in.live_groups, r_int / 64 + 1));
+  p.ngroups = std::min<uint32_t>((p.V + 63) / 64, std::max<uint32_t>(in.live_groups, p.r_int / 64 + 1));
   const uint64_t pw = (uint64_t)opt.bfs_pull_waves;
-  const unsigned pull_grid = grid_waves(pw ? std::min<uint64_t>(ngroups, pw) : ngroups);
-  const unsigned push_blocks = grid_waves((uint64_t)opt.huge_grid_waves);
+  p.pull_grid = grid_waves(pw ? std::min<uint64_t>(p.ngroups, pw) : p.ngroups);
+  p.push_blocks = grid_waves((uint64_t)opt.huge_grid_waves);
   const uint64_t nch = std::max(in.n_chunks, 1u);
-  const unsigned heavy_blocks = grid_waves(pw ? std::min<uint64_t>(nch, pw) : nch);
-  const unsigned grid_b = std::max(push_blocks, in.n_chunks ? heavy_blocks : 1u);
-  // iterations queued ahead of the device: before issuing iteration n, wait
-  // until iteration n - ahead has been decided (iteration n - ahead - 1 has
-  // finished).  Short levels take less device time than the host needs to
-  // see a decision and issue four launches, so one queued iteration left
-  // the device idle between them; each iteration queued past the end runs
-  // as four launches that return at once.
-  const uint32_t ahead = (uint32_t)std::max<int64_t>(1, opt.bfs_ahead);
-  for (uint32_t n = 0;; n++) {
-    if (n >= ahead) {
-      const uint32_t need = n - ahead + 1;
-      uint64_t spins = 0;
-      while (!hf->done && hf->decided < need) {
-        if ((++spins & 0xFFF) == 0 && hipStreamQuery(st) == hipSuccess && !hf->done &&
-            hf->decided < need)
-          return fail(GG_EHIP, "gg_bfs: device loop stopped without a decision");
-      }
-      if (hf->done) break;
-    }
-    if (n > V + 2) return fail(GG_EHIP, "gg_bfs: traversal did not terminate");
-    uint64_t *fin = w->f[n & 1].bits, *fout = w->f[(n + 1) & 1].bits;
-    uint64_t *set_out = s->sets + (size_t)((n + 1) & 1) * set_words;
-    BfsOut o{d_parent, d_depth, in.vinfo, w->visited, fout, V};
-    {
-      ProfScope p("compact");
-      BfsBinArgs b{fin, words, out.degree, (uint32_t)opt.push_small_max,
-                   (uint32_t)opt.push_big_max, w->f[0].q[0], w->f[0].q[1], w->f[0].q[2], s->qc};
-      hipLaunchKernelGGL(bfs_bin_kernel, dim3(bin_blocks), dim3(kCompactBlock), 0, st, s->ctl, o,
-                         b);
-    }
-    {
-      ProfScope p("bfs_pull");
-      BfsPullArgs a{in.index, in.nbr, in.first_orig, fin, V, in.light_max,
-                    (uint32_t)opt.pull_serial_max, ngroups};
-      hipLaunchKernelGGL(bfs_pull_kernel, dim3(pull_grid), dim3(kBlock), 0, st, s->ctl, o, a,
-                         set_out, exam);
-    }
-    {
-      ProfScope p("bfs_push_heavy");
-      BfsPushArgs a{out.index, out.nbr, w->f[0].q[0], w->f[0].q[1], w->f[0].q[2], s->qc};
-      BfsHeavyArgs h{in.nbr, in.chunk_v, in.chunk_beg, in.chunk_end, fin, in.n_chunks};
-      hipLaunchKernelGGL(bfs_push_heavy_kernel, dim3(grid_b), dim3(kBlock), 0, st, s->ctl, o, a,
-                         h, set_out, exam, push_blocks, heavy_blocks);
-    }
-    hipLaunchKernelGGL(bfs_decide_kernel, dim3(1), dim3(kBlock), 0, st, s->ctl, set_out, s->qc,
-                       exam, s->lv, s->h_flag);
-    GG_HIP(hipGetLastError());
+  p.heavy_blocks = grid_waves(pw ? std::min<uint64_t>(nch, pw) : nch);
+  p.grid_b = std::max(p.push_blocks, in.n_chunks ? p.heavy_blocks : 1u);
+  return p;
+}
+
+// Output arrays and buffers to their initial state, the root, and the first
+// decision.
+static void bfs_issue_start(const gg_graph *g, const BfsPlan &p, const BfsBufs &b, hipStream_t st) {
+  const Dir &in = in_dir(g);
+  const Options &opt = options();
+  const size_t set_words = (size_t)kScalarSlots * kCountStride;
+  const uint64_t threshold = g->edges / (uint64_t)opt.switch_divisor;
+  hipLaunchKernelGGL(bfs_init_kernel, dim3(grid1(p.V)), dim3(kBlock), 0, st, p.parent, p.depth,
+                     p.V, b.visited, in.empty, b.f[0], b.f[1], b.words, b.sets, 2 * set_words,
+                     b.exam);
+  hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(kBlock), 0, st, b.ctl, b.sets, b.qc, b.exam,
+                     b.lv, b.hf, b.visited, b.f[0], p.r_int, p.root, p.parent, p.depth, in.vinfo,
+                     p.V, (int32_t)opt.force_mode, threshold, g->arcs, (uint64_t)opt.bfs_alpha,
+                     b.lv_cap);
+}
+
+// The four launches of iteration n (bin, pull, push-or-heavy, decide).
+static void bfs_issue_iteration(const gg_graph *g, const BfsPlan &p, const BfsBufs &b, uint32_t n,
+                                hipStream_t st, bool prof = true) {
+  const Dir &out = out_dir(g), &in = in_dir(g);
+  const Options &opt = options();
+  const size_t set_words = (size_t)kScalarSlots * kCountStride;
+  uint64_t *fin = b.f[n & 1], *fout = b.f[(n + 1) & 1];
+  uint64_t *set_out = b.sets + (size_t)((n + 1) & 1) * set_words;
+  BfsOut o{p.parent, p.depth, in.vinfo, b.visited, fout, p.V};
+  {
+    ProfScope ps(prof ? "compact" : nullptr);
+    BfsBinArgs a{fin, b.words, out.degree, (uint32_t)opt.push_small_max,
+                 (uint32_t)opt.push_big_max, b.q[0], b.q[1], b.q[2], b.qc};
+    hipLaunchKernelGGL(bfs_bin_kernel, dim3(p.bin_blocks), dim3(kCompactBlock), 0, st, b.ctl, o, a);
   }
-  GG_HIP(hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, st));
-  const double ms = tm.stop();
-  const BfsCtl &c = *s->h_ctl;
-  if (h_frontier && cap > 0) {
-    const uint32_t levels = std::min<uint32_t>(c.iter, (uint32_t)cap);
-    if (levels) {
-      GG_HIP(hipMemcpyAsync(h_frontier, s->lv, (size_t)levels * 4, hipMemcpyDeviceToHost, st));
-      GG_HIP(hipStreamSynchronize(st));
-    }
+  {
+    ProfScope ps(prof ? "bfs_pull" : nullptr);
+    BfsPullArgs a{in.index, in.nbr, in.first_orig, fin, p.V, in.light_max,
+                  (uint32_t)opt.pull_serial_max, p.ngroups};
+    hipLaunchKernelGGL(bfs_pull_kernel, dim3(p.pull_grid), dim3(kBlock), 0, st, b.ctl, o, a,
+                       set_out, b.exam);
   }
+  {
+    ProfScope ps(prof ? "bfs_push_heavy" : nullptr);
+    BfsPushArgs a{out.index, out.nbr, b.q[0], b.q[1], b.q[2], b.qc};
+    BfsHeavyArgs h{in.nbr, in.chunk_v, in.chunk_beg, in.chunk_end, fin, in.n_chunks};
+    hipLaunchKernelGGL(bfs_push_heavy_kernel, dim3(p.grid_b), dim3(kBlock), 0, st, b.ctl, o, a, h,
+                       set_out, b.exam, p.push_blocks, p.heavy_blocks);
+  }
+  hipLaunchKernelGGL(bfs_decide_kernel, dim3(1), dim3(kBlock), 0, st, b.ctl, set_out, b.qc, b.exam,
+                     b.lv, b.hf);
+}
+
+// Statistics from the control block read back after the traversal.
+static void bfs_stats(const BfsCtl &c, double ms, gg_run_stats *stats) {
   gg_run_stats r;
   memset(&r, 0, sizeof(r));
   r.iterations = c.iter;
@@ -812,5 +808,189 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
     r.it_unvisited[i] = c.unvisited[i];
   }
   put_stats(stats, r);
+}
+
+// ------------------------------------------------- allocation-free variant
+// gg_bfs_noalloc keeps every buffer of the traversal in `scratch`, one
+// array per GPU that lives as long as the process (the loader places it
+// with the code object), so the call itself allocates nothing: no
+// hipMalloc, no pinned host memory, no events.  Its size bounds the graphs
+// it takes (kNoAllocMaxVertices) and the level counts it records.  Calls on
+// one GPU share the array: calls that overlap on a GPU race.
+constexpr uint32_t kNoAllocMaxVertices = 1u << 20;
+constexpr uint32_t kNoAllocMaxLevels = 1u << 16;
+constexpr size_t kNoAllocWords = kNoAllocMaxVertices / 64 + 1;
+constexpr size_t na_align(size_t x) { return (x + 255) & ~(size_t)255; }
+// byte offsets of the buffers in the array
+constexpr size_t kNaCtl = 0;
+constexpr size_t kNaSets = na_align(kNaCtl + sizeof(BfsCtl));
+constexpr size_t kNaQc = na_align(kNaSets + 2 * kScalarSlots * kCountStride * 8);
+constexpr size_t kNaFlag = na_align(kNaQc + sizeof(Counters));
+constexpr size_t kNaExam = na_align(kNaFlag + sizeof(BfsHostFlag));
+constexpr size_t kNaLv = na_align(kNaExam + 1024 * 8);
+constexpr size_t kNaVisited = na_align(kNaLv + (size_t)kNoAllocMaxLevels * 4);
+constexpr size_t kNaF0 = na_align(kNaVisited + kNoAllocWords * 8);
+constexpr size_t kNaF1 = na_align(kNaF0 + kNoAllocWords * 8);
+constexpr size_t kNaQ0 = na_align(kNaF1 + kNoAllocWords * 8);
+constexpr size_t kNaQ1 = na_align(kNaQ0 + (size_t)kNoAllocMaxVertices * 4);
+constexpr size_t kNaQ2 = na_align(kNaQ1 + (size_t)kNoAllocMaxVertices * 4);
+constexpr size_t kNoAllocBytes = na_align(kNaQ2 + (size_t)kNoAllocMaxVertices * 4);
+static_assert(kNoAllocBytes < ((size_t)14 << 20), "scratch stays a modest 14 MiB");
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+// The array's device address on each GPU (the host-side name of `scratch`
+// is not one), looked up once per device with that device current.
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+int bfs_noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(GG_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return GG_OK;
+  void *ptr = nullptr;
+  GG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return fail(GG_EHIP, "scratch array has no device address");
+  g_scratch_ptr[device] = ptr;
+  return GG_OK;
+}
+
+static int scratch_bufs(uint32_t V, int cap, BfsBufs *b) {
+  int dev = -1;
+  GG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(GG_EINVAL, "device index out of range");
+  uint8_t *base;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = (uint8_t *)g_scratch_ptr[dev];
+  }
+  if (!base) {  // initialised before this device was current: first call here
+    GG_TRY(bfs_noalloc_setup(dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = (uint8_t *)g_scratch_ptr[dev];
+  }
+  b->ctl = (BfsCtl *)(base + kNaCtl);
+  b->sets = (uint64_t *)(base + kNaSets);
+  b->qc = (Counters *)(base + kNaQc);
+  b->hf = (BfsHostFlag *)(base + kNaFlag);
+  b->exam = options().count_examined ? (unsigned long long *)(base + kNaExam) : nullptr;
+  b->lv = (uint32_t *)(base + kNaLv);
+  b->lv_cap = (uint32_t)std::min<int64_t>(std::max(cap, 1), (int64_t)V + 1);
+  b->visited = (uint64_t *)(base + kNaVisited);
+  b->f[0] = (uint64_t *)(base + kNaF0);
+  b->f[1] = (uint64_t *)(base + kNaF1);
+  b->q[0] = (uint32_t *)(base + kNaQ0);
+  b->q[1] = (uint32_t *)(base + kNaQ1);
+  b->q[2] = (uint32_t *)(base + kNaQ2);
+  b->words = ((size_t)V >> 6) + 1;
+  return GG_OK;
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *d_depth,
+                      uint32_t *h_frontier, int cap, gg_run_stats *stats) {
+  GG_TRY(check_graph(g));
+  if (root >= g->vertices) return fail(GG_EINVAL, "root out of range");
+  if (!d_parent) return fail(GG_EINVAL, "null output");
+  Work *w;
+  GG_TRY(work_get(g, &w));
+  const uint32_t V = g->vertices;
+  const uint32_t lv_need = (uint32_t)std::min<int64_t>(std::max(cap, 1), (int64_t)V + 1);
+  BfsState *s;
+  GG_TRY(bfs_state(w, lv_need, &s));
+  const Options &opt = options();
+  BfsBufs b{s->ctl, s->sets, s->qc, s->lv, s->lv_cap, s->h_flag,
+            opt.count_examined ? w->exam : nullptr, w->visited, {w->f[0].bits, w->f[1].bits},
+            {w->f[0].q[0], w->f[0].q[1], w->f[0].q[2]}, w->words};
+  const BfsPlan p = bfs_plan(g, root, d_parent, d_depth, b.words);
+  hipStream_t st = stream();
+  volatile BfsHostFlag *hf = s->h_flag;
+  hf->decided = 0;
+  hf->done = 0;
+  DeviceTimer tm;
+  bfs_issue_start(g, p, b, st);
+  // iterations queued ahead of the device: before issuing iteration n, wait
+  // until iteration n - ahead has been decided (iteration n - ahead - 1 has
+  // finished).  Short levels take less device time than the host needs to
+  // see a decision and issue four launches; each iteration queued past the
+  // end runs as four launches that return at once.
+  const uint32_t ahead = (uint32_t)std::max<int64_t>(1, opt.bfs_ahead);
+  for (uint32_t n = 0;; n++) {
+    if (n >= ahead) {
+      const uint32_t need = n - ahead + 1;
+      uint64_t spins = 0;
+      while (!hf->done && hf->decided < need) {
+        if ((++spins & 0xFFF) == 0 && hipStreamQuery(st) == hipSuccess && !hf->done &&
+            hf->decided < need)
+          return fail(GG_EHIP, "gg_bfs: device loop stopped without a decision");
+      }
+      if (hf->done) break;
+    }
+    if (n > V + 2) return fail(GG_EHIP, "gg_bfs: traversal did not terminate");
+    bfs_issue_iteration(g, p, b, n, st);
+    GG_HIP(hipGetLastError());
+  }
+  GG_HIP(hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, st));
+  const double ms = tm.stop();
+  const BfsCtl &c = *s->h_ctl;
+  if (h_frontier && cap > 0) {
+    const uint32_t levels = std::min<uint32_t>(c.iter, (uint32_t)cap);
+    if (levels) {
+      GG_HIP(hipMemcpyAsync(h_frontier, s->lv, (size_t)levels * 4, hipMemcpyDeviceToHost, st));
+      GG_HIP(hipStreamSynchronize(st));
+    }
+  }
+  bfs_stats(c, ms, stats);
+  return GG_OK;
+}
+
+// No host flag to poll: the host issues iterations in rounds and reads the
+// control block's `done` word back after each round (stream order: a copy
+// into host memory it owns, no pinned buffer).  Iterations issued past the
+// end return at once.
+extern "C" int gg_bfs_noalloc(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *d_depth,
+                              uint32_t *h_frontier, int cap, gg_run_stats *stats) {
+  GG_TRY(check_graph(g));
+  if (root >= g->vertices) return fail(GG_EINVAL, "root out of range");
+  if (!d_parent) return fail(GG_EINVAL, "null output");
+  const uint32_t V = g->vertices;
+  if (V > kNoAllocMaxVertices)
+    return fail(GG_EINVAL, "gg_bfs_noalloc: the graph has more than 2^20 vertices, the most its "
+                           "scratch array holds (use gg_bfs)");
+  if (cap > (int)kNoAllocMaxLevels)
+    return fail(GG_EINVAL, "gg_bfs_noalloc: cap above 65536 level counts, the most its scratch "
+                           "array records");
+  BfsBufs b;
+  GG_TRY(scratch_bufs(V, cap, &b));
+  const BfsPlan p = bfs_plan(g, root, d_parent, d_depth, b.words);
+  hipStream_t st = stream();
+  const auto t0 = std::chrono::steady_clock::now();
+  bfs_issue_start(g, p, b, st);
+  constexpr uint32_t kRound = 4;  // iterations per round
+  uint32_t done = 0;
+  for (uint32_t n = 0; !done; n += kRound) {
+    if (n > V + 2) return fail(GG_EHIP, "gg_bfs_noalloc: traversal did not terminate");
+    for (uint32_t k = 0; k < kRound; k++) bfs_issue_iteration(g, p, b, n + k, st, false);
+    GG_HIP(hipGetLastError());
+    GG_HIP(hipMemcpyAsync(&done, &b.ctl->done, sizeof(done), hipMemcpyDeviceToHost, st));
+    GG_HIP(hipStreamSynchronize(st));
+  }
+  BfsCtl c;
+  GG_HIP(hipMemcpyAsync(&c, b.ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, st));
+  GG_HIP(hipStreamSynchronize(st));
+  const double ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (h_frontier && cap > 0) {
+    const uint32_t levels = std::min<uint32_t>(c.iter, (uint32_t)cap);
+    if (levels) {
+      GG_HIP(hipMemcpyAsync(h_frontier, b.lv, (size_t)levels * 4, hipMemcpyDeviceToHost, st));
+      GG_HIP(hipStreamSynchronize(st));
+    }
+  }
+  bfs_stats(c, ms, stats);
   return GG_OK;
 }

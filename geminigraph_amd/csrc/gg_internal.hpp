@@ -267,6 +267,9 @@ int graph_finish(gg_graph *g);
 int work_get(gg_graph *g, Work **w);
 void work_free(Work *w);
 void bfs_state_free(void *state);
+// Looks up the device address of gg_bfs_noalloc's scratch array on `device`
+// (current); called once per device at initialisation.
+int bfs_noalloc_setup(int device);
 void msd_state_free(void *state);
 // PageRank plan (prplan.hip): *out = null when the graph gets the plain pull
 // (not relabelled, option pr_hot_sources = 0, or 2^32 arcs or more).

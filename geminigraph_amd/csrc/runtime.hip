@@ -44,7 +44,8 @@ int init_device(int device) {
   }
   GG_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   g_device = device;
-  return GG_OK;
+  // the allocation-free entry point's scratch array, on this device
+  return bfs_noalloc_setup(device);
 }
 
 int ensure_init() {
@@ -93,7 +94,7 @@ static std::mutex g_prof_mu;
 bool prof_on() { return g_prof_on; }
 
 ProfScope::ProfScope(const char *n) : name(n) {
-  if (!g_prof_on) return;
+  if (!g_prof_on || !n) return;  // a null name: never timed (no events created)
   if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess) {
     start = stop = nullptr;
     return;

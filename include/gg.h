@@ -225,6 +225,20 @@ int gg_pagerank(gg_graph *g, int iterations, double *d_curr_out, double *h_delta
  * Returns the number of levels in stats->iterations. */
 int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *d_depth,
            uint32_t *h_frontier, int cap, gg_run_stats *stats);
+/* gg_bfs without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device or pinned memory during a call (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and statistics as gg_bfs (depths and
+ * level sizes identical; parents valid, as races may pick other ones), but
+ * every traversal buffer lives in one fixed scratch array per GPU that the
+ * library holds until the process ends (about 13 MiB), so the call makes no
+ * hipMalloc, hipFree, pinned-memory or event calls; device_ms is host wall
+ * time.  Limits: at most 2^20 vertices and cap <= 65536 (GG_EINVAL beyond).
+ * gg_init (or any earlier call) must have initialised the library.  Calls
+ * on one GPU share the array: calls that overlap on one GPU race, so
+ * serialise them (one stream, or a lock). */
+int gg_bfs_noalloc(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *d_depth,
+                   uint32_t *h_frontier, int cap, gg_run_stats *stats);
 /* Connected components by min-label propagation, toolkits/cc.cpp:22-109,
  * on a graph built with undirected != 0. */
 int gg_cc(gg_graph *g, uint32_t *d_label, uint32_t *h_frontier, int cap,
