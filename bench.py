@@ -223,6 +223,9 @@ def pagerank_extra(gg, g, V, E, iterations=20, runs=3):
     rel = float(np.max(np.abs(ranks - pulled) / np.abs(pulled)))
     b_iter = 12 * E + 28 * V
     return {
+        "timed_region": "ms: gg_pagerank calls after the first (the plan built), output "
+                        "preallocated; first_call_ms: the first call, which builds the bin plan "
+                        "(the reference-equivalent first compute(), pagerank.cpp:27-105)",
         "iterations": iterations,
         "ms": round(pr_s * 1e3, 3),
         "gteps": round(iterations * E / pr_s / 1e9, 2),
@@ -255,7 +258,9 @@ def cc_extra(gg, scale, ef, seed, reorder):
     cc_s = time.perf_counter() - t
     lab.free()
     gu.close()
-    return {"ms": round(cc_s * 1e3, 3), "iterations": st.iterations,
+    return {"timed_region": "one gg_cc call after a warm call (which builds the bin plan), "
+                            "labels preallocated",
+            "ms": round(cc_s * 1e3, 3), "iterations": st.iterations,
             "push_iterations": st.push_iterations, "gteps_2E": round(2 * E / cc_s / 1e9, 2)}
 
 
@@ -319,12 +324,136 @@ def rmat26_extra(gg, seed, scale=26, ef=16, n_roots=8, iterations=20):
     return out
 
 
-def cpu_baseline(scale, ef, seed, roots, gpu_depths, sample_roots, repeats):
-    import oracle
+def cpu_threads():
     threads = os.environ.get("OMP_NUM_THREADS")
     if not threads:
         threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(int(threads), 64))
+    return max(1, min(int(threads), 64))
+
+
+CPU_KIND = ("the oracle: this build's C/OpenMP restatement of the reference's engine and "
+            "toolkits (not the reference binary, which needs MPI), timed as pagerank.cpp:144-147 "
+            "times compute(): one untimed run, then timed runs, graph load excluded")
+
+
+def timed_runs(fn, runs):
+    """One untimed run (the reference's first compute()), then `runs` timed
+    ones; returns (median, min) seconds."""
+    import oracle
+    fn()
+    ts = []
+    for _ in range(runs):
+        t = oracle.wall_time()
+        fn()
+        ts.append(oracle.wall_time() - t)
+    return float(np.median(ts)), float(np.min(ts))
+
+
+def cpu_extras(scale, ef, seed, gpu):
+    """CPU columns beside the GPU figures of `extras`, on the same graphs:
+    PageRank on RMAT-16 (the reference's own CPU config) and RMAT-24, CC on
+    the symmetrised RMAT-24 graph, SSSP on the weighted RMAT-24 graph.
+    Bounded samples keep the host work to about a minute."""
+    import oracle
+    threads = cpu_threads()
+    oracle.set_threads(threads)
+    out = {"cores": threads, "kind": "port", "what": CPU_KIND}
+    # PageRank, RMAT-16 edge factor 16, 20 iterations: the whole config, 1 + 5 runs
+    e16 = oracle.rmat_edges(16, 16, seed=seed, permute=True)
+    o16 = oracle.Graph(e16, 1 << 16)
+    med, mn = timed_runs(lambda: o16.pagerank(20), 5)
+    out["pagerank_rmat16"] = {"ms_median": round(med * 1e3, 2), "ms_min": round(mn * 1e3, 2),
+                              "gteps": round(20 * len(e16) / med / 1e9, 3),
+                              "sample": "20 iterations, 1 untimed + 5 timed runs"}
+    del o16, e16
+    V = 1 << scale
+    e = oracle.rmat_edges(scale, ef, seed=seed, permute=True)
+    o = oracle.Graph(e, V)
+    it = 4
+    med, mn = timed_runs(lambda: o.pagerank(it), 5)
+    out["pagerank"] = {"ms_per_iter_median": round(med * 1e3 / it, 2),
+                       "ms_per_iter_min": round(mn * 1e3 / it, 2),
+                       "gteps": round(it * len(e) / med / 1e9, 3),
+                       "sample": f"{it} iterations per run (the per-iteration cost of the 20-"
+                                 f"iteration run), 1 untimed + 5 timed runs, RMAT-{scale}"}
+    del o
+    ou = oracle.Graph(e, V, undirected=True)
+    med, mn = timed_runs(lambda: ou.cc(), 3)
+    out["cc"] = {"ms_median": round(med * 1e3, 1), "ms_min": round(mn * 1e3, 1),
+                 "gteps_2E": round(2 * len(e) / med / 1e9, 3),
+                 "sample": "the whole CC, 1 untimed + 3 timed runs"}
+    del ou, e
+    ew = oracle.rmat_edges(scale, ef, seed=seed + 3, permute=True, weighted=True)
+    ow = oracle.Graph(ew, V)
+    del ew
+    roots = gpu.get("sssp_roots", [])[:3]
+    od = ow.out_degree().astype(np.int64)
+    ts, m = [], 0
+    ow.sssp(roots[0])
+    for r in roots:
+        t = oracle.wall_time()
+        dist, _ = ow.sssp(r)
+        ts.append(oracle.wall_time() - t)
+        m += int(od[dist < 1e9].sum())
+    out["sssp"] = {"ms_mean": round(1e3 * float(np.mean(ts)), 1),
+                   "gteps": round(m / sum(ts) / 1e9, 3),
+                   "sample": f"{len(roots)} of the GPU leg's roots after one untimed run, "
+                             f"weighted RMAT-{scale}"}
+    del ow
+    return out
+
+
+def sssp24_extra(gg, scale, ef, seed, n_roots=8):
+    """SSSP (plain method) from 8 seeded roots on the weighted RMAT-24 graph
+    (relabelled, the weighted bin plan): the GPU figure the CPU column of the
+    same graph stands beside."""
+    V = 1 << scale
+    g = gg.Graph.rmat(scale, ef, seed=seed, permute=True, weighted=True, reorder=True)
+    roots = pick_roots(g.out_degree(), n_roots, seed)
+    dist = gg.DeviceArray(V, np.float32)
+    g.sssp_into(roots[0], dist)
+    gg.synchronize()
+    times, m = [], 0
+    for r in roots:
+        t = time.perf_counter()
+        st = g.sssp_into(r, dist)
+        gg.synchronize()
+        times.append(time.perf_counter() - t)
+        m += st.m_reach
+    dist.free()
+    g.close()
+    return {"workload": f"SSSP from {n_roots} roots, weighted RMAT-{scale} edge factor {ef}, "
+                        f"seed {seed}, relabelled",
+            "ms_mean": round(1e3 * float(np.mean(times)), 2), "gteps": round(m / sum(times) / 1e9, 2),
+            "sssp_roots": roots, "timed_region": "each gg_sssp call, outputs preallocated"}
+
+
+def pagerank16_extra(gg, seed, iterations=20, runs=5):
+    """PageRank on the reference's CPU config (RMAT-16 edge factor 16, 20
+    iterations): the GPU's figure for the config itself, 1 + 5 runs as
+    pagerank.cpp:144-147, each with its own output allocation."""
+    g = gg.Graph.rmat(16, 16, seed=seed, permute=True, reorder=True)
+    ts = []
+    for i in range(runs + 1):
+        t = time.perf_counter()
+        pr = gg.DeviceArray(1 << 16, np.float64)
+        g.pagerank_into(pr, iterations)
+        gg.synchronize()
+        el = time.perf_counter() - t
+        pr.free()
+        if i:
+            ts.append(el)
+    g.close()
+    med = float(np.median(ts))
+    return {"ms_median": round(med * 1e3, 3), "ms_min": round(min(ts) * 1e3, 3),
+            "gteps": round(iterations * (1 << 20) / med / 1e9, 2),
+            "timed_region": "compute() as pagerank.cpp:27-105: output allocation, the plan build "
+                            "of the first call (run 0, untimed), 20 iterations"}
+
+
+def cpu_baseline(scale, ef, seed, roots, gpu_depths, sample_roots, repeats):
+    import oracle
+    threads = cpu_threads()
     oracle.set_threads(threads)
     t0 = time.time()
     e = oracle.rmat_edges(scale, ef, seed=seed, permute=True)
@@ -446,7 +575,7 @@ def main():
     ap.add_argument("--seed", type=int, default=24)
     ap.add_argument("--cpu-sample-roots", type=int, default=16)
     ap.add_argument("--cpu-repeats", type=int, default=6)
-    ap.add_argument("--parity-roots", type=int, default=2,
+    ap.add_argument("--parity-roots", type=int, default=16,
                     help="roots whose parent-tree depths are checked against the oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
@@ -588,7 +717,30 @@ def main():
             roofline["traffic_over_algorithmic"] = round(traffic / roofline["bytes_per_launch"], 3)
         log(f"traffic: {roofline['traffic']} B/launch {detail}")
 
-    extras = {"build_s": round(build_s, 2)}
+    extras = {"build_s": round(build_s, 2),
+              "bfs_timed_region": "value: the 16 gg_bfs calls of each step, parents written into "
+                                  "one preallocated array, the traversal buffers (frontiers, "
+                                  "visited) kept by the graph from the first call on; graph "
+                                  "build excluded as in the reference"}
+    # the reference's region (bfs.cpp:23-86): compute() allocates the parent
+    # array inside exec_time; here each call also allocates (and, after the
+    # timer, frees) its own parent array
+    ref_ts, ref_m = [], 0
+    for r in roots:
+        t = time.perf_counter()
+        par = gg.DeviceArray(V, np.uint32)
+        st = g.bfs_into(r, par)
+        gg.synchronize()
+        ref_ts.append(time.perf_counter() - t)
+        ref_m += st.m_reach
+        par.free()
+    extras["bfs_reference_region"] = {
+        "ms_per_bfs": round(1e3 * float(np.mean(ref_ts)), 4),
+        "gteps": round(ref_m / sum(ref_ts) / 1e9, 2),
+        "timed_region": "each call allocates its parent array (hipMalloc) inside the timer, as "
+                        "bfs.cpp:23-29 does with alloc_vertex_array; the traversal buffers, which "
+                        "bfs.cpp also allocates per call (alloc_vertex_subset), stay with the graph"}
+    log(f"BFS, reference region: {extras['bfs_reference_region']}")
     # the same roots with the depth array written as well (8 bytes per vertex
     # found instead of 4)
     step(with_depth=True)
@@ -620,8 +772,12 @@ def main():
         del g
         parent.free()
         depth.free()
+        extras["pagerank_rmat16"] = pagerank16_extra(gg, args.seed)
+        log(f"PageRank RMAT-16: {extras['pagerank_rmat16']}")
         extras["cc"] = cc_extra(gg, scale, ef, args.seed, reorder)
         log(f"CC: {extras['cc']}")
+        extras["sssp"] = sssp24_extra(gg, scale, ef, args.seed + 3)
+        log(f"SSSP: {extras['sssp']}")
         extras["batched"] = batch_extra(gg, args.seed + 7, reorder)
         log(f"batched: {extras['batched']}")
         extras["rmat26"] = rmat26_extra(gg, args.seed + 2)
@@ -636,6 +792,9 @@ def main():
                                    args.cpu_sample_roots, args.cpu_repeats)
         extras["depth_parity_vs_oracle"] = parity
         log(f"CPU baseline: {cpu}, parity={parity}")
+        if not args.no_extras:
+            extras["cpu"] = cpu_extras(scale, ef, args.seed, extras.get("sssp", {}))
+            log(f"CPU columns: {extras['cpu']}")
 
     out = {
         "metric": "BFS GTEPS (RMAT-24, 16 roots, plain method)",
