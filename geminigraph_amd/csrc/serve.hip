@@ -4,9 +4,12 @@
 //
 // Workers are threads, each bound to its own HIP stream and traversal-context
 // slot, so the batches they run overlap on the device.  A free worker takes
-// the oldest pending query and every other pending query of the same graph
-// and kind (up to 64) -- queries that arrived while the previous batches ran
-// join the next one.  Device memory is charged against the budget when a
+// the oldest pending query whose (graph, kind) has no batch running, and
+// every other pending query of the same graph and kind (up to 64) --
+// queries that arrived while the previous batch of their kind ran join the
+// next one.  One batch per (graph, kind) at a time keeps batches full (a
+// batched traversal costs nearly as much for a few roots as for 64) and
+// stops one graph and kind from taking every worker.  Device memory is charged against the budget when a
 // graph is added and the first time a worker serves a graph (its traversal
 // context and 64-root result rows); a worker that would exceed the budget
 // first drops its own buffers for other graphs, and fails the batch with
@@ -73,6 +76,7 @@ struct gg_server {
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<gg::SQuery *> pending;
+  std::vector<unsigned char> running;  // [graph * 3 + kind]: a batch is running
   std::map<uint64_t, gg::SQuery *> queries;
   uint64_t next_id = 1, submitted = 0, completed = 0, batches = 0, max_batch = 0;
   bool stop = false;
@@ -154,17 +158,31 @@ static int run_batch(gg_graph *g, WorkerBuf &b, int kind, std::vector<SQuery *> 
   return GG_OK;
 }
 
+// The oldest pending query whose graph and kind have no batch running
+// (caller holds mu); pending.end() if there is none.
+static std::deque<SQuery *>::iterator runnable_locked(gg_server *s) {
+  for (auto it = s->pending.begin(); it != s->pending.end(); ++it)
+    if (!s->running[(size_t)(*it)->graph * 3 + (*it)->kind]) return it;
+  return s->pending.end();
+}
+
 static void worker_main(gg_server *s, int k) {
   hipSetDevice(s->device);
   bind_thread(s->streams[k], k + 1);
   std::unique_lock<std::mutex> lk(s->mu);
   while (true) {
-    s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
-    if (s->pending.empty()) break;  // stop requested and drained
-    SQuery *head = s->pending.front();
+    std::deque<SQuery *>::iterator first;
+    s->cv_work.wait(lk, [&] {
+      if (s->stop && s->pending.empty()) return true;
+      first = runnable_locked(s);
+      return first != s->pending.end();
+    });
+    if (s->stop && s->pending.empty()) break;  // stop requested and drained
+    SQuery *head = *first;
     int gi = head->graph, kind = head->kind;
+    s->running[(size_t)gi * 3 + kind] = 1;
     std::vector<SQuery *> batch;
-    for (auto it = s->pending.begin(); it != s->pending.end() && batch.size() < 64;) {
+    for (auto it = first; it != s->pending.end() && batch.size() < 64;) {
       if ((*it)->graph == gi && (*it)->kind == kind) {
         batch.push_back(*it);
         it = s->pending.erase(it);
@@ -183,6 +201,8 @@ static void worker_main(gg_server *s, int k) {
     }
     double t = now_s();
     lk.lock();
+    s->running[(size_t)gi * 3 + kind] = 0;
+    s->cv_work.notify_all();  // queries of this graph and kind may be waiting
     for (SQuery *q : batch) {
       q->t_done = t;
       q->status = r;
@@ -249,6 +269,7 @@ int gg_server_add_graph(gg_server *s, gg_graph *g, int *graph_id) {
     if (h == g) return fail(GG_EINVAL, "gg_server_add_graph: graph already added");
   s->used += g->device_bytes;
   s->graphs.push_back(g);
+  s->running.resize(s->graphs.size() * 3, 0);
   for (auto &v : s->bufs) v.emplace_back();
   *graph_id = (int)s->graphs.size() - 1;
   return GG_OK;
