@@ -183,6 +183,18 @@ def serve_extra(gg, seed, reorder, seconds, rate=2000.0, workers=4, budget=200e9
         m = kind == k
         out[f"{name}_p50_ms"] = round(float(np.percentile(lat[m], 50)), 2)
         out[f"{name}_p99_ms"] = round(float(np.percentile(lat[m], 99)), 2)
+    # a backlog shows as latency growing over the trace: both halves by arrival
+    half = arr < arr[-1] / 2
+    for tag, m in (("first_half", half), ("second_half", ~half)):
+        out[f"{tag}_p50_ms"] = round(float(np.percentile(lat[m], 50)), 2)
+        out[f"{tag}_p99_ms"] = round(float(np.percentile(lat[m], 99)), 2)
+    # steady-state throughput: completions per second over the middle 80% of
+    # the trace (n / wall also counts the drain of the last batches)
+    done_s = arr + lat / 1e3
+    t0, t1 = 0.1 * arr[-1], 0.9 * arr[-1]
+    out["completion_rate_per_s"] = round(float(((done_s >= t0) & (done_s < t1)).sum() / (t1 - t0)), 1)
+    out["sustained"] = bool(out["completion_rate_per_s"] >= 0.99 * rate and
+                            out["second_half_p99_ms"] <= 1.5 * out["first_half_p99_ms"])
     del graphs
     return out
 
@@ -321,6 +333,81 @@ def rmat26_extra(gg, seed, scale=26, ef=16, n_roots=8, iterations=20):
                 "pagerank_frac": round(iterations * b_iter / pr_s / 1e9 / HBM_PEAK_GBS, 4),
                 "pagerank_pull_ms_per_iter": round(pull_s * 1e3 / iterations, 3),
                 "pagerank_max_rel_vs_pull": float(np.max(np.abs(plan - pulled) / np.abs(pulled)))})
+    return out
+
+
+def facade_extra(gg, scale, ef, seed, roots, iterations=20):
+    """The facade toolkits (geminigraph_amd/toolkits: the reference's BFS and
+    PageRank compute() over Graph<Empty>::process_edges with device
+    callbacks) run as the reference's programs are: on a binary edge-list
+    file, 1 + 5 compute() calls, each printing its exec_time.  Beside them,
+    the C-ABI calls on the same graph (file ids, no relabeling, as the facade
+    keeps them), timed the same way."""
+    import re
+    import subprocess
+    import tempfile
+    V, E = 1 << scale, (1 << scale) * ef
+    tk = os.path.join(ROOT, "geminigraph_amd", "toolkits")
+    if not all(os.path.exists(os.path.join(tk, n)) for n in ("bfs", "pagerank")):
+        return {"error": "facade toolkits not built (make -C geminigraph_amd toolkits)"}
+    buf = gg.rmat_device(scale, E, seed, permute=True)
+    host = np.empty(E * 8, dtype=np.uint8)
+    gg.lib().gg_memcpy_d2h(host.ctypes.data, gg.ctypes.c_void_p(buf.ptr), E * 8)
+    g = gg.Graph.build_device(buf, E, V, hub_first=True)  # the facade's own load flags
+    buf.free()
+    d = tempfile.mkdtemp(prefix="ggfacade", dir="/tmp")
+    path = os.path.join(d, "rmat.bin")
+    host.tofile(path)
+    del host
+
+    def run(name, arg):
+        p = subprocess.run([os.path.join(tk, name), path, str(V), str(arg)], capture_output=True,
+                           text=True, timeout=300, env=dict(os.environ, GG_RUNS="6"))
+        if p.returncode != 0:
+            raise RuntimeError(f"{name}: {p.stderr[-500:]}")
+        t = [float(x) for x in re.findall(r"exec_time=([0-9.eE+-]+)\(s\)", p.stdout)]
+        return t[1:]  # the first compute() is the warm-up, as pagerank.cpp:144-147
+
+    out = {"workload": f"RMAT-{scale} edge factor {ef}, permuted ids, not relabelled (the facade "
+                       f"keeps file ids; lists hub-first), {len(roots)} of the bench roots",
+           "timed_region": "exec_time of compute() as the toolkits print it (allocation of the "
+                           "vertex arrays and subsets included, load excluded), median of the "
+                           "5 runs after the first; C-ABI: the same calls with per-call output "
+                           "allocation"}
+    try:
+        bfs_t = [float(np.median(run("bfs", r))) for r in roots]
+        pr_t = float(np.median(run("pagerank", iterations)))
+    finally:
+        os.remove(path)
+        os.rmdir(d)
+    cabi_bfs = []
+    for r in roots:
+        ts = []
+        for _ in range(6):
+            t = time.perf_counter()
+            par = gg.DeviceArray(V, np.uint32)
+            g.bfs_into(r, par)
+            gg.synchronize()
+            ts.append(time.perf_counter() - t)
+            par.free()
+        cabi_bfs.append(float(np.median(ts[1:])))
+    ts = []
+    for _ in range(6):
+        t = time.perf_counter()
+        pr = gg.DeviceArray(V, np.float64)
+        g.pagerank_into(pr, iterations)
+        gg.synchronize()
+        ts.append(time.perf_counter() - t)
+        pr.free()
+    cabi_pr = float(np.median(ts[1:]))
+    g.close()
+    out.update({
+        "bfs_facade_ms": round(1e3 * float(np.mean(bfs_t)), 3),
+        "bfs_cabi_ms": round(1e3 * float(np.mean(cabi_bfs)), 3),
+        "bfs_facade_over_cabi": round(float(np.mean(bfs_t)) / float(np.mean(cabi_bfs)), 2),
+        "pagerank_facade_ms": round(1e3 * pr_t, 2),
+        "pagerank_cabi_ms": round(1e3 * cabi_pr, 2),
+        "pagerank_facade_over_cabi": round(pr_t / cabi_pr, 2)})
     return out
 
 
@@ -564,6 +651,113 @@ def measure_traffic(args):
                      "launches": pull_n, "workload": "one bench step (same graph and roots)"}
 
 
+LEVEL_PASSES = (
+    "TCC_EA0_RDREQ_sum,TCC_EA0_RDREQ_32B_sum,TCC_EA0_WRREQ_sum,TCC_EA0_WRREQ_64B_sum",
+    "TCC_EA0_RDREQ_64B_sum,TCC_EA0_RDREQ_128B_sum,TCC_HIT_sum,TCC_MISS_sum",
+    "TCC_READ_sum,TCC_WRITE_sum,TCC_ATOMIC_sum,TCC_REQ_sum",
+)
+
+
+def level_child(args):
+    """Workload of the per-level passes: the bench graph, one untimed BFS,
+    then BFS from the first bench root with the edge counters on; prints that
+    BFS's per-level records as JSON."""
+    import geminigraph_amd as gg
+    gg.init(0)
+    g = gg.Graph.rmat(args.scale, args.edge_factor, seed=args.seed, permute=True,
+                      reorder=not args.no_reorder)
+    roots = pick_roots(g.out_degree(), args.roots, args.seed + 1)
+    parent = gg.DeviceArray(1 << args.scale, np.uint32)
+    g.bfs_into(roots[1], parent)
+    with gg.options(count_examined=1):
+        st = g.bfs_into(roots[0], parent).as_dict()
+    gg.synchronize()
+    n = st["iterations"]
+    print(json.dumps({k: st[k][:n] for k in ("modes", "it_active", "it_new", "it_examined",
+                                               "it_unvisited")}), flush=True)
+
+
+def _csv_rows(out, suffix):
+    import csv
+    for root, _, files in os.walk(out):
+        for f in files:
+            if f.endswith(suffix):
+                return list(csv.DictReader(open(os.path.join(root, f))))
+    return []
+
+
+def _last_bfs(rows, key):
+    """Dispatches of the last BFS (after its seed kernel) grouped per level:
+    [{kernel: row}] with the level advanced at each decide."""
+    rows = sorted(rows, key=key)
+    name = lambda r: r["Kernel_Name"].split("(")[0].replace("void ", "")
+    seeds = [i for i, r in enumerate(rows) if name(r) == "gg::bfs_seed_kernel"]
+    levels, cur = [], {}
+    for r in rows[seeds[-1] + 1:]:
+        k = name(r).replace("gg::bfs_", "").replace("_kernel", "")
+        if k in ("init", "seed"):
+            break
+        cur[k] = r
+        if k == "decide":
+            levels.append(cur)
+            cur = {}
+    return levels
+
+
+def level_table(args):
+    """Per-level table of one bench BFS: mode, frontier, kernel times (kernel
+    trace) and HBM requests of the bottom-up and push/heavy kernels (three
+    counter passes), printed as markdown."""
+    import subprocess
+    import tempfile
+    prof = "rocprofv3"
+    tmp = tempfile.mkdtemp(prefix="gglev", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    child = [sys.executable, os.path.abspath(__file__), "--level-child", "--scale", str(args.scale),
+             "--edge-factor", str(args.edge_factor), "--roots", str(args.roots),
+             "--seed", str(args.seed)] + (["--no-reorder"] if args.no_reorder else [])
+    runs = [("trace", ["--kernel-trace"])] + [(f"p{i}", ["--pmc", c]) for i, c in
+                                               enumerate(LEVEL_PASSES)]
+    stats, per = None, {}
+    for tag, flags in runs:
+        out = os.path.join(tmp, tag)
+        r = subprocess.run(["timeout", "-s", "KILL", "120", prof] + flags +
+                           ["-d", out, "-o", "run", "-f", "csv", "--"] + child, cwd="/tmp", env=env,
+                           capture_output=True, text=True)
+        if tag == "trace":
+            stats = json.loads([l for l in r.stdout.splitlines() if l.startswith("{")][-1])
+            per["trace"] = _last_bfs(_csv_rows(out, "kernel_trace.csv"),
+                                     key=lambda x: int(x["Dispatch_Id"]))
+        else:
+            rows = _csv_rows(out, "counter_collection.csv")
+            merged = {}
+            for x in rows:
+                d = merged.setdefault(int(x["Dispatch_Id"]), dict(x, counters={}))
+                d["counters"][x["Counter_Name"]] = float(x["Counter_Value"])
+            per[tag] = _last_bfs(list(merged.values()), key=lambda x: int(x["Dispatch_Id"]))
+    print("| level | mode | frontier | new | examined | bin us | pull us | push/heavy us | "
+          "decide us | read MB (128B/64B/32B req) | write req (64B) | TCC req | hit/miss |")
+    print("|---|---|---|---|---|---|---|---|---|---|---|---|---|")
+    for i, lv in enumerate(per["trace"]):
+        us = {k: (int(v["End_Timestamp"]) - int(v["Start_Timestamp"])) / 1e3 for k, v in lv.items()}
+        c = {}
+        for tag in ("p0", "p1", "p2"):
+            if i < len(per[tag]):
+                for k in ("pull", "push_heavy"):
+                    for n, v in per[tag][i].get(k, {}).get("counters", {}).items():
+                        c[n] = c.get(n, 0.0) + v
+        r128, r64, r32 = (c.get("TCC_EA0_RDREQ_128B_sum", 0), c.get("TCC_EA0_RDREQ_64B_sum", 0),
+                          c.get("TCC_EA0_RDREQ_32B_sum", 0))
+        read_mb = (128 * r128 + 64 * r64 + 32 * r32) / 1e6
+        print(f"| {i} | {'push' if stats['modes'][i] == 0 else 'pull'} | {stats['it_active'][i]} | "
+              f"{stats['it_new'][i]} | {stats['it_examined'][i]} | {us.get('bin', 0):.1f} | "
+              f"{us.get('pull', 0):.1f} | {us.get('push_heavy', 0):.1f} | {us.get('decide', 0):.1f} | "
+              f"{read_mb:.1f} ({int(r128)}/{int(r64)}/{int(r32)}) | "
+              f"{int(c.get('TCC_EA0_WRREQ_sum', 0))} ({int(c.get('TCC_EA0_WRREQ_64B_sum', 0))}) | "
+              f"{int(c.get('TCC_REQ_sum', 0))} | {int(c.get('TCC_HIT_sum', 0))}/"
+              f"{int(c.get('TCC_MISS_sum', 0))} |", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -581,11 +775,14 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-reorder", action="store_true",
                     help="keep file vertex ids (no degree-sorted relabeling)")
-    ap.add_argument("--serve-seconds", type=float, default=5.0,
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the serving trace (0 skips the serving leg)")
     ap.add_argument("--no-pmc", action="store_true",
                     help="skip the rocprofv3 counter passes behind roofline.traffic")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--level-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--level-table", action="store_true",
+                    help="per-level kernel times and HBM requests of one bench BFS (rocprofv3)")
     ap.add_argument("--pagerank-only", action="store_true",
                     help="run only the PageRank leg (for profiling) and print its figures")
     ap.add_argument("--batched-only", action="store_true",
@@ -594,9 +791,20 @@ def main():
                     help="run only the CC leg (for profiling) and print its figures")
     ap.add_argument("--rmat26-only", action="store_true",
                     help="run only the scale-26 SSSP/PageRank leg (for profiling)")
+    ap.add_argument("--facade-only", action="store_true",
+                    help="run only the facade-toolkit leg and print its figures")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving leg and print its figures")
+    ap.add_argument("--serve-workers", type=int, default=4)
     args = ap.parse_args()
     if args.pmc_child:
         pmc_child(args)
+        return
+    if args.level_child:
+        level_child(args)
+        return
+    if args.level_table:
+        level_table(args)
         return
     if args.pagerank_only:
         import geminigraph_amd as gg
@@ -624,6 +832,19 @@ def main():
         import geminigraph_amd as gg
         gg.init(0)
         log(f"rmat26: {rmat26_extra(gg, args.seed + 2)}")
+        return
+    if args.facade_only:
+        import geminigraph_amd as gg
+        gg.init(0)
+        g = gg.Graph.rmat(args.scale, args.edge_factor, seed=args.seed, permute=True)
+        roots = pick_roots(g.out_degree(), args.roots, args.seed + 1)[:4]
+        g.close()
+        log(f"facade: {facade_extra(gg, args.scale, args.edge_factor, args.seed, roots)}")
+        return
+    if args.serve_only:
+        import geminigraph_amd as gg
+        gg.init(0)
+        log(f"serving: {serve_extra(gg, args.seed + 11, not args.no_reorder, args.serve_seconds, workers=args.serve_workers)}")
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -772,6 +993,8 @@ def main():
         del g
         parent.free()
         depth.free()
+        extras["facade"] = facade_extra(gg, scale, ef, args.seed, roots[:4])
+        log(f"facade: {extras['facade']}")
         extras["pagerank_rmat16"] = pagerank16_extra(gg, args.seed)
         log(f"PageRank RMAT-16: {extras['pagerank_rmat16']}")
         extras["cc"] = cc_extra(gg, scale, ef, args.seed, reorder)
@@ -783,7 +1006,8 @@ def main():
         extras["rmat26"] = rmat26_extra(gg, args.seed + 2)
         log(f"rmat26: {extras['rmat26']}")
         if args.serve_seconds > 0:
-            extras["serving"] = serve_extra(gg, args.seed + 11, reorder, args.serve_seconds)
+            extras["serving"] = serve_extra(gg, args.seed + 11, reorder, args.serve_seconds,
+                                            workers=args.serve_workers)
             log(f"serving: {extras['serving']}")
 
     cpu = None

@@ -95,6 +95,11 @@ class GraphInfo(ctypes.Structure):
 
 
 BUILD_REORDER = 1
+BUILD_HUB_FIRST = 2
+
+
+def _flags(reorder, hub_first):
+    return (BUILD_REORDER if reorder else 0) | (BUILD_HUB_FIRST if hub_first else 0)
 
 
 class TuneResult(ctypes.Structure):
@@ -322,50 +327,52 @@ class Graph:
 
     # -- construction -----------------------------------------------------
     @classmethod
-    def build(cls, edges, vertices, undirected=False, reorder=False):
+    def build(cls, edges, vertices, undirected=False, reorder=False, hub_first=False):
         """From a host structured array of packed records (oracle.EDGE_DTYPE / WEDGE_DTYPE).
-        reorder=True relabels vertices internally by descending out-degree."""
+        reorder=True relabels vertices internally by descending out-degree;
+        hub_first=True keeps file ids and orders each list hubs first."""
         edges = np.ascontiguousarray(edges)
         weighted = edges.dtype.itemsize == 12
         h = ctypes.c_void_p()
         _check(lib().gg_graph_build_ex(ctypes.c_void_p(edges.ctypes.data), 0, len(edges),
                                        int(vertices), int(weighted), int(undirected),
-                                       BUILD_REORDER if reorder else 0, ctypes.byref(h)))
+                                       _flags(reorder, hub_first), ctypes.byref(h)))
         return cls(h.value)
 
     @classmethod
     def build_device(cls, dev_edges, num_edges, vertices, weighted=False, undirected=False,
-                     reorder=False):
+                     reorder=False, hub_first=False):
         h = ctypes.c_void_p()
         _check(lib().gg_graph_build_ex(ctypes.c_void_p(dev_edges.ptr), 1, int(num_edges),
                                        int(vertices), int(weighted), int(undirected),
-                                       BUILD_REORDER if reorder else 0, ctypes.byref(h)))
+                                       _flags(reorder, hub_first), ctypes.byref(h)))
         return cls(h.value)
 
     @classmethod
     def rmat(cls, scale, edge_factor=16, seed=1, a=RMAT_A, b=RMAT_B, c=RMAT_C, permute=True,
-             weighted=False, undirected=False, reorder=False):
+             weighted=False, undirected=False, reorder=False, hub_first=False):
         ne = (1 << scale) * edge_factor
         buf = rmat_device(scale, ne, seed, a, b, c, permute, weighted)
         try:
-            return cls.build_device(buf, ne, 1 << scale, weighted, undirected, reorder)
+            return cls.build_device(buf, ne, 1 << scale, weighted, undirected, reorder, hub_first)
         finally:
             buf.free()
 
     @classmethod
-    def load_directed(cls, path, vertices, weighted=False, reorder=False):
+    def load_directed(cls, path, vertices, weighted=False, reorder=False, hub_first=False):
         """core/graph.hpp:772 -- |E| = file size / record size."""
         h = ctypes.c_void_p()
         _check(lib().gg_graph_load_ex(str(path).encode(), int(vertices), int(weighted), 0,
-                                      BUILD_REORDER if reorder else 0, ctypes.byref(h)))
+                                      _flags(reorder, hub_first), ctypes.byref(h)))
         return cls(h.value)
 
     @classmethod
-    def load_undirected_from_directed(cls, path, vertices, weighted=False, reorder=False):
+    def load_undirected_from_directed(cls, path, vertices, weighted=False, reorder=False,
+                                      hub_first=False):
         """core/graph.hpp:350 -- symmetric CSR, CSC aliased, |E| kept."""
         h = ctypes.c_void_p()
         _check(lib().gg_graph_load_ex(str(path).encode(), int(vertices), int(weighted), 1,
-                                      BUILD_REORDER if reorder else 0, ctypes.byref(h)))
+                                      _flags(reorder, hub_first), ctypes.byref(h)))
         return cls(h.value)
 
     @classmethod
@@ -400,9 +407,16 @@ class Graph:
                                             array.dtype.itemsize, out.ctypes.data))
         return out
 
-    def _check_vertex_array(self, array):
-        if array.n < self.vertices:
-            raise ValueError("vertex array shorter than the graph's vertex count")
+    def _check_vertex_array(self, array, dtype=None, per_vertex=1):
+        """A DeviceArray of at least per_vertex * V elements (of `dtype`):
+        the kernels write that many, so a shorter buffer would be overrun."""
+        if not isinstance(array, DeviceArray):
+            raise TypeError("expected a DeviceArray")
+        if array.n < self.vertices * per_vertex:
+            raise ValueError(f"vertex array of {array.n} elements; the call writes "
+                             f"{self.vertices * per_vertex}")
+        if dtype is not None and array.dtype != np.dtype(dtype):
+            raise ValueError(f"vertex array of {array.dtype}; the call writes {np.dtype(dtype)}")
 
     def close(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -481,6 +495,9 @@ class Graph:
 
     # -- toolkits (device outputs; *_into variants keep results on the GPU) --
     def pagerank_into(self, out, iterations=20, deltas=None):
+        self._check_vertex_array(out, np.float64)
+        if deltas is not None and len(deltas) < iterations + 1:
+            raise ValueError("deltas needs iterations + 1 entries")
         st = RunStats()
         dptr = deltas.ctypes.data if deltas is not None else None
         _check(lib().gg_pagerank(self.handle, int(iterations), ctypes.c_void_p(out.ptr),
@@ -496,6 +513,9 @@ class Graph:
     def bfs_into(self, root, parent, depth=None, frontier=None):
         """BFS into device arrays; depth None keeps parents only (the
         reference's output, bfs.cpp:26-94)."""
+        self._check_vertex_array(parent, np.uint32)
+        if depth is not None:
+            self._check_vertex_array(depth, np.uint32)
         st = RunStats()
         fr_ptr = frontier.ctypes.data if frontier is not None else None
         cap = len(frontier) if frontier is not None else 0
@@ -508,9 +528,9 @@ class Graph:
     def bfs_noalloc_into(self, root, parent, depth=None, frontier=None):
         """bfs_into through gg_bfs_noalloc: no allocation inside the call
         (graphs of at most 2^20 vertices; overlapping calls on a GPU race)."""
-        self._check_vertex_array(parent)
+        self._check_vertex_array(parent, np.uint32)
         if depth is not None:
-            self._check_vertex_array(depth)
+            self._check_vertex_array(depth, np.uint32)
         st = RunStats()
         fr_ptr = frontier.ctypes.data if frontier is not None else None
         cap = len(frontier) if frontier is not None else 0
@@ -529,6 +549,7 @@ class Graph:
         return parent.numpy(), depth.numpy(), fr[:levels].copy(), st.as_dict()
 
     def cc_into(self, label, frontier=None):
+        self._check_vertex_array(label, np.uint32)
         st = RunStats()
         fr_ptr = frontier.ctypes.data if frontier is not None else None
         cap = len(frontier) if frontier is not None else 0
@@ -544,6 +565,7 @@ class Graph:
         return label.numpy(), fr[:n].copy(), st.as_dict()
 
     def sssp_into(self, root, dist, frontier=None):
+        self._check_vertex_array(dist, np.float32)
         st = RunStats()
         fr_ptr = frontier.ctypes.data if frontier is not None else None
         cap = len(frontier) if frontier is not None else 0
@@ -559,6 +581,8 @@ class Graph:
         return dist.numpy(), fr[:n].copy(), st.as_dict()
 
     def bc_into(self, root, dep, inv, frontier=None):
+        self._check_vertex_array(dep, np.float64)
+        self._check_vertex_array(inv, np.float64)
         st = RunStats()
         fr_ptr = frontier.ctypes.data if frontier is not None else None
         cap = len(frontier) if frontier is not None else 0
@@ -599,6 +623,7 @@ def bfs_batch_into(graph, roots, depth):
     """Up to 64 BFS roots in one traversal; depth is a DeviceArray of V*64
     uint32, vertex-major (depth[v*64 + i] = level of v from roots[i])."""
     roots = _batch_roots(roots)
+    graph._check_vertex_array(depth, np.uint32, 64)
     st = RunStats()
     _check(lib().gg_bfs_batch(graph.handle, ctypes.c_void_p(roots.ctypes.data), len(roots),
                               ctypes.c_void_p(depth.ptr), ctypes.byref(st)))
@@ -616,6 +641,7 @@ def bfs_batch(graph, roots):
 
 def sssp_batch_into(graph, roots, dist):
     roots = _batch_roots(roots)
+    graph._check_vertex_array(dist, np.float32, 64)
     st = RunStats()
     _check(lib().gg_sssp_batch(graph.handle, ctypes.c_void_p(roots.ctypes.data), len(roots),
                                ctypes.c_void_p(dist.ptr), ctypes.byref(st)))
@@ -633,6 +659,8 @@ def sssp_batch(graph, roots):
 
 def bc_batch_into(graph, roots, dep, inv):
     roots = _batch_roots(roots)
+    graph._check_vertex_array(dep, np.float64, 64)
+    graph._check_vertex_array(inv, np.float64, 64)
     st = RunStats()
     _check(lib().gg_bc_batch(graph.handle, ctypes.c_void_p(roots.ctypes.data), len(roots),
                              ctypes.c_void_p(dep.ptr), ctypes.c_void_p(inv.ptr), ctypes.byref(st)))

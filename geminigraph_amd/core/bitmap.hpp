@@ -43,8 +43,14 @@ __global__ void bitmap_fill_words(unsigned long *d, size_t words, size_t size) {
     d[i] = x;
   }
 }
+template <int = 0>
+__global__ void bitmap_set_bit(unsigned long *d, size_t i) {
+  d[WORD_OFFSET(i)] |= 1ul << BIT_OFFSET(i);
+}
 }  // namespace gg_facade
 
+// Host calls are queued on the null stream, in order with the engine's
+// launches; get_bit waits for them (a read-back).
 class Bitmap {
  public:
   size_t size;
@@ -62,24 +68,21 @@ class Bitmap {
   Bitmap &operator=(const Bitmap &) = delete;
   size_t words() const { return WORD_OFFSET(size) + 1; }
   void clear() {
-    gg_facade::hip_check(hipMemset(data, 0, sizeof(unsigned long) * words()), "Bitmap clear");
+    gg_facade::hip_check(hipMemsetAsync(data, 0, sizeof(unsigned long) * words(), 0),
+                         "Bitmap clear");
   }
   void fill() {
     hipLaunchKernelGGL(gg_facade::bitmap_fill_words<0>, dim3((words() + 255) / 256), dim3(256), 0, 0,
                        data, words(), size);
-    gg_facade::hip_check(hipDeviceSynchronize(), "Bitmap fill");
   }
-  // host-side single-bit access (a device round trip each)
+  // host-side single-bit access: a read-back, or one queued store
   unsigned long get_bit(size_t i) const {
     unsigned long w = 0;
     gg_facade::hip_check(hipMemcpy(&w, data + WORD_OFFSET(i), 8, hipMemcpyDeviceToHost), "get_bit");
     return w & (1ul << BIT_OFFSET(i));
   }
   void set_bit(size_t i) {
-    unsigned long w = 0;
-    gg_facade::hip_check(hipMemcpy(&w, data + WORD_OFFSET(i), 8, hipMemcpyDeviceToHost), "set_bit");
-    w |= 1ul << BIT_OFFSET(i);
-    gg_facade::hip_check(hipMemcpy(data + WORD_OFFSET(i), &w, 8, hipMemcpyHostToDevice), "set_bit");
+    hipLaunchKernelGGL(gg_facade::bitmap_set_bit<0>, dim3(1), dim3(1), 0, 0, data, i);
   }
   BitmapView view() const { return BitmapView{size, data}; }
 };

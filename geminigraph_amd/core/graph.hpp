@@ -47,8 +47,11 @@ inline void check(int code, const char *what) {
   }
 }
 
-constexpr int kLightMax = 256;  // lists above this go to one wave with lane sub-lists
+constexpr int kLightMax = 256;  // dense: longer in-lists go to one wave with lane sub-lists
+constexpr int kSubList = 32;    // sparse: out-lists reach slots in sub-lists of at most 32
+constexpr int kWaveList = 64 * kSubList;  // sparse: longer out-lists go to the heavy kernel
 constexpr int kSlots = 256;     // spread reducer slots (16 x 8 bytes apart)
+constexpr int kPhaseGrid = 8192;  // blocks of the persistent phase kernels
 
 template <typename M>
 struct MsgUnit {  // core/graph.hpp:61-65
@@ -56,16 +59,50 @@ struct MsgUnit {  // core/graph.hpp:61-65
   M msg_data;
 } __attribute__((packed));
 
+constexpr int kEmitSegs = 1024;  // message segments, each with its own counter
+constexpr int kPart = 2048;      // dense: heavy in-lists reach signals in parts of this many
+
+// Messages of one call.  A signal's emit goes to one of kEmitSegs segments
+// (chosen by the emitting wave, so no counter takes more than a small share
+// of the atomics; one counter for all would serialise them at one L2
+// address), or, once its segment is full, to an overflow region; a pack
+// pass then lays the messages out densely in `buf` for the slots.
 struct EmitBlock {
-  void *buf;
-  unsigned long long count;
-  unsigned long long cap;
+  void *buf;                    // packed messages the slots read
+  unsigned long long count;     // messages packed
+  void *seg_buf;                // kEmitSegs x seg_cap messages, then ovf_cap more
+  unsigned *seg_cnt;            // kEmitSegs counters, 32 apart (one 128-byte line each)
+  unsigned long long seg_cap, ovf_cap;
+  unsigned long long ovf_count;
+  unsigned long long dropped;   // messages beyond every region (the host aborts)
 };
 
-// spread reduction of R into kSlots slots
+// Device state of one process_edges / process_vertices call.  The switch is
+// decided on the device (`mode`), every phase kernel of both modes is
+// launched and returns at once in the mode not chosen, and the message and
+// heavy-list counts stay on the device, so a call makes one host round
+// trip: the read-back of this block, for the reducer the API returns.
+struct Ctl {
+  EmitBlock eb;
+  unsigned long long active_edges;
+  unsigned long long result;  // the reducer's bits (R of the call)
+  unsigned mode;              // 0 = sparse (push), 1 = dense (pull)
+  unsigned n_heavy;           // sparse messages whose lists exceed kWaveList
+  unsigned long long pack_off[kEmitSegs + 1];  // segment offsets of the packed messages
+};
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// One atomic per wave into one of kSlots spread slots (the slots are zero
+// between calls: the reduce kernel clears what it reads).
 template <typename R>
-__device__ inline void slot_add(R *slots, R v) {
-  if (v == R(0)) return;
+__device__ inline void slot_add_wave(R *slots, R v) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) || v == R(0)) return;
   R *p = slots + (size_t)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % kSlots) * 16;
   if constexpr (sizeof(R) == 8 && __is_same(R, double))
     unsafeAtomicAdd(p, v);
@@ -77,90 +114,276 @@ __device__ inline void slot_add(R *slots, R v) {
     atomicAdd((unsigned *)p, (unsigned)v);
 }
 
+// A trivially copyable value from lane `src` (4-byte pieces).
+template <typename T>
+__device__ inline T shfl_any(const T &x, int src) {
+  constexpr int n = (sizeof(T) + 3) / 4;
+  unsigned w[n] = {};
+  __builtin_memcpy(w, &x, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < n; i++) w[i] = (unsigned)__shfl((int)w[i], src);
+  T y;
+  __builtin_memcpy(&y, w, sizeof(T));
+  return y;
+}
+
 template <typename R, typename F>
 __global__ void process_vertices_kernel(F f, const unsigned long *active, VertexId V, R *slots) {
   for (VertexId base = blockIdx.x * blockDim.x; base < V; base += gridDim.x * blockDim.x) {
     VertexId v = base + threadIdx.x;
-    if (v < V && (active[WORD_OFFSET(v)] >> BIT_OFFSET(v)) & 1ul) slot_add<R>(slots, (R)f(v));
+    R x = R(0);
+    if (v < V && (active[WORD_OFFSET(v)] >> BIT_OFFSET(v)) & 1ul) x = (R)f(v);
+    slot_add_wave<R>(slots, x);
+  }
+}
+
+// One block: the sum of the slots as R into *out, the slots back to zero.
+template <typename R>
+__global__ void reduce_slots_kernel(R *slots, unsigned long long *out) {
+  __shared__ R part[4];
+  R x = slots[(size_t)threadIdx.x * 16];
+  slots[(size_t)threadIdx.x * 16] = R(0);
+  x = wave_sum(x);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    R t = part[0] + part[1] + part[2] + part[3];
+    unsigned long long bits = 0;
+    __builtin_memcpy(&bits, &t, sizeof(R));
+    *out = bits;
+  }
+}
+
+// One block: the switch (core/graph.hpp:1503-1509) from the active out-degree
+// sum in the slots, which it clears for the call's reducer; resets the
+// message count for the mode chosen.
+template <int = 0>
+__global__ void decide_kernel(Ctl *c, EdgeId *slots, EdgeId threshold) {
+  __shared__ EdgeId part[4];
+  EdgeId x = slots[(size_t)threadIdx.x * 16];
+  slots[(size_t)threadIdx.x * 16] = 0;
+  x = wave_sum(x);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const EdgeId active_edges = part[0] + part[1] + part[2] + part[3];
+    const unsigned mode = active_edges < threshold ? 0u : 1u;
+    c->active_edges = active_edges;
+    c->mode = mode;
+    c->n_heavy = 0;
+    c->eb.count = 0;
+    c->eb.ovf_count = 0;
+    c->eb.dropped = 0;
+  }
+  for (int k = threadIdx.x; k < kEmitSegs; k += blockDim.x) c->eb.seg_cnt[(size_t)k * 32] = 0;
+}
+
+// One block of kEmitSegs threads: offsets of the segments' messages in the
+// packed array, and the packed count.
+template <int = 0>
+__global__ void pack_offsets_kernel(Ctl *c) {
+  __shared__ unsigned long long wsum[kEmitSegs / 64];
+  const int t = threadIdx.x, lane = t & 63;
+  const unsigned long long cnt = c->eb.seg_cnt[(size_t)t * 32];
+  const unsigned long long n = cnt < c->eb.seg_cap ? cnt : c->eb.seg_cap;
+  unsigned long long inc = n;
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned long long y = __shfl_up(inc, off);
+    if (lane >= off) inc += y;
+  }
+  if (lane == 63) wsum[t >> 6] = inc;
+  __syncthreads();
+  unsigned long long base = 0;
+  for (int w = 0; w < (t >> 6); w++) base += wsum[w];
+  c->pack_off[t] = base + inc - n;
+  if (t == kEmitSegs - 1) {
+    c->pack_off[kEmitSegs] = base + inc;
+    const unsigned long long o = c->eb.ovf_count < c->eb.ovf_cap ? c->eb.ovf_count : c->eb.ovf_cap;
+    c->eb.count = base + inc + o;
+  }
+}
+
+// Block s copies segment s (s == kEmitSegs: the overflow region) into place.
+template <typename M>
+__global__ void pack_copy_kernel(Ctl *c) {
+  const MsgUnit<M> *src = (const MsgUnit<M> *)c->eb.seg_buf;
+  MsgUnit<M> *dst = (MsgUnit<M> *)c->eb.buf;
+  for (unsigned s = blockIdx.x; s <= (unsigned)kEmitSegs; s += gridDim.x) {
+    const unsigned long long b = c->pack_off[s < kEmitSegs ? s : kEmitSegs];
+    const unsigned long long n = s < kEmitSegs ? c->pack_off[s + 1] - b : c->eb.count - b;
+    const MsgUnit<M> *from = src + (size_t)s * c->eb.seg_cap;
+    for (unsigned long long i = threadIdx.x; i < n; i += blockDim.x) dst[b + i] = from[i];
   }
 }
 
 template <typename F>
-__global__ void sparse_signal_kernel(F f, const unsigned long *active, VertexId V) {
+__global__ void sparse_signal_kernel(F f, const Ctl *c, const unsigned long *active, VertexId V) {
+  if (c->mode != 0) return;
   for (VertexId base = blockIdx.x * blockDim.x; base < V; base += gridDim.x * blockDim.x) {
     VertexId v = base + threadIdx.x;
     if (v < V && (active[WORD_OFFSET(v)] >> BIT_OFFSET(v)) & 1ul) f(v);
   }
 }
 
-// One thread per message for short lists; long lists are queued for waves.
+// Sparse slots, 64 messages per wave step: lists up to kSubList entries go
+// to their own lane whole; longer ones (up to kWaveList) are cut into
+// kSubList-entry sub-lists spread over the wave's lanes; the longest go to
+// the heavy kernel.  Sub-lists are disjoint, as the reference's per-socket
+// ones are (core/graph.hpp:1600-1633), and the reducer is their sum.
 template <typename R, typename M, typename E, typename F>
-__global__ void sparse_slot_kernel(F f, const MsgUnit<M> *msgs, unsigned long long n,
-                                   const EdgeId *index, AdjUnit<E> *adj, R *slots,
-                                   unsigned *heavy, unsigned *n_heavy) {
-  for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
-       i += (unsigned long long)gridDim.x * blockDim.x) {
-    VertexId v = msgs[i].vertex;
-    EdgeId b = index[v], e = index[v + 1];
-    if (e == b) continue;  // outgoing_adj_bitmap test (core/graph.hpp:1630)
-    if (e - b > (EdgeId)kLightMax) {
-      heavy[atomicAdd(n_heavy, 1u)] = (unsigned)i;
-      continue;
+__global__ void sparse_slot_kernel(F f, Ctl *c, const EdgeId *index, AdjUnit<E> *adj, R *slots,
+                                   unsigned *heavy, unsigned heavy_cap) {
+  if (c->mode != 0) return;
+  const unsigned long long n = c->eb.count;
+  const MsgUnit<M> *msgs = (const MsgUnit<M> *)c->eb.buf;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long W = (unsigned long long)gridDim.x * (blockDim.x >> 6);
+  const unsigned long long gw = (unsigned long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  for (unsigned long long base = gw * 64; base < n; base += W * 64) {
+    const unsigned long long i = base + lane;
+    VertexId v = 0;
+    M msg{};
+    EdgeId b = 0, e = 0;
+    if (i < n) {
+      v = msgs[i].vertex;
+      msg = msgs[i].msg_data;
+      b = index[v];
+      e = index[v + 1];
     }
-    slot_add<R>(slots, (R)f(v, msgs[i].msg_data, VertexAdjList<E>(adj + b, adj + e)));
+    const EdgeId d = e - b;
+    R acc = R(0);
+    unsigned chunks = 0;
+    if (d > (EdgeId)kWaveList) {
+      const unsigned k = atomicAdd(&c->n_heavy, 1u);
+      if (k < heavy_cap)
+        heavy[k] = (unsigned)i;
+      else
+        atomicAdd(&c->eb.dropped, 1ull);  // more than one per vertex: the host aborts
+    } else if (d > (EdgeId)kSubList) {
+      chunks = (unsigned)((d + kSubList - 1) / kSubList);
+    } else if (d) {
+      acc += (R)f(v, msg, VertexAdjList<E>(adj + b, adj + e));
+    }
+    // exclusive scan of the chunk counts over the wave
+    unsigned inc = chunks;
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned y = (unsigned)__shfl_up((int)inc, off);
+      if (lane >= off) inc += y;
+    }
+    const unsigned total = (unsigned)__shfl((int)inc, 63), excl = inc - chunks;
+    for (unsigned j0 = 0; j0 < total; j0 += 64) {
+      const unsigned j = j0 + lane;
+      // owner: the last lane whose exclusive start is <= j (a lane without
+      // chunks shares its start with the next lane, so it is never the
+      // last one); six steps for every lane, so the shuffles stay uniform
+      int lo = 0;
+      for (int step = 32; step > 0; step >>= 1) {
+        const unsigned x = (unsigned)__shfl((int)excl, lo + step < 64 ? lo + step : 63);
+        if (lo + step < 64 && x <= j) lo += step;
+      }
+      // all lanes take part in the shuffles above; only j < total calls f
+      const VertexId vk = (VertexId)__shfl((int)v, lo);
+      const M mk = shfl_any(msg, lo);
+      const EdgeId bk = shfl_any(b, lo), ek = shfl_any(e, lo);
+      const unsigned ex = (unsigned)__shfl((int)excl, lo);
+      if (j < total) {
+        const EdgeId lb = bk + (EdgeId)(j - ex) * kSubList;
+        const EdgeId le = lb + kSubList < ek ? lb + kSubList : ek;
+        acc += (R)f(vk, mk, VertexAdjList<E>(adj + lb, adj + le));
+      }
+    }
+    slot_add_wave<R>(slots, acc);
   }
 }
 
+// Sparse slots of the longest lists: one wave per message, kSubList-entry
+// sub-lists strided over its lanes.
 template <typename R, typename M, typename E, typename F>
-__global__ void sparse_slot_heavy_kernel(F f, const MsgUnit<M> *msgs, const unsigned *heavy,
-                                         unsigned n_heavy, const EdgeId *index, AdjUnit<E> *adj,
-                                         R *slots) {
-  unsigned wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  unsigned lane = threadIdx.x & 63, waves = gridDim.x * (blockDim.x >> 6);
-  for (unsigned h = wave; h < n_heavy; h += waves) {
+__global__ void sparse_slot_heavy_kernel(F f, const Ctl *c, const unsigned *heavy,
+                                         unsigned heavy_cap, const EdgeId *index,
+                                         AdjUnit<E> *adj, R *slots) {
+  if (c->mode != 0) return;
+  const unsigned nh = c->n_heavy < heavy_cap ? c->n_heavy : heavy_cap;
+  const MsgUnit<M> *msgs = (const MsgUnit<M> *)c->eb.buf;
+  const unsigned lane = threadIdx.x & 63, waves = gridDim.x * (blockDim.x >> 6);
+  for (unsigned h = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); h < nh; h += waves) {
     const MsgUnit<M> &m = msgs[heavy[h]];
-    EdgeId b = index[m.vertex], e = index[m.vertex + 1];
-    EdgeId per = (e - b + 63) / 64;
-    EdgeId lb = b + lane * per, le = lb + per < e ? lb + per : e;
-    if (lb < le) slot_add<R>(slots, (R)f(m.vertex, m.msg_data, VertexAdjList<E>(adj + lb, adj + le)));
+    const EdgeId b = index[m.vertex], e = index[m.vertex + 1];
+    R acc = R(0);
+    for (EdgeId lb = b + (EdgeId)lane * kSubList; lb < e; lb += 64 * (EdgeId)kSubList) {
+      const EdgeId le = lb + kSubList < e ? lb + kSubList : e;
+      acc += (R)f(m.vertex, m.msg_data, VertexAdjList<E>(adj + lb, adj + le));
+    }
+    slot_add_wave<R>(slots, acc);
   }
 }
 
+// Dense signals: one thread per vertex over its whole in-list (up to
+// kLightMax entries), as one partition does.  The vertices come in
+// descending in-degree order (`order`, built at load), so the lanes of a
+// wave walk lists of about the same length.
 template <typename E, typename F>
-__global__ void dense_signal_kernel(F f, VertexId V, const EdgeId *index, AdjUnit<E> *adj) {
-  for (VertexId base = blockIdx.x * blockDim.x; base < V; base += gridDim.x * blockDim.x) {
-    VertexId v = base + threadIdx.x;
-    if (v >= V) continue;
-    EdgeId b = index[v], e = index[v + 1];
-    if (e > b && e - b <= (EdgeId)kLightMax) f(v, VertexAdjList<E>(adj + b, adj + e));
+__global__ void dense_signal_kernel(F f, const Ctl *c, const VertexId *order, VertexId n,
+                                    const EdgeId *index, AdjUnit<E> *adj) {
+  if (c->mode != 1) return;
+  for (VertexId i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const VertexId v = order[i];
+    const EdgeId b = index[v], e = index[v + 1];
+    f(v, VertexAdjList<E>(adj + b, adj + e));
   }
 }
 
+// Longer in-lists: parts of at most kPart entries, one thread each, each
+// part its own dense_signal call, as the reference's per-socket parts are
+// (core/graph.hpp:1767-1770); dense_slot combines them.  A list's first part
+// holds its highest-degree in-neighbours (lists are hub-first).
 template <typename E, typename F>
-__global__ void dense_signal_heavy_kernel(F f, const VertexId *heavy, unsigned n_heavy,
+__global__ void dense_signal_parts_kernel(F f, const Ctl *c, const VertexId *part_v,
+                                          const EdgeId *part_b, unsigned n_parts,
                                           const EdgeId *index, AdjUnit<E> *adj) {
-  unsigned wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  unsigned lane = threadIdx.x & 63, waves = gridDim.x * (blockDim.x >> 6);
-  for (unsigned h = wave; h < n_heavy; h += waves) {
-    VertexId v = heavy[h];
-    EdgeId b = index[v], e = index[v + 1];
-    EdgeId per = (e - b + 63) / 64;
-    EdgeId lb = b + lane * per, le = lb + per < e ? lb + per : e;
-    if (lb < le) f(v, VertexAdjList<E>(adj + lb, adj + le));
+  if (c->mode != 1) return;
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n_parts;
+       i += gridDim.x * blockDim.x) {
+    const VertexId v = part_v[i];
+    const EdgeId b = part_b[i], e = index[v + 1];
+    f(v, VertexAdjList<E>(adj + b, adj + (b + kPart < e ? b + kPart : e)));
   }
 }
 
 template <typename R, typename M, typename F>
-__global__ void dense_slot_kernel(F f, const MsgUnit<M> *msgs, unsigned long long n, R *slots) {
-  for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
-       i += (unsigned long long)gridDim.x * blockDim.x)
-    slot_add<R>(slots, (R)f(msgs[i].vertex, msgs[i].msg_data));
+__global__ void dense_slot_kernel(F f, const Ctl *c, R *slots) {
+  if (c->mode != 1) return;
+  const unsigned long long n = c->eb.count;
+  const MsgUnit<M> *msgs = (const MsgUnit<M> *)c->eb.buf;
+  const unsigned long long S = (unsigned long long)gridDim.x * blockDim.x;
+  for (unsigned long long base = (unsigned long long)blockIdx.x * blockDim.x; base < n;
+       base += S) {
+    const unsigned long long i = base + threadIdx.x;
+    R x = R(0);
+    if (i < n) x = (R)f(msgs[i].vertex, msgs[i].msg_data);
+    slot_add_wave<R>(slots, x);
+  }
 }
 
 template <typename T>
 __global__ void fill_kernel(T *p, T v, VertexId n) {
   for (VertexId i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     p[i] = v;
+}
+
+template <typename T>
+__global__ void set_value_kernel(T *p, T v) {
+  *p = v;
+}
+
+template <int = 0>
+__global__ void set_buf_kernel(Ctl *c, void *buf, void *seg_buf, unsigned *seg_cnt,
+                               unsigned long long seg_cap, unsigned long long ovf_cap) {
+  c->eb.buf = buf;
+  c->eb.seg_buf = seg_buf;
+  c->eb.seg_cnt = seg_cnt;
+  c->eb.seg_cap = seg_cap;
+  c->eb.ovf_cap = ovf_cap;
 }
 
 template <int = 0>
@@ -190,15 +413,38 @@ struct GraphView {
   int partition_id;
   int partitions;
   gg_facade::EmitBlock *eb;
-  // emit a message to a vertex (core/graph.hpp:1479-1490); legal inside signals
+  // emit a message to a vertex (core/graph.hpp:1479-1490); legal inside
+  // signals.  The lanes of a wave that emit at the same instant reserve
+  // their slots in the wave's segment with one atomic (the reference fills
+  // thread-local buffers for the same reason, core/graph.hpp:1482-1489).
   template <typename M>
   __device__ void emit(VertexId vtx, M msg) const {
-    unsigned long long i = atomicAdd(&eb->count, 1ull);
-    if (i < eb->cap) {
-      gg_facade::MsgUnit<M> *b = (gg_facade::MsgUnit<M> *)eb->buf;
-      b[i].vertex = vtx;
-      b[i].msg_data = msg;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned long long lanes = __ballot(1);  // the lanes emitting now
+    const int leader = __ffsll((long long)lanes) - 1;
+    const unsigned seg =
+        (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % gg_facade::kEmitSegs;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(&eb->seg_cnt[(size_t)seg * 32], (unsigned)__popcll(lanes));
+    base = (unsigned)__shfl((int)base, leader);
+    const unsigned long long i = (unsigned long long)base + __popcll(lanes & below);
+    gg_facade::MsgUnit<M> *m = (gg_facade::MsgUnit<M> *)eb->seg_buf;
+    if (i < eb->seg_cap) {
+      m[(size_t)seg * eb->seg_cap + i] = gg_facade::MsgUnit<M>{vtx, msg};
+      return;
     }
+    // the segment is full: the overflow region
+    const unsigned long long over = __ballot(1);
+    const int l2 = __ffsll((long long)over) - 1;
+    unsigned long long ob = 0;
+    if (lane == l2) ob = atomicAdd(&eb->ovf_count, (unsigned long long)__popcll(over));
+    ob = __shfl(ob, l2);
+    const unsigned long long j = ob + __popcll(over & below);
+    if (j < eb->ovf_cap)
+      m[(size_t)gg_facade::kEmitSegs * eb->seg_cap + j] = gg_facade::MsgUnit<M>{vtx, msg};
+    else
+      atomicAdd(&eb->dropped, 1ull);
   }
 };
 
@@ -214,16 +460,20 @@ class Graph {
   bool symmetric = false;
 
   Graph() {
-    gg_facade::hip_check(hipMalloc((void **)&eb_, sizeof(gg_facade::EmitBlock)), "emit block");
-    gg_facade::hip_check(hipMalloc((void **)&slots_, 16 * 8 * gg_facade::kSlots), "slots");
-    gg_facade::hip_check(hipMalloc((void **)&n_heavy_, 4), "n_heavy");
+    gg_facade::hip_check(hipMalloc((void **)&ctl_, sizeof(gg_facade::Ctl)), "control block");
+    gg_facade::hip_check(hipMemset(ctl_, 0, sizeof(gg_facade::Ctl)), "control block");
+    gg_facade::hip_check(hipMalloc((void **)&slots_, kSlotBytes), "slots");
+    gg_facade::hip_check(hipMemset(slots_, 0, kSlotBytes), "slots");  // zero between calls
+    gg_facade::hip_check(hipMalloc((void **)&seg_cnt_, sizeof(unsigned) * 32 * gg_facade::kEmitSegs),
+                         "segment counters");
   }
   ~Graph() {
     for (void *p : owned_) (void)hipFree(p);
-    (void)hipFree(eb_);
+    (void)hipFree(ctl_);
     (void)hipFree(slots_);
-    (void)hipFree(n_heavy_);
     (void)hipFree(msg_buf_);
+    (void)hipFree(seg_buf_);
+    (void)hipFree(seg_cnt_);
     (void)hipFree(heavy_msgs_);
     if (g_) gg_graph_destroy(g_);
   }
@@ -253,11 +503,11 @@ class Graph {
   T *alloc_interleaved_vertex_array() {
     return alloc_vertex_array<T>();
   }
+  // queued on the null stream: later launches and copies see it in order
   template <typename T>
   void fill_vertex_array(T *array, T value) {
     hipLaunchKernelGGL(gg_facade::fill_kernel<T>, dim3(gg_facade::grid_for(vertices)), dim3(256), 0,
                        0, array, value, vertices);
-    gg_facade::hip_check(hipDeviceSynchronize(), "fill_vertex_array");
   }
   template <typename T>
   void dealloc_vertex_array(T *array) {
@@ -280,7 +530,7 @@ class Graph {
   }
   template <typename T>
   void set_value(T *array, VertexId v, T value) {
-    gg_facade::hip_check(hipMemcpy(array + v, &value, sizeof(T), hipMemcpyHostToDevice), "set_value");
+    hipLaunchKernelGGL(gg_facade::set_value_kernel<T>, dim3(1), dim3(1), 0, 0, array + v, value);
   }
   template <typename T>
   T get_value(const T *array, VertexId v) const {
@@ -291,90 +541,115 @@ class Graph {
   VertexSubset *alloc_vertex_subset() { return new VertexSubset(vertices); }
 
   GraphView<EdgeData> view() const {
-    return GraphView<EdgeData>{vertices, edges, out_degree, in_degree, partition_id, partitions, eb_};
+    return GraphView<EdgeData>{vertices, edges, out_degree, in_degree, partition_id, partitions,
+                               &ctl_->eb};
   }
 
   // ---- engine (core/graph.hpp:1406-1867) ----
   template <typename R, typename F>
   R process_vertices(F process, Bitmap *active) {
-    clear_slots<R>();
+    static_assert(sizeof(R) <= 8, "reducer of at most 8 bytes");
     hipLaunchKernelGGL((gg_facade::process_vertices_kernel<R, F>), dim3(gg_facade::grid_for(vertices)),
                        dim3(256), 0, 0, process, (const unsigned long *)active->data, vertices,
                        (R *)slots_);
-    return read_slots<R>();
+    hipLaunchKernelGGL((gg_facade::reduce_slots_kernel<R>), dim3(1), dim3(gg_facade::kSlots), 0, 0,
+                       (R *)slots_, &ctl_->result);
+    return read_result<R>();
   }
 
+  // Every launch is queued at once: the device decides the mode, the other
+  // mode's kernels return at once, message counts stay on the device; the
+  // host waits once, for the reducer.
   template <typename R, typename M, typename SS, typename SL, typename DS, typename DL>
   R process_edges(SS sparse_signal, SL sparse_slot, DS dense_signal, DL dense_slot, Bitmap *active,
                   Bitmap *dense_selective = nullptr) {
     (void)dense_selective;  // only synchronises replicas across partitions
+    static_assert(sizeof(R) <= 8, "reducer of at most 8 bytes");
+    ensure_messages<M>();
     const VertexId *od = out_degree;
-    EdgeId active_edges = process_vertices<EdgeId>(
-        [=] __device__(VertexId v) { return (EdgeId)od[v]; }, active);
-    bool sparse = active_edges < edges / 20;
-    last_mode_sparse = sparse;
-    clear_slots<R>();
-    if (sparse) {
-      reset_emits<M>((size_t)vertices);
-      hipLaunchKernelGGL((gg_facade::sparse_signal_kernel<SS>), dim3(gg_facade::grid_for(vertices)),
-                         dim3(256), 0, 0, sparse_signal, (const unsigned long *)active->data, vertices);
-      unsigned long long n = emitted();
-      gg_facade::hip_check(hipMemset(n_heavy_, 0, 4), "n_heavy");
-      if (n) {
-        hipLaunchKernelGGL((gg_facade::sparse_slot_kernel<R, M, EdgeData, SL>),
-                           dim3(gg_facade::grid_for(n)), dim3(256), 0, 0, sparse_slot,
-                           (const gg_facade::MsgUnit<M> *)msg_buf_, n, out_.index, out_.adj,
-                           (R *)slots_, heavy_msgs_, n_heavy_);
-        unsigned nh = 0;
-        gg_facade::hip_check(hipMemcpy(&nh, n_heavy_, 4, hipMemcpyDeviceToHost), "n_heavy");
-        if (nh)
-          hipLaunchKernelGGL((gg_facade::sparse_slot_heavy_kernel<R, M, EdgeData, SL>),
-                             dim3(gg_facade::grid_for(64ull * nh)), dim3(256), 0, 0, sparse_slot,
-                             (const gg_facade::MsgUnit<M> *)msg_buf_, heavy_msgs_, nh, out_.index,
-                             out_.adj, (R *)slots_);
-      }
-    } else {
-      reset_emits<M>((size_t)vertices + 64ull * in_.n_heavy);
-      hipLaunchKernelGGL((gg_facade::dense_signal_kernel<EdgeData, DS>),
-                         dim3(gg_facade::grid_for(vertices)), dim3(256), 0, 0, dense_signal, vertices,
-                         in_.index, in_.adj);
-      if (in_.n_heavy)
-        hipLaunchKernelGGL((gg_facade::dense_signal_heavy_kernel<EdgeData, DS>),
-                           dim3(gg_facade::grid_for(64ull * in_.n_heavy)), dim3(256), 0, 0,
-                           dense_signal, in_.heavy, in_.n_heavy, in_.index, in_.adj);
-      unsigned long long n = emitted();
-      if (n)
-        hipLaunchKernelGGL((gg_facade::dense_slot_kernel<R, M, DL>), dim3(gg_facade::grid_for(n)),
-                           dim3(256), 0, 0, dense_slot, (const gg_facade::MsgUnit<M> *)msg_buf_, n,
-                           (R *)slots_);
+    hipLaunchKernelGGL((gg_facade::process_vertices_kernel<EdgeId, gg_facade_out_degree>),
+                       dim3(gg_facade::grid_for(vertices)), dim3(256), 0, 0,
+                       gg_facade_out_degree{od}, (const unsigned long *)active->data, vertices,
+                       (EdgeId *)slots_);
+    hipLaunchKernelGGL(gg_facade::decide_kernel<0>, dim3(1), dim3(gg_facade::kSlots), 0, 0, ctl_,
+                       (EdgeId *)slots_, (EdgeId)(edges / 20));  // core/graph.hpp:1509
+    const dim3 grid(gg_facade::kPhaseGrid), block(256);
+    R *slots = (R *)slots_;
+    // signals of the mode chosen (BSP: all signals and emits complete before
+    // any slot, core/graph.hpp:1541-1556 then :1585; :1755-1793 then :1801)
+    hipLaunchKernelGGL((gg_facade::sparse_signal_kernel<SS>), dim3(gg_facade::grid_for(vertices)),
+                       block, 0, 0, sparse_signal, ctl_, (const unsigned long *)active->data,
+                       vertices);
+    hipLaunchKernelGGL((gg_facade::dense_signal_kernel<EdgeData, DS>),
+                       dim3(gg_facade::grid_for(in_.n_light)), block, 0, 0, dense_signal, ctl_,
+                       in_.light, in_.n_light, in_.index, in_.adj);
+    if (in_.n_parts)
+      hipLaunchKernelGGL((gg_facade::dense_signal_parts_kernel<EdgeData, DS>),
+                         dim3(gg_facade::grid_for(in_.n_parts)), block, 0, 0, dense_signal, ctl_,
+                         in_.part_v, in_.part_b, in_.n_parts, in_.index, in_.adj);
+    // the messages, packed
+    hipLaunchKernelGGL(gg_facade::pack_offsets_kernel<0>, dim3(1), dim3(gg_facade::kEmitSegs), 0, 0,
+                       ctl_);
+    hipLaunchKernelGGL((gg_facade::pack_copy_kernel<M>), dim3(gg_facade::kEmitSegs + 1), block, 0,
+                       0, ctl_);
+    // slots of the mode chosen
+    hipLaunchKernelGGL((gg_facade::sparse_slot_kernel<R, M, EdgeData, SL>), grid, block, 0, 0,
+                       sparse_slot, ctl_, out_.index, out_.adj, slots, heavy_msgs_,
+                       vertices + 1);
+    hipLaunchKernelGGL((gg_facade::sparse_slot_heavy_kernel<R, M, EdgeData, SL>), grid, block, 0,
+                       0, sparse_slot, ctl_, heavy_msgs_, vertices + 1, out_.index, out_.adj,
+                       slots);
+    hipLaunchKernelGGL((gg_facade::dense_slot_kernel<R, M, DL>), grid, block, 0, 0, dense_slot,
+                       ctl_, slots);
+    hipLaunchKernelGGL((gg_facade::reduce_slots_kernel<R>), dim3(1), dim3(gg_facade::kSlots), 0, 0,
+                       slots, &ctl_->result);
+    gg_facade::Ctl h;
+    gg_facade::hip_check(hipMemcpy(&h, ctl_, sizeof(h), hipMemcpyDeviceToHost), "process_edges");
+    if (h.eb.dropped) {
+      fprintf(stderr, "gg: message buffer overflow (%llu messages dropped)\n", h.eb.dropped);
+      abort();
     }
-    return read_slots<R>();
+    last_mode_sparse = h.mode == 0;
+    R r;
+    __builtin_memcpy(&r, &h.result, sizeof(R));
+    return r;
   }
 
   bool last_mode_sparse = false;
 
  private:
+  static constexpr size_t kSlotBytes = 16 * 8 * gg_facade::kSlots;
+  struct gg_facade_out_degree {
+    const VertexId *od;
+    __device__ EdgeId operator()(VertexId v) const { return (EdgeId)od[v]; }
+  };
   struct DirView {
     EdgeId *index = nullptr;
     AdjUnit<EdgeData> *adj = nullptr;
-    VertexId *heavy = nullptr;  // vertices with more than kLightMax entries
-    unsigned n_heavy = 0;
+    VertexId *light = nullptr;  // vertices with 1..kLightMax entries, by descending degree
+    VertexId n_light = 0;
+    VertexId *part_v = nullptr; // longer lists: parts of kPart entries (vertex, first entry)
+    EdgeId *part_b = nullptr;
+    unsigned n_parts = 0;
   };
   gg_graph *g_ = nullptr;
   DirView out_, in_;
-  gg_facade::EmitBlock *eb_ = nullptr;
-  void *msg_buf_ = nullptr;
+  gg_facade::Ctl *ctl_ = nullptr;
+  void *msg_buf_ = nullptr;   // packed messages
+  void *seg_buf_ = nullptr;   // segments and overflow region
+  unsigned *seg_cnt_ = nullptr;
   size_t msg_cap_bytes_ = 0;
   void *slots_ = nullptr;
   unsigned *heavy_msgs_ = nullptr;
-  unsigned *n_heavy_ = nullptr;
   std::vector<void *> owned_;
 
   void load(const std::string &path, VertexId v, bool undirected) {
     int weighted = sizeof(EdgeData) == 0 || __is_same(EdgeData, Empty) ? 0 : 1;
     static_assert(__is_same(EdgeData, Empty) || __is_same(EdgeData, float),
                   "edge data: Empty or float");
-    gg_facade::check(gg_graph_load_ex(path.c_str(), v, weighted, undirected ? 1 : 0, 0u, &g_),
+    // file ids, as the callbacks index vertex arrays by them; lists hubs first
+    gg_facade::check(gg_graph_load_ex(path.c_str(), v, weighted, undirected ? 1 : 0,
+                                      GG_BUILD_HUB_FIRST, &g_),
                      "load");
     gg_graph_info info;
     gg_facade::check(gg_graph_get_info(g_, &info), "info");
@@ -405,53 +680,75 @@ class Graph {
                            nbr, w, (EdgeId)arcs, a);
       d.adj = (AdjUnit<EdgeData> *)a;
     }
-    std::vector<VertexId> hd(vertices), hv;
+    std::vector<VertexId> hd(vertices);
+    std::vector<EdgeId> hidx((size_t)vertices + 1);
     gg_facade::hip_check(hipMemcpy(hd.data(), deg, sizeof(VertexId) * vertices, hipMemcpyDeviceToHost),
                          "degrees");
+    gg_facade::hip_check(hipMemcpy(hidx.data(), index, sizeof(EdgeId) * ((size_t)vertices + 1),
+                                   hipMemcpyDeviceToHost),
+                         "offsets");
+    // counting sort of the light vertices by descending degree; parts of the
+    // heavy lists
+    std::vector<VertexId> start(gg_facade::kLightMax + 2, 0), pv;
+    std::vector<EdgeId> pb;
+    for (VertexId v = 0; v < vertices; v++) {
+      if (hd[v] > (VertexId)gg_facade::kLightMax) {
+        for (EdgeId b = hidx[v]; b < hidx[v + 1]; b += gg_facade::kPart) {
+          pv.push_back(v);
+          pb.push_back(b);
+        }
+      } else if (hd[v]) {
+        start[gg_facade::kLightMax - hd[v] + 1]++;
+      }
+    }
+    for (int k = 1; k <= gg_facade::kLightMax + 1; k++) start[k] += start[k - 1];
+    std::vector<VertexId> lv(start[gg_facade::kLightMax + 1]);
     for (VertexId v = 0; v < vertices; v++)
-      if (hd[v] > (VertexId)gg_facade::kLightMax) hv.push_back(v);
-    d.n_heavy = (unsigned)hv.size();
-    gg_facade::hip_check(hipMalloc((void **)&d.heavy, sizeof(VertexId) * (hv.size() + 1)), "heavy");
-    owned_.push_back(d.heavy);
-    if (!hv.empty())
-      gg_facade::hip_check(hipMemcpy(d.heavy, hv.data(), sizeof(VertexId) * hv.size(),
-                                     hipMemcpyHostToDevice),
-                           "heavy");
+      if (hd[v] && hd[v] <= (VertexId)gg_facade::kLightMax)
+        lv[start[gg_facade::kLightMax - hd[v]]++] = v;
+    d.n_light = (VertexId)lv.size();
+    d.n_parts = (unsigned)pv.size();
+    auto upload = [&](void **dst, const void *src, size_t bytes, const char *what) {
+      gg_facade::hip_check(hipMalloc(dst, bytes ? bytes : 8), what);
+      owned_.push_back(*dst);
+      if (bytes) gg_facade::hip_check(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), what);
+    };
+    upload((void **)&d.light, lv.data(), sizeof(VertexId) * lv.size(), "light vertices");
+    upload((void **)&d.part_v, pv.data(), sizeof(VertexId) * pv.size(), "parts");
+    upload((void **)&d.part_b, pb.data(), sizeof(EdgeId) * pb.size(), "parts");
     gg_facade::hip_check(hipDeviceSynchronize(), "make_dir");
     return d;
   }
 
+  // Message buffers for a call with messages of type M: at most one message
+  // per vertex (sparse) or per light vertex and heavy-list part (dense);
+  // segments hold 1.5 times that spread over kEmitSegs, the overflow region
+  // that again.  Grown on the host only when M needs more bytes than any
+  // earlier call.
   template <typename M>
-  void reset_emits(size_t cap) {
-    size_t bytes = sizeof(gg_facade::MsgUnit<M>) * (cap ? cap : 1);
+  void ensure_messages() {
+    const unsigned long long cap = (unsigned long long)vertices + in_.n_parts + 1;
+    const unsigned long long seg_cap = (cap + cap / 2) / gg_facade::kEmitSegs + 64;
+    const unsigned long long total = seg_cap * gg_facade::kEmitSegs + cap;
+    const size_t bytes = sizeof(gg_facade::MsgUnit<M>) * (size_t)total;
     if (bytes > msg_cap_bytes_) {
+      gg_facade::hip_check(hipDeviceSynchronize(), "message buffer");
       (void)hipFree(msg_buf_);
+      (void)hipFree(seg_buf_);
       gg_facade::hip_check(hipMalloc(&msg_buf_, bytes), "message buffer");
+      gg_facade::hip_check(hipMalloc(&seg_buf_, bytes), "message segments");
       msg_cap_bytes_ = bytes;
+      hipLaunchKernelGGL(gg_facade::set_buf_kernel<0>, dim3(1), dim3(1), 0, 0, ctl_, msg_buf_,
+                         seg_buf_, seg_cnt_, seg_cap, cap);
     }
-    gg_facade::EmitBlock h{msg_buf_, 0ull, (unsigned long long)cap};
-    gg_facade::hip_check(hipMemcpy(eb_, &h, sizeof(h), hipMemcpyHostToDevice), "emit block");
-  }
-  unsigned long long emitted() {
-    gg_facade::EmitBlock h;
-    gg_facade::hip_check(hipMemcpy(&h, eb_, sizeof(h), hipMemcpyDeviceToHost), "emitted");
-    if (h.count > h.cap) {
-      fprintf(stderr, "gg: message buffer overflow (%llu > %llu)\n", h.count, h.cap);
-      abort();
-    }
-    return h.count;
   }
   template <typename R>
-  void clear_slots() {
-    gg_facade::hip_check(hipMemset(slots_, 0, 16 * 8 * gg_facade::kSlots), "slots");
-  }
-  template <typename R>
-  R read_slots() {
-    std::vector<unsigned char> h(16 * 8 * gg_facade::kSlots);
-    gg_facade::hip_check(hipMemcpy(h.data(), slots_, h.size(), hipMemcpyDeviceToHost), "slots");
-    R s = 0;
-    for (int i = 0; i < gg_facade::kSlots; i++) s += *(R *)(h.data() + (size_t)i * 16 * sizeof(R));
-    return s;
+  R read_result() {
+    unsigned long long bits = 0;
+    gg_facade::hip_check(hipMemcpy(&bits, &ctl_->result, 8, hipMemcpyDeviceToHost), "reducer");
+    R r;
+    __builtin_memcpy(&r, &bits, sizeof(R));
+    return r;
   }
 };
 

@@ -1205,10 +1205,37 @@ __global__ void msbc_inv_kernel(double *sigma_inv, double *dd, uint64_t n) {
     dd[k] = inv;
   }
 }
-__global__ void msbc_final_kernel(double *dd, const double *inv, uint64_t n) {
+// Entries k < live come from the traversal; the rows after them belong to
+// vertices no root can reach (no in-arcs, not a root): sigma 0, so
+// inv = 1/0 = +inf and D = (inf - inf) / inf = NaN, written without reading.
+__global__ void msbc_final_kernel(double *dd, double *inv, uint64_t live, uint64_t n) {
   uint64_t s = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += s)
-    dd[k] = (dd[k] - inv[k]) / inv[k];  // bc.cpp:170-176
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += s) {
+    if (k < live) {
+      dd[k] = (dd[k] - inv[k]) / inv[k];  // bc.cpp:170-176
+    } else {
+      inv[k] = __builtin_inf();
+      dd[k] = __builtin_nan("");
+    }
+  }
+}
+// The same on a relabelled graph, with both rows written in original order
+// in the same pass (one read of each row instead of three passes).
+__global__ void msbc_final_unpermute_kernel(const double *dd, const double *inv,
+                                            const uint32_t *iperm, uint32_t V, uint64_t live,
+                                            double *dep_out, double *inv_out) {
+  const uint64_t n = (uint64_t)V * 64, s = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += s) {
+    const uint64_t o = (uint64_t)iperm[k >> 6] * 64 + (k & 63);
+    if (k < live) {
+      const double x = inv[k];
+      dep_out[o] = (dd[k] - x) / x;  // bc.cpp:170-176
+      inv_out[o] = x;
+    } else {
+      dep_out[o] = __builtin_nan("");
+      inv_out[o] = __builtin_inf();
+    }
+  }
 }
 
 // ------------------------------------------------- batched-query plumbing
@@ -2616,7 +2643,12 @@ int bc_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_de
   double *sigma = g->reordered || keep ? (double *)w->mb_a : d_inv_paths;
   double *dd = g->reordered || keep ? (double *)w->mb_b : d_dep;
   if (rows) *rows = dd;
-  fill<double>(sigma, 0.0, n);
+  // rows that a root can reach: vertices up to the last with in-arcs (the
+  // relabeling puts the vertices without arcs last) and the roots
+  uint64_t live_v = std::min<uint64_t>(V, (uint64_t)in.live_groups * 64);
+  for (int i = 0; i < n_roots; i++) live_v = std::max<uint64_t>(live_v, (uint64_t)roots[i] + 1);
+  const uint64_t live = live_v * 64;
+  fill<double>(sigma, 0.0, live);
   GG_HIP(hipMemcpyAsync(w->ms_roots, roots, 4 * (size_t)n_roots, hipMemcpyHostToDevice, stream()));
   hipLaunchKernelGGL(msbc_seed_sigma_kernel, dim3(1), dim3(1), 0, stream(), w->ms_roots, n_roots,
                      sigma);
@@ -2647,7 +2679,8 @@ int bc_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_de
   MsLevels lv;
   GG_TRY(ms_bfs_run(g, w, en, roots, n_roots, depth, hook, true, &lv));
   const bool masks = lv.masks;
-  hipLaunchKernelGGL(msbc_inv_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(), sigma, dd, n);
+  hipLaunchKernelGGL(msbc_inv_kernel, dim3(grid1(live)), dim3(kBlock), 0, stream(), sigma, dd,
+                     live);
   // backward: level L-1 pulls D from level L along out-edges (the reference
   // pushes along the transposed graph, bc.cpp:121-168)
   for (uint32_t L = levels; L >= 1; L--) {
@@ -2664,7 +2697,9 @@ int bc_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_de
       hipLaunchKernelGGL(lane_vertex_kernel<MsbcBack>, dim3(grid_waves(en.in.n[b])), dim3(kBlock),
                          0, stream(), a, op);
     }
-    if (out.n_chunks) {
+    // the level's vertices are binned by out-degree: with the big bin empty
+    // and its bound at most the heavy-list bound, no heavy list is active
+    if (out.n_chunks && (en.in.n[2] || en.big_max > out.light_max)) {
       HeavyArgs h{out.nbr, nullptr, out.chunk_v, out.chunk_h, out.chunk_beg, out.chunk_end,
                   out.n_chunks, nullptr, nullptr};
       hipLaunchKernelGGL(lane_heavy_kernel<MsbcBack>, dim3(grid_waves(out.n_chunks)), dim3(kBlock),
@@ -2672,13 +2707,12 @@ int bc_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_de
     }
     GG_TRY(en.note_iteration(false));
   }
-  hipLaunchKernelGGL(msbc_final_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(), dd, sigma, n);
-  if (g->reordered && !keep) {
-    hipLaunchKernelGGL(unpermute_rows_kernel<double>, dim3(grid1(n)), dim3(kBlock), 0, stream(), dd,
-                       d_dep, g->iperm, V);
-    hipLaunchKernelGGL(unpermute_rows_kernel<double>, dim3(grid1(n)), dim3(kBlock), 0, stream(),
-                       sigma, d_inv_paths, g->iperm, V);
-  }
+  if (g->reordered && !keep)
+    hipLaunchKernelGGL(msbc_final_unpermute_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(), dd,
+                       sigma, g->iperm, V, live, d_dep, d_inv_paths);
+  else
+    hipLaunchKernelGGL(msbc_final_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(), dd, sigma,
+                       live, n);
   en.st.device_ms = tm.stop();
   GG_HIP(hipGetLastError());
   en.st.reached = (uint64_t)n_roots;

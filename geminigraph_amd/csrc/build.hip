@@ -139,8 +139,12 @@ __global__ void check_range_kernel(const uint32_t *edges, uint64_t ne, int rec_w
 
 // keys for one direction: key vertex in the high bits, neighbour in the low
 // vbits.  mode 0: key = src, 1: key = dst, 2: both (symmetric, 2E keys).
+// perm (relabeling) renames both endpoints; rank (hub-first lists without
+// relabeling) replaces only the neighbour by its degree rank, so each list
+// sorts hubs first and split_keys_kernel maps the ranks back to ids.
 __global__ void make_keys_kernel(const uint32_t *edges, uint64_t ne, int rec_words, int vbits,
-                                 int mode, const uint32_t *perm, uint64_t *keys, float *vals) {
+                                 int mode, const uint32_t *perm, const uint32_t *rank,
+                                 uint64_t *keys, float *vals) {
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += stride) {
     uint64_t s = edges[i * rec_words], d = edges[i * rec_words + 1];
@@ -148,16 +152,17 @@ __global__ void make_keys_kernel(const uint32_t *edges, uint64_t ne, int rec_wor
       s = perm[s];
       d = perm[d];
     }
+    const uint64_t rs = rank ? rank[s] : s, rd = rank ? rank[d] : d;
     float w = rec_words == 3 ? __uint_as_float(edges[i * rec_words + 2]) : 0.f;
     if (mode == 0) {
-      keys[i] = (s << vbits) | d;
+      keys[i] = (s << vbits) | rd;
       if (vals) vals[i] = w;
     } else if (mode == 1) {
-      keys[i] = (d << vbits) | s;
+      keys[i] = (d << vbits) | rs;
       if (vals) vals[i] = w;
     } else {
-      keys[2 * i] = (s << vbits) | d;
-      keys[2 * i + 1] = (d << vbits) | s;
+      keys[2 * i] = (s << vbits) | rd;
+      keys[2 * i + 1] = (d << vbits) | rs;
       if (vals) {
         vals[2 * i] = w;
         vals[2 * i + 1] = w;
@@ -193,11 +198,14 @@ __global__ void gather_u32_kernel(const uint32_t *src, const uint32_t *idx, uint
   for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < V; n += s) dst[n] = src[idx[n]];
 }
 
-__global__ void split_keys_kernel(const uint64_t *keys, uint64_t n, int vbits, uint32_t *nbr) {
+__global__ void split_keys_kernel(const uint64_t *keys, uint64_t n, int vbits,
+                                  const uint32_t *unrank, uint32_t *nbr) {
   const uint64_t mask = (1ULL << vbits) - 1ULL;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    nbr[i] = (uint32_t)(keys[i] & mask);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t low = (uint32_t)(keys[i] & mask);
+    nbr[i] = unrank ? unrank[low] : low;
+  }
 }
 
 static unsigned grid_for(uint64_t n, unsigned block = 256, unsigned cap = 1u << 20) {
@@ -366,7 +374,7 @@ int graph_finish(gg_graph *g) {
 
 // One direction from the device edge list.
 static int build_dir(gg_graph *g, Dir &d, const uint32_t *edges, uint64_t ne, int rec_words,
-                     int mode) {
+                     int mode, const uint32_t *rank, const uint32_t *unrank) {
   uint32_t V = g->vertices;
   int vbits = 1;
   while (((uint64_t)1 << vbits) < (uint64_t)V) vbits++;
@@ -379,7 +387,8 @@ static int build_dir(gg_graph *g, Dir &d, const uint32_t *edges, uint64_t ne, in
   if (r == GG_OK && g->weighted) r = device_alloc((void **)&vals_alt, n * 4);
   if (r == GG_OK) {
     hipLaunchKernelGGL(make_keys_kernel, dim3(grid_for(ne)), dim3(256), 0, stream(), edges, ne,
-                       rec_words, vbits, mode, g->reordered ? g->perm : nullptr, keys, vals);
+                       rec_words, vbits, mode, g->reordered ? g->perm : nullptr, rank, keys,
+                       vals);
     r = sort_keys(keys, keys_alt, vals, vals_alt, n, 2 * vbits);
   }
   device_free(keys_alt);
@@ -387,7 +396,7 @@ static int build_dir(gg_graph *g, Dir &d, const uint32_t *edges, uint64_t ne, in
   if (r == GG_OK) r = graph_alloc(g, (void **)&d.nbr, n * 4);
   if (r == GG_OK) {
     hipLaunchKernelGGL(split_keys_kernel, dim3(grid_for(n)), dim3(256), 0, stream(), keys, n,
-                       vbits, d.nbr);
+                       vbits, unrank, d.nbr);
     hipError_t e = hipStreamSynchronize(stream());
     if (e != hipSuccess) r = fail(GG_EHIP, std::string("split_keys: ") + hipGetErrorString(e));
   }
@@ -402,10 +411,9 @@ static int build_dir(gg_graph *g, Dir &d, const uint32_t *edges, uint64_t ne, in
   return r;
 }
 
-// Degree-sorted relabeling: iperm = ids sorted by descending out-degree
-// (ties: vertices with in-arcs first, then by id), perm its inverse; degree
-// arrays are then permuted into internal order.
-static int relabel(gg_graph *g) {
+// Degree order: order[k] = the k-th vertex by descending out-degree (ties:
+// vertices with in-arcs first, then by id), rank its inverse.
+static int degree_order(gg_graph *g, uint32_t *order, uint32_t *rank) {
   uint32_t V = g->vertices;
   uint64_t *keys = nullptr, *keys_alt = nullptr;
   uint32_t *ids = nullptr, *ids_alt = nullptr;
@@ -413,8 +421,6 @@ static int relabel(gg_graph *g) {
   if (r == GG_OK) r = device_alloc((void **)&keys_alt, (size_t)V * 8);
   if (r == GG_OK) r = device_alloc((void **)&ids, (size_t)V * 4);
   if (r == GG_OK) r = device_alloc((void **)&ids_alt, (size_t)V * 4);
-  if (r == GG_OK) r = graph_alloc(g, (void **)&g->perm, (size_t)V * 4);
-  if (r == GG_OK) r = graph_alloc(g, (void **)&g->iperm, (size_t)V * 4);
   if (r == GG_OK) {
     hipLaunchKernelGGL(relabel_keys_kernel, dim3(grid_for(V)), dim3(256), 0, stream(),
                        g->out.degree, g->symmetric ? nullptr : g->in.degree, V, keys, ids);
@@ -427,29 +433,45 @@ static int relabel(gg_graph *g) {
     if (e == hipSuccess && r == GG_OK)
       e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kb, vb, (size_t)V, 0, 33, stream());
     if (e == hipSuccess && r == GG_OK)
-      e = hipMemcpyAsync(g->iperm, vb.current(), (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
-    if (e == hipSuccess && r == GG_OK) {
-      hipLaunchKernelGGL(invert_perm_kernel, dim3(grid_for(V)), dim3(256), 0, stream(), g->iperm, V,
-                         g->perm);
-      // degrees into internal order (keys buffer reused as scratch)
-      uint32_t *scratch = (uint32_t *)keys;
-      hipLaunchKernelGGL(gather_u32_kernel, dim3(grid_for(V)), dim3(256), 0, stream(),
-                         g->out.degree, g->iperm, V, scratch);
-      e = hipMemcpyAsync(g->out.degree, scratch, (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
-      if (e == hipSuccess && !g->symmetric) {
-        hipLaunchKernelGGL(gather_u32_kernel, dim3(grid_for(V)), dim3(256), 0, stream(),
-                           g->in.degree, g->iperm, V, scratch);
-        e = hipMemcpyAsync(g->in.degree, scratch, (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
-      }
-    }
+      e = hipMemcpyAsync(order, vb.current(), (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
+    if (e == hipSuccess && r == GG_OK)
+      hipLaunchKernelGGL(invert_perm_kernel, dim3(grid_for(V)), dim3(256), 0, stream(), order, V,
+                         rank);
     if (e == hipSuccess) e = hipStreamSynchronize(stream());
     device_free(tmp);
-    if (e != hipSuccess && r == GG_OK) r = fail(GG_EHIP, std::string("relabel: ") + hipGetErrorString(e));
+    if (e != hipSuccess && r == GG_OK)
+      r = fail(GG_EHIP, std::string("degree order: ") + hipGetErrorString(e));
   }
   device_free(keys);
   device_free(keys_alt);
   device_free(ids);
   device_free(ids_alt);
+  return r;
+}
+
+// Degree-sorted relabeling: iperm = the degree order, perm its inverse;
+// degree arrays are then permuted into internal order.
+static int relabel(gg_graph *g) {
+  uint32_t V = g->vertices;
+  int r = graph_alloc(g, (void **)&g->perm, (size_t)V * 4);
+  if (r == GG_OK) r = graph_alloc(g, (void **)&g->iperm, (size_t)V * 4);
+  if (r == GG_OK) r = degree_order(g, g->iperm, g->perm);
+  uint32_t *scratch = nullptr;
+  if (r == GG_OK) r = device_alloc((void **)&scratch, (size_t)V * 4);
+  if (r == GG_OK) {
+    hipLaunchKernelGGL(gather_u32_kernel, dim3(grid_for(V)), dim3(256), 0, stream(),
+                       g->out.degree, g->iperm, V, scratch);
+    hipError_t e =
+        hipMemcpyAsync(g->out.degree, scratch, (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
+    if (e == hipSuccess && !g->symmetric) {
+      hipLaunchKernelGGL(gather_u32_kernel, dim3(grid_for(V)), dim3(256), 0, stream(),
+                         g->in.degree, g->iperm, V, scratch);
+      e = hipMemcpyAsync(g->in.degree, scratch, (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    if (e != hipSuccess) r = fail(GG_EHIP, std::string("relabel: ") + hipGetErrorString(e));
+  }
+  device_free(scratch);
   if (r == GG_OK) {
     g->h_perm.resize(V);
     hipError_t e = hipMemcpy(g->h_perm.data(), g->perm, (size_t)V * 4, hipMemcpyDeviceToHost);
@@ -500,6 +522,14 @@ static int graph_build_device(const uint32_t *d_edges, uint64_t ne, uint32_t V, 
     if (r == GG_OK) r = scan_degrees(g->out.degree, g->out.index, V);
     if (r == GG_OK && !undirected) r = scan_degrees(g->in.degree, g->in.index, V);
   }
+  // hub-first lists on file ids: neighbours sorted by their degree rank (a
+  // relabelled graph's lists are in that order already)
+  uint32_t *order = nullptr, *rank = nullptr;
+  if (r == GG_OK && ne && (flags & GG_BUILD_HUB_FIRST) && !(flags & GG_BUILD_REORDER)) {
+    r = device_alloc((void **)&order, (size_t)V * 4);
+    if (r == GG_OK) r = device_alloc((void **)&rank, (size_t)V * 4);
+    if (r == GG_OK) r = degree_order(g, order, rank);
+  }
   if (r == GG_OK) {
     if (ne == 0) {
       r = graph_alloc(g, (void **)&g->out.nbr, 4);
@@ -507,12 +537,14 @@ static int graph_build_device(const uint32_t *d_edges, uint64_t ne, uint32_t V, 
       if (r == GG_OK && weighted) r = graph_alloc(g, (void **)&g->out.w, 4);
       if (r == GG_OK && weighted && !undirected) r = graph_alloc(g, (void **)&g->in.w, 4);
     } else if (undirected) {
-      r = build_dir(g, g->out, d_edges, ne, rec_words, 2);
+      r = build_dir(g, g->out, d_edges, ne, rec_words, 2, rank, order);
     } else {
-      r = build_dir(g, g->out, d_edges, ne, rec_words, 0);
-      if (r == GG_OK) r = build_dir(g, g->in, d_edges, ne, rec_words, 1);
+      r = build_dir(g, g->out, d_edges, ne, rec_words, 0, rank, order);
+      if (r == GG_OK) r = build_dir(g, g->in, d_edges, ne, rec_words, 1, rank, order);
     }
   }
+  device_free(order);
+  device_free(rank);
   if (r == GG_OK) r = graph_finish(g);
   if (r != GG_OK) {
     std::string msg = gg_last_error();
@@ -555,7 +587,8 @@ int gg_graph_build_ex(const void *edges, int edges_on_device, uint64_t num_edges
                       gg_graph **out) {
   if (!out) return fail(GG_EINVAL, "gg_graph_build: null out");
   if (!edges && num_edges) return fail(GG_EINVAL, "gg_graph_build: null edges");
-  if (flags & ~GG_BUILD_REORDER) return fail(GG_EINVAL, "gg_graph_build: unknown flags");
+  if (flags & ~(GG_BUILD_REORDER | GG_BUILD_HUB_FIRST))
+    return fail(GG_EINVAL, "gg_graph_build: unknown flags");
   // host input is checked before any device work (the device check covers
   // edge lists already on the device)
   if (!edges_on_device && weighted && !host_weights_ok((const uint32_t *)edges, num_edges))
@@ -590,7 +623,8 @@ int gg_graph_load(const char *path, uint32_t vertices, int weighted, int undirec
 int gg_graph_load_ex(const char *path, uint32_t vertices, int weighted, int undirected,
                      unsigned flags, gg_graph **out) {
   if (!path || !out) return fail(GG_EINVAL, "gg_graph_load: null argument");
-  if (flags & ~GG_BUILD_REORDER) return fail(GG_EINVAL, "gg_graph_load: unknown flags");
+  if (flags & ~(GG_BUILD_REORDER | GG_BUILD_HUB_FIRST))
+    return fail(GG_EINVAL, "gg_graph_load: unknown flags");
   GG_TRY(ensure_init());
   struct stat st;
   if (stat(path, &st) != 0) return fail(GG_EIO, std::string("cannot stat ") + path);

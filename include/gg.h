@@ -172,6 +172,12 @@ int gg_graph_load(const char *path, uint32_t vertices, int weighted, int undirec
  * boundary.  The reference's loaders keep file ids (core/graph.hpp:772-1339);
  * its locality step is the vertex chunking of core/graph.hpp:824-871. */
 #define GG_BUILD_REORDER 1u
+/* GG_BUILD_HUB_FIRST keeps file ids but orders every adjacency list by
+ * descending out-degree of the neighbour (ties as the relabeling orders
+ * them), so a bottom-up step meets the likeliest frontier member first; the
+ * reference's list order is arbitrary (core/graph.hpp:1064-1075).  Implied
+ * by GG_BUILD_REORDER. */
+#define GG_BUILD_HUB_FIRST 2u
 int gg_graph_build_ex(const void *edges, int edges_on_device, uint64_t num_edges,
                       uint32_t vertices, int weighted, int undirected, unsigned flags,
                       gg_graph **out);

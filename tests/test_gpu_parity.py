@@ -695,3 +695,83 @@ def test_bfs_batch_partial_and_repeated_roots(gg):
         assert np.array_equal(depths[i], o.bfs(root)[1])
     with pytest.raises(ValueError):
         gg.bfs_batch(g, list(range(65)))
+
+
+# --------------------------------------------------- hub-first lists (file ids)
+
+def _check_hub_first(idx, nbr, ref_idx, ref_nbr, od, noin):
+    """Same lists as the oracle's (as multisets), each ordered by descending
+    out-degree of the neighbour, then with-in-arcs first, then id."""
+    assert np.array_equal(idx, ref_idx)
+    V = len(idx) - 1
+    owner = np.repeat(np.arange(V), np.diff(idx).astype(np.int64))
+    a = np.lexsort((nbr, owner))
+    b = np.lexsort((ref_nbr, owner))
+    assert np.array_equal(nbr[a], ref_nbr[b])
+    same = owner[1:] == owner[:-1]
+    x, y = nbr[:-1][same].astype(np.int64), nbr[1:][same].astype(np.int64)
+    ok = (od[x] > od[y]) | ((od[x] == od[y]) & ((noin[x] < noin[y]) |
+                                                ((noin[x] == noin[y]) & (x <= y))))
+    assert ok.all()
+
+
+@pytest.mark.parametrize("scale,seed", [(12, 31), (15, 32)])
+def test_hub_first_build_and_toolkits(gg, scale, seed):
+    e = oracle.rmat_edges(scale, seed=seed, weighted=True)
+    V = 1 << scale
+    g = gg.Graph.build(e, V, hub_first=True)
+    assert not g.reordered
+    o = oracle.Graph(e, V)
+    od = o.out_degree().astype(np.int64)
+    noin = (o.in_degree() == 0).astype(np.int64)
+    idx, nbr, _ = g.csc()
+    ridx, rnbr, _ = o.csc()
+    _check_hub_first(idx, nbr, ridx, rnbr, od, noin)
+    idx, nbr, _ = g.csr()
+    ridx, rnbr, _ = o.csr()
+    _check_hub_first(idx, nbr, ridx, rnbr, od, noin)
+    for root in roots_of(o.out_degree(), 2, seed):
+        say("hub-first", scale, root)
+        parent, depth, fr, _ = g.bfs(root)
+        _, rd, rf = o.bfs(root)
+        assert np.array_equal(depth, rd) and np.array_equal(fr, rf)
+        check_parents(e, V, root, parent, depth)
+        dist, fr, _ = g.sssp(root)
+        rdist, rfr = o.sssp(root)
+        assert np.array_equal(dist.view(np.uint32), rdist.view(np.uint32))
+        assert np.array_equal(fr, rfr)
+        dep, inv, fr, _ = g.bc(root)
+        rdep, rinv, rfr = o.bc(root)
+        assert np.array_equal(fr, rfr)
+        assert_rel(inv, rinv)
+        assert_rel(dep, rdep)
+    pr, _, _ = g.pagerank(20)
+    rpr, _ = o.pagerank(20)
+    assert_rel(pr, rpr)
+    gu = gg.Graph.build(e, V, undirected=True, hub_first=True)
+    label, fr, _ = gu.cc()
+    ref, ref_fr = oracle.Graph(e, V, undirected=True).cc()
+    assert np.array_equal(label, ref) and np.array_equal(fr, ref_fr)
+
+
+def test_into_wrappers_refuse_short_or_mistyped_buffers(gg):
+    # the kernels write V (or V * 64) elements: a shorter buffer is refused
+    # before the call, as is one of another element type
+    e = oracle.rmat_edges(10, seed=33, weighted=True)
+    V = 1 << 10
+    g = gg.Graph.build(e, V)
+    short = gg.DeviceArray(V - 1, np.uint32)
+    rows_short = gg.DeviceArray(V, np.uint32)
+    f64 = gg.DeviceArray(V, np.float64)
+    with pytest.raises(ValueError):
+        g.bfs_into(0, short)
+    with pytest.raises(ValueError):
+        g.sssp_into(0, f64)
+    with pytest.raises(ValueError):
+        g.cc_into(f64)
+    with pytest.raises(ValueError):
+        gg.bfs_batch_into(g, [0, 1], rows_short)
+    with pytest.raises(ValueError):
+        g.pagerank_into(short)
+    for x in (short, rows_short, f64):
+        x.free()
