@@ -738,7 +738,9 @@ def level_table(args):
     print("| level | mode | frontier | new | examined | bin us | pull us | push/heavy us | "
           "decide us | read MB (128B/64B/32B req) | write req (64B) | TCC req | hit/miss |")
     print("|---|---|---|---|---|---|---|---|---|---|---|---|---|")
-    for i, lv in enumerate(per["trace"]):
+    # (the trace also holds the iterations queued past the end, which return
+    # at once: only the levels the BFS ran are listed)
+    for i, lv in enumerate(per["trace"][:len(stats["modes"])]):
         us = {k: (int(v["End_Timestamp"]) - int(v["Start_Timestamp"])) / 1e3 for k, v in lv.items()}
         c = {}
         for tag in ("p0", "p1", "p2"):

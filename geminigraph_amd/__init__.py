@@ -39,6 +39,9 @@ def build():
     subprocess.run(["make", "-s", "-B", "-j4", "-C", _HERE, "toolkits"], check=True)
 
 
+STATS_RECORDS = 256  # GG_STATS_RECORDS in gg.h
+
+
 class RunStats(ctypes.Structure):
     _fields_ = [
         ("iterations", ctypes.c_int),
@@ -47,11 +50,11 @@ class RunStats(ctypes.Structure):
         ("m_reach", ctypes.c_uint64),
         ("reached", ctypes.c_uint64),
         ("device_ms", ctypes.c_double),
-        ("modes", ctypes.c_uint32 * 64),
-        ("it_active", ctypes.c_uint32 * 64),
-        ("it_new", ctypes.c_uint32 * 64),
-        ("it_examined", ctypes.c_uint64 * 64),
-        ("it_unvisited", ctypes.c_uint64 * 64),
+        ("modes", ctypes.c_uint32 * STATS_RECORDS),
+        ("it_active", ctypes.c_uint32 * STATS_RECORDS),
+        ("it_new", ctypes.c_uint32 * STATS_RECORDS),
+        ("it_examined", ctypes.c_uint64 * STATS_RECORDS),
+        ("it_unvisited", ctypes.c_uint64 * STATS_RECORDS),
     ]
 
     def as_dict(self):

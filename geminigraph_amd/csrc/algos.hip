@@ -327,7 +327,7 @@ struct Engine {
   }
   int note_iteration(bool was_sparse) {
     int i = st.iterations;
-    if (i < 64) {
+    if (i < GG_STATS_RECORDS) {
       st.modes[i] = was_sparse ? 0u : 1u;
       st.it_active[i] = in.total;
     }
@@ -340,7 +340,7 @@ struct Engine {
       uint64_t x = 0;
       for (int k = 0; k < 64; k++) x += w->h_exam[k * 16];
       st.edges_examined += x;
-      if (i < 64) st.it_examined[i] = x;
+      if (i < GG_STATS_RECORDS) st.it_examined[i] = x;
     }
     return GG_OK;
   }
@@ -349,7 +349,7 @@ struct Engine {
     GG_TRY(note_iteration(was_sparse));
     Counters c;
     GG_TRY(compact(fout(), bin_deg, visited_or, &c));
-    if (st.iterations - 1 < 64) st.it_new[st.iterations - 1] = c.total;
+    if (st.iterations - 1 < GG_STATS_RECORDS) st.it_new[st.iterations - 1] = c.total;
     cur ^= 1;
     in = c;
     return GG_OK;
@@ -1451,8 +1451,8 @@ struct MsCtl {
   uint32_t push_iters;
   int32_t force;
   uint64_t threshold;
-  uint32_t modes[64];
-  uint64_t active[64], fresh[64];
+  uint32_t modes[GG_STATS_RECORDS];
+  uint64_t active[GG_STATS_RECORDS], fresh[GG_STATS_RECORDS];
 };
 struct MsHostFlag {
   uint32_t decided;  // iterations decided so far
@@ -1544,7 +1544,7 @@ __device__ void msd_decide(MsCtl *c, uint64_t *set, Counters *qc, MsHostFlag *hf
     total += s_part[0][k];
     edges += s_part[1][k];
   }
-  if (i > 0 && i - 1 < 64) c->fresh[i - 1] = total;
+  if (i > 0 && i - 1 < GG_STATS_RECORDS) c->fresh[i - 1] = total;
   if (total == 0) {
     c->done = 1;
     c->mode = kMsDone;
@@ -1557,7 +1557,7 @@ __device__ void msd_decide(MsCtl *c, uint64_t *set, Counters *qc, MsHostFlag *hf
                         : force == 2        ? kMsPull
                         : edges < threshold ? kMsPush
                                             : kMsPull;
-  if (i < 64) {
+  if (i < GG_STATS_RECORDS) {
     c->modes[i] = mode == kMsPush ? 0u : 1u;
     c->active[i] = total;
   }
@@ -1946,7 +1946,7 @@ static int ms_bfs_device(gg_graph *g, Work *w, const uint32_t *roots, int n_root
   const MsCtl &c = *s->h_ctl;
   st->iterations = c.iter;
   st->push_iterations = c.push_iters;
-  for (uint32_t i = 0; i < std::min<uint32_t>(c.iter, 64); i++) {
+  for (uint32_t i = 0; i < std::min<uint32_t>(c.iter, GG_STATS_RECORDS); i++) {
     st->modes[i] = c.modes[i];
     st->it_active[i] = c.active[i];
     st->it_new[i] = c.fresh[i];
@@ -2147,7 +2147,7 @@ static int mss_device(gg_graph *g, Work *w, const uint32_t *roots, int n_roots, 
   const MsCtl &c = *s->h_ctl;
   st->iterations = c.iter;
   st->push_iterations = c.push_iters;
-  for (uint32_t i = 0; i < std::min<uint32_t>(c.iter, 64); i++) {
+  for (uint32_t i = 0; i < std::min<uint32_t>(c.iter, GG_STATS_RECORDS); i++) {
     st->modes[i] = c.modes[i];
     st->it_active[i] = c.active[i];
     st->it_new[i] = c.fresh[i];
@@ -2264,7 +2264,7 @@ int gg_pagerank(gg_graph *g, int iterations, double *d_curr_out, double *h_delta
         hipLaunchKernelGGL(pr_heavy_finish_kernel, dim3((in.n_heavy + kBlock - 1) / kBlock),
                            dim3(kBlock), 0, stream(), in.heavy_v, in.n_heavy, w->hacc, op, acc);
     }
-    if (en.st.iterations < 64) en.st.modes[en.st.iterations] = 1;
+    if (en.st.iterations < GG_STATS_RECORDS) en.st.modes[en.st.iterations] = 1;
     en.st.iterations++;
     en.st.edges_examined += g->arcs;
     std::swap(curr, next);

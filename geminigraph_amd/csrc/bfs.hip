@@ -54,8 +54,9 @@ struct BfsCtl {
   uint64_t reached, m_reach, examined, threshold;
   uint64_t m_unvisited;  // in-arcs of the vertices not visited yet (pull's scan bound)
   uint64_t alpha;        // 0, or: pull only if frontier out-arcs * alpha > m_unvisited
-  uint32_t modes[64];
-  uint64_t active[64], fresh[64], exam[64], unvisited[64];
+  uint32_t modes[GG_STATS_RECORDS];
+  uint64_t active[GG_STATS_RECORDS], fresh[GG_STATS_RECORDS], exam[GG_STATS_RECORDS],
+      unvisited[GG_STATS_RECORDS];
 };
 
 // Where found vertices go.  parent is also the claim word (none = V until
@@ -184,7 +185,7 @@ __device__ void bfs_decide(BfsCtl *c, uint64_t *set, Counters *qc, unsigned long
     ind += s_part[2][k];
     ex += s_part[3][k];
   }
-  if (i > 0 && i - 1 < 64) {
+  if (i > 0 && i - 1 < GG_STATS_RECORDS) {
     c->fresh[i - 1] = total;
     c->exam[i - 1] = ex;
   }
@@ -209,7 +210,7 @@ __device__ void bfs_decide(BfsCtl *c, uint64_t *set, Counters *qc, unsigned long
                   : force == 2 ? kBfsPull
                   : sparse     ? kBfsPush
                                : kBfsPull;
-  if (i < 64) {
+  if (i < GG_STATS_RECORDS) {
     c->modes[i] = mode == kBfsPush ? 0u : 1u;
     c->active[i] = total;
     c->unvisited[i] = (uint64_t)V - reached;
@@ -800,7 +801,7 @@ static void bfs_stats(const BfsCtl &c, double ms, gg_run_stats *stats) {
   r.m_reach = c.m_reach;
   r.reached = c.reached;
   r.device_ms = ms;
-  for (uint32_t i = 0; i < std::min<uint32_t>(c.iter, 64); i++) {
+  for (uint32_t i = 0; i < std::min<uint32_t>(c.iter, GG_STATS_RECORDS); i++) {
     r.modes[i] = c.modes[i];
     r.it_active[i] = c.active[i];
     r.it_new[i] = c.fresh[i];

@@ -203,6 +203,9 @@ uint64_t *gg_subset_device_words(gg_subset *s);
 
 /* ------------------------------------------------------------ toolkits */
 
+/* Per-call records kept in gg_run_stats (process_edges calls beyond this
+ * are counted in `iterations` but not recorded). */
+#define GG_STATS_RECORDS 256
 typedef struct gg_run_stats {
   int iterations;          /* process_edges calls */
   int push_iterations;     /* of which sparse (push) */
@@ -210,12 +213,12 @@ typedef struct gg_run_stats {
   uint64_t m_reach;        /* sum of out-degrees of reached vertices */
   uint64_t reached;        /* reached vertices (BFS/SSSP/BC) or components (CC) */
   double device_ms;        /* device time of the compute, by HIP events */
-  /* per process_edges call (first 64 calls): */
-  uint32_t modes[64];          /* 0 = push (sparse), 1 = pull (dense) */
-  uint32_t it_active[64];      /* active vertices passed in */
-  uint32_t it_new[64];         /* vertices activated by the call */
-  uint64_t it_examined[64];    /* adjacency entries read (option count_examined) */
-  uint64_t it_unvisited[64];   /* BFS: vertices not yet visited before the call */
+  /* per process_edges call (the first GG_STATS_RECORDS calls): */
+  uint32_t modes[GG_STATS_RECORDS];          /* 0 = push (sparse), 1 = pull (dense) */
+  uint32_t it_active[GG_STATS_RECORDS];      /* active vertices passed in */
+  uint32_t it_new[GG_STATS_RECORDS];         /* vertices activated by the call */
+  uint64_t it_examined[GG_STATS_RECORDS];    /* adjacency entries read (option count_examined) */
+  uint64_t it_unvisited[GG_STATS_RECORDS];   /* BFS: vertices not yet visited before the call */
 } gg_run_stats;
 
 /* PageRank, toolkits/pagerank.cpp:24-129: FP64, d = 0.85, non-normalised,

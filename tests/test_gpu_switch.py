@@ -36,7 +36,7 @@ def roots_of(out_degree, k, seed):
 
 def gpu_calls(st, n=None):
     n = st["iterations"] if n is None else n
-    assert n <= 64, "more process_edges calls than the stats record"
+    assert n <= 256, "more process_edges calls than the stats record"
     return np.asarray(st["modes"][:n], dtype=np.int64)
 
 
