@@ -921,6 +921,19 @@ __global__ void ms_reach_kernel(const uint64_t *vis, const uint32_t *outdeg, uin
   x = wave_sum_u64(x);
   if (lane_id() == 0 && x) atomicAdd(&acc[(global_wave() & 63) * 16], (unsigned long long)x);
 }
+// Roots of a batch by value, and the order of their lanes (order_lanes).
+struct Roots64 {
+  uint32_t r[64];
+};
+struct LaneMap {
+  uint8_t to_caller[64];  // internal lane -> the caller's root index
+  uint8_t to_lane[64];    // the caller's root index -> internal lane
+};
+static LaneMap identity_lanes() {
+  LaneMap m;
+  for (int i = 0; i < 64; i++) m.to_caller[i] = m.to_lane[i] = (uint8_t)i;
+  return m;
+}
 template <class T>
 __global__ void unpermute_rows_kernel(const T *in, T *out, const uint32_t *iperm, uint32_t V) {
   uint64_t n = (uint64_t)V * 64, s = (uint64_t)gridDim.x * blockDim.x;
@@ -944,12 +957,13 @@ struct LaneArgs {
 // lanes first load 64 arcs at once -- neighbour, weight and the neighbour's
 // selection mask (the roots for which the arc matters, e.g. the roots whose
 // distance at the neighbour changed) -- then walk only the selected arcs, in
-// arc order, four row loads in flight at a time.
-// Op: T, L; uint64_t select(u); L load(u, lane); void combine(T&, L, w).
+// arc order, Op::kRows row loads in flight at a time.
+// Op: T, L, kRows; uint64_t select(u); L load(u, lane); void combine(T&, L, w).
 template <class Op>
 __device__ __forceinline__ void lane_range(const Op &op, const uint32_t *nbr, const float *wt,
                                            uint64_t b, uint64_t e, uint64_t vm, int lane,
                                            typename Op::T &acc) {
+  constexpr int kRows = Op::kRows;
   for (uint64_t base = b; base < e; base += 64) {
     const uint64_t k = base + lane;
     uint32_t u = 0;
@@ -962,11 +976,11 @@ __device__ __forceinline__ void lane_range(const Op &op, const uint32_t *nbr, co
     }
     uint64_t any = __ballot(sel != 0);
     while (any) {
-      uint32_t uj[4];
-      float wj[4];
-      bool on[4];
+      uint32_t uj[kRows];
+      float wj[kRows];
+      bool on[kRows];
 #pragma unroll
-      for (int t = 0; t < 4; t++) {
+      for (int t = 0; t < kRows; t++) {
         uj[t] = 0;
         wj[t] = 0.f;
         on[t] = false;
@@ -978,11 +992,11 @@ __device__ __forceinline__ void lane_range(const Op &op, const uint32_t *nbr, co
           on[t] = (readlane_u64(sel, j) >> lane) & 1ULL;
         }
       }
-      typename Op::L x[4];
+      typename Op::L x[kRows];
 #pragma unroll
-      for (int t = 0; t < 4; t++) x[t] = on[t] ? op.load(uj[t], lane) : typename Op::L{};
+      for (int t = 0; t < kRows; t++) x[t] = on[t] ? op.load(uj[t], lane) : typename Op::L{};
 #pragma unroll
-      for (int t = 0; t < 4; t++)
+      for (int t = 0; t < kRows; t++)
         if (on[t]) op.combine(acc, x[t], wj[t]);
     }
   }
@@ -1030,6 +1044,7 @@ __global__ __launch_bounds__(kBlock) void lane_heavy_kernel(HeavyArgs a, Op op) 
 struct MssPull {
   using T = float;
   using L = float;
+  static constexpr int kRows = 8;  // f32 rows: 8 loads in flight beat 4 by 4% (RMAT-24, 64 roots)
   float *dist;
   const uint64_t *cm_cur;  // roots whose distance at a vertex changed last step
   uint64_t *cm_next, *out_bits;
@@ -1140,6 +1155,7 @@ __global__ void mss_seed_kernel(const uint32_t *roots, int n, float *dist, uint6
 struct MsbcSigma {
   using T = double;
   using L = double;
+  static constexpr int kRows = 4;
   __device__ bool enabled() const { return true; }
   double *sigma;
   const uint64_t *fprev, *fnew;
@@ -1160,6 +1176,7 @@ struct MsbcSigma {
 // D(v) += sum of D(w) over out-edges v->w with w at depth L (bc.cpp:125-167).
 struct MsbcBack {
   using T = double;
+  static constexpr int kRows = 4;
   __device__ bool enabled() const { return true; }
   struct L {
     uint32_t d;
@@ -1208,11 +1225,20 @@ __global__ void msbc_inv_kernel(double *sigma_inv, double *dd, uint64_t n) {
 // Entries k < live come from the traversal; the rows after them belong to
 // vertices no root can reach (no in-arcs, not a root): sigma 0, so
 // inv = 1/0 = +inf and D = (inf - inf) / inf = NaN, written without reading.
-__global__ void msbc_final_kernel(double *dd, double *inv, uint64_t live, uint64_t n) {
+// In place; a wave holds one 64-lane row per step (n and live are whole
+// rows, blocks whole waves), so the lanes go back to the caller's order by a
+// shuffle: caller lane c takes internal lane m.to_lane[c].
+__global__ void msbc_final_kernel(double *dd, double *inv, uint64_t live, uint64_t n, LaneMap m) {
   uint64_t s = (uint64_t)gridDim.x * blockDim.x;
+  const int lane = lane_id();
+  const int from = m.to_lane[lane];
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += s) {
     if (k < live) {
-      dd[k] = (dd[k] - inv[k]) / inv[k];  // bc.cpp:170-176
+      const double x = inv[k];
+      const double dep = (dd[k] - x) / x;  // bc.cpp:170-176
+      const double xo = shfl_f64(x, from), depo = shfl_f64(dep, from);
+      inv[k] = xo;
+      dd[k] = depo;
     } else {
       inv[k] = __builtin_inf();
       dd[k] = __builtin_nan("");
@@ -1223,10 +1249,10 @@ __global__ void msbc_final_kernel(double *dd, double *inv, uint64_t live, uint64
 // in the same pass (one read of each row instead of three passes).
 __global__ void msbc_final_unpermute_kernel(const double *dd, const double *inv,
                                             const uint32_t *iperm, uint32_t V, uint64_t live,
-                                            double *dep_out, double *inv_out) {
+                                            double *dep_out, double *inv_out, LaneMap m) {
   const uint64_t n = (uint64_t)V * 64, s = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += s) {
-    const uint64_t o = (uint64_t)iperm[k >> 6] * 64 + (k & 63);
+    const uint64_t o = (uint64_t)iperm[k >> 6] * 64 + m.to_caller[k & 63];
     if (k < live) {
       const double x = inv[k];
       dep_out[o] = (dd[k] - x) / x;  // bc.cpp:170-176
@@ -1245,6 +1271,47 @@ static int batch_roots(const gg_graph *g, const uint32_t *h_roots, int n_roots, 
   for (int i = 0; i < n_roots; i++) {
     if (h_roots[i] >= g->vertices) return fail(GG_EINVAL, "batched query: root out of range");
     out[i] = g->to_internal(h_roots[i]);
+  }
+  return GG_OK;
+}
+
+// Lane order of a batch.  A row gather loads only the lanes (roots) an arc
+// selects, and a 64-lane row of doubles spans four 128-byte lines, so roots
+// whose searches reach each level at the same pace are put side by side:
+// their lanes tend to be selected together and fall into fewer lines.  The
+// pace is estimated by the root's two-hop reach (the out-degree sum of its
+// out-neighbours), lanes in descending order of it.  Results do not depend
+// on the order; the outputs are written back in the caller's order.
+__global__ void root_reach_kernel(const uint64_t *index, const uint32_t *nbr, const uint32_t *deg,
+                                  Roots64 roots, uint64_t *out) {
+  const uint32_t r = roots.r[blockIdx.x];
+  const int lane = lane_id();
+  const uint64_t b = index[r], e = index[r + 1];
+  uint64_t s = 0;
+  for (uint64_t k = b + lane; k < e; k += 64) s += deg[nbr[k]];
+  for (int d = 32; d >= 1; d >>= 1) s += shfl_u64(s, lane ^ d);
+  if (lane == 0) out[blockIdx.x] = s;
+}
+// roots (internal ids, caller order) -> roots in lane order, and the map;
+// d_tmp: device scratch of tmp_bytes (n x 8 needed, else the order is kept).
+static int order_lanes(const Dir &out, uint32_t *roots, int n, uint64_t *d_tmp, size_t tmp_bytes,
+                       LaneMap *m) {
+  *m = identity_lanes();
+  if (n < 2 || tmp_bytes < 8 * (size_t)n) return GG_OK;
+  Roots64 r;
+  for (int i = 0; i < n; i++) r.r[i] = roots[i];
+  hipLaunchKernelGGL(root_reach_kernel, dim3(n), dim3(64), 0, stream(), out.index, out.nbr,
+                     out.degree, r, d_tmp);
+  uint64_t reach[64];
+  GG_HIP(hipMemcpyAsync(reach, d_tmp, 8 * (size_t)n, hipMemcpyDeviceToHost, stream()));
+  GG_HIP(hipStreamSynchronize(stream()));
+  uint8_t ord[64];
+  for (int i = 0; i < n; i++) ord[i] = (uint8_t)i;
+  std::stable_sort(ord, ord + n, [&](uint8_t a, uint8_t b) { return reach[a] > reach[b]; });
+  for (int j = 0; j < n; j++) {
+    m->to_caller[j] = ord[j];
+    m->to_lane[ord[j]] = (uint8_t)j;
+    roots[j] = r.r[ord[j]];
   }
   return GG_OK;
 }
@@ -2640,6 +2707,11 @@ int bc_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_de
   const uint64_t n = (uint64_t)V * 64;
   DeviceTimer tm;
   uint32_t *depth = (uint32_t *)w->ms_tmp;
+  // (kept rows are read back per root index: those keep the caller's order)
+  LaneMap lanes = identity_lanes();
+  if (!keep)  // the depth rows' buffer is free until the forward pass
+    GG_TRY(order_lanes(out_dir(g), roots, n_roots, (uint64_t *)w->ms_tmp, (size_t)V * 64 * 4,
+                       &lanes));
   double *sigma = g->reordered || keep ? (double *)w->mb_a : d_inv_paths;
   double *dd = g->reordered || keep ? (double *)w->mb_b : d_dep;
   if (rows) *rows = dd;
@@ -2709,10 +2781,10 @@ int bc_batch_run(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_de
   }
   if (g->reordered && !keep)
     hipLaunchKernelGGL(msbc_final_unpermute_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(), dd,
-                       sigma, g->iperm, V, live, d_dep, d_inv_paths);
+                       sigma, g->iperm, V, live, d_dep, d_inv_paths, lanes);
   else
     hipLaunchKernelGGL(msbc_final_kernel, dim3(grid1(n)), dim3(kBlock), 0, stream(), dd, sigma,
-                       live, n);
+                       live, n, lanes);
   en.st.device_ms = tm.stop();
   GG_HIP(hipGetLastError());
   en.st.reached = (uint64_t)n_roots;
