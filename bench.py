@@ -26,6 +26,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
+# The BFS graph's relabeling: the 2^19 highest-out-degree vertices first, in
+# degree order, the others in file order (the build option relabel_hubs).
+# Consecutive internal ids then have ascending file ids, so the parents that
+# a wave writes (indexed by file id) share lines; with every vertex in degree
+# order they land on random lines (1.13x the BFS rate with the hybrid order
+# on RMAT-24; PageRank, CC and batched BC prefer the full order, which their
+# legs keep).
+BFS_RELABEL_HUBS = 1 << 19
 
 
 def log(*a):
@@ -583,13 +591,20 @@ COPY_KERNEL = "gg::gg_copy_kernel"
 COPY_BYTES = 256 << 20
 
 
+def bfs_graph(gg, args):
+    """The bench's BFS graph: RMAT, permuted ids, relabelled with the hybrid
+    order (relabel_hubs = args.relabel_hubs) unless --no-reorder."""
+    with gg.options(relabel_hubs=args.relabel_hubs):
+        return gg.Graph.rmat(args.scale, args.edge_factor, seed=args.seed, permute=True,
+                             reorder=not args.no_reorder)
+
+
 def pmc_child(args):
     """Workload of the counter passes: one step of the bench (the same graph
     and roots), then one calibration copy of COPY_BYTES."""
     import geminigraph_amd as gg
     gg.init(0)
-    g = gg.Graph.rmat(args.scale, args.edge_factor, seed=args.seed, permute=True,
-                      reorder=not args.no_reorder)
+    g = bfs_graph(gg, args)
     V = 1 << args.scale
     roots = pick_roots(g.out_degree(), args.roots, args.seed + 1)
     parent = gg.DeviceArray(V, np.uint32)
@@ -664,8 +679,7 @@ def level_child(args):
     BFS's per-level records as JSON."""
     import geminigraph_amd as gg
     gg.init(0)
-    g = gg.Graph.rmat(args.scale, args.edge_factor, seed=args.seed, permute=True,
-                      reorder=not args.no_reorder)
+    g = bfs_graph(gg, args)
     roots = pick_roots(g.out_degree(), args.roots, args.seed + 1)
     parent = gg.DeviceArray(1 << args.scale, np.uint32)
     g.bfs_into(roots[1], parent)
@@ -715,7 +729,8 @@ def level_table(args):
     env = dict(os.environ, TMPDIR="/tmp")
     child = [sys.executable, os.path.abspath(__file__), "--level-child", "--scale", str(args.scale),
              "--edge-factor", str(args.edge_factor), "--roots", str(args.roots),
-             "--seed", str(args.seed)] + (["--no-reorder"] if args.no_reorder else [])
+             "--seed", str(args.seed), "--relabel-hubs", str(args.relabel_hubs)] + \
+        (["--no-reorder"] if args.no_reorder else [])
     runs = [("trace", ["--kernel-trace"])] + [(f"p{i}", ["--pmc", c]) for i, c in
                                                enumerate(LEVEL_PASSES)]
     stats, per = None, {}
@@ -777,6 +792,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-reorder", action="store_true",
                     help="keep file vertex ids (no degree-sorted relabeling)")
+    ap.add_argument("--relabel-hubs", type=int, default=BFS_RELABEL_HUBS,
+                    help="BFS graph: vertices kept in degree order before the file-order rest "
+                         "(0 = all in degree order)")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the serving trace (0 skips the serving leg)")
     ap.add_argument("--no-pmc", action="store_true",
@@ -861,7 +879,7 @@ def main():
 
     t0 = time.time()
     reorder = not args.no_reorder
-    g = gg.Graph.rmat(scale, ef, seed=args.seed, permute=True, reorder=reorder)
+    g = bfs_graph(gg, args)
     gg.synchronize()
     build_s = time.time() - t0
     log(f"RMAT-{scale} ef{ef}: V={V} E={E} built on device in {build_s:.2f}s")
@@ -990,11 +1008,15 @@ def main():
     src.free()
     dst.free()
     if not args.no_extras:
-        extras["pagerank"] = pagerank_extra(gg, g, V, E)
-        log(f"PageRank: {extras['pagerank']}")
         del g
         parent.free()
         depth.free()
+        # PageRank on the same RMAT graph with every vertex in degree order
+        # (the default relabeling)
+        gp = gg.Graph.rmat(scale, ef, seed=args.seed, permute=True, reorder=reorder)
+        extras["pagerank"] = pagerank_extra(gg, gp, V, E)
+        log(f"PageRank: {extras['pagerank']}")
+        del gp
         extras["facade"] = facade_extra(gg, scale, ef, args.seed, roots[:4])
         log(f"facade: {extras['facade']}")
         extras["pagerank_rmat16"] = pagerank16_extra(gg, args.seed)
@@ -1041,6 +1063,7 @@ def main():
                    "outputs": "parent[V] per root, as bfs.cpp:26-94 (depths checked from "
                               "the parent tree)",
                    "reorder": reorder,
+                   "relabel_hubs": args.relabel_hubs if reorder else None,
                    "switch_divisor": gg.get_option("switch_divisor")},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -188,6 +188,21 @@ __global__ void relabel_keys_kernel(const uint32_t *deg, const uint32_t *in_deg,
     ids[v] = v;
   }
 }
+// Hybrid order: the vertices of degree rank < hubs keep that rank; the others
+// follow in file order, those without in-arcs last.  Consecutive internal
+// ids outside the hubs then have ascending file ids, so outputs indexed by
+// file id (BFS parents, depths) are written near one another by the waves
+// that find neighbouring vertices.
+__global__ void hybrid_keys_kernel(const uint32_t *rank, const uint32_t *deg,
+                                   const uint32_t *in_deg, uint32_t V, uint32_t hubs,
+                                   uint64_t *keys, uint32_t *ids) {
+  uint32_t s = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s) {
+    const uint64_t noin = (in_deg ? in_deg[v] : deg[v]) == 0;
+    keys[v] = rank[v] < hubs ? rank[v] : hubs + (noin << 32) + v;
+    ids[v] = v;
+  }
+}
 __global__ void invert_perm_kernel(const uint32_t *iperm, uint32_t V, uint32_t *perm) {
   uint32_t s = gridDim.x * blockDim.x;
   for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < V; n += s) perm[iperm[n]] = n;
@@ -412,8 +427,9 @@ static int build_dir(gg_graph *g, Dir &d, const uint32_t *edges, uint64_t ne, in
 }
 
 // Degree order: order[k] = the k-th vertex by descending out-degree (ties:
-// vertices with in-arcs first, then by id), rank its inverse.
-static int degree_order(gg_graph *g, uint32_t *order, uint32_t *rank) {
+// vertices with in-arcs first, then by id), rank its inverse.  With hubs in
+// (0, V), the hybrid order of hybrid_keys_kernel instead.
+static int degree_order(gg_graph *g, uint32_t *order, uint32_t *rank, uint32_t hubs = 0) {
   uint32_t V = g->vertices;
   uint64_t *keys = nullptr, *keys_alt = nullptr;
   uint32_t *ids = nullptr, *ids_alt = nullptr;
@@ -421,26 +437,40 @@ static int degree_order(gg_graph *g, uint32_t *order, uint32_t *rank) {
   if (r == GG_OK) r = device_alloc((void **)&keys_alt, (size_t)V * 8);
   if (r == GG_OK) r = device_alloc((void **)&ids, (size_t)V * 4);
   if (r == GG_OK) r = device_alloc((void **)&ids_alt, (size_t)V * 4);
-  if (r == GG_OK) {
-    hipLaunchKernelGGL(relabel_keys_kernel, dim3(grid_for(V)), dim3(256), 0, stream(),
-                       g->out.degree, g->symmetric ? nullptr : g->in.degree, V, keys, ids);
+  void *tmp = nullptr;
+  // sorts (keys, ids) over end_bit bits; order = the sorted ids, rank its inverse
+  auto sort_order = [&](int end_bit) -> hipError_t {
     rocprim::double_buffer<uint64_t> kb(keys, keys_alt);
     rocprim::double_buffer<uint32_t> vb(ids, ids_alt);
     size_t tmp_bytes = 0;
-    hipError_t e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, kb, vb, (size_t)V, 0, 33, stream());
-    void *tmp = nullptr;
-    if (e == hipSuccess) r = device_alloc(&tmp, tmp_bytes);
-    if (e == hipSuccess && r == GG_OK)
-      e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kb, vb, (size_t)V, 0, 33, stream());
-    if (e == hipSuccess && r == GG_OK)
+    hipError_t e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, kb, vb, (size_t)V, 0, end_bit,
+                                             stream());
+    if (e == hipSuccess && !tmp && device_alloc(&tmp, tmp_bytes) != GG_OK) return hipErrorOutOfMemory;
+    if (e == hipSuccess)
+      e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kb, vb, (size_t)V, 0, end_bit, stream());
+    if (e == hipSuccess)
       e = hipMemcpyAsync(order, vb.current(), (size_t)V * 4, hipMemcpyDeviceToDevice, stream());
-    if (e == hipSuccess && r == GG_OK)
+    if (e == hipSuccess)
       hipLaunchKernelGGL(invert_perm_kernel, dim3(grid_for(V)), dim3(256), 0, stream(), order, V,
                          rank);
+    return e;
+  };
+  if (r == GG_OK) {
+    const uint32_t *in_deg = g->symmetric ? nullptr : g->in.degree;
+    hipLaunchKernelGGL(relabel_keys_kernel, dim3(grid_for(V)), dim3(256), 0, stream(),
+                       g->out.degree, in_deg, V, keys, ids);
+    hipError_t e = sort_order(33);
+    if (e == hipSuccess && hubs > 0 && hubs < V) {
+      // the temporary space of the first sort serves the second (same sizes)
+      hipLaunchKernelGGL(hybrid_keys_kernel, dim3(grid_for(V)), dim3(256), 0, stream(), rank,
+                         g->out.degree, in_deg, V, hubs, keys, ids);
+      e = sort_order(34);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(stream());
     device_free(tmp);
-    if (e != hipSuccess && r == GG_OK)
-      r = fail(GG_EHIP, std::string("degree order: ") + hipGetErrorString(e));
+    if (e != hipSuccess)
+      r = fail(e == hipErrorOutOfMemory ? GG_ENOMEM : GG_EHIP,
+               std::string("degree order: ") + hipGetErrorString(e));
   }
   device_free(keys);
   device_free(keys_alt);
@@ -455,7 +485,9 @@ static int relabel(gg_graph *g) {
   uint32_t V = g->vertices;
   int r = graph_alloc(g, (void **)&g->perm, (size_t)V * 4);
   if (r == GG_OK) r = graph_alloc(g, (void **)&g->iperm, (size_t)V * 4);
-  if (r == GG_OK) r = degree_order(g, g->iperm, g->perm);
+  const int64_t hubs = options().relabel_hubs;
+  if (r == GG_OK)
+    r = degree_order(g, g->iperm, g->perm, hubs > 0 && hubs < V ? (uint32_t)hubs : 0);
   uint32_t *scratch = nullptr;
   if (r == GG_OK) r = device_alloc((void **)&scratch, (size_t)V * 4);
   if (r == GG_OK) {

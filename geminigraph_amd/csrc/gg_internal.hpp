@@ -88,6 +88,10 @@ struct Options {
   int64_t batch_graphs = 0;      // batched BFS levels replayed from captured hipGraphs
   int64_t batch_device_loop = 1; // batched BFS/SSSP iterations decided on the device
   int64_t bfs_ahead = 1;         // BFS: iterations the host issues ahead of the decisions
+  // relabeling (fixed at build): the relabel_hubs highest-out-degree vertices
+  // first, in degree order, then the others in file order, those without
+  // in-arcs last (0: every vertex in degree order)
+  int64_t relabel_hubs = 0;
 };
 Options &options();
 bool prof_on();  // per-kernel event timing (gg_prof_enable) is on

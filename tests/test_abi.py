@@ -56,6 +56,14 @@ def test_last_error_and_options_without_gpu():
         gg.set_option("no_such_option", 1)
     with pytest.raises(gg.GGError):
         gg.set_option("force_mode", 7)
+    # the relabeling's hub count: 0 (every vertex in degree order) up to 2^32 - 1
+    assert gg.get_option("relabel_hubs") == 0
+    with gg.options(relabel_hubs=1 << 19):
+        assert gg.get_option("relabel_hubs") == 1 << 19
+    assert gg.get_option("relabel_hubs") == 0
+    for bad in (-1, 1 << 32):
+        with pytest.raises(gg.GGError):
+            gg.set_option("relabel_hubs", bad)
 
 
 @pytest.mark.parametrize("bad", [-0.5, float("nan"), -float("inf")])

@@ -184,9 +184,11 @@ def test_sssp_rmat26_relabelled_plan_fixed_point(gg):
 
 @pytest.mark.timeout(900)
 def test_bfs_rmat24_bench_roots(gg, rmat24):
+    # the bench's graph: hybrid relabeling (top 2^19 vertices in degree order)
     e, o = rmat24
     V = 1 << 24
-    g = gg.Graph.rmat(24, 16, seed=24, permute=True, reorder=True)
+    with gg.options(relabel_hubs=1 << 19):
+        g = gg.Graph.rmat(24, 16, seed=24, permute=True, reorder=True)
     roots = pick_roots(g.out_degree(), 16, 25)
     od = o.out_degree().astype(np.int64)
     assert np.array_equal(g.out_degree(), o.out_degree())

@@ -532,15 +532,40 @@ def test_reordered_build_matches_oracle(gg, undirected, weighted):
     _segments_equal(*g.csc()[:2], *o.csc()[:2], g.csc()[2], o.csc()[2])
 
 
+@pytest.mark.parametrize("undirected", [False, True])
+def test_hybrid_relabel_order(gg, undirected):
+    # relabel_hubs = K: the K highest-out-degree vertices first, in the full
+    # degree order; then the others in file order, those without in-arcs last
+    e = oracle.rmat_edges(11, seed=44)
+    V, K = 1 << 11, 150
+    with gg.options(relabel_hubs=K):
+        g = gg.Graph.build(e, V, undirected=undirected, reorder=True)
+    full = gg.Graph.build(e, V, undirected=undirected, reorder=True)
+    iperm, iperm_full = np.argsort(g.perm()), np.argsort(full.perm())
+    assert np.array_equal(iperm[:K], iperm_full[:K])
+    o = oracle.Graph(e, V, undirected=undirected)
+    indeg = o.out_degree() if undirected else o.in_degree()
+    rest = np.setdiff1d(np.arange(V), iperm_full[:K])
+    want = np.concatenate([rest[indeg[rest] > 0], rest[indeg[rest] == 0]])
+    assert np.array_equal(iperm[K:], want)
+    assert np.array_equal(g.out_degree(), o.out_degree())
+    _segments_equal(*g.csr()[:2], *o.csr()[:2], g.csr()[2], o.csr()[2])
+    _segments_equal(*g.csc()[:2], *o.csc()[:2], g.csc()[2], o.csc()[2])
+
+
+@pytest.mark.parametrize("hubs", [0, 700])
 @pytest.mark.parametrize("mode", ["auto", "push", "pull"])
-def test_reordered_toolkits_match_oracle(gg, mode):
+def test_reordered_toolkits_match_oracle(gg, mode, hubs):
+    # hubs > 0: the hybrid relabeling (the top vertices in degree order, the
+    # rest in file order), the BFS bench graph's
     scale = 13
     V = 1 << scale
     e = oracle.rmat_edges(scale, seed=43)
     ew = oracle.rmat_edges(scale, seed=43, weighted=True)
-    g = gg.Graph.build(e, V, reorder=True)
-    gu = gg.Graph.build(e, V, undirected=True, reorder=True)
-    gw = gg.Graph.build(ew, V, reorder=True)
+    with gg.options(relabel_hubs=hubs):
+        g = gg.Graph.build(e, V, reorder=True)
+        gu = gg.Graph.build(e, V, undirected=True, reorder=True)
+        gw = gg.Graph.build(ew, V, reorder=True)
     o, ou, ow = oracle.Graph(e, V), oracle.Graph(e, V, undirected=True), oracle.Graph(ew, V)
     roots = roots_of(o.out_degree(), 2, 43)
     say("reordered", mode)
