@@ -104,6 +104,13 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *                      decided one (1..8, default 1); more hides the host's
  *                      reaction time on short levels, each one past the end
  *                      costs launches that return at once
+ *   "relabel_hubs"     GG_BUILD_REORDER builds (read at build time): K > 0
+ *                      keeps the K highest-out-degree vertices first, in
+ *                      degree order, and the others in file order, those
+ *                      without in-arcs last, so that BFS parents and depths
+ *                      (indexed by file id) found by one wave share lines;
+ *                      0 = every vertex in degree order (default; PageRank,
+ *                      CC and the batched BC run faster on it)
  *   "batch_graphs"     batched BFS: 1 = replay each level (after the first)
  *                      from a hipGraph captured once per buffer parity;
  *                      0 = launch the level's kernels (default: measured
