@@ -736,8 +736,10 @@ static BfsPlan bfs_plan(const gg_graph *g, uint32_t root, uint32_t *parent, uint
   const uint64_t pw = (uint64_t)opt.bfs_pull_waves;
   p.pull_grid = grid_waves(pw ? std::min<uint64_t>(p.ngroups, pw) : p.ngroups);
   p.push_blocks = grid_waves((uint64_t)opt.huge_grid_waves);
-  const uint64_t nch = std::max(in.n_chunks, 1u);
-  p.heavy_blocks = grid_waves(pw ? std::min<uint64_t>(nch, pw) : nch);
+  // the heavy-chunk step finds little left after the bottom-up step's head
+  // tests: half the pull grid (its launch cost grows with the grid)
+  const uint64_t nch = std::max(in.n_chunks, 1u), hw = pw / 2;
+  p.heavy_blocks = grid_waves(hw ? std::min<uint64_t>(nch, hw) : nch);
   p.grid_b = std::max(p.push_blocks, in.n_chunks ? p.heavy_blocks : 1u);
   return p;
 }

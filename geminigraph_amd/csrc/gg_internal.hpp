@@ -84,7 +84,9 @@ struct Options {
   int64_t pr_bin_bits = 13;      // PageRank bin plan: 2^bits destinations per bin
   int64_t pr_range_bits = 14;    // PageRank bin plan: 2^bits sources per cold range
   int64_t heavy_waves = 8192;    // dense heavy-chunk grids: waves (0 = one per chunk)
-  int64_t bfs_pull_waves = 16384; // BFS pull grids: waves (0 = one per group / chunk)
+  // BFS pull grid: waves (0 = one per group); the heavy-chunk step takes half.
+  // 8192 = one wave per wave slot of the chip (256 CUs x 4 SIMDs x 8 waves)
+  int64_t bfs_pull_waves = 8192;
   int64_t batch_graphs = 0;      // batched BFS levels replayed from captured hipGraphs
   int64_t batch_device_loop = 1; // batched BFS/SSSP iterations decided on the device
   int64_t bfs_ahead = 1;         // BFS: iterations the host issues ahead of the decisions

@@ -90,7 +90,9 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *   "pr_range_bits"    that plan: 2^bits sources per cold range (8..14,
  *                      default 14)
  *   "heavy_waves"      waves of the dense heavy-chunk grids (0 = one per chunk)
- *   "bfs_pull_waves"   waves of the BFS pull grids (0 = one per group / chunk)
+ *   "bfs_pull_waves"   waves of the BFS bottom-up grid (default 8192, one per
+ *                      wave slot of MI355X; 0 = one per 64-vertex group);
+ *                      the heavy-chunk step uses half as many
  *   "level_masks"      batched BFS/BC: 1 = keep per-level root masks
  *                      (default; depth rows are built once at the end and
  *                      BC selects backward arcs exactly), 0 = write depth
