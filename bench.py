@@ -315,32 +315,21 @@ def rmat26_extra(gg, seed, scale=26, ef=16, n_roots=8, iterations=20):
            "sssp_roots": len(roots), "sssp_ms_mean": round(1e3 * float(np.mean(times)), 1),
            "sssp_ms_max": round(1e3 * float(np.max(times)), 1),
            "sssp_iterations": iters, "sssp_gteps": round(m / sum(times) / 1e9, 2)}
+    # PageRank measured as on the RMAT-24 graph (pagerank_extra): wall time of
+    # 3 calls after the plan-building first one, the roofline fraction from
+    # the kernels' event time; the wall-time fraction beside it
     h = gg.Graph.rmat(scale, ef, seed=seed, permute=True, reorder=True)
-    pr = gg.DeviceArray(V, np.float64)
-    t = time.perf_counter()
-    h.pagerank_into(pr, iterations)
-    gg.synchronize()
-    first_s = time.perf_counter() - t
-    t = time.perf_counter()
-    h.pagerank_into(pr, iterations)
-    gg.synchronize()
-    pr_s = time.perf_counter() - t
-    plan = pr.numpy()
-    with gg.options(pr_hot_sources=0):
-        t = time.perf_counter()
-        h.pagerank_into(pr, iterations)
-        gg.synchronize()
-        pull_s = time.perf_counter() - t
-    pulled = pr.numpy()
-    pr.free()
+    r = pagerank_extra(gg, h, V, E, iterations)
     h.close()
     b_iter = 12 * E + 28 * V
-    out.update({"pagerank_ms_per_iter": round(pr_s * 1e3 / iterations, 3),
-                "pagerank_first_call_ms": round(first_s * 1e3, 1),
-                "pagerank_gteps": round(iterations * E / pr_s / 1e9, 2),
-                "pagerank_frac": round(iterations * b_iter / pr_s / 1e9 / HBM_PEAK_GBS, 4),
-                "pagerank_pull_ms_per_iter": round(pull_s * 1e3 / iterations, 3),
-                "pagerank_max_rel_vs_pull": float(np.max(np.abs(plan - pulled) / np.abs(pulled)))})
+    out.update({"pagerank_ms_per_iter": r["ms_per_iter"],
+                "pagerank_first_call_ms": r["first_call_ms"],
+                "pagerank_gteps": r["gteps"],
+                "pagerank_frac": r["frac"],
+                "pagerank_frac_wall": round(b_iter / (r["ms_per_iter"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                "pagerank_kernel_ms_per_iter": r["kernel_ms_per_iter"],
+                "pagerank_pull_ms_per_iter": r["pull_ms_per_iter"],
+                "pagerank_max_rel_vs_pull": r["max_rel_vs_pull"]})
     return out
 
 
