@@ -18,8 +18,21 @@ struct BitmapView {
   __device__ unsigned long get_bit(size_t i) const {
     return data[WORD_OFFSET(i)] & (1ul << BIT_OFFSET(i));
   }
+  // Lanes of a wave that set the bits of one word in lane order (a wave over
+  // 64 consecutive vertices, as process_vertices runs) take one atomic for
+  // the word; any other pattern one atomic per lane.
   __device__ void set_bit(size_t i) const {
-    atomicOr((unsigned long long *)(data + WORD_OFFSET(i)), 1ull << BIT_OFFSET(i));
+    const size_t w = WORD_OFFSET(i);
+    const unsigned lane = __lane_id();
+    const unsigned long long act = __ballot(1);
+    const int first = __ffsll((long long)act) - 1;
+    const size_t w0 = ((size_t)(unsigned)__shfl((int)(w >> 32), first) << 32) |
+                      (unsigned)__shfl((int)(unsigned)w, first);
+    if (__ballot(w != w0 || BIT_OFFSET(i) != lane) == 0) {
+      if ((int)lane == first) atomicOr((unsigned long long *)(data + w0), act);
+    } else {
+      atomicOr((unsigned long long *)(data + w), 1ull << BIT_OFFSET(i));
+    }
   }
 };
 

@@ -107,3 +107,56 @@ def test_bc_toolkit(files, tmp_path):
     assert active_lines(out) == rfr.tolist()
     assert_rel(inv, rinv)
     assert_rel(dep, rdep)
+
+
+# A graph whose lists take the facade's long-list paths: vertex HUB has 5000
+# out-arcs (sparse slots above 2048 entries, spread over the grid), SINK has
+# 5000 in-arcs (dense signals in parts of 2048), joined to an RMAT part.
+HV = 1 << 14
+HUB, SINK = 8192, 14000
+
+
+@pytest.fixture(scope="module")
+def hub_files(tmp_path_factory):
+    d = tmp_path_factory.mktemp("hub")
+    r = oracle.rmat_edges(13, seed=64)
+    pairs = np.stack([r["src"], r["dst"]], axis=1).astype(np.int64).tolist()
+    pairs += [(0, HUB)] + [(HUB, HUB + 1 + i) for i in range(5000)]
+    pairs += [(HUB + 1 + i, SINK) for i in range(5000)] + [(SINK, SINK + 1 + i) for i in range(99)]
+    pairs += [(SINK + 50, 3)]
+    e = oracle.edges_from_pairs(pairs)
+    w = ((np.arange(len(e)) % 13) + 1).astype(np.float32) / 16
+    ew = oracle.edges_from_pairs(np.stack([e["src"], e["dst"]], 1), w)
+    e.tofile(d / "h.bin")
+    ew.tofile(d / "hw.bin")
+    return d, e, ew
+
+
+@pytest.mark.parametrize("root", [0, HUB])
+def test_bfs_toolkit_long_lists(hub_files, tmp_path, root):
+    d, e, _ = hub_files
+    o = oracle.Graph(e, HV)
+    out = run_toolkit("bfs", [d / "h.bin", HV, root], tmp_path)
+    parent = load(tmp_path, "parent", np.uint32)
+    _, depth, fr = o.bfs(root)
+    assert active_lines(out) == fr.tolist()
+    check_parents(e, HV, root, parent, depth)
+
+
+def test_pagerank_sssp_bc_toolkits_long_lists(hub_files, tmp_path):
+    d, e, ew = hub_files
+    run_toolkit("pagerank", [d / "h.bin", HV, 20], tmp_path)
+    ref, _ = oracle.Graph(e, HV).pagerank(20)
+    assert_rel(load(tmp_path, "pagerank", np.float64), ref)
+    run_toolkit("sssp", [d / "hw.bin", HV, 0], tmp_path)
+    rdist, _ = oracle.Graph(ew, HV).sssp(0)
+    assert np.array_equal(load(tmp_path, "distance", np.float32).view(np.uint32),
+                          rdist.view(np.uint32))
+    out = run_toolkit("bc", [d / "h.bin", HV, 0], tmp_path)
+    rdep, rinv, rfr = oracle.Graph(e, HV).bc(0)
+    assert active_lines(out) == rfr.tolist()
+    assert_rel(load(tmp_path, "inv_num_paths", np.float64), rinv)
+    assert_rel(load(tmp_path, "dependencies", np.float64), rdep)
+    out = run_toolkit("cc", [d / "h.bin", HV], tmp_path)
+    ref_l, _ = oracle.Graph(e, HV, undirected=True).cc()
+    assert np.array_equal(load(tmp_path, "label", np.uint32), ref_l)
