@@ -58,7 +58,7 @@ class RunStats(ctypes.Structure):
     ]
 
     def as_dict(self):
-        n = min(self.iterations, 64)
+        n = min(self.iterations, STATS_RECORDS)
         return {
             "iterations": self.iterations,
             "push_iterations": self.push_iterations,
