@@ -154,3 +154,20 @@ def test_pagerank_always_dense(gg):
     modes, _ = ref_calls(tr)
     assert modes.tolist() == [1] * 10
     assert np.array_equal(gpu_calls(st), modes)
+
+
+def test_long_chain_records_every_iteration(gg):
+    # 200 levels: the per-call records (256 of them) hold every iteration
+    n = 200
+    e = oracle.edges_from_pairs([(i, i + 1) for i in range(n - 1)])
+    g = gg.Graph.build(e, n)
+    with gg.options(**REFERENCE):
+        _, depth, fr, st = g.bfs(0)
+    tr = oracle.Trace()
+    _, rdepth, rfr = oracle.Graph(e, n).bfs(0, trace=tr)
+    modes, active = ref_calls(tr)
+    assert np.array_equal(depth, rdepth) and np.array_equal(fr, rfr)
+    assert st["iterations"] == len(modes) > 64
+    assert len(st["modes"]) == len(modes)
+    assert np.array_equal(gpu_calls(st), modes)
+    assert np.array_equal(np.asarray(st["it_active"]), active)
