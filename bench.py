@@ -149,7 +149,8 @@ def batch_extra(gg, seed, reorder):
     return out
 
 
-def serve_extra(gg, seed, reorder, seconds, rate=2000.0, workers=4, budget=200e9):
+def serve_extra(gg, seed, reorder, seconds, rate=2000.0, workers=4, budget=200e9,
+                bfs_workers=None):
     """Serving: a Poisson stream at `rate` queries/s, BFS 50% / SSSP 30% /
     BC 20%, uniformly random roots, over 4 resident weighted RMAT graphs
     (scales 18, 20, 22, 24, edge factor 16) admitted by a 200 GB budget; the
@@ -167,7 +168,8 @@ def serve_extra(gg, seed, reorder, seconds, rate=2000.0, workers=4, budget=200e9
         arrival = np.cumsum(rng.exponential(1.0 / rate, n))
         return arrival, gi, kind, root
 
-    with gg.Server(int(budget), workers=workers) as s:
+    bw = gg.get_option("serve_bfs_workers") if bfs_workers is None else bfs_workers
+    with gg.options(serve_bfs_workers=bw), gg.Server(int(budget), workers=workers) as s:
         ids = [s.add_graph(g) for g in graphs]
         # warm-up: every worker allocates its buffers for each (graph, kind)
         arr, gi, kind, root = trace(400)
@@ -180,7 +182,8 @@ def serve_extra(gg, seed, reorder, seconds, rate=2000.0, workers=4, budget=200e9
     batches = info["batches"] - before["batches"]
     out = {"workload": f"Poisson {rate:.0f} q/s for {seconds} s, BFS/SSSP/BC 50/30/20, "
                        f"weighted RMAT scales {list(scales)} edge factor 16",
-           "queries": n, "workers": workers, "budget_GB": budget / 1e9,
+           "queries": n, "workers": workers, "bfs_workers": min(bw, workers - 1),
+           "budget_GB": budget / 1e9,
            "queries_per_s": round(n / wall, 1), "offered_per_s": rate,
            "p50_ms": round(float(np.percentile(lat, 50)), 2),
            "p99_ms": round(float(np.percentile(lat, 99)), 2),
@@ -805,6 +808,8 @@ def main():
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving leg and print its figures")
     ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-bfs-workers", type=int, default=None,
+                    help="serving: workers reserved for BFS queries (default: the option's)")
     args = ap.parse_args()
     if args.pmc_child:
         pmc_child(args)
@@ -853,7 +858,7 @@ def main():
     if args.serve_only:
         import geminigraph_amd as gg
         gg.init(0)
-        log(f"serving: {serve_extra(gg, args.seed + 11, not args.no_reorder, args.serve_seconds, workers=args.serve_workers)}")
+        log(f"serving: {serve_extra(gg, args.seed + 11, not args.no_reorder, args.serve_seconds, workers=args.serve_workers, bfs_workers=args.serve_bfs_workers)}")
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -1020,7 +1025,8 @@ def main():
         log(f"rmat26: {extras['rmat26']}")
         if args.serve_seconds > 0:
             extras["serving"] = serve_extra(gg, args.seed + 11, reorder, args.serve_seconds,
-                                            workers=args.serve_workers)
+                                            workers=args.serve_workers,
+                                            bfs_workers=args.serve_bfs_workers)
             log(f"serving: {extras['serving']}")
 
     cpu = None

@@ -94,6 +94,7 @@ struct Options {
   // first, in degree order, then the others in file order, those without
   // in-arcs last (0: every vertex in degree order)
   int64_t relabel_hubs = 0;
+  int64_t serve_bfs_workers = 2;  // servers: workers that take BFS queries only (read at create)
 };
 Options &options();
 bool prof_on();  // per-kernel event timing (gg_prof_enable) is on

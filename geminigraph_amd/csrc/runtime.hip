@@ -212,6 +212,7 @@ static int64_t *option_slot(const char *name) {
   if (!strcmp(name, "batch_device_loop")) return &o.batch_device_loop;
   if (!strcmp(name, "bfs_ahead")) return &o.bfs_ahead;
   if (!strcmp(name, "relabel_hubs")) return &o.relabel_hubs;
+  if (!strcmp(name, "serve_bfs_workers")) return &o.serve_bfs_workers;
   return nullptr;
 }
 
@@ -229,6 +230,7 @@ int gg_set_option(const char *name, int64_t value) {
     return fail(GG_EINVAL, "bfs_alpha must be in [0, 2^20]");
   const bool zero_ok = !strcmp(name, "bfs_pull_waves") || !strcmp(name, "pr_hot_sources") ||
                        !strcmp(name, "bfs_alpha") || !strcmp(name, "relabel_hubs") ||
+                       !strcmp(name, "serve_bfs_workers") ||
                        !strcmp(name, "heavy_waves");
   if (zero_ok && value < 0) return fail(GG_EINVAL, std::string(name) + " must be >= 0");
   if (strcmp(name, "force_mode") && !zero_ok && !flag && value <= 0)
@@ -239,6 +241,8 @@ int gg_set_option(const char *name, int64_t value) {
       (value < 8 || value > 14))
     return fail(GG_EINVAL, std::string(name) + " must be in [8, 14]");
   if (!strcmp(name, "bfs_ahead") && value > 8) return fail(GG_EINVAL, "bfs_ahead must be in [1, 8]");
+  if (!strcmp(name, "serve_bfs_workers") && value > 31)
+    return fail(GG_EINVAL, "serve_bfs_workers must be in [0, 31]");
   if (!strcmp(name, "relabel_hubs") && value > 0xFFFFFFFFll)
     return fail(GG_EINVAL, "relabel_hubs must be < 2^32");
   if (!strcmp(name, "pull_light_max") && value > (1 << 20))

@@ -4,7 +4,8 @@
 //
 // Workers are threads, each bound to its own HIP stream and traversal-context
 // slot, so the batches they run overlap on the device.  A free worker takes
-// the oldest pending query whose (graph, kind) has no batch running, and
+// the oldest pending query whose (graph, kind) has no batch running (of kind
+// BFS only, for the serve_bfs_workers workers reserved for it), and
 // every other pending query of the same graph and kind (up to 64) --
 // queries that arrived while the previous batch of their kind ran join the
 // next one.  One batch per (graph, kind) at a time keeps batches full (a
@@ -77,6 +78,7 @@ struct gg_server {
   std::condition_variable cv_work, cv_done;
   std::deque<gg::SQuery *> pending;
   std::vector<unsigned char> running;  // [graph * 3 + kind]: a batch is running
+  int bfs_workers = 0;  // workers 0 .. bfs_workers-1 take BFS queries only
   std::map<uint64_t, gg::SQuery *> queries;
   uint64_t next_id = 1, submitted = 0, completed = 0, batches = 0, max_batch = 0;
   bool stop = false;
@@ -158,11 +160,14 @@ static int run_batch(gg_graph *g, WorkerBuf &b, int kind, std::vector<SQuery *> 
   return GG_OK;
 }
 
-// The oldest pending query whose graph and kind have no batch running
-// (caller holds mu); pending.end() if there is none.
-static std::deque<SQuery *>::iterator runnable_locked(gg_server *s) {
+// The oldest pending query whose graph and kind have no batch running, of
+// kind BFS only for a BFS worker (caller holds mu); pending.end() if there is
+// none.
+static std::deque<SQuery *>::iterator runnable_locked(gg_server *s, bool bfs_only) {
   for (auto it = s->pending.begin(); it != s->pending.end(); ++it)
-    if (!s->running[(size_t)(*it)->graph * 3 + (*it)->kind]) return it;
+    if (!s->running[(size_t)(*it)->graph * 3 + (*it)->kind] &&
+        (!bfs_only || (*it)->kind == GG_QUERY_BFS))
+      return it;
   return s->pending.end();
 }
 
@@ -170,11 +175,12 @@ static void worker_main(gg_server *s, int k) {
   hipSetDevice(s->device);
   bind_thread(s->streams[k], k + 1);
   std::unique_lock<std::mutex> lk(s->mu);
+  const bool bfs_only = k < s->bfs_workers;
   while (true) {
     std::deque<SQuery *>::iterator first;
     s->cv_work.wait(lk, [&] {
       if (s->stop && s->pending.empty()) return true;
-      first = runnable_locked(s);
+      first = runnable_locked(s, bfs_only);
       return first != s->pending.end();
     });
     if (s->stop && s->pending.empty()) break;  // stop requested and drained
@@ -231,6 +237,10 @@ int gg_server_create(size_t budget_bytes, int workers, gg_server **out) {
   hipGetDevice(&s->device);
   s->streams.resize(workers);
   s->bufs.resize(workers);
+  // BFS batches take milliseconds where SSSP and BC batches of the same
+  // graph take tens to hundreds: reserved workers keep BFS queries from
+  // waiting behind them (at least one worker takes every kind)
+  s->bfs_workers = (int)std::min<int64_t>(options().serve_bfs_workers, workers - 1);
   for (int k = 0; k < workers; k++) {
     if (hipStreamCreateWithFlags(&s->streams[k], hipStreamNonBlocking) != hipSuccess) {
       for (int j = 0; j < k; j++) hipStreamDestroy(s->streams[j]);

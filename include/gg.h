@@ -113,6 +113,10 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *                      (indexed by file id) found by one wave share lines;
  *                      0 = every vertex in degree order (default; PageRank,
  *                      CC and the batched BC run faster on it)
+ *   "serve_bfs_workers" gg_server_create: workers that take BFS queries only
+ *                      (default 2, at most workers - 1; 0 = every worker takes
+ *                      every kind): BFS batches take milliseconds, SSSP and
+ *                      BC batches of the same graph tens to hundreds
  *   "batch_graphs"     batched BFS: 1 = replay each level (after the first)
  *                      from a hipGraph captured once per buffer parity;
  *                      0 = launch the level's kernels (default: measured
@@ -304,10 +308,11 @@ int gg_bc_batch(gg_graph *g, const uint32_t *h_roots, int n_roots, double *d_dep
  * reference, whose Graph is not re-entrant: core/graph.hpp:132-141).
  * Queries (BFS depths, SSSP distances, BC dependencies from one root) are
  * admitted as they arrive; each worker, on its own HIP stream and its own
- * traversal buffers, takes the oldest pending query and every pending query
- * of the same graph and kind (up to 64) and runs them as one batched
- * traversal, so batches from different workers overlap on the device while
- * new queries queue.  Device memory is admitted against `budget_bytes`: the
+ * traversal buffers, takes the oldest pending query whose graph and kind
+ * have no batch running (BFS queries only, for the serve_bfs_workers workers
+ * reserved for them) and every pending query of the same graph and kind (up
+ * to 64) and runs them as one batched traversal, so batches from different
+ * workers overlap on the device while new queries queue.  Device memory is admitted against `budget_bytes`: the
  * resident graphs plus each worker's buffers for the graphs it serves. */
 typedef struct gg_server gg_server;
 #define GG_QUERY_BFS 0

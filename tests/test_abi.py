@@ -64,6 +64,10 @@ def test_last_error_and_options_without_gpu():
     for bad in (-1, 1 << 32):
         with pytest.raises(gg.GGError):
             gg.set_option("relabel_hubs", bad)
+    assert gg.get_option("serve_bfs_workers") == 2
+    for bad in (-1, 32):
+        with pytest.raises(gg.GGError):
+            gg.set_option("serve_bfs_workers", bad)
 
 
 @pytest.mark.parametrize("bad", [-0.5, float("nan"), -float("inf")])

@@ -156,3 +156,24 @@ def test_serve_bad_requests(gg, resident):
             s.submit(gid, gg.QUERY_BFS, g.vertices)
     with pytest.raises(gg.GGError):
         gg.Server(1 << 30, workers=0)
+
+
+@pytest.mark.parametrize("bfs_workers", [0, 3])
+def test_serve_reserved_bfs_workers(gg, resident, bfs_workers):
+    # 0: every worker takes every kind; 3 of 4: SSSP and BC all go to the one
+    # general worker, BFS to any; results as the oracle's either way
+    q = _trace(resident, 60, 13)
+    with gg.options(serve_bfs_workers=bfs_workers), gg.Server(64 << 30, workers=4) as s:
+        ids = [s.add_graph(g) for g, _ in resident]
+        tickets = [s.submit(ids[gi], kind, root) for gi, kind, root in q]
+        res = [s.wait(t)[0] for t in tickets]
+        info = s.info()
+    assert info["completed"] == len(q)
+    for (gi, kind, root), m in zip(q, res):
+        o = resident[gi][1]
+        if kind == 0:
+            assert np.array_equal(m, o.bfs(root)[1]), (gi, root)
+        elif kind == 1:
+            assert np.array_equal(m.view(np.uint32), o.sssp(root)[0].view(np.uint32)), (gi, root)
+        else:
+            assert_rel(m, o.bc(root)[0])
