@@ -34,6 +34,13 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
 # on RMAT-24; PageRank, CC and batched BC prefer the full order, which their
 # legs keep).
 BFS_RELABEL_HUBS = 1 << 19
+# The BFS leg's engine options, from a sweep on this graph (RMAT-24, the 16
+# bench roots): push while the frontier's out-arcs are below |E|/800 or
+# below 1/16 of the unvisited vertices' in-arcs, small-degree push bin up to
+# 32 arcs (defaults 200 / 24 / 64: 509 -> 528 GTEPS).  Batched BFS and the
+# plain BC run faster with the defaults (1.94 vs 2.05 ms, 203 vs 214 ms), so
+# only this leg uses them.
+BFS_OPTIONS = {"switch_divisor": 800, "bfs_alpha": 16, "push_small_max": 32}
 
 
 def log(*a):
@@ -592,10 +599,12 @@ def bfs_graph(gg, args):
 
 
 def pmc_child(args):
-    """Workload of the counter passes: one step of the bench (the same graph
-    and roots), then one calibration copy of COPY_BYTES."""
+    """Workload of the counter passes: one step of the bench (the same graph,
+    roots and options), then one calibration copy of COPY_BYTES."""
     import geminigraph_amd as gg
     gg.init(0)
+    for k, v in BFS_OPTIONS.items():
+        gg.set_option(k, v)
     g = bfs_graph(gg, args)
     V = 1 << args.scale
     roots = pick_roots(g.out_degree(), args.roots, args.seed + 1)
@@ -671,6 +680,8 @@ def level_child(args):
     BFS's per-level records as JSON."""
     import geminigraph_amd as gg
     gg.init(0)
+    for k, v in BFS_OPTIONS.items():
+        gg.set_option(k, v)
     g = bfs_graph(gg, args)
     roots = pick_roots(g.out_degree(), args.roots, args.seed + 1)
     parent = gg.DeviceArray(1 << args.scale, np.uint32)
@@ -873,6 +884,8 @@ def main():
 
     t0 = time.time()
     reorder = not args.no_reorder
+    bfs_opts = gg.options(**BFS_OPTIONS)  # the BFS leg's options, restored before the extras
+    bfs_opts.__enter__()
     g = bfs_graph(gg, args)
     gg.synchronize()
     build_s = time.time() - t0
@@ -1001,6 +1014,8 @@ def main():
     roofline["frac_of_stream_copy"] = round(achieved / extras["stream_copy_GBs"], 4)
     src.free()
     dst.free()
+    switch_divisor = gg.get_option("switch_divisor")
+    bfs_opts.__exit__(None, None, None)
     if not args.no_extras:
         del g
         parent.free()
@@ -1059,7 +1074,7 @@ def main():
                               "the parent tree)",
                    "reorder": reorder,
                    "relabel_hubs": args.relabel_hubs if reorder else None,
-                   "switch_divisor": gg.get_option("switch_divisor")},
+                   "switch_divisor": switch_divisor, "options": BFS_OPTIONS},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extras": extras,

@@ -184,7 +184,8 @@ def test_sssp_rmat26_relabelled_plan_fixed_point(gg):
 
 @pytest.mark.timeout(900)
 def test_bfs_rmat24_bench_roots(gg, rmat24):
-    # the bench's graph: hybrid relabeling (top 2^19 vertices in degree order)
+    # the bench's graph (hybrid relabeling: top 2^19 vertices in degree order)
+    # and the bench's BFS options
     e, o = rmat24
     V = 1 << 24
     with gg.options(relabel_hubs=1 << 19):
@@ -197,22 +198,23 @@ def test_bfs_rmat24_bench_roots(gg, rmat24):
     key.sort()
     parent = gg.DeviceArray(V, np.uint32)
     fr = np.zeros(256, dtype=np.uint32)
-    for r in roots:
-        say("bfs rmat24 root", r)
-        fr[:] = 0
-        st = g.bfs_into(r, parent, None, fr)
-        p = parent.numpy()
-        _, ref_depth, ref_fr = o.bfs(r)
-        assert np.array_equal(tree_depths(p, r), ref_depth), r
-        assert np.array_equal(fr[:len(ref_fr)], ref_fr) and not fr[len(ref_fr):].any()
-        assert st.iterations == len(ref_fr)
-        reached = ref_depth != 0xFFFFFFFF
-        assert st.m_reach == int(od[reached].sum())
-        vs = np.nonzero(reached)[0]
-        vs = vs[vs != r]
-        k = vs.astype(np.uint64) * np.uint64(V) + p[vs].astype(np.uint64)
-        pos = np.minimum(np.searchsorted(key, k), len(key) - 1)
-        assert (key[pos] == k).all(), "a parent is not an in-neighbour"
+    with gg.options(switch_divisor=800, bfs_alpha=16, push_small_max=32):
+        for r in roots:
+            say("bfs rmat24 root", r)
+            fr[:] = 0
+            st = g.bfs_into(r, parent, None, fr)
+            p = parent.numpy()
+            _, ref_depth, ref_fr = o.bfs(r)
+            assert np.array_equal(tree_depths(p, r), ref_depth), r
+            assert np.array_equal(fr[:len(ref_fr)], ref_fr) and not fr[len(ref_fr):].any()
+            assert st.iterations == len(ref_fr)
+            reached = ref_depth != 0xFFFFFFFF
+            assert st.m_reach == int(od[reached].sum())
+            vs = np.nonzero(reached)[0]
+            vs = vs[vs != r]
+            k = vs.astype(np.uint64) * np.uint64(V) + p[vs].astype(np.uint64)
+            pos = np.minimum(np.searchsorted(key, k), len(key) - 1)
+            assert (key[pos] == k).all(), "a parent is not an in-neighbour"
     parent.free()
     g.close()
 
