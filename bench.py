@@ -705,19 +705,19 @@ def _csv_rows(out, suffix):
 
 def _last_bfs(rows, key):
     """Dispatches of the last BFS (after its seed kernel) grouped per level:
-    [{kernel: row}] with the level advanced at each decide."""
+    [{kernel: row}], a level starting at each decide_bin launch."""
     rows = sorted(rows, key=key)
     name = lambda r: r["Kernel_Name"].split("(")[0].replace("void ", "")
     seeds = [i for i, r in enumerate(rows) if name(r) == "gg::bfs_seed_kernel"]
-    levels, cur = [], {}
+    levels = []
     for r in rows[seeds[-1] + 1:]:
         k = name(r).replace("gg::bfs_", "").replace("_kernel", "")
         if k in ("init", "seed"):
             break
-        cur[k] = r
-        if k == "decide":
-            levels.append(cur)
-            cur = {}
+        if k == "decide_bin":
+            levels.append({})
+        if levels:
+            levels[-1][k] = r
     return levels
 
 
@@ -753,9 +753,9 @@ def level_table(args):
                 d = merged.setdefault(int(x["Dispatch_Id"]), dict(x, counters={}))
                 d["counters"][x["Counter_Name"]] = float(x["Counter_Value"])
             per[tag] = _last_bfs(list(merged.values()), key=lambda x: int(x["Dispatch_Id"]))
-    print("| level | mode | frontier | new | examined | bin us | pull us | push/heavy us | "
-          "decide us | read MB (128B/64B/32B req) | write req (64B) | TCC req | hit/miss |")
-    print("|---|---|---|---|---|---|---|---|---|---|---|---|---|")
+    print("| level | mode | frontier | new | examined | decide+bin us | pull us | push/heavy us | "
+          "read MB (128B/64B/32B req) | write req (64B) | TCC req | hit/miss |")
+    print("|---|---|---|---|---|---|---|---|---|---|---|---|")
     # (the trace also holds the iterations queued past the end, which return
     # at once: only the levels the BFS ran are listed)
     for i, lv in enumerate(per["trace"][:len(stats["modes"])]):
@@ -770,8 +770,8 @@ def level_table(args):
                           c.get("TCC_EA0_RDREQ_32B_sum", 0))
         read_mb = (128 * r128 + 64 * r64 + 32 * r32) / 1e6
         print(f"| {i} | {'push' if stats['modes'][i] == 0 else 'pull'} | {stats['it_active'][i]} | "
-              f"{stats['it_new'][i]} | {stats['it_examined'][i]} | {us.get('bin', 0):.1f} | "
-              f"{us.get('pull', 0):.1f} | {us.get('push_heavy', 0):.1f} | {us.get('decide', 0):.1f} | "
+              f"{stats['it_new'][i]} | {stats['it_examined'][i]} | {us.get('decide_bin', 0):.1f} | "
+              f"{us.get('pull', 0):.1f} | {us.get('push_heavy', 0):.1f} | "
               f"{read_mb:.1f} ({int(r128)}/{int(r64)}/{int(r32)}) | "
               f"{int(c.get('TCC_EA0_WRREQ_sum', 0))} ({int(c.get('TCC_EA0_WRREQ_64B_sum', 0))}) | "
               f"{int(c.get('TCC_REQ_sum', 0))} | {int(c.get('TCC_HIT_sum', 0))}/"
@@ -919,7 +919,7 @@ def main():
     gg.prof_enable(True)
     step()
     gg.prof_enable(False)
-    kernels = {k: gg.prof_read(k) for k in ("bfs_pull", "bfs_push_heavy", "compact")}
+    kernels = {k: gg.prof_read(k) for k in ("bfs_decide_bin", "bfs_pull", "bfs_push_heavy")}
     log(f"BFS x{len(roots)}: {ms_per_step:.2f} ms/step, {gteps:.2f} GTEPS; kernels {kernels}")
 
     # one instrumented pass (edge counters on) for the algorithmic byte model
@@ -935,9 +935,11 @@ def main():
             pull_b += lb
             if len(gpu_depths) < args.parity_roots:
                 gpu_depths.append(depths_from_parents(parent.numpy(), r))
-    # Both step kernels are launched once per iteration: bfs_pull_kernel works
-    # in pull iterations and returns at once in push ones; bfs_push_heavy_kernel
-    # runs the push, or the heavy in-list chunks of a pull.  Per launch = per
+    # Three launches per iteration: bfs_decide_bin_kernel (the push/pull
+    # decision, and the frontier binning of a push), bfs_pull_kernel (the
+    # bottom-up step; returns at once in a push) and bfs_push_heavy_kernel
+    # (the push, or the heavy in-list chunks of a pull).  The roofline covers
+    # the two edge-processing launches, pull and push/heavy; per launch = per
     # iteration (per bfs_pull_kernel dispatch, as rocprof counts); the bytes
     # are the model's for every iteration, push and pull.
     step_ms = kernels["bfs_pull"][0] + kernels["bfs_push_heavy"][0]

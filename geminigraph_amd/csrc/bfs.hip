@@ -1,21 +1,26 @@
 // BFS (toolkits/bfs.cpp) with its iteration loop driven from the device.
 //
-// Every iteration is the same fixed sequence of four launches, issued ahead
-// of time on one stream:
-//   decide     -- one block: reduces the size and out-degree sum of the
-//                 frontier (counted by the kernels that produced it),
-//                 applies the push/pull switch (core/graph.hpp:1503-1509),
-//                 records the iteration's statistics, or marks the
-//                 traversal done;
-//   bin        -- push only: frontier bitmap -> three degree-binned queues,
-//                 and clears the output bitmap;
-//   pull       -- pull only: bottom-up over 64-vertex groups (bfs.cpp:60-76);
+// Every iteration is the same fixed pair of launches, issued ahead of time
+// on one stream:
+//   step       -- every block first makes the iteration's decision from the
+//                 counts of the frontier just produced (its size, out-arcs
+//                 and in-arcs, summed over spread slots by the kernels that
+//                 produced it): the push/pull switch
+//                 (core/graph.hpp:1503-1509), or the end of the traversal.
+//                 All blocks reach the same decision from the same counts;
+//                 block 0 also records it (statistics, control block, the
+//                 flag the host polls).  Then, by mode:
+//                   push: frontier bitmap -> three degree-binned queues, the
+//                         frontier joins `visited`, the output bitmap is
+//                         cleared;
+//                   pull: bottom-up over 64-vertex groups (bfs.cpp:60-76);
 //   push_heavy -- push: a persistent grid over the queues (bfs.cpp:49-58);
 //                 pull: the chunks of the heavy in-lists.
-// Each launch runs the step of the mode the decide chose or returns at once,
-// so the host never waits for an iteration before issuing the next one: it
+// The host never waits for an iteration before issuing the next one: it
 // stays one iteration ahead and polls the decision the device writes to
-// coherent host memory.
+// coherent host memory.  Per-iteration buffers that one launch reads while
+// the next fills them (count slots, examined counters, queue counters,
+// Beamer's bound) alternate between two copies by iteration parity.
 //
 // Claims: a push edge, and each chunk of a heavy in-list, claims its vertex
 // with a compare-and-swap on the vertex's parent entry, as the reference
@@ -43,16 +48,26 @@ constexpr uint32_t kHubBatch = 512;  // huge frontier vertices staged per push r
 constexpr uint32_t kPushChunk = 256;  // arcs of a huge frontier vertex per wave step
 static_assert(kHubBatch % kBlock == 0, "whole rows of the staged batch per thread");
 constexpr int kCountStride = 16;  // u64 per count slot: one 128-byte line
+// Count slots of one frontier: the step kernels' waves add their counts to
+// slot (wave % kBfsSlots); every block of the next decision reads all of
+// them, one per lane of its first wave.
+constexpr uint32_t kBfsSlots = 64;
+constexpr size_t kBfsSetWords = (size_t)kBfsSlots * kCountStride;
+// examined counters of one iteration: 64 slots, one 128-byte line each
+constexpr size_t kExamWords = 64 * 16;
 
 // Device control block of one traversal (done and iter first: the host
-// reads those 8 bytes after every decide).
+// reads those 8 bytes after every decision).
 struct BfsCtl {
   uint32_t done, iter;  // iter = iterations started
   uint32_t mode, level, push_iters, V;
   int32_t force;
   uint32_t lv_cap;
   uint64_t reached, m_reach, examined, threshold;
-  uint64_t m_unvisited;  // in-arcs of the vertices not visited yet (pull's scan bound)
+  // in-arcs of the vertices not visited yet (the pull's scan bound) before
+  // iteration n, in m_unv[n & 1]: every block of iteration n reads it while
+  // block 0 writes the next copy
+  uint64_t m_unv[2];
   uint64_t alpha;        // 0, or: pull only if frontier out-arcs * alpha > m_unvisited
   uint32_t modes[GG_STATS_RECORDS];
   uint64_t active[GG_STATS_RECORDS], fresh[GG_STATS_RECORDS], exam[GG_STATS_RECORDS],
@@ -70,11 +85,11 @@ struct BfsOut {
 };
 
 // Lane 0 of a wave adds its counts (vertices found, their out-arcs and their
-// in-arcs) to one of kScalarSlots spread slots.
+// in-arcs) to one of kBfsSlots spread slots.
 __device__ __forceinline__ void bfs_count(uint64_t *set, uint32_t slot, uint64_t n, uint64_t e,
                                           uint64_t ie) {
   if (!n) return;
-  uint64_t *p = set + (size_t)(slot % kScalarSlots) * kCountStride;
+  uint64_t *p = set + (size_t)(slot % kBfsSlots) * kCountStride;
   atomicAdd((unsigned long long *)p, (unsigned long long)n);
   atomicAdd((unsigned long long *)(p + 1), (unsigned long long)e);
   atomicAdd((unsigned long long *)(p + 2), (unsigned long long)ie);
@@ -123,111 +138,87 @@ struct BfsHostFlag {
   uint32_t done;
 };
 
-// Reduces the counts of the frontier just produced (kScalarSlots spread
-// slots) and decides the next iteration: the push/pull switch
-// (core/graph.hpp:1509), the iteration's records, or the end of the
-// traversal.  Called by all kBlock threads of one block.  Every load is
-// issued before any result is used (the slots by all threads, the control
-// block's scalars by thread 0), so the decide waits for one round of memory
-// latency instead of three dependent ones.
-__device__ void bfs_decide(BfsCtl *c, uint64_t *set, Counters *qc, unsigned long long *exam,
-                           uint32_t *lv, BfsHostFlag *hf) {
-  __shared__ uint64_t s_part[4][kWavesPerBlock];
-  const int t = threadIdx.x;  // kBlock == kScalarSlots: one slot per thread
-  const int lane = lane_id(), wv = t >> 6;
-  uint32_t done = 0, i = 0, lv_cap = 0, V = 0, push_iters = 0;
-  int32_t force = 0;
-  uint64_t examined = 0, reached = 0, m_reach = 0, m_unvisited = 0, threshold = 0, alpha = 0;
-  if (t == 0) {
-    done = c->done;
-    i = c->iter;
-    lv_cap = c->lv_cap;
-    V = c->V;
-    push_iters = c->push_iters;
-    force = c->force;
-    examined = c->examined;
-    reached = c->reached;
-    m_reach = c->m_reach;
-    m_unvisited = c->m_unvisited;
-    threshold = c->threshold;
-    alpha = c->alpha;
-  }
-  uint64_t *p = set + (size_t)t * kCountStride;
-  uint64_t n = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  uint64_t e = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  uint64_t in = __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  uint64_t x = 0;
-  if (exam && t < 64) x = __hip_atomic_load(&exam[t * 16], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  // a slot is only ever added to with n > 0, so an empty slot is already 0
-  // (after the end every step returns at once and the slots stay 0)
-  if (n) {
-    __hip_atomic_store(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(p + 1, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(p + 2, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  if (x) __hip_atomic_store(&exam[t * 16], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  n = wave_sum_u64(n);
-  e = wave_sum_u64(e);
-  in = wave_sum_u64(in);
-  x = wave_sum_u64(x);
-  if (lane == 0) {
-    s_part[0][wv] = n;
-    s_part[1][wv] = e;
-    s_part[2][wv] = in;
-    s_part[3][wv] = x;
+// The decision of iteration n, made by every block of the iteration's first
+// launch from the counts of the frontier iteration n - 1 produced
+// (kBfsSlots spread slots, read by the block's first wave): the push/pull
+// switch (core/graph.hpp:1509), or the end of the traversal.  Every block
+// reads the same counts and the same control block values, so all reach the
+// same mode; block 0 also records the iteration (statistics, level counts,
+// the running sums, the next copy of Beamer's bound) and publishes the
+// decision to the host.  The slots are cleared by the iteration's second
+// launch, once every block here has read them.  Returns the mode.
+__device__ uint32_t bfs_decide_block(BfsCtl *c, uint32_t n, const uint64_t *set,
+                                     const unsigned long long *exam, uint32_t *lv,
+                                     BfsHostFlag *hf) {
+  __shared__ uint32_t s_mode;
+  if (threadIdx.x < 64) {
+    const int lane = lane_id();
+    const uint32_t done = c->done;
+    const int32_t force = c->force;
+    const uint64_t threshold = c->threshold, alpha = c->alpha, m_unv = c->m_unv[n & 1];
+    uint64_t cnt = 0, edg = 0, ind = 0, x = 0;
+    if (!done) {
+      const uint64_t *p = set + (size_t)lane * kCountStride;  // kBfsSlots == 64
+      cnt = p[0];
+      edg = p[1];
+      ind = p[2];
+      if (exam) x = exam[lane * 16];
+    }
+    cnt = wave_sum_u64(cnt);
+    edg = wave_sum_u64(edg);
+    ind = wave_sum_u64(ind);
+    x = wave_sum_u64(x);
+    if (lane == 0) {
+      uint32_t mode = kBfsDone;
+      const uint64_t m_left = m_unv - (ind < m_unv ? ind : m_unv);
+      if (!done && cnt) {
+        // core/graph.hpp:1509: sparse iff active out-edges < |E| / divisor;
+        // with alpha, also sparse while the pull would have to scan more
+        // than alpha times the frontier's out-arcs (the unvisited
+        // vertices' in-arcs bound it)
+        const bool sparse = edg < threshold || (alpha && edg * alpha <= m_left);
+        mode = force == 1 ? kBfsPush : force == 2 ? kBfsPull : sparse ? kBfsPush : kBfsPull;
+      }
+      s_mode = mode;
+      if (blockIdx.x == 0 && !done) {
+        const uint32_t i = n;
+        if (i > 0 && i - 1 < GG_STATS_RECORDS) {
+          c->fresh[i - 1] = cnt;
+          c->exam[i - 1] = x;
+        }
+        c->examined += x;
+        if (!cnt) {
+          c->done = 1;
+          c->mode = kBfsDone;
+          __hip_atomic_store(&hf->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        } else {
+          const uint64_t reached = c->reached + cnt;
+          c->reached = reached;
+          c->m_reach += edg;
+          c->m_unv[(n + 1) & 1] = m_left;
+          if (i < c->lv_cap) lv[i] = (uint32_t)cnt;
+          if (i < GG_STATS_RECORDS) {
+            c->modes[i] = mode == kBfsPush ? 0u : 1u;
+            c->active[i] = cnt;
+            c->unvisited[i] = (uint64_t)c->V - reached;
+          }
+          c->mode = mode;
+          c->level = i + 1;
+          c->iter = i + 1;
+          if (mode == kBfsPush) c->push_iters += 1;
+          __hip_atomic_store(&hf->decided, i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+      }
+    }
   }
   __syncthreads();
-  if (t != 0 || done) return;
-  uint64_t total = 0, edges = 0, ind = 0, ex = 0;
-  for (int k = 0; k < kWavesPerBlock; k++) {
-    total += s_part[0][k];
-    edges += s_part[1][k];
-    ind += s_part[2][k];
-    ex += s_part[3][k];
-  }
-  if (i > 0 && i - 1 < GG_STATS_RECORDS) {
-    c->fresh[i - 1] = total;
-    c->exam[i - 1] = ex;
-  }
-  c->examined = examined + ex;
-  if (total == 0) {
-    c->done = 1;
-    c->mode = kBfsDone;
-    __hip_atomic_store(&hf->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    return;
-  }
-  reached += total;
-  c->reached = reached;
-  c->m_reach = m_reach + edges;
-  m_unvisited -= ind < m_unvisited ? ind : m_unvisited;
-  c->m_unvisited = m_unvisited;
-  if (i < lv_cap) lv[i] = (uint32_t)total;
-  // core/graph.hpp:1509: sparse iff active out-edges < |E| / divisor; with
-  // alpha, also sparse while the pull would have to scan more than alpha
-  // times the frontier's out-arcs (the unvisited vertices' in-arcs bound it)
-  const bool sparse = edges < threshold || (alpha && edges * alpha <= m_unvisited);
-  uint32_t mode = force == 1   ? kBfsPush
-                  : force == 2 ? kBfsPull
-                  : sparse     ? kBfsPush
-                               : kBfsPull;
-  if (i < GG_STATS_RECORDS) {
-    c->modes[i] = mode == kBfsPush ? 0u : 1u;
-    c->active[i] = total;
-    c->unvisited[i] = (uint64_t)V - reached;
-  }
-  c->mode = mode;
-  c->level = i + 1;
-  c->iter = i + 1;
-  if (mode == kBfsPush) c->push_iters = push_iters + 1;
-  qc->n[0] = qc->n[1] = qc->n[2] = 0;
-  qc->total = 0;
-  qc->edges = 0;
-  __hip_atomic_store(&hf->decided, i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  return s_mode;
 }
 
-// Output arrays to "unreached" (bfs.cpp:35-36), frontiers and count slots to
-// 0.  Vertices without in-arcs start in `visited`: no step can find them (the
-// root is seeded separately), so the bottom-up step skips their groups.
+// Output arrays to "unreached" (bfs.cpp:35-36), frontiers, count slots and
+// examined counters (both copies) to 0.  Vertices without in-arcs start in
+// `visited`: no step can find them (the root is seeded separately), so the
+// bottom-up step skips their groups.
 __global__ void bfs_init_kernel(uint32_t *parent, uint32_t *depth, uint32_t V, uint64_t *visited,
                                 const uint64_t *no_in, uint64_t *f0, uint64_t *f1, size_t words,
                                 uint64_t *sets, size_t set_words, unsigned long long *exam) {
@@ -244,16 +235,15 @@ __global__ void bfs_init_kernel(uint32_t *parent, uint32_t *depth, uint32_t V, u
   }
   for (size_t i = i0; i < set_words; i += s) sets[i] = 0;
   if (exam)
-    for (size_t i = i0; i < 1024; i += s) exam[i] = 0;
+    for (size_t i = i0; i < 2 * kExamWords; i += s) exam[i] = 0;
 }
 
-// One block: the root (bfs.cpp:37-39), the control block, and the decision
-// for the first iteration.
+// One block: the root (bfs.cpp:37-39), the control block, the root's counts
+// (the frontier that iteration 0 decides on) and both queue counters.
 __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
-    BfsCtl *c, uint64_t *set0, Counters *qc, unsigned long long *exam, uint32_t *lv,
-    BfsHostFlag *hf, uint64_t *visited, uint64_t *fin, uint32_t root, uint32_t root_orig,
-    uint32_t *parent, uint32_t *depth, const uint4 *vinfo, uint32_t V, int32_t force,
-    uint64_t threshold, uint64_t arcs, uint64_t alpha, uint32_t lv_cap) {
+    BfsCtl *c, uint64_t *set0, Counters *qc, uint64_t *visited, uint64_t *fin, uint32_t root,
+    uint32_t root_orig, uint32_t *parent, uint32_t *depth, const uint4 *vinfo, uint32_t V,
+    int32_t force, uint64_t threshold, uint64_t arcs, uint64_t alpha, uint32_t lv_cap) {
   if (threadIdx.x == 0) {
     c->done = 0;
     c->iter = 0;
@@ -267,7 +257,8 @@ __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
     c->m_reach = 0;
     c->examined = 0;
     c->threshold = threshold;
-    c->m_unvisited = arcs;
+    c->m_unv[0] = arcs;
+    c->m_unv[1] = arcs;
     c->alpha = alpha;
     const uint64_t bit = 1ULL << (root & 63);
     visited[root >> 6] |= bit;
@@ -277,17 +268,12 @@ __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
     set0[0] = 1;
     set0[1] = vinfo[root].y;
     set0[2] = vinfo[root].z;
-    __threadfence();
+    for (int k = 0; k < 2; k++) {
+      qc[k].n[0] = qc[k].n[1] = qc[k].n[2] = 0;
+      qc[k].total = 0;
+      qc[k].edges = 0;
+    }
   }
-  __syncthreads();
-  bfs_decide(c, set0, qc, exam, lv, hf);
-}
-
-__global__ __launch_bounds__(kBlock) void bfs_decide_kernel(BfsCtl *c, uint64_t *set,
-                                                            Counters *qc,
-                                                            unsigned long long *exam,
-                                                            uint32_t *lv, BfsHostFlag *hf) {
-  bfs_decide(c, set, qc, exam, lv, hf);
 }
 
 // Frontier binning of a push iteration: bitmap -> three degree-binned
@@ -300,6 +286,83 @@ struct BfsBinArgs {
   uint32_t *q0, *q1, *q2;
   Counters *qc;
 };
+
+// The binning as one part of the step launch, every block taking one word
+// per thread and grid-wide round.  Push frontiers are sparse (most words
+// hold no set bit, few more than one), so each thread walks its own word's
+// bits: their degrees decide the bins, kept as two masks for the second
+// walk that writes the queue entries.  A block adds its bin sizes to the
+// queue counters once per round; a round whose words are all empty skips
+// that exchange.  Light on registers (the bottom-up step shares the launch
+// and needs 8 waves per SIMD).
+__device__ __forceinline__ void bfs_bin_step(const BfsBinArgs &b, uint64_t *visited,
+                                             uint64_t *clear) {
+  __shared__ uint32_t s_tot[3][kWavesPerBlock];
+  __shared__ uint32_t s_base[3];
+  const int lane = lane_id(), wv = wave_in_block();
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  for (size_t base = (size_t)blockIdx.x * kBlock; base < b.words; base += stride) {
+    const size_t wi = base + threadIdx.x;
+    uint64_t word = 0;
+    if (wi < b.words) {
+      word = b.fin[wi];
+      clear[wi] = 0;
+      if (word) visited[wi] |= word;
+    }
+    if (!__syncthreads_or(word != 0)) continue;
+    // first walk: the bin of every set bit (small / mid masks; the rest huge)
+    uint64_t m_small = 0, m_mid = 0;
+    for (uint64_t w = word; w; w &= w - 1) {
+      const int k = __builtin_ctzll(w);
+      const uint32_t d = b.deg[wi * 64 + k];
+      if (d <= b.small_max)
+        m_small |= 1ULL << k;
+      else if (d <= b.big_max)
+        m_mid |= 1ULL << k;
+    }
+    uint32_t t0, t1, t2;
+    const uint32_t c0 = (uint32_t)__popcll(m_small), c1 = (uint32_t)__popcll(m_mid);
+    const uint32_t x0 = wave_excl_scan(c0, &t0);
+    const uint32_t x1 = wave_excl_scan(c1, &t1);
+    const uint32_t x2 = wave_excl_scan((uint32_t)__popcll(word) - c0 - c1, &t2);
+    if (lane == 0) {
+      s_tot[0][wv] = t0;
+      s_tot[1][wv] = t1;
+      s_tot[2][wv] = t2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t b0 = 0, b1 = 0, b2 = 0;
+      for (int i = 0; i < kWavesPerBlock; i++) {
+        b0 += s_tot[0][i];
+        b1 += s_tot[1][i];
+        b2 += s_tot[2][i];
+      }
+      s_base[0] = b0 ? atomicAdd(&b.qc->n[0], b0) : 0;
+      s_base[1] = b1 ? atomicAdd(&b.qc->n[1], b1) : 0;
+      s_base[2] = b2 ? atomicAdd(&b.qc->n[2], b2) : 0;
+    }
+    __syncthreads();
+    uint32_t o0 = s_base[0] + x0, o1 = s_base[1] + x1, o2 = s_base[2] + x2;
+    for (int i = 0; i < wv; i++) {
+      o0 += s_tot[0][i];
+      o1 += s_tot[1][i];
+      o2 += s_tot[2][i];
+    }
+    // second walk: the queue entries
+    for (uint64_t w = word; w; w &= w - 1) {
+      const int k = __builtin_ctzll(w);
+      const uint32_t v = (uint32_t)(wi * 64 + k);
+      if ((m_small >> k) & 1ULL)
+        b.q0[o0++] = v;
+      else if ((m_mid >> k) & 1ULL)
+        b.q1[o1++] = v;
+      else
+        b.q2[o2++] = v;
+    }
+    __syncthreads();  // s_tot / s_base are reused by the next round
+  }
+}
 
 struct BfsPushArgs {
   const uint64_t *index;
@@ -433,23 +496,34 @@ struct BfsPullArgs {
   const uint32_t *nbr;
   const uint32_t *first_orig;  // original id of each vertex's first in-neighbour
   const uint64_t *fin;
-  uint32_t V, light_max, serial_max, ngroups;
+  uint32_t V, light_max, ngroups;
 };
 
 // Bottom-up step (bfs.cpp:60-76) over 64-vertex groups, a persistent grid:
 // an unvisited vertex scans its in-neighbours until one is in the frontier.
-// Short lists are scanned by their own lane, longer ones by the whole wave
-// 64 entries at a time; lists above light_max are left to the chunk kernel.
+// After the first-in-neighbour test, the lists still open in a group (up to
+// light_max entries; longer ones are left to the chunk kernel) are scanned
+// as one sequence spread evenly over the wave's 64 lanes, 64 entries per
+// round: lane j takes entry j of the round, whatever list it belongs to, so
+// one long list no longer holds the wave while the other lanes idle (a lane
+// per list scanned its own list before, 4 entries per round: 3.8% slower
+// per BFS at RMAT-24).  A hit is handed to the list's owner lane through a
+// per-wave LDS slot; any hit is a valid parent, as any in-neighbour in the
+// frontier is in bfs.cpp:66-70.  Rounds stop once every open list has a hit.
 // The wave owns its group's words of visited and of the output frontier
 // (the chunk step runs after it, in the next launch, and only ORs bits in).
-// W waves take part, of any block size; gw is this wave's index.
-__device__ __forceinline__ void bfs_pull_step(const BfsCtl *c, const BfsOut &o,
+// W waves take part, blocks of kBlock threads; gw is this wave's index.
+__device__ __forceinline__ void bfs_pull_step(uint32_t level, const BfsOut &o,
                                               const BfsPullArgs &a, uint64_t *set_out,
                                               unsigned long long *exam, uint32_t W,
                                               uint32_t gw) {
-  const uint32_t level = c->level;
   const int lane = lane_id();
   uint64_t cnt = 0, edg = 0, ind = 0, examined = 0;
+  // a hit of the open lists' balanced scan, per owner lane
+  __shared__ uint32_t s_hit[kWavesPerBlock][64];
+  const int wv = (int)(threadIdx.x >> 6) % kWavesPerBlock;
+  s_hit[wv][lane] = kUnset;
+  wave_sync();
   // the next group's bitmap words are loaded while this group is processed
   uint32_t g = gw;
   uint64_t vis_n = 0, fin_n = 0;
@@ -490,48 +564,37 @@ __device__ __forceinline__ void bfs_pull_step(const BfsCtl *c, const BfsOut &o,
       examined += 1;
     }
     cand = cand && found == kUnset && deg > 1 && deg <= a.light_max;
-    uint64_t b = 0, e1 = 0;
+    uint64_t b = 0;
+    uint32_t len = 0;
     if (cand) {
-      b = a.index[v];
-      e1 = b + deg;
-      b += 1;
+      b = a.index[v] + 1;
+      len = deg - 1;
     }
-    const bool serial = cand && deg <= a.serial_max;
-    if (serial) {
-      // four neighbours and their frontier words in flight per step
-      for (uint64_t e = b; e < e1 && found == kUnset; e += 4) {
-        uint32_t u[4];
-        bool h[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) u[t] = e + t < e1 ? stream_load(&a.nbr[e + t]) : 0;
-#pragma unroll
-        for (int t = 0; t < 4; t++) h[t] = e + t < e1 && test_bit(a.fin, u[t]);
-#pragma unroll
-        for (int t = 3; t >= 0; t--)
-          if (h[t]) found = u[t];
-        uint64_t n = e1 - e < 4 ? e1 - e : 4;
-        examined += n;
+    uint32_t total;
+    const uint32_t excl = wave_excl_scan(len, &total);
+    for (uint32_t j0 = 0; j0 < total; j0 += 64) {
+      const uint32_t j = j0 + lane;
+      const bool ok = j < total;
+      const int k = wave_owner(excl, ok ? j : total - 1);
+      const uint64_t bk = shfl_u64(b, k);
+      const uint32_t exk = (uint32_t)__shfl((int)excl, k);
+      const bool open_k = (uint32_t)__shfl((int)found, k) == kUnset;
+      bool hit = false;
+      uint32_t u = 0;
+      if (ok && open_k) {
+        u = stream_load(&a.nbr[bk + (j - exk)]);
+        hit = test_bit(a.fin, u);
       }
-    }
-    uint64_t mmask = __ballot(cand && !serial);
-    while (mmask) {
-      const int k = __builtin_ctzll(mmask);
-      mmask &= mmask - 1;
-      const uint64_t bk = shfl_u64(b, k), ek = shfl_u64(e1, k);
-      uint32_t fk = kUnset;
-      for (uint64_t e0 = bk; e0 < ek; e0 += 64) {
-        const uint64_t e = e0 + lane;
-        const uint32_t u = e < ek ? stream_load(&a.nbr[e]) : 0;
-        const bool hit = e < ek && test_bit(a.fin, u);
-        const uint64_t hm = __ballot(hit);
-        if (lane == k)
-          examined += hm ? (uint64_t)(__builtin_ctzll(hm) + 1) : (ek - e0 < 64 ? ek - e0 : 64);
-        if (hm) {
-          fk = __shfl(u, __builtin_ctzll(hm));
-          break;
-        }
+      examined += (uint64_t)__popcll(__ballot(ok && open_k)) * (lane == 0);
+      if (hit) s_hit[wv][k] = u;
+      wave_sync();
+      const uint32_t h = s_hit[wv][lane];
+      if (h != kUnset) {
+        if (found == kUnset) found = h;
+        s_hit[wv][lane] = kUnset;
       }
-      if (lane == k) found = fk;
+      wave_sync();
+      if (!__ballot(cand && found == kUnset)) break;
     }
     const bool got = found != kUnset;
     const uint64_t m = __ballot(got);
@@ -613,34 +676,52 @@ __device__ __forceinline__ void bfs_heavy_step(const BfsCtl *c, const BfsOut &o,
   bfs_wave_flush(set_out, gw, cnt, edg, ind, exam, examined);
 }
 
-// Launches of an iteration besides the decide.  A launch of the mode not
-// chosen returns at once but still costs a kernel boundary, so the push and
-// the heavy chunks, which have the same block shape and register needs,
-// share one launch; the binning and the bottom-up step stay apart (the
-// binning needs 84 VGPRs, which would cut the step's occupancy from 8 waves
-// per SIMD to 5, measured 8% slower as one launch).
-//   bin        -- push: the frontier binning;
-//   pull       -- pull: the bottom-up step;
+// The three launches of an iteration.  A launch of the mode not chosen
+// returns at once but still costs a kernel boundary, so the decision shares
+// a launch with the binning, and the push with the heavy chunks (the same
+// block shape and register needs).  The bottom-up step keeps a launch of its
+// own: in one kernel with the binning it ran 30% slower per level, with the
+// same loop code, the same 8 waves per SIMD, LDS and barriers ruled out.
+//   decide_bin -- every block: the decision of iteration n; then, push: the
+//                 frontier binning over all blocks;
+//   pull       -- pull: the bottom-up step (every other mode returns);
 //   push_heavy -- push: the persistent push over the queues (push_blocks);
-//                 pull: the heavy in-list chunks (heavy_blocks).
-__global__ __launch_bounds__(kCompactBlock) void bfs_bin_kernel(const BfsCtl *c, BfsOut o,
-                                                               BfsBinArgs b) {
-  if (c->mode != kBfsPush) return;
-  compact_words_spread(b.fin, b.words, b.deg, b.small_max, b.big_max, b.q0, b.q1, b.q2, b.qc,
-                       o.visited, o.fout, gridDim.x);
+//                 pull: the heavy in-list chunks (heavy_blocks).  Block 0
+//                 first clears what the decision read or what the next
+//                 iteration's binning fills: the count slots and examined
+//                 counters of iteration n's decision, and the queue counters
+//                 of iteration n + 1.
+__global__ __launch_bounds__(kBlock) void bfs_decide_bin_kernel(
+    BfsCtl *c, uint32_t n, const uint64_t *set_in, const unsigned long long *exam_in,
+    uint32_t *lv, BfsHostFlag *hf, BfsOut o, BfsBinArgs b) {
+  if (bfs_decide_block(c, n, set_in, exam_in, lv, hf) == kBfsPush)
+    bfs_bin_step(b, o.visited, o.fout);
 }
 
-__global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, BfsOut o,
+__global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, uint32_t n, BfsOut o,
                                                           BfsPullArgs a, uint64_t *set_out,
                                                           unsigned long long *exam) {
-  if (c->mode != kBfsPull) return;
-  bfs_pull_step(c, o, a, set_out, exam, gridDim.x * kWavesPerBlock, global_wave());
+  if (c->done || c->mode != kBfsPull) return;
+  bfs_pull_step(n + 1, o, a, set_out, exam, gridDim.x * kWavesPerBlock, global_wave());
 }
 
 // Blocks past the mode's count return at once.
 __global__ __launch_bounds__(kBlock) void bfs_push_heavy_kernel(
     const BfsCtl *c, BfsOut o, BfsPushArgs p, BfsHeavyArgs h, uint64_t *set_out,
-    unsigned long long *exam, uint32_t push_blocks, uint32_t heavy_blocks) {
+    unsigned long long *exam, uint32_t push_blocks, uint32_t heavy_blocks, uint64_t *set_spent,
+    unsigned long long *exam_spent, Counters *qc_next) {
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    uint64_t *sp = set_spent + (size_t)threadIdx.x * kCountStride;  // kBfsSlots == 64
+    sp[0] = 0;
+    sp[1] = 0;
+    sp[2] = 0;
+    if (exam_spent) exam_spent[threadIdx.x * 16] = 0;
+    if (threadIdx.x == 0) {
+      qc_next->n[0] = qc_next->n[1] = qc_next->n[2] = 0;
+      qc_next->total = 0;
+      qc_next->edges = 0;
+    }
+  }
   const uint32_t mode = c->mode;
   if (mode == kBfsPush) {
     if (blockIdx.x < push_blocks)
@@ -653,8 +734,9 @@ __global__ __launch_bounds__(kBlock) void bfs_push_heavy_kernel(
 // Device state of the BFS loop, kept in the traversal context.
 struct BfsState {
   BfsCtl *ctl = nullptr;
-  uint64_t *sets = nullptr;   // 2 x kScalarSlots x kCountStride
-  Counters *qc = nullptr;
+  uint64_t *sets = nullptr;   // 2 x kBfsSetWords
+  Counters *qc = nullptr;     // 2 (by iteration parity)
+  unsigned long long *exam = nullptr;  // 2 x kExamWords
   uint32_t *lv = nullptr;
   uint32_t lv_cap = 0;
   BfsCtl *h_ctl = nullptr;      // pinned
@@ -667,6 +749,7 @@ void bfs_state_free(void *p) {
   device_free(s->ctl);
   device_free(s->sets);
   device_free(s->qc);
+  device_free(s->exam);
   device_free(s->lv);
   if (s->h_ctl) (void)hipHostFree(s->h_ctl);
   if (s->h_flag) (void)hipHostFree(s->h_flag);
@@ -679,8 +762,9 @@ static int bfs_state(Work *w, uint32_t lv_need, BfsState **out) {
     s = new BfsState;
     w->bfs = s;
     GG_TRY(device_alloc((void **)&s->ctl, sizeof(BfsCtl)));
-    GG_TRY(device_alloc((void **)&s->sets, 2 * kScalarSlots * kCountStride * 8));
-    GG_TRY(device_alloc((void **)&s->qc, sizeof(Counters)));
+    GG_TRY(device_alloc((void **)&s->sets, 2 * kBfsSetWords * 8));
+    GG_TRY(device_alloc((void **)&s->qc, 2 * sizeof(Counters)));
+    GG_TRY(device_alloc((void **)&s->exam, 2 * kExamWords * 8));
     GG_HIP(hipHostMalloc((void **)&s->h_ctl, sizeof(BfsCtl)));
     GG_HIP(hipHostMalloc((void **)&s->h_flag, sizeof(BfsHostFlag), hipHostMallocCoherent));
   }
@@ -699,12 +783,12 @@ static int bfs_state(Work *w, uint32_t lv_need, BfsState **out) {
 // traversal context, or the allocation-free variant's scratch array).
 struct BfsBufs {
   BfsCtl *ctl;
-  uint64_t *sets;  // 2 x kScalarSlots x kCountStride
-  Counters *qc;
+  uint64_t *sets;  // 2 x kBfsSetWords
+  Counters *qc;    // 2
   uint32_t *lv;
   uint32_t lv_cap;
-  BfsHostFlag *hf;  // where the decide publishes its progress
-  unsigned long long *exam;  // null unless count_examined
+  BfsHostFlag *hf;  // where the decision is published to the host
+  unsigned long long *exam;  // 2 x kExamWords; null unless count_examined
   uint64_t *visited, *f[2];
   uint32_t *q[3];
   size_t words;  // V / 64 + 1
@@ -728,6 +812,8 @@ static BfsPlan bfs_plan(const gg_graph *g, uint32_t root, uint32_t *parent, uint
   p.r_int = g->to_internal(root);
   p.parent = parent;
   p.depth = depth;
+  // one block per 1024 bitmap words (at most 1024 blocks): every block makes
+  // the decision, so fewer blocks read the count slots
   p.bin_blocks = compact_grid(words);
   // the bottom-up step walks the groups that hold a vertex with in-arcs
   // (the ones after them can never be found) and the root's: its frontier
@@ -749,48 +835,52 @@ static BfsPlan bfs_plan(const gg_graph *g, uint32_t root, uint32_t *parent, uint
 static void bfs_issue_start(const gg_graph *g, const BfsPlan &p, const BfsBufs &b, hipStream_t st) {
   const Dir &in = in_dir(g);
   const Options &opt = options();
-  const size_t set_words = (size_t)kScalarSlots * kCountStride;
   const uint64_t threshold = g->edges / (uint64_t)opt.switch_divisor;
   hipLaunchKernelGGL(bfs_init_kernel, dim3(grid1(p.V)), dim3(kBlock), 0, st, p.parent, p.depth,
-                     p.V, b.visited, in.empty, b.f[0], b.f[1], b.words, b.sets, 2 * set_words,
+                     p.V, b.visited, in.empty, b.f[0], b.f[1], b.words, b.sets, 2 * kBfsSetWords,
                      b.exam);
-  hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(kBlock), 0, st, b.ctl, b.sets, b.qc, b.exam,
-                     b.lv, b.hf, b.visited, b.f[0], p.r_int, p.root, p.parent, p.depth, in.vinfo,
-                     p.V, (int32_t)opt.force_mode, threshold, g->arcs, (uint64_t)opt.bfs_alpha,
+  hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(kBlock), 0, st, b.ctl, b.sets, b.qc,
+                     b.visited, b.f[0], p.r_int, p.root, p.parent, p.depth, in.vinfo, p.V,
+                     (int32_t)opt.force_mode, threshold, g->arcs, (uint64_t)opt.bfs_alpha,
                      b.lv_cap);
 }
 
-// The four launches of iteration n (bin, pull, push-or-heavy, decide).
+// The three launches of iteration n (decide_bin, pull, push_heavy).  Buffers by
+// parity: the frontier bitmaps, the count slots (iteration n decides on
+// sets[n & 1] and counts into sets[(n + 1) & 1]), the examined counters
+// likewise, and the queue counters (qc[n & 1] binned and pushed in
+// iteration n).
 static void bfs_issue_iteration(const gg_graph *g, const BfsPlan &p, const BfsBufs &b, uint32_t n,
                                 hipStream_t st, bool prof = true) {
   const Dir &out = out_dir(g), &in = in_dir(g);
   const Options &opt = options();
-  const size_t set_words = (size_t)kScalarSlots * kCountStride;
   uint64_t *fin = b.f[n & 1], *fout = b.f[(n + 1) & 1];
-  uint64_t *set_out = b.sets + (size_t)((n + 1) & 1) * set_words;
+  uint64_t *set_in = b.sets + (size_t)(n & 1) * kBfsSetWords;
+  uint64_t *set_out = b.sets + (size_t)((n + 1) & 1) * kBfsSetWords;
+  unsigned long long *exam_in = b.exam ? b.exam + (n & 1) * kExamWords : nullptr;
+  unsigned long long *exam_out = b.exam ? b.exam + ((n + 1) & 1) * kExamWords : nullptr;
+  Counters *qc = b.qc + (n & 1), *qc_next = b.qc + ((n + 1) & 1);
   BfsOut o{p.parent, p.depth, in.vinfo, b.visited, fout, p.V};
   {
-    ProfScope ps(prof ? "compact" : nullptr);
-    BfsBinArgs a{fin, b.words, out.degree, (uint32_t)opt.push_small_max,
-                 (uint32_t)opt.push_big_max, b.q[0], b.q[1], b.q[2], b.qc};
-    hipLaunchKernelGGL(bfs_bin_kernel, dim3(p.bin_blocks), dim3(kCompactBlock), 0, st, b.ctl, o, a);
+    ProfScope ps(prof ? "bfs_decide_bin" : nullptr);
+    BfsBinArgs ba{fin, b.words, out.degree, (uint32_t)opt.push_small_max,
+                  (uint32_t)opt.push_big_max, b.q[0], b.q[1], b.q[2], qc};
+    hipLaunchKernelGGL(bfs_decide_bin_kernel, dim3(p.bin_blocks), dim3(kBlock), 0, st, b.ctl, n,
+                       set_in, exam_in, b.lv, b.hf, o, ba);
   }
   {
     ProfScope ps(prof ? "bfs_pull" : nullptr);
-    BfsPullArgs a{in.index, in.nbr, in.first_orig, fin, p.V, in.light_max,
-                  (uint32_t)opt.pull_serial_max, p.ngroups};
-    hipLaunchKernelGGL(bfs_pull_kernel, dim3(p.pull_grid), dim3(kBlock), 0, st, b.ctl, o, a,
-                       set_out, b.exam);
+    BfsPullArgs pa{in.index, in.nbr, in.first_orig, fin, p.V, in.light_max, p.ngroups};
+    hipLaunchKernelGGL(bfs_pull_kernel, dim3(p.pull_grid), dim3(kBlock), 0, st, b.ctl, n, o, pa,
+                       set_out, exam_out);
   }
   {
     ProfScope ps(prof ? "bfs_push_heavy" : nullptr);
-    BfsPushArgs a{out.index, out.nbr, b.q[0], b.q[1], b.q[2], b.qc};
+    BfsPushArgs a{out.index, out.nbr, b.q[0], b.q[1], b.q[2], qc};
     BfsHeavyArgs h{in.nbr, in.chunk_v, in.chunk_beg, in.chunk_end, fin, in.n_chunks};
     hipLaunchKernelGGL(bfs_push_heavy_kernel, dim3(p.grid_b), dim3(kBlock), 0, st, b.ctl, o, a, h,
-                       set_out, b.exam, p.push_blocks, p.heavy_blocks);
+                       set_out, exam_out, p.push_blocks, p.heavy_blocks, set_in, exam_in, qc_next);
   }
-  hipLaunchKernelGGL(bfs_decide_kernel, dim3(1), dim3(kBlock), 0, st, b.ctl, set_out, b.qc, b.exam,
-                     b.lv, b.hf);
 }
 
 // Statistics from the control block read back after the traversal.
@@ -827,10 +917,10 @@ constexpr size_t na_align(size_t x) { return (x + 255) & ~(size_t)255; }
 // byte offsets of the buffers in the array
 constexpr size_t kNaCtl = 0;
 constexpr size_t kNaSets = na_align(kNaCtl + sizeof(BfsCtl));
-constexpr size_t kNaQc = na_align(kNaSets + 2 * kScalarSlots * kCountStride * 8);
-constexpr size_t kNaFlag = na_align(kNaQc + sizeof(Counters));
+constexpr size_t kNaQc = na_align(kNaSets + 2 * kBfsSetWords * 8);
+constexpr size_t kNaFlag = na_align(kNaQc + 2 * sizeof(Counters));
 constexpr size_t kNaExam = na_align(kNaFlag + sizeof(BfsHostFlag));
-constexpr size_t kNaLv = na_align(kNaExam + 1024 * 8);
+constexpr size_t kNaLv = na_align(kNaExam + 2 * kExamWords * 8);
 constexpr size_t kNaVisited = na_align(kNaLv + (size_t)kNoAllocMaxLevels * 4);
 constexpr size_t kNaF0 = na_align(kNaVisited + kNoAllocWords * 8);
 constexpr size_t kNaF1 = na_align(kNaF0 + kNoAllocWords * 8);
@@ -907,7 +997,7 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
   GG_TRY(bfs_state(w, lv_need, &s));
   const Options &opt = options();
   BfsBufs b{s->ctl, s->sets, s->qc, s->lv, s->lv_cap, s->h_flag,
-            opt.count_examined ? w->exam : nullptr, w->visited, {w->f[0].bits, w->f[1].bits},
+            opt.count_examined ? s->exam : nullptr, w->visited, {w->f[0].bits, w->f[1].bits},
             {w->f[0].q[0], w->f[0].q[1], w->f[0].q[2]}, w->words};
   const BfsPlan p = bfs_plan(g, root, d_parent, d_depth, b.words);
   hipStream_t st = stream();

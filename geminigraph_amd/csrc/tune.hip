@@ -88,7 +88,7 @@ extern "C" int gg_tune(gg_graph *g, int kind, int samples, int apply, gg_tune_re
     knobs = {{"switch_divisor", {20, 50, 100, 200, 400, 1000}},
              {"bfs_alpha", {0, 8, 16, 24, 32, 64}},
              {"push_big_max", {256, 1024, 4096}},
-             {"pull_serial_max", {16, 64, 128, 1024}},
+             {"push_small_max", {16, 32, 64}},
              {"bfs_pull_waves", {4096, 8192, 16384}}};
   } else {
     knobs = {{"pr_hot_sources", {0, 1 << 17, 1 << 18, 1 << 19}},

@@ -70,7 +70,8 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *   "push_small_max"   largest out-degree handled by in-wave load balancing
  *   "push_big_max"     largest out-degree handled by one wave per vertex
  *   "pull_light_max"   largest in-degree handled inside a 64-vertex group
- *   "pull_serial_max"  BFS pull: largest in-degree scanned by a single lane
+ *   "pull_serial_max"  batched BFS pull without a bin plan: largest in-degree
+ *                      scanned by a single lane
  *   "huge_grid_waves"  waves sharing the adjacency of the largest-degree bin
  *   "count_examined"   1 = count adjacency entries read (gg_run_stats)
  *   "bfs_alpha"        BFS: also push while the frontier's out-arcs times
@@ -375,7 +376,7 @@ int gg_graph_open(const char *path, gg_graph **out);
  * own work and keeps the fastest, one option at a time (no counterpart in
  * the reference, whose switch is fixed at |E| / 20, core/graph.hpp:1509).
  * GG_TUNE_BFS: `samples` BFS roots (spread over the ids, with out-arcs);
- * tunes switch_divisor, bfs_alpha, push_big_max, pull_serial_max,
+ * tunes switch_divisor, bfs_alpha, push_big_max, push_small_max,
  * bfs_pull_waves.  GG_TUNE_PAGERANK: `samples` iterations (the plan build
  * is not timed); tunes pr_hot_sources, pr_bin_bits, pr_range_bits.
  * Each candidate is timed twice (best of two) with device events.  apply != 0

@@ -10,7 +10,7 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 BFS_KNOBS = {"switch_divisor": {20, 50, 100, 200, 400, 1000}, "bfs_alpha": {0, 8, 16, 24, 32, 64},
-             "push_big_max": {256, 1024, 4096}, "pull_serial_max": {16, 64, 128, 1024},
+             "push_big_max": {256, 1024, 4096}, "push_small_max": {16, 32, 64},
              "bfs_pull_waves": {4096, 8192, 16384}}
 PR_KNOBS = {"pr_hot_sources": {0, 1 << 17, 1 << 18, 1 << 19}, "pr_bin_bits": {12, 13, 14},
             "pr_range_bits": {12, 13, 14}}
