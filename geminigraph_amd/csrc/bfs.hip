@@ -287,44 +287,83 @@ struct BfsBinArgs {
   Counters *qc;
 };
 
-// The binning as one part of the step launch, every block taking one word
-// per thread and grid-wide round.  Push frontiers are sparse (most words
-// hold no set bit, few more than one), so each thread walks its own word's
-// bits: their degrees decide the bins, kept as two masks for the second
-// walk that writes the queue entries.  A block adds its bin sizes to the
-// queue counters once per round; a round whose words are all empty skips
-// that exchange.  Light on registers (the bottom-up step shares the launch
-// and needs 8 waves per SIMD).
+// The binning in the decision's launch: each block takes one contiguous
+// range of bitmap words (whole multiples of kBlock), and adds its bin sizes to
+// the queue counters once, so a few hundred blocks, not one per word round,
+// meet on those three counters.  Push frontiers are sparse (most words hold
+// no set bit, few more than one), so each thread handles kBinWords words of
+// the range at a time: it loads them together, issues the degree loads of up
+// to two set bits per word together (more rounds only for denser words), and
+// keeps each word's bins as two masks, so the second walk, which writes the
+// queue entries, loads nothing.  A block whose range holds no set bit returns
+// after the first walk.  The launch has one wave per SIMD (the decision runs
+// in every block), so registers are not scarce here.
+constexpr int kBinWords = 4;
 __device__ __forceinline__ void bfs_bin_step(const BfsBinArgs &b, uint64_t *visited,
                                              uint64_t *clear) {
   __shared__ uint32_t s_tot[3][kWavesPerBlock];
   __shared__ uint32_t s_base[3];
   const int lane = lane_id(), wv = wave_in_block();
-  const size_t stride = (size_t)gridDim.x * kBlock;
-  for (size_t base = (size_t)blockIdx.x * kBlock; base < b.words; base += stride) {
-    const size_t wi = base + threadIdx.x;
-    uint64_t word = 0;
-    if (wi < b.words) {
-      word = b.fin[wi];
-      clear[wi] = 0;
-      if (word) visited[wi] |= word;
+  const size_t span = (size_t)gridDim.x * kBlock;
+  const size_t per = (b.words + span - 1) / span * kBlock;
+  const size_t beg = (size_t)blockIdx.x * per;
+  const size_t end = beg + per < b.words ? beg + per : b.words;
+  for (size_t base = beg; base < end; base += (size_t)kBinWords * kBlock) {  // block-uniform
+    uint64_t wd[kBinWords], ms[kBinWords], mm[kBinWords], rem[kBinWords];
+#pragma unroll
+    for (int t = 0; t < kBinWords; t++) {
+      const size_t wi = base + (size_t)t * kBlock + threadIdx.x;
+      wd[t] = wi < end ? b.fin[wi] : 0;
     }
-    if (!__syncthreads_or(word != 0)) continue;
-    // first walk: the bin of every set bit (small / mid masks; the rest huge)
-    uint64_t m_small = 0, m_mid = 0;
-    for (uint64_t w = word; w; w &= w - 1) {
-      const int k = __builtin_ctzll(w);
-      const uint32_t d = b.deg[wi * 64 + k];
-      if (d <= b.small_max)
-        m_small |= 1ULL << k;
-      else if (d <= b.big_max)
-        m_mid |= 1ULL << k;
+#pragma unroll
+    for (int t = 0; t < kBinWords; t++) {
+      const size_t wi = base + (size_t)t * kBlock + threadIdx.x;
+      if (wi < end) {
+        clear[wi] = 0;
+        if (wd[t]) visited[wi] |= wd[t];
+      }
+      ms[t] = mm[t] = 0;
+      rem[t] = wd[t];
     }
+    bool more = true;
+    while (more) {
+      uint32_t k0[kBinWords], k1[kBinWords], d0[kBinWords], d1[kBinWords];
+#pragma unroll
+      for (int t = 0; t < kBinWords; t++) {
+        const size_t wi = base + (size_t)t * kBlock + threadIdx.x;
+        k0[t] = rem[t] ? (uint32_t)__builtin_ctzll(rem[t]) : 64u;
+        rem[t] &= rem[t] - 1;
+        k1[t] = rem[t] ? (uint32_t)__builtin_ctzll(rem[t]) : 64u;
+        rem[t] &= rem[t] - 1;
+        d0[t] = k0[t] < 64 ? b.deg[wi * 64 + k0[t]] : 0;
+        d1[t] = k1[t] < 64 ? b.deg[wi * 64 + k1[t]] : 0;
+      }
+      more = false;
+#pragma unroll
+      for (int t = 0; t < kBinWords; t++) {
+        if (k0[t] < 64) {
+          if (d0[t] <= b.small_max) ms[t] |= 1ULL << k0[t];
+          else if (d0[t] <= b.big_max) mm[t] |= 1ULL << k0[t];
+        }
+        if (k1[t] < 64) {
+          if (d1[t] <= b.small_max) ms[t] |= 1ULL << k1[t];
+          else if (d1[t] <= b.big_max) mm[t] |= 1ULL << k1[t];
+        }
+        more = more || rem[t] != 0;
+      }
+    }
+    uint32_t c0 = 0, c1 = 0, c2 = 0;
+#pragma unroll
+    for (int t = 0; t < kBinWords; t++) {
+      c0 += (uint32_t)__popcll(ms[t]);
+      c1 += (uint32_t)__popcll(mm[t]);
+      c2 += (uint32_t)__popcll(wd[t] & ~ms[t] & ~mm[t]);
+    }
+    if (!__syncthreads_or((c0 | c1 | c2) != 0)) continue;
     uint32_t t0, t1, t2;
-    const uint32_t c0 = (uint32_t)__popcll(m_small), c1 = (uint32_t)__popcll(m_mid);
-    const uint32_t x0 = wave_excl_scan(c0, &t0);
-    const uint32_t x1 = wave_excl_scan(c1, &t1);
-    const uint32_t x2 = wave_excl_scan((uint32_t)__popcll(word) - c0 - c1, &t2);
+    uint32_t o0 = wave_excl_scan(c0, &t0);
+    uint32_t o1 = wave_excl_scan(c1, &t1);
+    uint32_t o2 = wave_excl_scan(c2, &t2);
     if (lane == 0) {
       s_tot[0][wv] = t0;
       s_tot[1][wv] = t1;
@@ -343,24 +382,29 @@ __device__ __forceinline__ void bfs_bin_step(const BfsBinArgs &b, uint64_t *visi
       s_base[2] = b2 ? atomicAdd(&b.qc->n[2], b2) : 0;
     }
     __syncthreads();
-    uint32_t o0 = s_base[0] + x0, o1 = s_base[1] + x1, o2 = s_base[2] + x2;
+    o0 += s_base[0];
+    o1 += s_base[1];
+    o2 += s_base[2];
     for (int i = 0; i < wv; i++) {
       o0 += s_tot[0][i];
       o1 += s_tot[1][i];
       o2 += s_tot[2][i];
     }
-    // second walk: the queue entries
-    for (uint64_t w = word; w; w &= w - 1) {
-      const int k = __builtin_ctzll(w);
-      const uint32_t v = (uint32_t)(wi * 64 + k);
-      if ((m_small >> k) & 1ULL)
-        b.q0[o0++] = v;
-      else if ((m_mid >> k) & 1ULL)
-        b.q1[o1++] = v;
-      else
-        b.q2[o2++] = v;
+#pragma unroll
+    for (int t = 0; t < kBinWords; t++) {
+      const size_t wi = base + (size_t)t * kBlock + threadIdx.x;
+      for (uint64_t w = wd[t]; w; w &= w - 1) {
+        const int k = __builtin_ctzll(w);
+        const uint32_t v = (uint32_t)(wi * 64 + k);
+        if ((ms[t] >> k) & 1ULL)
+          b.q0[o0++] = v;
+        else if ((mm[t] >> k) & 1ULL)
+          b.q1[o1++] = v;
+        else
+          b.q2[o2++] = v;
+      }
     }
-    __syncthreads();  // s_tot / s_base are reused by the next round
+    __syncthreads();  // s_tot / s_base are reused by the next chunk
   }
 }
 
@@ -501,114 +545,117 @@ struct BfsPullArgs {
 
 // Bottom-up step (bfs.cpp:60-76) over 64-vertex groups, a persistent grid:
 // an unvisited vertex scans its in-neighbours until one is in the frontier.
-// After the first-in-neighbour test, the lists still open in a group (up to
-// light_max entries; longer ones are left to the chunk kernel) are scanned
-// as one sequence spread evenly over the wave's 64 lanes, 64 entries per
-// round: lane j takes entry j of the round, whatever list it belongs to, so
-// one long list no longer holds the wave while the other lanes idle (a lane
-// per list scanned its own list before, 4 entries per round: 3.8% slower
-// per BFS at RMAT-24).  A hit is handed to the list's owner lane through a
-// per-wave LDS slot; any hit is a valid parent, as any in-neighbour in the
-// frontier is in bfs.cpp:66-70.  Rounds stop once every open list has a hit.
-// The wave owns its group's words of visited and of the output frontier
-// (the chunk step runs after it, in the next launch, and only ORs bits in).
+// A wave takes its groups 64 at a time (lane j loads the bitmap words of
+// group gw + j*W), lines up the unvisited vertices of all of them, and works
+// through that sequence in batches of 64, one vertex per lane: so a sparse
+// level (a few unvisited vertices per group) pays one dependent chain per 64
+// vertices instead of one per group, and a dense level one per group as
+// before.  In a batch, the first in-neighbour (the highest-degree one on a
+// relabelled graph) decides most vertices without touching the list; its
+// original id is read alongside, so the parent needs no further load.  The
+// lists still open (up to light_max entries; longer ones are left to the
+// chunk kernel, which runs after this one) are then scanned as one sequence
+// spread evenly over the lanes, 64 entries per round, a hit handed to the
+// list's owner lane through a per-wave LDS slot (any hit is a valid parent,
+// as any in-neighbour in the frontier is in bfs.cpp:66-70); rounds stop once
+// every open list has a hit.  Found vertices set their bit in their group's
+// LDS word; the wave then writes each group's words of the output frontier
+// and of visited (it owns them; the chunk step only ORs bits in later).
 // W waves take part, blocks of kBlock threads; gw is this wave's index.
 __device__ __forceinline__ void bfs_pull_step(uint32_t level, const BfsOut &o,
                                               const BfsPullArgs &a, uint64_t *set_out,
                                               unsigned long long *exam, uint32_t W,
                                               uint32_t gw) {
   const int lane = lane_id();
-  uint64_t cnt = 0, edg = 0, ind = 0, examined = 0;
-  // a hit of the open lists' balanced scan, per owner lane
-  __shared__ uint32_t s_hit[kWavesPerBlock][64];
   const int wv = (int)(threadIdx.x >> 6) % kWavesPerBlock;
+  __shared__ uint32_t s_hit[kWavesPerBlock][64];             // a hit per owner lane
+  __shared__ unsigned long long s_found[kWavesPerBlock][64];  // found bits per group
   s_hit[wv][lane] = kUnset;
+  s_found[wv][lane] = 0;
   wave_sync();
-  // the next group's bitmap words are loaded while this group is processed
-  uint32_t g = gw;
-  uint64_t vis_n = 0, fin_n = 0;
-  if (g < a.ngroups) {
-    vis_n = o.visited[g];
-    fin_n = a.fin[g];
-  }
-  for (; g < a.ngroups; g += W) {
-    const uint64_t visw = vis_n | fin_n;
-    if (g + W < a.ngroups) {
-      vis_n = o.visited[g + W];
-      fin_n = a.fin[g + W];
-    }
-    if (visw == ~0ULL) {
-      if (lane == 0) {
-        o.fout[g] = 0;
-        o.visited[g] = visw;
-      }
-      continue;
-    }
-    const uint32_t v = g * 64 + lane;
-    bool cand = v < a.V && !((visw >> lane) & 1ULL);
-    const uint4 vi = cand ? o.vinfo[v] : make_uint4(0, 0, 0, 0);
-    const uint32_t deg = vi.z;
-    cand = cand && deg > 0;
-    // the first in-neighbour (the highest-degree one on a relabelled graph)
-    // decides most vertices without touching the list; its original id is
-    // read alongside, so the parent needs no further load.  Heavy vertices
-    // (in-lists above light_max) get this test too; the heavy-chunk kernel,
-    // which runs after this one, scans the lists of those it leaves.
-    uint32_t found = kUnset;
-    bool head = false;  // found at the first in-neighbour
-    uint32_t head_orig = 0;
-    if (cand) {
-      head_orig = a.first_orig[v];
-      head = test_bit(a.fin, vi.w);
-      if (head) found = vi.w;
-      examined += 1;
-    }
-    cand = cand && found == kUnset && deg > 1 && deg <= a.light_max;
-    uint64_t b = 0;
-    uint32_t len = 0;
-    if (cand) {
-      b = a.index[v] + 1;
-      len = deg - 1;
+  uint64_t cnt = 0, edg = 0, ind = 0, examined = 0;
+  for (uint64_t r0 = gw; r0 < a.ngroups; r0 += (uint64_t)W * 64) {
+    const uint64_t gj = r0 + (uint64_t)lane * W;  // this lane's group
+    const bool gvalid = gj < a.ngroups;
+    uint64_t visw = ~0ULL, open = 0;
+    if (gvalid) {
+      visw = o.visited[gj] | a.fin[gj];
+      open = ~visw;  // vertices without in-arcs start visited
+      const uint64_t vb = gj * 64;
+      if (vb + 64 > a.V) open &= (1ULL << (a.V - vb)) - 1;  // 0 < V - vb < 64
     }
     uint32_t total;
-    const uint32_t excl = wave_excl_scan(len, &total);
-    for (uint32_t j0 = 0; j0 < total; j0 += 64) {
-      const uint32_t j = j0 + lane;
-      const bool ok = j < total;
-      const int k = wave_owner(excl, ok ? j : total - 1);
-      const uint64_t bk = shfl_u64(b, k);
-      const uint32_t exk = (uint32_t)__shfl((int)excl, k);
-      const bool open_k = (uint32_t)__shfl((int)found, k) == kUnset;
-      bool hit = false;
-      uint32_t u = 0;
-      if (ok && open_k) {
-        u = stream_load(&a.nbr[bk + (j - exk)]);
-        hit = test_bit(a.fin, u);
+    const uint32_t gexcl = wave_excl_scan((uint32_t)__popcll(open), &total);
+    for (uint32_t c0 = 0; c0 < total; c0 += 64) {
+      const uint32_t c = c0 + lane;
+      const bool ok = c < total;
+      const int gk = wave_owner(gexcl, ok ? c : total - 1);  // the group of vertex c
+      const uint64_t ok_open = shfl_u64(open, gk);
+      const uint32_t rk = c - (uint32_t)__shfl((int)gexcl, gk);
+      const uint32_t v = (uint32_t)((r0 + (uint64_t)gk * W) * 64 + (ok ? nth_set_bit(ok_open, rk) : 0));
+      bool cand = ok;
+      const uint4 vi = cand ? o.vinfo[v] : make_uint4(0, 0, 0, 0);
+      const uint32_t deg = vi.z;
+      cand = cand && deg > 0;
+      uint32_t found = kUnset;
+      bool head = false;  // found at the first in-neighbour
+      uint32_t head_orig = 0;
+      if (cand) {
+        head_orig = a.first_orig[v];
+        head = test_bit(a.fin, vi.w);
+        if (head) found = vi.w;
+        examined += 1;
       }
-      examined += (uint64_t)__popcll(__ballot(ok && open_k)) * (lane == 0);
-      if (hit) s_hit[wv][k] = u;
-      wave_sync();
-      const uint32_t h = s_hit[wv][lane];
-      if (h != kUnset) {
-        if (found == kUnset) found = h;
-        s_hit[wv][lane] = kUnset;
+      cand = cand && found == kUnset && deg > 1 && deg <= a.light_max;
+      uint64_t b = 0;
+      uint32_t len = 0;
+      if (cand) {
+        b = a.index[v] + 1;
+        len = deg - 1;
       }
-      wave_sync();
-      if (!__ballot(cand && found == kUnset)) break;
+      uint32_t ltotal;
+      const uint32_t lexcl = wave_excl_scan(len, &ltotal);
+      for (uint32_t j0 = 0; j0 < ltotal; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        const bool jok = j < ltotal;
+        const int k = wave_owner(lexcl, jok ? j : ltotal - 1);
+        const uint64_t bk = shfl_u64(b, k);
+        const uint32_t exk = (uint32_t)__shfl((int)lexcl, k);
+        const bool open_k = (uint32_t)__shfl((int)found, k) == kUnset;
+        bool hit = false;
+        uint32_t u = 0;
+        if (jok && open_k) {
+          u = stream_load(&a.nbr[bk + (j - exk)]);
+          hit = test_bit(a.fin, u);
+        }
+        examined += (uint64_t)__popcll(__ballot(jok && open_k)) * (lane == 0);
+        if (hit) s_hit[wv][k] = u;
+        wave_sync();
+        const uint32_t h = s_hit[wv][lane];
+        if (h != kUnset) {
+          if (found == kUnset) found = h;
+          s_hit[wv][lane] = kUnset;
+        }
+        wave_sync();
+        if (!__ballot(cand && found == kUnset)) break;
+      }
+      if (found != kUnset) {
+        o.parent[vi.x] = head ? head_orig : o.vinfo[found].x;
+        if (o.depth) o.depth[vi.x] = level;
+        edg += vi.y;
+        ind += deg;
+        atomicOr(&s_found[wv][gk], 1ULL << (v & 63));
+      }
     }
-    const bool got = found != kUnset;
-    const uint64_t m = __ballot(got);
-    if (got) {
-      o.parent[vi.x] = head ? head_orig : o.vinfo[found].x;
-      if (o.depth) o.depth[vi.x] = level;
-      edg += vi.y;
-      ind += deg;
-    }
-    if (lane == 0) {
-      o.fout[g] = m;
-      o.visited[g] = visw;  // this level joins visited; m joins next iteration
+    wave_sync();
+    if (gvalid) {
+      const uint64_t m = s_found[wv][lane];
+      o.fout[gj] = m;
+      o.visited[gj] = visw;  // this level joins visited; m joins next iteration
       cnt += (uint64_t)__popcll(m);
+      s_found[wv][lane] = 0;
     }
+    wave_sync();
   }
   bfs_wave_flush(set_out, gw, cnt, edg, ind, exam, examined);
 }
