@@ -35,12 +35,13 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
 # legs keep).
 BFS_RELABEL_HUBS = 1 << 19
 # The BFS leg's engine options, from a sweep on this graph (RMAT-24, the 16
-# bench roots): push while the frontier's out-arcs are below |E|/800 or
-# below 1/16 of the unvisited vertices' in-arcs, small-degree push bin up to
-# 32 arcs (defaults 200 / 24 / 64: 509 -> 528 GTEPS).  Batched BFS and the
-# plain BC run faster with the defaults (1.94 vs 2.05 ms, 203 vs 214 ms), so
-# only this leg uses them.
-BFS_OPTIONS = {"switch_divisor": 800, "bfs_alpha": 16, "push_small_max": 32}
+# bench roots): push while the frontier's out-arcs are below |E|/3200 or
+# below 1/32 of the unvisited vertices' in-arcs, small-degree push bin up to
+# 32 arcs.  Divisor 800-6400 with alpha 24-48 lie within 1% of each other;
+# alpha 16 (0.454 ms per BFS) and below push too long for the current
+# bottom-up step, alpha 64 pulls too early.  Batched BFS and the plain BC run
+# faster with the defaults (200 / 24 / 64), so only this leg uses them.
+BFS_OPTIONS = {"switch_divisor": 3200, "bfs_alpha": 32, "push_small_max": 32}
 
 
 def log(*a):

@@ -198,7 +198,7 @@ def test_bfs_rmat24_bench_roots(gg, rmat24):
     key.sort()
     parent = gg.DeviceArray(V, np.uint32)
     fr = np.zeros(256, dtype=np.uint32)
-    with gg.options(switch_divisor=800, bfs_alpha=16, push_small_max=32):
+    with gg.options(switch_divisor=3200, bfs_alpha=32, push_small_max=32):
         for r in roots:
             say("bfs rmat24 root", r)
             fr[:] = 0
