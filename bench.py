@@ -587,6 +587,12 @@ def cpu_baseline(scale, ef, seed, roots, gpu_depths, sample_roots, repeats):
 # the edge-processing launches of every BFS iteration: the bottom-up step, and
 # the push (push iterations) or the heavy in-list chunks (pull iterations)
 STEP_KERNELS = ("gg::bfs_pull_kernel", "gg::bfs_push_heavy_kernel")
+
+
+def kernel_name(row):
+    """A profiler row's kernel name without return type, template arguments
+    or parameters (gg::bfs_pull_kernel<false>(...) -> gg::bfs_pull_kernel)."""
+    return row["Kernel_Name"].split("(")[0].replace("void ", "").split("<")[0]
 COPY_KERNEL = "gg::gg_copy_kernel"
 COPY_BYTES = 256 << 20
 
@@ -651,7 +657,7 @@ def measure_traffic(args):
             return None, {"error": f"{counter} pass produced no counters (exit {r.returncode})"}
         pull, pull_n, copy = 0.0, 0, None
         for row in csv.DictReader(open(path)):
-            name = row["Kernel_Name"].split("(")[0].replace("void ", "")
+            name = kernel_name(row)
             v = float(row["Counter_Value"])
             if name in STEP_KERNELS:
                 pull += v
@@ -708,7 +714,7 @@ def _last_bfs(rows, key):
     """Dispatches of the last BFS (after its seed kernel) grouped per level:
     [{kernel: row}], a level starting at each decide_bin launch."""
     rows = sorted(rows, key=key)
-    name = lambda r: r["Kernel_Name"].split("(")[0].replace("void ", "")
+    name = kernel_name
     seeds = [i for i, r in enumerate(rows) if name(r) == "gg::bfs_seed_kernel"]
     levels = []
     for r in rows[seeds[-1] + 1:]:

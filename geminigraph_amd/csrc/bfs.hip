@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <chrono>
 #include <mutex>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -562,15 +563,23 @@ struct BfsPullArgs {
 // LDS word; the wave then writes each group's words of the output frontier
 // and of visited (it owns them; the chunk step only ORs bits in later).
 // W waves take part, blocks of kBlock threads; gw is this wave's index.
+// kCount: the launch counts adjacency entries for the roofline's byte model
+// (count_examined): a list's entries up to its first hit in list order, as a
+// serial scan stops (bfs.cpp:66-67), so each round hands the owner its first
+// hit (the lowest entry, an LDS atomic min) instead of any one.
+template <bool kCount>
 __device__ __forceinline__ void bfs_pull_step(uint32_t level, const BfsOut &o,
                                               const BfsPullArgs &a, uint64_t *set_out,
                                               unsigned long long *exam, uint32_t W,
                                               uint32_t gw) {
   const int lane = lane_id();
   const int wv = (int)(threadIdx.x >> 6) % kWavesPerBlock;
-  __shared__ uint32_t s_hit[kWavesPerBlock][64];             // a hit per owner lane
+  // a hit per owner lane: its vertex; counting: (entry << 32) | vertex
+  using Hit = typename std::conditional<kCount, unsigned long long, uint32_t>::type;
+  constexpr Hit kNoHit = (Hit)~(Hit)0;
+  __shared__ Hit s_hit[kWavesPerBlock][64];
   __shared__ unsigned long long s_found[kWavesPerBlock][64];  // found bits per group
-  s_hit[wv][lane] = kUnset;
+  s_hit[wv][lane] = kNoHit;
   s_found[wv][lane] = 0;
   wave_sync();
   uint64_t cnt = 0, edg = 0, ind = 0, examined = 0;
@@ -628,13 +637,21 @@ __device__ __forceinline__ void bfs_pull_step(uint32_t level, const BfsOut &o,
           u = stream_load(&a.nbr[bk + (j - exk)]);
           hit = test_bit(a.fin, u);
         }
-        examined += (uint64_t)__popcll(__ballot(jok && open_k)) * (lane == 0);
-        if (hit) s_hit[wv][k] = u;
-        wave_sync();
-        const uint32_t h = s_hit[wv][lane];
-        if (h != kUnset) {
-          if (found == kUnset) found = h;
-          s_hit[wv][lane] = kUnset;
+        if constexpr (kCount) {
+          if (hit) atomicMin(&s_hit[wv][k], ((unsigned long long)j << 32) | u);
+          wave_sync();
+          const uint32_t first_k = (uint32_t)(s_hit[wv][k] >> 32);
+          examined += (uint64_t)__popcll(__ballot(jok && open_k && first_k >= j)) * (lane == 0);
+          wave_sync();
+        } else {
+          examined += (uint64_t)__popcll(__ballot(jok && open_k)) * (lane == 0);
+          if (hit) s_hit[wv][k] = u;
+          wave_sync();
+        }
+        const Hit h = s_hit[wv][lane];
+        if (h != kNoHit) {
+          if (found == kUnset) found = (uint32_t)h;
+          s_hit[wv][lane] = kNoHit;
         }
         wave_sync();
         if (!__ballot(cand && found == kUnset)) break;
@@ -745,11 +762,12 @@ __global__ __launch_bounds__(kBlock) void bfs_decide_bin_kernel(
     bfs_bin_step(b, o.visited, o.fout);
 }
 
+template <bool kCount>
 __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, uint32_t n, BfsOut o,
                                                           BfsPullArgs a, uint64_t *set_out,
                                                           unsigned long long *exam) {
   if (c->done || c->mode != kBfsPull) return;
-  bfs_pull_step(n + 1, o, a, set_out, exam, gridDim.x * kWavesPerBlock, global_wave());
+  bfs_pull_step<kCount>(n + 1, o, a, set_out, exam, gridDim.x * kWavesPerBlock, global_wave());
 }
 
 // Blocks past the mode's count return at once.
@@ -918,8 +936,12 @@ static void bfs_issue_iteration(const gg_graph *g, const BfsPlan &p, const BfsBu
   {
     ProfScope ps(prof ? "bfs_pull" : nullptr);
     BfsPullArgs pa{in.index, in.nbr, in.first_orig, fin, p.V, in.light_max, p.ngroups};
-    hipLaunchKernelGGL(bfs_pull_kernel, dim3(p.pull_grid), dim3(kBlock), 0, st, b.ctl, n, o, pa,
-                       set_out, exam_out);
+    if (exam_out)
+      hipLaunchKernelGGL(bfs_pull_kernel<true>, dim3(p.pull_grid), dim3(kBlock), 0, st, b.ctl, n,
+                         o, pa, set_out, exam_out);
+    else
+      hipLaunchKernelGGL(bfs_pull_kernel<false>, dim3(p.pull_grid), dim3(kBlock), 0, st, b.ctl, n,
+                         o, pa, set_out, nullptr);
   }
   {
     ProfScope ps(prof ? "bfs_push_heavy" : nullptr);
