@@ -683,8 +683,9 @@ LEVEL_PASSES = (
 
 def level_child(args):
     """Workload of the per-level passes: the bench graph, one untimed BFS,
-    then BFS from the first bench root with the edge counters on; prints that
-    BFS's per-level records as JSON."""
+    BFS from the first bench root with the edge counters on (its per-level
+    records, printed as JSON), then the same BFS as the bench runs it (no
+    counting), the one the passes read (the last BFS of the run)."""
     import geminigraph_amd as gg
     gg.init(0)
     for k, v in BFS_OPTIONS.items():
@@ -695,6 +696,7 @@ def level_child(args):
     g.bfs_into(roots[1], parent)
     with gg.options(count_examined=1):
         st = g.bfs_into(roots[0], parent).as_dict()
+    g.bfs_into(roots[0], parent)
     gg.synchronize()
     n = st["iterations"]
     print(json.dumps({k: st[k][:n] for k in ("modes", "it_active", "it_new", "it_examined",
