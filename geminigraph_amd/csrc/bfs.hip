@@ -1,19 +1,18 @@
 // BFS (toolkits/bfs.cpp) with its iteration loop driven from the device.
 //
-// Every iteration is the same fixed pair of launches, issued ahead of time
-// on one stream:
-//   step       -- every block first makes the iteration's decision from the
+// Every iteration is the same fixed sequence of three launches, issued ahead
+// of time on one stream:
+//   decide_bin -- every block first makes the iteration's decision from the
 //                 counts of the frontier just produced (its size, out-arcs
 //                 and in-arcs, summed over spread slots by the kernels that
 //                 produced it): the push/pull switch
 //                 (core/graph.hpp:1503-1509), or the end of the traversal.
 //                 All blocks reach the same decision from the same counts;
 //                 block 0 also records it (statistics, control block, the
-//                 flag the host polls).  Then, by mode:
-//                   push: frontier bitmap -> three degree-binned queues, the
-//                         frontier joins `visited`, the output bitmap is
-//                         cleared;
-//                   pull: bottom-up over 64-vertex groups (bfs.cpp:60-76);
+//                 flag the host polls).  Then, in a push: frontier bitmap ->
+//                 three degree-binned queues, the frontier joins `visited`,
+//                 the output bitmap is cleared;
+//   pull       -- pull: bottom-up over 64-vertex groups (bfs.cpp:60-76);
 //   push_heavy -- push: a persistent grid over the queues (bfs.cpp:49-58);
 //                 pull: the chunks of the heavy in-lists.
 // The host never waits for an iteration before issuing the next one: it
@@ -1078,8 +1077,8 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
   // iterations queued ahead of the device: before issuing iteration n, wait
   // until iteration n - ahead has been decided (iteration n - ahead - 1 has
   // finished).  Short levels take less device time than the host needs to
-  // see a decision and issue four launches; each iteration queued past the
-  // end runs as four launches that return at once.
+  // see a decision and issue three launches; each iteration queued past the
+  // end runs as three launches that return at once.
   const uint32_t ahead = (uint32_t)std::max<int64_t>(1, opt.bfs_ahead);
   for (uint32_t n = 0;; n++) {
     if (n >= ahead) {
