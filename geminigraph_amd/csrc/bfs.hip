@@ -63,6 +63,10 @@ struct BfsCtl {
   uint32_t mode, level, push_iters, V;
   int32_t force;
   uint32_t lv_cap;
+  // levels done before the host's first iteration: 1 when the seed kernel
+  // pushed the root's out-arcs itself (host iteration n is level n + n0)
+  uint32_t n0;
+  uint32_t pad;
   uint64_t reached, m_reach, examined, threshold;
   // in-arcs of the vertices not visited yet (the pull's scan bound) before
   // iteration n, in m_unv[n & 1]: every block of iteration n reads it while
@@ -138,6 +142,16 @@ struct BfsHostFlag {
   uint32_t done;
 };
 
+// The push/pull switch (core/graph.hpp:1509): sparse iff the frontier's
+// out-arcs are below |E| / divisor; with alpha, also sparse while the pull
+// would have to scan more than alpha times the frontier's out-arcs (the
+// unvisited vertices' in-arcs, m_left, bound it).
+__device__ __forceinline__ uint32_t bfs_mode(int32_t force, uint64_t threshold, uint64_t alpha,
+                                             uint64_t edg, uint64_t m_left) {
+  const bool sparse = edg < threshold || (alpha && edg * alpha <= m_left);
+  return force == 1 ? kBfsPush : force == 2 ? kBfsPull : sparse ? kBfsPush : kBfsPull;
+}
+
 // The decision of iteration n, made by every block of the iteration's first
 // launch from the counts of the frontier iteration n - 1 produced
 // (kBfsSlots spread slots, read by the block's first wave): the push/pull
@@ -171,17 +185,10 @@ __device__ uint32_t bfs_decide_block(BfsCtl *c, uint32_t n, const uint64_t *set,
     if (lane == 0) {
       uint32_t mode = kBfsDone;
       const uint64_t m_left = m_unv - (ind < m_unv ? ind : m_unv);
-      if (!done && cnt) {
-        // core/graph.hpp:1509: sparse iff active out-edges < |E| / divisor;
-        // with alpha, also sparse while the pull would have to scan more
-        // than alpha times the frontier's out-arcs (the unvisited
-        // vertices' in-arcs bound it)
-        const bool sparse = edg < threshold || (alpha && edg * alpha <= m_left);
-        mode = force == 1 ? kBfsPush : force == 2 ? kBfsPull : sparse ? kBfsPush : kBfsPull;
-      }
+      if (!done && cnt) mode = bfs_mode(force, threshold, alpha, edg, m_left);
       s_mode = mode;
       if (blockIdx.x == 0 && !done) {
-        const uint32_t i = n;
+        const uint32_t i = n + c->n0;  // the level this iteration runs
         if (i > 0 && i - 1 < GG_STATS_RECORDS) {
           c->fresh[i - 1] = cnt;
           c->exam[i - 1] = x;
@@ -206,7 +213,7 @@ __device__ uint32_t bfs_decide_block(BfsCtl *c, uint32_t n, const uint64_t *set,
           c->level = i + 1;
           c->iter = i + 1;
           if (mode == kBfsPush) c->push_iters += 1;
-          __hip_atomic_store(&hf->decided, i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+          __hip_atomic_store(&hf->decided, n + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
       }
     }
@@ -238,12 +245,27 @@ __global__ void bfs_init_kernel(uint32_t *parent, uint32_t *depth, uint32_t V, u
     for (size_t i = i0; i < 2 * kExamWords; i += s) exam[i] = 0;
 }
 
-// One block: the root (bfs.cpp:37-39), the control block, the root's counts
-// (the frontier that iteration 0 decides on) and both queue counters.
+// One block: the root (bfs.cpp:37-39), the control block and both queue
+// counters.  When the first level is a push (the switch on the root's own
+// counts) of at most kSeedArcs out-arcs, the block runs it: it claims the
+// root's out-neighbours with a compare-and-swap on their parent entries, as
+// the push does (bfs.cpp:53), sets their bits in the frontier that the
+// host's iteration 0 reads, leaves their counts in iteration 0's slots, and
+// records level 0 as the decision would; the host's iteration n then runs
+// level n + 1 (n0 = 1), one iteration of three launches fewer.  Otherwise
+// the frontier is the root, with its counts, and n0 = 0.
+constexpr uint32_t kSeedArcs = 1024;
 __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
-    BfsCtl *c, uint64_t *set0, Counters *qc, uint64_t *visited, uint64_t *fin, uint32_t root,
+    BfsCtl *c, uint64_t *set0, Counters *qc, unsigned long long *exam0, uint32_t *lv,
+    uint64_t *visited, uint64_t *fin, const uint64_t *oindex, const uint32_t *onbr, uint32_t root,
     uint32_t root_orig, uint32_t *parent, uint32_t *depth, const uint4 *vinfo, uint32_t V,
     int32_t force, uint64_t threshold, uint64_t arcs, uint64_t alpha, uint32_t lv_cap) {
+  __shared__ uint64_t s_sum[3][kWavesPerBlock];
+  const uint4 rv = vinfo[root];
+  const uint64_t rb = oindex[root], re = oindex[root + 1];
+  const uint64_t m_left = arcs - (rv.z < arcs ? rv.z : arcs);
+  const bool seed_push =
+      bfs_mode(force, threshold, alpha, rv.y, m_left) == kBfsPush && re - rb <= kSeedArcs;
   if (threadIdx.x == 0) {
     c->done = 0;
     c->iter = 0;
@@ -253,6 +275,7 @@ __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
     c->V = V;
     c->force = force;
     c->lv_cap = lv_cap;
+    c->n0 = seed_push ? 1u : 0u;
     c->reached = 0;
     c->m_reach = 0;
     c->examined = 0;
@@ -262,17 +285,66 @@ __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
     c->alpha = alpha;
     const uint64_t bit = 1ULL << (root & 63);
     visited[root >> 6] |= bit;
-    fin[root >> 6] |= bit;
     parent[root_orig] = root_orig;
     if (depth) depth[root_orig] = 0;
-    set0[0] = 1;
-    set0[1] = vinfo[root].y;
-    set0[2] = vinfo[root].z;
+    if (!seed_push) {
+      fin[root >> 6] |= bit;
+      set0[0] = 1;
+      set0[1] = rv.y;
+      set0[2] = rv.z;
+    }
     for (int k = 0; k < 2; k++) {
       qc[k].n[0] = qc[k].n[1] = qc[k].n[2] = 0;
       qc[k].total = 0;
       qc[k].edges = 0;
     }
+  }
+  if (!seed_push) return;
+  __syncthreads();  // the root's parent entry is set before any claim (self-loops)
+  uint64_t cnt = 0, edg = 0, ind = 0;
+  for (uint64_t e = rb + threadIdx.x; e < re; e += kBlock) {
+    const uint32_t u = onbr[e];
+    const uint4 vi = vinfo[u];
+    if (atomicCAS(&parent[vi.x], V, root_orig) == V) {
+      if (depth) depth[vi.x] = 1;
+      atomicOr((unsigned long long *)&fin[u >> 6], 1ULL << (u & 63));
+      cnt++;
+      edg += vi.y;
+      ind += vi.z;
+    }
+  }
+  cnt = wave_sum_u64(cnt);
+  edg = wave_sum_u64(edg);
+  ind = wave_sum_u64(ind);
+  if (lane_id() == 0) {
+    s_sum[0][threadIdx.x >> 6] = cnt;
+    s_sum[1][threadIdx.x >> 6] = edg;
+    s_sum[2][threadIdx.x >> 6] = ind;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    cnt = edg = ind = 0;
+    for (int k = 0; k < kWavesPerBlock; k++) {
+      cnt += s_sum[0][k];
+      edg += s_sum[1][k];
+      ind += s_sum[2][k];
+    }
+    set0[0] = cnt;  // the frontier of level 1, for the host's iteration 0
+    set0[1] = edg;
+    set0[2] = ind;
+    if (exam0) exam0[0] = re - rb;
+    // level 0's records, as bfs_decide_block makes them for the root
+    c->reached = 1;
+    c->m_reach = rv.y;
+    c->m_unv[0] = m_left;
+    if (lv_cap) lv[0] = 1;
+    c->modes[0] = 0;
+    c->active[0] = 1;
+    c->unvisited[0] = (uint64_t)V - 1;
+    c->mode = kBfsPush;
+    c->level = 1;
+    c->iter = 1;
+    c->push_iters = 1;
   }
 }
 
@@ -766,7 +838,7 @@ __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, uint3
                                                           BfsPullArgs a, uint64_t *set_out,
                                                           unsigned long long *exam) {
   if (c->done || c->mode != kBfsPull) return;
-  bfs_pull_step<kCount>(n + 1, o, a, set_out, exam, gridDim.x * kWavesPerBlock, global_wave());
+  bfs_pull_step<kCount>(c->level, o, a, set_out, exam, gridDim.x * kWavesPerBlock, global_wave());
 }
 
 // Blocks past the mode's count return at once.
@@ -903,10 +975,11 @@ static void bfs_issue_start(const gg_graph *g, const BfsPlan &p, const BfsBufs &
   hipLaunchKernelGGL(bfs_init_kernel, dim3(grid1(p.V)), dim3(kBlock), 0, st, p.parent, p.depth,
                      p.V, b.visited, in.empty, b.f[0], b.f[1], b.words, b.sets, 2 * kBfsSetWords,
                      b.exam);
-  hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(kBlock), 0, st, b.ctl, b.sets, b.qc,
-                     b.visited, b.f[0], p.r_int, p.root, p.parent, p.depth, in.vinfo, p.V,
-                     (int32_t)opt.force_mode, threshold, g->arcs, (uint64_t)opt.bfs_alpha,
-                     b.lv_cap);
+  const Dir &out = out_dir(g);
+  hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(kBlock), 0, st, b.ctl, b.sets, b.qc, b.exam,
+                     b.lv, b.visited, b.f[0], out.index, out.nbr, p.r_int, p.root, p.parent,
+                     p.depth, in.vinfo, p.V, (int32_t)opt.force_mode, threshold, g->arcs,
+                     (uint64_t)opt.bfs_alpha, b.lv_cap);
 }
 
 // The three launches of iteration n (decide_bin, pull, push_heavy).  Buffers by
