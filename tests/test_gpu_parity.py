@@ -257,6 +257,47 @@ def test_bfs_edge_cases(gg):
         check_parents(e, V, root, parent, depth)
 
 
+@pytest.mark.parametrize("mode", ["push", "auto"])
+@pytest.mark.parametrize("with_depth", [True, False])
+def test_bfs_root_push_at_the_seed_limit(gg, with_depth, mode):
+    # the first level is pushed by the one-block seed kernel up to 1024 root
+    # out-arcs, by the grid above: roots with exactly 1024 and 1025 out-arcs
+    # (duplicates and a self-loop among them), each leaf with two arcs to the
+    # next level, and a chain behind that
+    pairs = [(0, 0), (0, 1)] + [(0, i) for i in range(1, 1023)]  # 1024 arcs, 1022 leaves
+    pairs += [(3000, i) for i in range(3001, 4026)]                # 1025 arcs
+    for base in (1, 3001):
+        for i in range(1000):
+            pairs += [(base + i, 6000 + (base + i) % 1500), (base + i, 7600 + i % 7)]
+    pairs += [(7600 + k, 7700 + k) for k in range(7)] + [(7700, 7701), (7701, 7702)]
+    e = oracle.edges_from_pairs(pairs)
+    V = 7800
+    for reorder in (False, True):
+        g = gg.Graph.build(e, V, reorder=reorder)
+        o = oracle.Graph(e, V)
+        for root in (0, 3000):
+            say("bfs seed limit", with_depth, mode, reorder, root)
+            _, ref_depth, ref_fr = o.bfs(root)
+            parent = gg.DeviceArray(V, np.uint32)
+            depth = gg.DeviceArray(V, np.uint32) if with_depth else None
+            fr = np.zeros(64, dtype=np.uint32)
+            with gg.options(force_mode=MODES[mode]):
+                st = g.bfs_into(root, parent, depth, fr).as_dict()
+            p = parent.numpy()
+            d = depth.numpy() if with_depth else depths_from_parents(p, root, V)
+            assert np.array_equal(d, ref_depth)
+            assert np.array_equal(fr[:len(ref_fr)], ref_fr) and not fr[len(ref_fr):].any()
+            assert st["iterations"] == len(ref_fr)
+            assert st["it_active"][:len(ref_fr)] == [int(x) for x in ref_fr]
+            check_parents(e, V, root, p, d)
+            reached = ref_depth != 0xFFFFFFFF
+            assert st["m_reach"] == int(o.out_degree()[reached].astype(np.int64).sum())
+            parent.free()
+            if depth is not None:
+                depth.free()
+        g.close()
+
+
 def test_bfs_empty_graph(gg):
     e = oracle.edges_from_pairs(np.zeros((0, 2), dtype=np.uint32))
     g = gg.Graph.build(e, 10)
