@@ -231,7 +231,14 @@ __global__ void bfs_init_kernel(uint32_t *parent, uint32_t *depth, uint32_t V, u
                                 uint64_t *sets, size_t set_words, unsigned long long *exam) {
   const size_t s = (size_t)gridDim.x * blockDim.x;
   const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (size_t i = i0; i < V; i += s) {
+  // 16-byte stores where the output arrays allow them (the parent array,
+  // 4V bytes, is most of what this kernel writes)
+  const size_t nq = ((uintptr_t)parent | (uintptr_t)depth) & 15 ? 0 : (size_t)V / 4;
+  for (size_t i = i0; i < nq; i += s) {
+    ((uint4 *)parent)[i] = make_uint4(V, V, V, V);
+    if (depth) ((uint4 *)depth)[i] = make_uint4(kUnset, kUnset, kUnset, kUnset);
+  }
+  for (size_t i = nq * 4 + i0; i < V; i += s) {
     parent[i] = V;
     if (depth) depth[i] = kUnset;
   }
@@ -972,7 +979,8 @@ static void bfs_issue_start(const gg_graph *g, const BfsPlan &p, const BfsBufs &
   const Dir &in = in_dir(g);
   const Options &opt = options();
   const uint64_t threshold = g->edges / (uint64_t)opt.switch_divisor;
-  hipLaunchKernelGGL(bfs_init_kernel, dim3(grid1(p.V)), dim3(kBlock), 0, st, p.parent, p.depth,
+  hipLaunchKernelGGL(bfs_init_kernel, dim3(grid1(std::max<uint64_t>((uint64_t)p.V / 4, b.words))),
+                     dim3(kBlock), 0, st, p.parent, p.depth,
                      p.V, b.visited, in.empty, b.f[0], b.f[1], b.words, b.sets, 2 * kBfsSetWords,
                      b.exam);
   const Dir &out = out_dir(g);
