@@ -120,16 +120,37 @@ __device__ __forceinline__ void bfs_wave_flush(uint64_t *set, uint32_t gw, uint6
 // claiming on those words instead would pile returning atomics onto a few
 // cache lines.  The winner writes the depth (if kept) and sets the frontier
 // bit (an atomic whose result is not needed).
-__device__ __forceinline__ bool bfs_claim(const BfsOut &o, uint32_t src_orig, uint32_t dst,
-                                          uint32_t level, uint4 &vi) {
-  const uint64_t bit = 1ULL << (dst & 63);
-  if ((o.visited[dst >> 6] | o.fout[dst >> 6]) & bit) return false;  // plain reads first
-  vi = o.vinfo[dst];
-  if (atomicCAS(&o.parent[vi.x], o.none, src_orig) != o.none) return false;
-  if (o.depth) o.depth[vi.x] = level;
-  __hip_atomic_fetch_or((unsigned long long *)&o.fout[dst >> 6], (unsigned long long)bit,
-                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
+// Each lane makes two claims at once (two 64-arc steps of a wave in flight):
+// the plain bitmap reads, the records and the compare-and-swaps of both are
+// issued before either result is used (a dependent chain each, so two chains
+// overlap their latencies).
+__device__ __forceinline__ void bfs_claim2(const BfsOut &o, uint32_t level, bool ok0,
+                                           uint32_t s0, uint32_t d0, bool ok1, uint32_t s1,
+                                           uint32_t d1, uint64_t &cnt, uint64_t &edg,
+                                           uint64_t &ind) {
+  const uint64_t t0 = ok0 ? (o.visited[d0 >> 6] | o.fout[d0 >> 6]) : ~0ULL;
+  const uint64_t t1 = ok1 ? (o.visited[d1 >> 6] | o.fout[d1 >> 6]) : ~0ULL;
+  const bool c0 = !((t0 >> (d0 & 63)) & 1ULL), c1 = !((t1 >> (d1 & 63)) & 1ULL);
+  const uint4 v0 = c0 ? o.vinfo[d0] : make_uint4(0, 0, 0, 0);
+  const uint4 v1 = c1 ? o.vinfo[d1] : make_uint4(0, 0, 0, 0);
+  const bool w0 = c0 && atomicCAS(&o.parent[v0.x], o.none, s0) == o.none;
+  const bool w1 = c1 && atomicCAS(&o.parent[v1.x], o.none, s1) == o.none;
+  if (w0) {
+    if (o.depth) o.depth[v0.x] = level;
+    __hip_atomic_fetch_or((unsigned long long *)&o.fout[d0 >> 6], 1ULL << (d0 & 63),
+                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    cnt++;
+    edg += v0.y;
+    ind += v0.z;
+  }
+  if (w1) {
+    if (o.depth) o.depth[v1.x] = level;
+    __hip_atomic_fetch_or((unsigned long long *)&o.fout[d1 >> 6], 1ULL << (d1 & 63),
+                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    cnt++;
+    edg += v1.y;
+    ind += v1.z;
+  }
 }
 
 // Progress the host polls, in coherent pinned memory: the deciding thread
@@ -507,14 +528,6 @@ __device__ __forceinline__ void bfs_push_step(const BfsCtl *c, const BfsOut &o,
   const uint32_t n0 = a.qc->n[0], n1 = a.qc->n[1], n2 = a.qc->n[2];
   const uint32_t waves0 = (n0 + 63) / 64;
   uint64_t cnt = 0, edg = 0, ind = 0, ex = 0;
-  auto visit = [&](uint32_t src_orig, uint32_t dst) {
-    uint4 vi;
-    if (bfs_claim(o, src_orig, dst, level, vi)) {
-      cnt++;
-      edg += vi.y;
-      ind += vi.z;
-    }
-  };
   for (uint32_t vw = gw; vw < waves0 + n1; vw += W) {
     if (vw < waves0) {
       const uint32_t i = vw * 64 + lane;
@@ -525,20 +538,28 @@ __device__ __forceinline__ void bfs_push_step(const BfsCtl *c, const BfsOut &o,
       const uint32_t d = valid ? (uint32_t)(a.index[v + 1] - b) : 0;
       uint32_t total;
       const uint32_t excl = wave_excl_scan(d, &total);
-      for (uint32_t j0 = 0; j0 < total; j0 += 64) {
-        const uint32_t j = j0 + lane;
-        const int k = wave_owner(excl, j);
-        const uint32_t src = __shfl(vo, k);
-        const uint64_t bb = shfl_u64(b, k);
-        const uint32_t exk = __shfl(excl, k);
-        if (j < total) visit(src, stream_load(&a.nbr[bb + (j - exk)]));
+      for (uint32_t j0 = 0; j0 < total; j0 += 128) {
+        const uint32_t ja = j0 + lane, jb = j0 + 64 + lane;
+        const int ka = wave_owner(excl, ja), kb = wave_owner(excl, jb);
+        const uint32_t sa = __shfl(vo, ka), sb = __shfl(vo, kb);
+        const uint64_t ba = shfl_u64(b, ka), bbb = shfl_u64(b, kb);
+        const uint32_t ea = __shfl(excl, ka), eb = __shfl(excl, kb);
+        const bool oka = ja < total, okb = jb < total;
+        const uint32_t da = oka ? stream_load(&a.nbr[ba + (ja - ea)]) : 0;
+        const uint32_t db = okb ? stream_load(&a.nbr[bbb + (jb - eb)]) : 0;
+        bfs_claim2(o, level, oka, sa, da, okb, sb, db, cnt, edg, ind);
       }
       if (lane == 0) ex += total;
     } else {
       const uint32_t v = a.q1[vw - waves0];
       const uint32_t vo = o.vinfo[v].x;
       const uint64_t b = a.index[v], e1 = a.index[v + 1];
-      for (uint64_t e = b + lane; e < e1; e += 64) visit(vo, stream_load(&a.nbr[e]));
+      for (uint64_t e = b + lane; e < e1; e += 128) {
+        const bool okb = e + 64 < e1;
+        const uint32_t da = stream_load(&a.nbr[e]);
+        const uint32_t db = okb ? stream_load(&a.nbr[e + 64]) : 0;
+        bfs_claim2(o, level, true, vo, da, okb, vo, db, cnt, edg, ind);
+      }
       if (lane == 0) ex += e1 - b;
     }
   }
@@ -606,7 +627,12 @@ __device__ __forceinline__ void bfs_push_step(const BfsCtl *c, const BfsOut &o,
       const uint64_t e1 = s_e[lo];
       const uint64_t cb = s_b[lo] + (j - s_pre[lo]) * kPushChunk;
       const uint64_t ce = cb + kPushChunk < e1 ? cb + kPushChunk : e1;
-      for (uint64_t e = cb + lane; e < ce; e += 64) visit(v, stream_load(&a.nbr[e]));
+      for (uint64_t e = cb + lane; e < ce; e += 128) {
+        const bool okb = e + 64 < ce;
+        const uint32_t da = stream_load(&a.nbr[e]);
+        const uint32_t db = okb ? stream_load(&a.nbr[e + 64]) : 0;
+        bfs_claim2(o, level, true, v, da, okb, v, db, cnt, edg, ind);
+      }
       if (lane == 0) ex += ce - cb;
     }
     coff += total;
