@@ -83,10 +83,30 @@ struct BfsCtl {
 // reference's output).
 struct BfsOut {
   uint32_t *parent, *depth;  // indexed by original ids
-  const uint4 *vinfo;        // by internal id: {original id, out-degree, in-degree, first in-nbr}
+  // by internal id: {original id, out-degree << 16 | in-degree (16 bits each,
+  // kDegEscape: read odeg / ideg), first in-neighbour, in-list start (low 32
+  // bits)}
+  const uint4 *vinfo;
   uint64_t *visited, *fout;
   uint32_t none;             // V
+  const uint32_t *odeg, *ideg;  // by internal id
 };
+
+// A record's degrees.  A capped field (a degree of 2^16 - 1 or more: a few
+// hubs) sends the read to the degree array, on a branch the wave takes only
+// when one of its lanes holds one.
+__device__ __forceinline__ uint32_t rec_odeg(uint32_t degs, const uint32_t *odeg, uint32_t v) {
+  uint32_t d = degs >> 16;
+  if (__builtin_expect(__ballot(d == kDegEscape) != 0, 0))
+    if (d == kDegEscape) d = odeg[v];
+  return d;
+}
+__device__ __forceinline__ uint32_t rec_ideg(uint32_t degs, const uint32_t *ideg, uint32_t v) {
+  uint32_t d = degs & kDegEscape;
+  if (__builtin_expect(__ballot(d == kDegEscape) != 0, 0))
+    if (d == kDegEscape) d = ideg[v];
+  return d;
+}
 
 // Lane 0 of a wave adds its counts (vertices found, their out-arcs and their
 // in-arcs) to one of kBfsSlots spread slots.
@@ -133,23 +153,27 @@ __device__ __forceinline__ void bfs_claim2(const BfsOut &o, uint32_t level, bool
   const bool c0 = !((t0 >> (d0 & 63)) & 1ULL), c1 = !((t1 >> (d1 & 63)) & 1ULL);
   const uint4 v0 = c0 ? o.vinfo[d0] : make_uint4(0, 0, 0, 0);
   const uint4 v1 = c1 ? o.vinfo[d1] : make_uint4(0, 0, 0, 0);
+  // the degrees before any write: a capped one's load then waits with the
+  // compare-and-swaps, not for this lane's stores
+  const uint32_t od0 = c0 ? rec_odeg(v0.y, o.odeg, d0) : 0, id0 = c0 ? rec_ideg(v0.y, o.ideg, d0) : 0;
+  const uint32_t od1 = c1 ? rec_odeg(v1.y, o.odeg, d1) : 0, id1 = c1 ? rec_ideg(v1.y, o.ideg, d1) : 0;
   const bool w0 = c0 && atomicCAS(&o.parent[v0.x], o.none, s0) == o.none;
   const bool w1 = c1 && atomicCAS(&o.parent[v1.x], o.none, s1) == o.none;
   if (w0) {
+    cnt++;
+    edg += od0;
+    ind += id0;
     if (o.depth) o.depth[v0.x] = level;
     __hip_atomic_fetch_or((unsigned long long *)&o.fout[d0 >> 6], 1ULL << (d0 & 63),
                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    cnt++;
-    edg += v0.y;
-    ind += v0.z;
   }
   if (w1) {
+    cnt++;
+    edg += od1;
+    ind += id1;
     if (o.depth) o.depth[v1.x] = level;
     __hip_atomic_fetch_or((unsigned long long *)&o.fout[d1 >> 6], 1ULL << (d1 & 63),
                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    cnt++;
-    edg += v1.y;
-    ind += v1.z;
   }
 }
 
@@ -286,14 +310,16 @@ constexpr uint32_t kSeedArcs = 1024;
 __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
     BfsCtl *c, uint64_t *set0, Counters *qc, unsigned long long *exam0, uint32_t *lv,
     uint64_t *visited, uint64_t *fin, const uint64_t *oindex, const uint32_t *onbr, uint32_t root,
-    uint32_t root_orig, uint32_t *parent, uint32_t *depth, const uint4 *vinfo, uint32_t V,
-    int32_t force, uint64_t threshold, uint64_t arcs, uint64_t alpha, uint32_t lv_cap) {
+    uint32_t root_orig, uint32_t *parent, uint32_t *depth, const uint4 *vinfo,
+    const uint32_t *odeg, const uint32_t *ideg, uint32_t V, int32_t force, uint64_t threshold,
+    uint64_t arcs, uint64_t alpha, uint32_t lv_cap) {
   __shared__ uint64_t s_sum[3][kWavesPerBlock];
   const uint4 rv = vinfo[root];
   const uint64_t rb = oindex[root], re = oindex[root + 1];
-  const uint64_t m_left = arcs - (rv.z < arcs ? rv.z : arcs);
+  const uint32_t r_out = rec_odeg(rv.y, odeg, root), r_in = rec_ideg(rv.y, ideg, root);
+  const uint64_t m_left = arcs - (r_in < arcs ? r_in : arcs);
   const bool seed_push =
-      bfs_mode(force, threshold, alpha, rv.y, m_left) == kBfsPush && re - rb <= kSeedArcs;
+      bfs_mode(force, threshold, alpha, r_out, m_left) == kBfsPush && re - rb <= kSeedArcs;
   if (threadIdx.x == 0) {
     c->done = 0;
     c->iter = 0;
@@ -318,8 +344,8 @@ __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
     if (!seed_push) {
       fin[root >> 6] |= bit;
       set0[0] = 1;
-      set0[1] = rv.y;
-      set0[2] = rv.z;
+      set0[1] = r_out;
+      set0[2] = r_in;
     }
     for (int k = 0; k < 2; k++) {
       qc[k].n[0] = qc[k].n[1] = qc[k].n[2] = 0;
@@ -337,8 +363,8 @@ __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
       if (depth) depth[vi.x] = 1;
       atomicOr((unsigned long long *)&fin[u >> 6], 1ULL << (u & 63));
       cnt++;
-      edg += vi.y;
-      ind += vi.z;
+      edg += rec_odeg(vi.y, odeg, u);
+      ind += rec_ideg(vi.y, ideg, u);
     }
   }
   cnt = wave_sum_u64(cnt);
@@ -363,7 +389,7 @@ __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
     if (exam0) exam0[0] = re - rb;
     // level 0's records, as bfs_decide_block makes them for the root
     c->reached = 1;
-    c->m_reach = rv.y;
+    c->m_reach = r_out;
     c->m_unv[0] = m_left;
     if (lv_cap) lv[0] = 1;
     c->modes[0] = 0;
@@ -646,6 +672,7 @@ struct BfsPullArgs {
   const uint32_t *first_orig;  // original id of each vertex's first in-neighbour
   const uint64_t *fin;
   uint32_t V, light_max, ngroups;
+  uint32_t start32;  // arcs < 2^32: the records hold the list starts
 };
 
 // Bottom-up step (bfs.cpp:60-76) over 64-vertex groups, a persistent grid:
@@ -671,7 +698,12 @@ struct BfsPullArgs {
 // (count_examined): a list's entries up to its first hit in list order, as a
 // serial scan stops (bfs.cpp:66-67), so each round hands the owner its first
 // hit (the lowest entry, an LDS atomic min) instead of any one.
-template <bool kCount>
+// kStart32 (arcs < 2^32): a list's start comes with the vertex's record, so
+// a scan needs no load of the offsets first.  The uncounted kStart32
+// instance fits in 80 scalar registers (no examined pointer, no offsets
+// array): built with 84, its bottom-up levels ran 14-25% slower at the same
+// 64 vector registers, as with fewer waves per SIMD.
+template <bool kCount, bool kStart32>
 __device__ __forceinline__ void bfs_pull_step(uint32_t level, const BfsOut &o,
                                               const BfsPullArgs &a, uint64_t *set_out,
                                               unsigned long long *exam, uint32_t W,
@@ -708,22 +740,22 @@ __device__ __forceinline__ void bfs_pull_step(uint32_t level, const BfsOut &o,
       const uint32_t v = (uint32_t)((r0 + (uint64_t)gk * W) * 64 + (ok ? nth_set_bit(ok_open, rk) : 0));
       bool cand = ok;
       const uint4 vi = cand ? o.vinfo[v] : make_uint4(0, 0, 0, 0);
-      const uint32_t deg = vi.z;
+      const uint32_t deg = cand ? rec_ideg(vi.y, o.ideg, v) : 0;
       cand = cand && deg > 0;
       uint32_t found = kUnset;
       bool head = false;  // found at the first in-neighbour
       uint32_t head_orig = 0;
       if (cand) {
         head_orig = a.first_orig[v];
-        head = test_bit(a.fin, vi.w);
-        if (head) found = vi.w;
+        head = test_bit(a.fin, vi.z);
+        if (head) found = vi.z;
         examined += 1;
       }
       cand = cand && found == kUnset && deg > 1 && deg <= a.light_max;
       uint64_t b = 0;
       uint32_t len = 0;
       if (cand) {
-        b = a.index[v] + 1;
+        b = (kStart32 ? (uint64_t)vi.w : a.index[v]) + 1;
         len = deg - 1;
       }
       uint32_t ltotal;
@@ -761,10 +793,12 @@ __device__ __forceinline__ void bfs_pull_step(uint32_t level, const BfsOut &o,
         if (!__ballot(cand && found == kUnset)) break;
       }
       if (found != kUnset) {
-        o.parent[vi.x] = head ? head_orig : o.vinfo[found].x;
-        if (o.depth) o.depth[vi.x] = level;
-        edg += vi.y;
+        // both loads (the parent's id, a capped degree) before the writes
+        const uint32_t pval = head ? head_orig : o.vinfo[found].x;
+        edg += rec_odeg(vi.y, o.odeg, v);
         ind += deg;
+        o.parent[vi.x] = pval;
+        if (o.depth) o.depth[vi.x] = level;
         atomicOr(&s_found[wv][gk], 1ULL << (v & 63));
       }
     }
@@ -778,7 +812,7 @@ __device__ __forceinline__ void bfs_pull_step(uint32_t level, const BfsOut &o,
     }
     wave_sync();
   }
-  bfs_wave_flush(set_out, gw, cnt, edg, ind, exam, examined);
+  bfs_wave_flush(set_out, gw, cnt, edg, ind, kCount ? exam : nullptr, examined);
 }
 
 struct BfsHeavyArgs {
@@ -817,6 +851,7 @@ __device__ __forceinline__ void bfs_heavy_step(const BfsCtl *c, const BfsOut &o,
       const uint32_t v = (uint32_t)__shfl((int)vl, j);
       const uint64_t bit = 1ULL << (v & 63);
       const uint4 vi = o.vinfo[v];
+      const uint32_t vod = rec_odeg(vi.y, o.odeg, v), vid = rec_ideg(vi.y, o.ideg, v);
       uint32_t *pv = &o.parent[vi.x];
       const uint64_t cb = a.chunk_beg[ch], ce = a.chunk_end[ch];
       for (uint64_t e0 = cb; e0 < ce; e0 += 64) {
@@ -832,8 +867,8 @@ __device__ __forceinline__ void bfs_heavy_step(const BfsCtl *c, const BfsOut &o,
             if (o.depth) o.depth[vi.x] = level;
             atomicOr((unsigned long long *)&o.fout[v >> 6], (unsigned long long)bit);
             cnt++;
-            edg += vi.y;
-            ind += vi.z;
+            edg += vod;
+            ind += vid;
           }
           break;
         }
@@ -866,19 +901,25 @@ __global__ __launch_bounds__(kBlock) void bfs_decide_bin_kernel(
     bfs_bin_step(b, o.visited, o.fout);
 }
 
-template <bool kCount>
+template <bool kCount, bool kStart32>
 __global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, uint32_t n, BfsOut o,
                                                           BfsPullArgs a, uint64_t *set_out,
                                                           unsigned long long *exam) {
   if (c->done || c->mode != kBfsPull) return;
-  bfs_pull_step<kCount>(c->level, o, a, set_out, exam, gridDim.x * kWavesPerBlock, global_wave());
+  bfs_pull_step<kCount, kStart32>(c->level, o, a, set_out, exam, gridDim.x * kWavesPerBlock, global_wave());
 }
 
-// Blocks past the mode's count return at once.
-__global__ __launch_bounds__(kBlock) void bfs_push_heavy_kernel(
-    const BfsCtl *c, BfsOut o, BfsPushArgs p, BfsHeavyArgs h, uint64_t *set_out,
-    unsigned long long *exam, uint32_t push_blocks, uint32_t heavy_blocks, uint64_t *set_spent,
-    unsigned long long *exam_spent, Counters *qc_next) {
+// Blocks past the mode's count return at once.  kCount: the examined
+// counters are kept (without, their pointers are null at compile time).  At
+// most 80 scalar registers (the rest spill to vector lanes, within the 64
+// vector registers of 8 waves per SIMD), as for the bottom-up kernel.
+template <bool kCount>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(80))) void
+bfs_push_heavy_kernel(const BfsCtl *c, BfsOut o, BfsPushArgs p, BfsHeavyArgs h, uint64_t *set_out,
+                      unsigned long long *exam_, uint32_t push_blocks, uint32_t heavy_blocks,
+                      uint64_t *set_spent, unsigned long long *exam_spent_, Counters *qc_next) {
+  unsigned long long *exam = kCount ? exam_ : nullptr;
+  unsigned long long *exam_spent = kCount ? exam_spent_ : nullptr;
   if (blockIdx.x == 0 && threadIdx.x < 64) {
     uint64_t *sp = set_spent + (size_t)threadIdx.x * kCountStride;  // kBfsSlots == 64
     sp[0] = 0;
@@ -1012,8 +1053,8 @@ static void bfs_issue_start(const gg_graph *g, const BfsPlan &p, const BfsBufs &
   const Dir &out = out_dir(g);
   hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(kBlock), 0, st, b.ctl, b.sets, b.qc, b.exam,
                      b.lv, b.visited, b.f[0], out.index, out.nbr, p.r_int, p.root, p.parent,
-                     p.depth, in.vinfo, p.V, (int32_t)opt.force_mode, threshold, g->arcs,
-                     (uint64_t)opt.bfs_alpha, b.lv_cap);
+                     p.depth, in.vinfo, out.degree, in.degree, p.V, (int32_t)opt.force_mode,
+                     threshold, g->arcs, (uint64_t)opt.bfs_alpha, b.lv_cap);
 }
 
 // The three launches of iteration n (decide_bin, pull, push_heavy).  Buffers by
@@ -1031,7 +1072,7 @@ static void bfs_issue_iteration(const gg_graph *g, const BfsPlan &p, const BfsBu
   unsigned long long *exam_in = b.exam ? b.exam + (n & 1) * kExamWords : nullptr;
   unsigned long long *exam_out = b.exam ? b.exam + ((n + 1) & 1) * kExamWords : nullptr;
   Counters *qc = b.qc + (n & 1), *qc_next = b.qc + ((n + 1) & 1);
-  BfsOut o{p.parent, p.depth, in.vinfo, b.visited, fout, p.V};
+  BfsOut o{p.parent, p.depth, in.vinfo, b.visited, fout, p.V, out.degree, in.degree};
   {
     ProfScope ps(prof ? "bfs_decide_bin" : nullptr);
     BfsBinArgs ba{fin, b.words, out.degree, (uint32_t)opt.push_small_max,
@@ -1041,20 +1082,20 @@ static void bfs_issue_iteration(const gg_graph *g, const BfsPlan &p, const BfsBu
   }
   {
     ProfScope ps(prof ? "bfs_pull" : nullptr);
-    BfsPullArgs pa{in.index, in.nbr, in.first_orig, fin, p.V, in.light_max, p.ngroups};
-    if (exam_out)
-      hipLaunchKernelGGL(bfs_pull_kernel<true>, dim3(p.pull_grid), dim3(kBlock), 0, st, b.ctl, n,
-                         o, pa, set_out, exam_out);
-    else
-      hipLaunchKernelGGL(bfs_pull_kernel<false>, dim3(p.pull_grid), dim3(kBlock), 0, st, b.ctl, n,
-                         o, pa, set_out, nullptr);
+    BfsPullArgs pa{in.index, in.nbr, in.first_orig, fin, p.V, in.light_max, p.ngroups,
+                   g->arcs < (1ULL << 32) && opt.bfs_record_starts ? 1u : 0u};
+    auto kern = exam_out ? (pa.start32 ? bfs_pull_kernel<true, true> : bfs_pull_kernel<true, false>)
+                         : (pa.start32 ? bfs_pull_kernel<false, true> : bfs_pull_kernel<false, false>);
+    hipLaunchKernelGGL(kern, dim3(p.pull_grid), dim3(kBlock), 0, st, b.ctl, n, o, pa, set_out,
+                       exam_out);
   }
   {
     ProfScope ps(prof ? "bfs_push_heavy" : nullptr);
     BfsPushArgs a{out.index, out.nbr, b.q[0], b.q[1], b.q[2], qc};
     BfsHeavyArgs h{in.nbr, in.chunk_v, in.chunk_beg, in.chunk_end, fin, in.n_chunks};
-    hipLaunchKernelGGL(bfs_push_heavy_kernel, dim3(p.grid_b), dim3(kBlock), 0, st, b.ctl, o, a, h,
-                       set_out, exam_out, p.push_blocks, p.heavy_blocks, set_in, exam_in, qc_next);
+    hipLaunchKernelGGL(exam_out ? bfs_push_heavy_kernel<true> : bfs_push_heavy_kernel<false>,
+                       dim3(p.grid_b), dim3(kBlock), 0, st, b.ctl, o, a, h, set_out, exam_out,
+                       p.push_blocks, p.heavy_blocks, set_in, exam_in, qc_next);
   }
 }
 

@@ -344,7 +344,9 @@ __global__ void vinfo_kernel(const uint64_t *index, const uint32_t *nbr, const u
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += s) {
     const uint64_t b = index[v], e = index[v + 1];
     const uint32_t f = b < e ? nbr[b] : kUnset;
-    vinfo[v] = make_uint4(iperm ? iperm[v] : v, other_deg[v], (uint32_t)(e - b), f);
+    const uint32_t od = other_deg[v] < kDegEscape ? other_deg[v] : kDegEscape;
+    const uint32_t id = e - b < kDegEscape ? (uint32_t)(e - b) : kDegEscape;
+    vinfo[v] = make_uint4(iperm ? iperm[v] : v, (od << 16) | id, f, (uint32_t)b);
     first_orig[v] = f == kUnset ? V : (iperm ? iperm[f] : f);
   }
 }

@@ -27,6 +27,7 @@ constexpr int kBlock = 256;                 // 4 waves per workgroup
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr uint32_t kChunkEdges = 1024;      // heavy-vertex chunk (one wave, 16 x 64)
 constexpr uint32_t kUnset = 0xFFFFFFFFu;
+constexpr uint32_t kDegEscape = 0xFFFFu;   // a vertex record's capped 16-bit degree
 constexpr uint32_t kScalarSlots = 256;      // spread slots of one scalar accumulator
 constexpr size_t kScalarStride = kScalarSlots * 16;  // doubles per accumulator
 
@@ -90,6 +91,7 @@ struct Options {
   int64_t batch_graphs = 0;      // batched BFS levels replayed from captured hipGraphs
   int64_t batch_device_loop = 1; // batched BFS/SSSP iterations decided on the device
   int64_t bfs_ahead = 1;         // BFS: iterations the host issues ahead of the decisions
+  int64_t bfs_record_starts = 1; // BFS pull: in-list starts from the vertex records (arcs < 2^32)
   // relabeling (fixed at build): the relabel_hubs highest-out-degree vertices
   // first, in degree order, then the others in file order, those without
   // in-arcs last (0: every vertex in degree order)
@@ -126,9 +128,12 @@ struct Dir {
   float *w = nullptr;         // arcs or null
   uint32_t *degree = nullptr; // V
   uint64_t *empty = nullptr;  // V/64 + 1 words: bit v set iff degree[v] == 0
-  // V: per vertex {original id, degree in the other direction, degree here,
-  // first (smallest) neighbour here or kUnset}: what a BFS step needs of a
-  // vertex it finds, in one 16-byte read
+  // V: per vertex {original id, degrees, first (smallest) neighbour here or
+  // kUnset, the list's start index[v] (low 32 bits)}: what a BFS step needs
+  // of a vertex it tests or finds, in one 16-byte read.  degrees = the degree
+  // in the other direction << 16 | the degree here, each capped at
+  // kDegEscape (a capped field means: read `degree` of that direction); the
+  // start is whole while arcs < 2^32
   uint4 *vinfo = nullptr;
   uint32_t *first_orig = nullptr;  // V: original id of the first neighbour (V if none)
   // heavy chunks (degree > light limit at build time)

@@ -211,6 +211,7 @@ static int64_t *option_slot(const char *name) {
   if (!strcmp(name, "batch_graphs")) return &o.batch_graphs;
   if (!strcmp(name, "batch_device_loop")) return &o.batch_device_loop;
   if (!strcmp(name, "bfs_ahead")) return &o.bfs_ahead;
+  if (!strcmp(name, "bfs_record_starts")) return &o.bfs_record_starts;
   if (!strcmp(name, "relabel_hubs")) return &o.relabel_hubs;
   if (!strcmp(name, "serve_bfs_workers")) return &o.serve_bfs_workers;
   return nullptr;
@@ -222,7 +223,8 @@ int gg_set_option(const char *name, int64_t value) {
   if (!strcmp(name, "force_mode") && (value < 0 || value > 2))
     return fail(GG_EINVAL, "force_mode must be 0, 1 or 2");
   const bool flag = !strcmp(name, "count_examined") || !strcmp(name, "level_masks") ||
-                    !strcmp(name, "batch_graphs") || !strcmp(name, "batch_device_loop");
+                    !strcmp(name, "batch_graphs") || !strcmp(name, "batch_device_loop") ||
+                    !strcmp(name, "bfs_record_starts");
   if (flag && (value < 0 || value > 1)) return fail(GG_EINVAL, std::string(name) + " must be 0 or 1");
   if (!strcmp(name, "bfs_pull_waves") && (value < 0 || value > (1 << 24)))
     return fail(GG_EINVAL, "bfs_pull_waves must be in [0, 2^24]");

@@ -107,6 +107,10 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *                      decided one (1..8, default 1); more hides the host's
  *                      reaction time on short levels, each one past the end
  *                      costs launches that return at once
+ *   "bfs_record_starts" BFS bottom-up step: 1 = take each in-list's start
+ *                      from the vertex's 16-byte record, read anyway, when
+ *                      the graph has fewer than 2^32 arcs (default); 0 =
+ *                      read the offsets array, as graphs with more arcs do
  *   "relabel_hubs"     GG_BUILD_REORDER builds (read at build time): K > 0
  *                      keeps the K highest-out-degree vertices first, in
  *                      degree order, and the others in file order, those

@@ -323,6 +323,61 @@ def test_bfs_hub_bins(gg):
         check_parents(e, n, 0, parent, depth)
 
 
+@pytest.mark.parametrize("light_max", [None, 100000])
+@pytest.mark.parametrize("reorder", [False, True])
+def test_bfs_degrees_past_16_bits(gg, reorder, light_max):
+    # the vertex records hold degrees in 16 bits; 65535 and above are read
+    # from the degree arrays.  Hub 1 has out-degree 70000 (m_reach counts it)
+    # and in-degree 70000, its one in-neighbour reachable from root 0 the
+    # last entry of its file-ordered list (140009, behind 69999 sources with
+    # no in-arcs): a bottom-up scan that stopped at 65535 entries would miss
+    # it.  pull_light_max 100000 (set before the build) scans the hub inside
+    # its group, the default leaves it to the heavy chunks.
+    n = 140010
+    pairs = [(0, 2), (2, 3), (3, 140009), (140009, 1)]
+    pairs += [(1, 10 + i) for i in range(70000)]
+    pairs += [(70010 + i, 1) for i in range(69999)]
+    e = oracle.edges_from_pairs(pairs)
+    opts = {} if light_max is None else {"pull_light_max": light_max}
+    with gg.options(**opts):
+        g = gg.Graph.build(e, n, reorder=reorder)
+    o = oracle.Graph(e, n)
+    for root in (0, 1, 70010):
+        _, rd, rf = o.bfs(root)
+        reached = rd != 0xFFFFFFFF
+        for mode in ("push", "pull", "auto"):
+            say("bfs 16-bit degrees", reorder, light_max, root, mode)
+            with gg.options(force_mode=MODES[mode]):
+                parent, depth, fr, st = g.bfs(root)
+            assert np.array_equal(depth, rd) and np.array_equal(fr, rf), (root, mode)
+            check_parents(e, n, root, parent, depth)
+            assert st["m_reach"] == int(o.out_degree()[reached].astype(np.int64).sum())
+    g.close()
+
+
+@pytest.mark.parametrize("record_starts", [1, 0])
+@pytest.mark.parametrize("reorder", [False, True])
+def test_bfs_list_starts(gg, reorder, record_starts):
+    # the bottom-up step takes each in-list's start from the vertex's record
+    # (graphs below 2^32 arcs) or, with bfs_record_starts 0 as graphs with
+    # more arcs do, from the offsets array: both give the oracle's levels
+    e = oracle.rmat_edges(14, seed=21)
+    V = 1 << 14
+    g = gg.Graph.build(e, V, reorder=reorder)
+    o = oracle.Graph(e, V)
+    for root in roots_of(o.out_degree(), 3, 21):
+        _, rd, rf = o.bfs(root)
+        reached = rd != 0xFFFFFFFF
+        for mode in ("pull", "auto"):
+            say("bfs list starts", reorder, record_starts, root, mode)
+            with gg.options(force_mode=MODES[mode], bfs_record_starts=record_starts):
+                parent, depth, fr, st = g.bfs(root)
+            assert np.array_equal(depth, rd) and np.array_equal(fr, rf), (root, mode)
+            check_parents(e, V, root, parent, depth)
+            assert st["m_reach"] == int(o.out_degree()[reached].astype(np.int64).sum())
+    g.close()
+
+
 def depths_from_parents(parent, root, V, max_levels=256):
     """Level of every vertex in the tree the parents form (0xFFFFFFFF where
     parent == V): each reached vertex walks up to the root."""
