@@ -584,16 +584,18 @@ def cpu_baseline(scale, ef, seed, roots, gpu_depths, sample_roots, repeats):
     }, parity
 
 
-# the edge-processing launches of every BFS iteration: the bottom-up step, and
-# the push (push iterations) or the heavy in-list chunks (pull iterations)
-STEP_KERNELS = ("gg::bfs_pull_kernel", "gg::bfs_push_heavy_kernel")
+# the two launches of every BFS iteration: the step (decision, then the
+# bottom-up step or the frontier binning), and the push (push iterations) or
+# the heavy in-list chunks (pull iterations)
+STEP_KERNELS = ("gg::bfs_step_kernel", "gg::bfs_push_heavy_kernel")
 
 
 def kernel_name(row):
     """A profiler row's kernel name without return type, template arguments
-    or parameters (gg::bfs_pull_kernel<false>(...) -> gg::bfs_pull_kernel)."""
+    or parameters (gg::bfs_step_kernel<false, true>(...) -> gg::bfs_step_kernel)."""
     return row["Kernel_Name"].split("(")[0].replace("void ", "").split("<")[0]
 COPY_KERNEL = "gg::gg_copy_kernel"
+SEED_ARCS = 1024  # gg_bfs: a first push of at most this many arcs runs in the seed kernel
 COPY_BYTES = 256 << 20
 
 
@@ -699,8 +701,11 @@ def level_child(args):
     g.bfs_into(roots[0], parent)
     gg.synchronize()
     n = st["iterations"]
-    print(json.dumps({k: st[k][:n] for k in ("modes", "it_active", "it_new", "it_examined",
-                                               "it_unvisited")}), flush=True)
+    out = {k: st[k][:n] for k in ("modes", "it_active", "it_new", "it_examined", "it_unvisited")}
+    # level 0 runs inside the seed kernel (no iteration launches) when it is a
+    # push of at most SEED_ARCS out-arcs, as gg_bfs decides it
+    out["seed_level"] = int(st["modes"][0] == 0 and int(g.out_degree()[roots[0]]) <= SEED_ARCS)
+    print(json.dumps(out), flush=True)
 
 
 def _csv_rows(out, suffix):
@@ -714,7 +719,7 @@ def _csv_rows(out, suffix):
 
 def _last_bfs(rows, key):
     """Dispatches of the last BFS (after its seed kernel) grouped per level:
-    [{kernel: row}], a level starting at each decide_bin launch."""
+    [{kernel: row}], a level starting at each step launch."""
     rows = sorted(rows, key=key)
     name = kernel_name
     seeds = [i for i, r in enumerate(rows) if name(r) == "gg::bfs_seed_kernel"]
@@ -723,7 +728,7 @@ def _last_bfs(rows, key):
         k = name(r).replace("gg::bfs_", "").replace("_kernel", "")
         if k in ("init", "seed"):
             break
-        if k == "decide_bin":
+        if k == "step":
             levels.append({})
         if levels:
             levels[-1][k] = r
@@ -732,7 +737,7 @@ def _last_bfs(rows, key):
 
 def level_table(args):
     """Per-level table of one bench BFS: mode, frontier, kernel times (kernel
-    trace) and HBM requests of the bottom-up and push/heavy kernels (three
+    trace) and HBM requests of the step and push/heavy kernels (three
     counter passes), printed as markdown."""
     import subprocess
     import tempfile
@@ -762,25 +767,30 @@ def level_table(args):
                 d = merged.setdefault(int(x["Dispatch_Id"]), dict(x, counters={}))
                 d["counters"][x["Counter_Name"]] = float(x["Counter_Value"])
             per[tag] = _last_bfs(list(merged.values()), key=lambda x: int(x["Dispatch_Id"]))
-    print("| level | mode | frontier | new | examined | decide+bin us | pull us | push/heavy us | "
+    print("| level | mode | frontier | new | examined | step us | push/heavy us | "
           "read MB (128B/64B/32B req) | write req (64B) | TCC req | hit/miss |")
-    print("|---|---|---|---|---|---|---|---|---|---|---|---|")
+    print("|---|---|---|---|---|---|---|---|---|---|---|")
     # (the trace also holds the iterations queued past the end, which return
-    # at once: only the levels the BFS ran are listed)
-    for i, lv in enumerate(per["trace"][:len(stats["modes"])]):
+    # at once: only the levels the BFS ran are listed; a level 0 run by the
+    # seed kernel has no launches of its own)
+    n0 = stats.get("seed_level", 0)
+    for i in range(len(stats["modes"])):
+        j = i - n0  # this level's iteration launches
+        lv = per["trace"][j] if 0 <= j < len(per["trace"]) else {}
         us = {k: (int(v["End_Timestamp"]) - int(v["Start_Timestamp"])) / 1e3 for k, v in lv.items()}
         c = {}
         for tag in ("p0", "p1", "p2"):
-            if i < len(per[tag]):
-                for k in ("pull", "push_heavy"):
-                    for n, v in per[tag][i].get(k, {}).get("counters", {}).items():
+            if 0 <= j < len(per[tag]):
+                for k in ("step", "push_heavy"):
+                    for n, v in per[tag][j].get(k, {}).get("counters", {}).items():
                         c[n] = c.get(n, 0.0) + v
         r128, r64, r32 = (c.get("TCC_EA0_RDREQ_128B_sum", 0), c.get("TCC_EA0_RDREQ_64B_sum", 0),
                           c.get("TCC_EA0_RDREQ_32B_sum", 0))
         read_mb = (128 * r128 + 64 * r64 + 32 * r32) / 1e6
         print(f"| {i} | {'push' if stats['modes'][i] == 0 else 'pull'} | {stats['it_active'][i]} | "
-              f"{stats['it_new'][i]} | {stats['it_examined'][i]} | {us.get('decide_bin', 0):.1f} | "
-              f"{us.get('pull', 0):.1f} | {us.get('push_heavy', 0):.1f} | "
+              f"{stats['it_new'][i]} | {stats['it_examined'][i]} | "
+              f"{'(seed kernel)' if j < 0 else format(us.get('step', 0), '.1f')} | "
+              f"{'-' if j < 0 else format(us.get('push_heavy', 0), '.1f')} | "
               f"{read_mb:.1f} ({int(r128)}/{int(r64)}/{int(r32)}) | "
               f"{int(c.get('TCC_EA0_WRREQ_sum', 0))} ({int(c.get('TCC_EA0_WRREQ_64B_sum', 0))}) | "
               f"{int(c.get('TCC_REQ_sum', 0))} | {int(c.get('TCC_HIT_sum', 0))}/"
@@ -928,7 +938,7 @@ def main():
     gg.prof_enable(True)
     step()
     gg.prof_enable(False)
-    kernels = {k: gg.prof_read(k) for k in ("bfs_decide_bin", "bfs_pull", "bfs_push_heavy")}
+    kernels = {k: gg.prof_read(k) for k in ("bfs_step", "bfs_push_heavy")}
     log(f"BFS x{len(roots)}: {ms_per_step:.2f} ms/step, {gteps:.2f} GTEPS; kernels {kernels}")
 
     # one instrumented pass (edge counters on) for the algorithmic byte model
@@ -944,21 +954,21 @@ def main():
             pull_b += lb
             if len(gpu_depths) < args.parity_roots:
                 gpu_depths.append(depths_from_parents(parent.numpy(), r))
-    # Three launches per iteration: bfs_decide_bin_kernel (the push/pull
-    # decision, and the frontier binning of a push), bfs_pull_kernel (the
-    # bottom-up step; returns at once in a push) and bfs_push_heavy_kernel
-    # (the push, or the heavy in-list chunks of a pull).  The roofline covers
-    # the two edge-processing launches, pull and push/heavy; per launch = per
-    # iteration (per bfs_pull_kernel dispatch, as rocprof counts); the bytes
-    # are the model's for every iteration, push and pull.
-    step_ms = kernels["bfs_pull"][0] + kernels["bfs_push_heavy"][0]
-    step_launches = kernels["bfs_pull"][1]
+    # Two launches per iteration: bfs_step_kernel (the push/pull decision,
+    # then the bottom-up step of a pull or the frontier binning of a push)
+    # and bfs_push_heavy_kernel (the push, or the heavy in-list chunks of a
+    # pull).  The roofline covers both, every launch of every iteration; per
+    # launch = per iteration (per bfs_step_kernel dispatch, as rocprof
+    # counts); the bytes are the model's for every iteration, push and pull.
+    step_ms = kernels["bfs_step"][0] + kernels["bfs_push_heavy"][0]
+    step_launches = kernels["bfs_step"][1]
     step_b = push_b + pull_b
     achieved = step_b / (step_ms / 1e3) / 1e9 if step_ms > 0 else 0.0
     roofline = {
         "bound": "hbm",
-        "kernel": "bfs_pull_kernel + bfs_push_heavy_kernel (the edge-processing step of every "
-                  "BFS iteration: bottom-up, or push; heavy in-lists)",
+        "kernel": "bfs_step_kernel + bfs_push_heavy_kernel (both launches of every BFS "
+                  "iteration: decision, then bottom-up or frontier binning; push or heavy "
+                  "in-lists)",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",

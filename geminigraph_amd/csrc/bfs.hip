@@ -1,18 +1,18 @@
 // BFS (toolkits/bfs.cpp) with its iteration loop driven from the device.
 //
-// Every iteration is the same fixed sequence of three launches, issued ahead
+// Every iteration is the same fixed sequence of two launches, issued ahead
 // of time on one stream:
-//   decide_bin -- every block first makes the iteration's decision from the
+//   step       -- every block first makes the iteration's decision from the
 //                 counts of the frontier just produced (its size, out-arcs
 //                 and in-arcs, summed over spread slots by the kernels that
 //                 produced it): the push/pull switch
 //                 (core/graph.hpp:1503-1509), or the end of the traversal.
 //                 All blocks reach the same decision from the same counts;
 //                 block 0 also records it (statistics, control block, the
-//                 flag the host polls).  Then, in a push: frontier bitmap ->
-//                 three degree-binned queues, the frontier joins `visited`,
-//                 the output bitmap is cleared;
-//   pull       -- pull: bottom-up over 64-vertex groups (bfs.cpp:60-76);
+//                 flag the host polls).  Then, in a pull: bottom-up over
+//                 64-vertex groups (bfs.cpp:60-76); in a push: frontier
+//                 bitmap -> three degree-binned queues, the frontier joins
+//                 `visited`, the output bitmap is cleared;
 //   push_heavy -- push: a persistent grid over the queues (bfs.cpp:49-58);
 //                 pull: the chunks of the heavy in-lists.
 // The host never waits for an iteration before issuing the next one: it
@@ -208,11 +208,12 @@ __device__ __forceinline__ uint32_t bfs_mode(int32_t force, uint64_t threshold, 
 // launch, once every block here has read them.  Returns the mode.
 __device__ uint32_t bfs_decide_block(BfsCtl *c, uint32_t n, const uint64_t *set,
                                      const unsigned long long *exam, uint32_t *lv,
-                                     BfsHostFlag *hf) {
-  __shared__ uint32_t s_mode;
+                                     BfsHostFlag *hf, uint32_t *level_out = nullptr) {
+  __shared__ uint32_t s_mode, s_level;
   if (threadIdx.x < 64) {
     const int lane = lane_id();
     const uint32_t done = c->done;
+    const uint32_t n0 = c->n0;
     const int32_t force = c->force;
     const uint64_t threshold = c->threshold, alpha = c->alpha, m_unv = c->m_unv[n & 1];
     uint64_t cnt = 0, edg = 0, ind = 0, x = 0;
@@ -232,8 +233,9 @@ __device__ uint32_t bfs_decide_block(BfsCtl *c, uint32_t n, const uint64_t *set,
       const uint64_t m_left = m_unv - (ind < m_unv ? ind : m_unv);
       if (!done && cnt) mode = bfs_mode(force, threshold, alpha, edg, m_left);
       s_mode = mode;
+      s_level = n + n0 + 1;
       if (blockIdx.x == 0 && !done) {
-        const uint32_t i = n + c->n0;  // the level this iteration runs
+        const uint32_t i = n + n0;  // the level this iteration runs
         if (i > 0 && i - 1 < GG_STATS_RECORDS) {
           c->fresh[i - 1] = cnt;
           c->exam[i - 1] = x;
@@ -264,6 +266,7 @@ __device__ uint32_t bfs_decide_block(BfsCtl *c, uint32_t n, const uint64_t *set,
     }
   }
   __syncthreads();
+  if (level_out) *level_out = s_level;
   return s_mode;
 }
 
@@ -304,7 +307,7 @@ __global__ void bfs_init_kernel(uint32_t *parent, uint32_t *depth, uint32_t V, u
 // the push does (bfs.cpp:53), sets their bits in the frontier that the
 // host's iteration 0 reads, leaves their counts in iteration 0's slots, and
 // records level 0 as the decision would; the host's iteration n then runs
-// level n + 1 (n0 = 1), one iteration of three launches fewer.  Otherwise
+// level n + 1 (n0 = 1), one iteration of two launches fewer.  Otherwise
 // the frontier is the root, with its counts, and n0 = 0.
 constexpr uint32_t kSeedArcs = 1024;
 __global__ __launch_bounds__(kBlock) void bfs_seed_kernel(
@@ -413,7 +416,7 @@ struct BfsBinArgs {
   Counters *qc;
 };
 
-// The binning in the decision's launch: each block takes one contiguous
+// The binning in the step launch of a push: each block takes one contiguous
 // range of bitmap words (whole multiples of kBlock), and adds its bin sizes to
 // the queue counters once, so a few hundred blocks, not one per word round,
 // meet on those three counters.  Push frontiers are sparse (most words hold
@@ -422,9 +425,10 @@ struct BfsBinArgs {
 // to two set bits per word together (more rounds only for denser words), and
 // keeps each word's bins as two masks, so the second walk, which writes the
 // queue entries, loads nothing.  A block whose range holds no set bit returns
-// after the first walk.  The launch has one wave per SIMD (the decision runs
-// in every block), so registers are not scarce here.
-constexpr int kBinWords = 4;
+// after the first walk.  Two words per thread keep the step kernel within
+// the 64 vector registers of the bottom-up step (the launch has a thread
+// for every word of an RMAT-25 bitmap, so one pass covers most ranges).
+constexpr int kBinWords = 2;
 __device__ __forceinline__ void bfs_bin_step(const BfsBinArgs &b, uint64_t *visited,
                                              uint64_t *clear) {
   __shared__ uint32_t s_tot[3][kWavesPerBlock];
@@ -699,10 +703,7 @@ struct BfsPullArgs {
 // serial scan stops (bfs.cpp:66-67), so each round hands the owner its first
 // hit (the lowest entry, an LDS atomic min) instead of any one.
 // kStart32 (arcs < 2^32): a list's start comes with the vertex's record, so
-// a scan needs no load of the offsets first.  The uncounted kStart32
-// instance fits in 80 scalar registers (no examined pointer, no offsets
-// array): built with 84, its bottom-up levels ran 14-25% slower at the same
-// 64 vector registers, as with fewer waves per SIMD.
+// a scan needs no load of the offsets first.
 template <bool kCount, bool kStart32>
 __device__ __forceinline__ void bfs_pull_step(uint32_t level, const BfsOut &o,
                                               const BfsPullArgs &a, uint64_t *set_out,
@@ -879,34 +880,37 @@ __device__ __forceinline__ void bfs_heavy_step(const BfsCtl *c, const BfsOut &o,
   bfs_wave_flush(set_out, gw, cnt, edg, ind, exam, examined);
 }
 
-// The three launches of an iteration.  A launch of the mode not chosen
-// returns at once but still costs a kernel boundary, so the decision shares
-// a launch with the binning, and the push with the heavy chunks (the same
-// block shape and register needs).  The bottom-up step keeps a launch of its
-// own: in one kernel with the binning it ran 30% slower per level, with the
-// same loop code, the same 8 waves per SIMD, LDS and barriers ruled out.
-//   decide_bin -- every block: the decision of iteration n; then, push: the
-//                 frontier binning over all blocks;
-//   pull       -- pull: the bottom-up step (every other mode returns);
+// The two launches of an iteration.  A launch of the mode not chosen would
+// return at once but still cost a kernel boundary, so each launch serves
+// both modes:
+//   step       -- every block: the decision of iteration n; then, pull: the
+//                 bottom-up step; push: the frontier binning;
 //   push_heavy -- push: the persistent push over the queues (push_blocks);
 //                 pull: the heavy in-list chunks (heavy_blocks).  Block 0
 //                 first clears what the decision read or what the next
 //                 iteration's binning fills: the count slots and examined
 //                 counters of iteration n's decision, and the queue counters
 //                 of iteration n + 1.
-__global__ __launch_bounds__(kBlock) void bfs_decide_bin_kernel(
-    BfsCtl *c, uint32_t n, const uint64_t *set_in, const unsigned long long *exam_in,
-    uint32_t *lv, BfsHostFlag *hf, BfsOut o, BfsBinArgs b) {
-  if (bfs_decide_block(c, n, set_in, exam_in, lv, hf) == kBfsPush)
-    bfs_bin_step(b, o.visited, o.fout);
-}
-
+// The step kernel is held to 80 scalar and 64 vector registers (8 waves per
+// SIMD; the few values that do not fit spill outside the batch loop): with
+// more scalar registers the bottom-up step ran 14-30% slower at the same
+// vector count, as with fewer waves per SIMD.  kCount: the examined
+// counters are kept (without, their pointers are null at compile time).
 template <bool kCount, bool kStart32>
-__global__ __launch_bounds__(kBlock) void bfs_pull_kernel(const BfsCtl *c, uint32_t n, BfsOut o,
-                                                          BfsPullArgs a, uint64_t *set_out,
-                                                          unsigned long long *exam) {
-  if (c->done || c->mode != kBfsPull) return;
-  bfs_pull_step<kCount, kStart32>(c->level, o, a, set_out, exam, gridDim.x * kWavesPerBlock, global_wave());
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(80), amdgpu_waves_per_eu(8, 8)))
+void bfs_step_kernel(BfsCtl *c, uint32_t n, const uint64_t *set_in,
+                     const unsigned long long *exam_in_, uint32_t *lv, BfsHostFlag *hf, BfsOut o,
+                     BfsBinArgs b, BfsPullArgs a, uint64_t *set_out,
+                     unsigned long long *exam_out_) {
+  const unsigned long long *exam_in = kCount ? exam_in_ : nullptr;
+  unsigned long long *exam_out = kCount ? exam_out_ : nullptr;
+  uint32_t level;
+  const uint32_t mode = bfs_decide_block(c, n, set_in, exam_in, lv, hf, &level);
+  if (mode == kBfsPush)
+    bfs_bin_step(b, o.visited, o.fout);
+  else if (mode == kBfsPull)
+    bfs_pull_step<kCount, kStart32>(level, o, a, set_out, exam_out, gridDim.x * kWavesPerBlock,
+                                    global_wave());
 }
 
 // Blocks past the mode's count return at once.  kCount: the examined
@@ -1057,7 +1061,7 @@ static void bfs_issue_start(const gg_graph *g, const BfsPlan &p, const BfsBufs &
                      threshold, g->arcs, (uint64_t)opt.bfs_alpha, b.lv_cap);
 }
 
-// The three launches of iteration n (decide_bin, pull, push_heavy).  Buffers by
+// The two launches of iteration n (step, push_heavy).  Buffers by
 // parity: the frontier bitmaps, the count slots (iteration n decides on
 // sets[n & 1] and counts into sets[(n + 1) & 1]), the examined counters
 // likewise, and the queue counters (qc[n & 1] binned and pushed in
@@ -1074,20 +1078,15 @@ static void bfs_issue_iteration(const gg_graph *g, const BfsPlan &p, const BfsBu
   Counters *qc = b.qc + (n & 1), *qc_next = b.qc + ((n + 1) & 1);
   BfsOut o{p.parent, p.depth, in.vinfo, b.visited, fout, p.V, out.degree, in.degree};
   {
-    ProfScope ps(prof ? "bfs_decide_bin" : nullptr);
+    ProfScope ps(prof ? "bfs_step" : nullptr);
     BfsBinArgs ba{fin, b.words, out.degree, (uint32_t)opt.push_small_max,
                   (uint32_t)opt.push_big_max, b.q[0], b.q[1], b.q[2], qc};
-    hipLaunchKernelGGL(bfs_decide_bin_kernel, dim3(p.bin_blocks), dim3(kBlock), 0, st, b.ctl, n,
-                       set_in, exam_in, b.lv, b.hf, o, ba);
-  }
-  {
-    ProfScope ps(prof ? "bfs_pull" : nullptr);
     BfsPullArgs pa{in.index, in.nbr, in.first_orig, fin, p.V, in.light_max, p.ngroups,
                    g->arcs < (1ULL << 32) && opt.bfs_record_starts ? 1u : 0u};
-    auto kern = exam_out ? (pa.start32 ? bfs_pull_kernel<true, true> : bfs_pull_kernel<true, false>)
-                         : (pa.start32 ? bfs_pull_kernel<false, true> : bfs_pull_kernel<false, false>);
-    hipLaunchKernelGGL(kern, dim3(p.pull_grid), dim3(kBlock), 0, st, b.ctl, n, o, pa, set_out,
-                       exam_out);
+    auto kern = exam_out ? (pa.start32 ? bfs_step_kernel<true, true> : bfs_step_kernel<true, false>)
+                         : (pa.start32 ? bfs_step_kernel<false, true> : bfs_step_kernel<false, false>);
+    hipLaunchKernelGGL(kern, dim3(std::max(p.pull_grid, p.bin_blocks)), dim3(kBlock), 0, st, b.ctl,
+                       n, set_in, exam_in, b.lv, b.hf, o, ba, pa, set_out, exam_out);
   }
   {
     ProfScope ps(prof ? "bfs_push_heavy" : nullptr);
@@ -1225,8 +1224,8 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
   // iterations queued ahead of the device: before issuing iteration n, wait
   // until iteration n - ahead has been decided (iteration n - ahead - 1 has
   // finished).  Short levels take less device time than the host needs to
-  // see a decision and issue three launches; each iteration queued past the
-  // end runs as three launches that return at once.
+  // see a decision and issue two launches; each iteration queued past the
+  // end runs as two launches that return at once.
   const uint32_t ahead = (uint32_t)std::max<int64_t>(1, opt.bfs_ahead);
   for (uint32_t n = 0;; n++) {
     if (n >= ahead) {
