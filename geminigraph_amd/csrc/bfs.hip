@@ -45,7 +45,7 @@ namespace gg {
 
 enum : uint32_t { kBfsPush = 0, kBfsPull = 1, kBfsDone = 2 };
 constexpr uint32_t kHubBatch = 512;  // huge frontier vertices staged per push round
-constexpr uint32_t kPushChunk = 256;  // arcs of a huge frontier vertex per wave step
+constexpr uint32_t kPushChunk = 128;  // arcs of a huge frontier vertex per wave step (one two-step round)
 static_assert(kHubBatch % kBlock == 0, "whole rows of the staged batch per thread");
 constexpr int kCountStride = 16;  // u64 per count slot: one 128-byte line
 // Count slots of one frontier: the step kernels' waves add their counts to
