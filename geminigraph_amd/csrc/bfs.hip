@@ -794,12 +794,14 @@ __device__ __forceinline__ void bfs_pull_step(uint32_t level, const BfsOut &o,
         if (!__ballot(cand && found == kUnset)) break;
       }
       if (found != kUnset) {
-        // both loads (the parent's id, a capped degree) before the writes
+        // both loads (the parent's id, a capped degree) before the writes;
+        // the writes are non-temporal: nothing in this launch reads them, and
+        // lines left dirty in L2 are written back at the kernel boundary
         const uint32_t pval = head ? head_orig : o.vinfo[found].x;
         edg += rec_odeg(vi.y, o.odeg, v);
         ind += deg;
-        o.parent[vi.x] = pval;
-        if (o.depth) o.depth[vi.x] = level;
+        __builtin_nontemporal_store(pval, &o.parent[vi.x]);
+        if (o.depth) __builtin_nontemporal_store(level, &o.depth[vi.x]);
         atomicOr(&s_found[wv][gk], 1ULL << (v & 63));
       }
     }
