@@ -51,7 +51,7 @@ constexpr int kLightMax = 256;  // dense: longer in-lists go to one wave with la
 constexpr int kSubList = 32;    // sparse: out-lists reach slots in sub-lists of at most 32
 constexpr int kWaveList = 64 * kSubList;  // sparse: longer out-lists go to the heavy kernel
 constexpr int kSlots = 256;     // spread reducer slots (16 x 8 bytes apart)
-constexpr int kPhaseGrid = 8192;  // blocks of the persistent phase kernels
+constexpr int kPhaseGrid = 2048;  // blocks of the persistent phase kernels (one round of residency)
 
 template <typename M>
 struct MsgUnit {  // core/graph.hpp:61-65
@@ -127,18 +127,34 @@ __device__ inline T shfl_any(const T &x, int src) {
   return y;
 }
 
+// A wave loads 64 bitmap words at once (one per lane), then visits only the
+// non-empty ones, each as 64 lanes over its 64 consecutive vertices (so
+// callbacks that set bits of a word in lane order still meet on one word).
+// Calls `g(v)` for every set bit v < V.
+template <typename G>
+__device__ inline void for_each_set_bit(const unsigned long *active, VertexId V, G g) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long nwords = ((unsigned long long)V + 63) / 64;
+  const unsigned long long W = (unsigned long long)gridDim.x * (blockDim.x >> 6);
+  const unsigned long long gw =
+      (unsigned long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  for (unsigned long long w0 = gw * 64; w0 < nwords; w0 += W * 64) {
+    const unsigned long long wi = w0 + lane;
+    const unsigned long word = wi < nwords ? active[wi] : 0ul;
+    for (unsigned long long m = __ballot(word != 0ul); m; m &= m - 1) {
+      const int k = __builtin_ctzll(m);
+      const unsigned long wk = shfl_any(word, k);
+      const unsigned long long v = (w0 + k) * 64 + lane;
+      if (v < V && (wk >> lane) & 1ul) g((VertexId)v);
+    }
+  }
+}
+
 template <typename R, typename F>
 __global__ void process_vertices_kernel(F f, const unsigned long *active, VertexId V, R *slots) {
-  for (VertexId base = blockIdx.x * blockDim.x; base < V; base += gridDim.x * blockDim.x) {
-    VertexId v = base + threadIdx.x;
-    // a wave covers one bitmap word (base and the block are multiples of
-    // 64): an empty word skips the wave's reduction
-    const unsigned long word = v < V ? active[WORD_OFFSET(v)] : 0ul;
-    if (!__ballot(word != 0ul)) continue;
-    R x = R(0);
-    if (v < V && (word >> BIT_OFFSET(v)) & 1ul) x = (R)f(v);
-    slot_add_wave<R>(slots, x);
-  }
+  R x = R(0);
+  for_each_set_bit(active, V, [&](VertexId v) { x += (R)f(v); });
+  slot_add_wave<R>(slots, x);
 }
 
 // One block: the sum of the slots as R into *out, the slots back to zero.
@@ -223,10 +239,7 @@ __global__ void pack_copy_kernel(Ctl *c) {
 template <typename F>
 __global__ void sparse_signal_kernel(F f, const Ctl *c, const unsigned long *active, VertexId V) {
   if (c->mode != 0) return;
-  for (VertexId base = blockIdx.x * blockDim.x; base < V; base += gridDim.x * blockDim.x) {
-    VertexId v = base + threadIdx.x;
-    if (v < V && (active[WORD_OFFSET(v)] >> BIT_OFFSET(v)) & 1ul) f(v);
-  }
+  for_each_set_bit(active, V, [&](VertexId v) { f(v); });
 }
 
 // Sparse slots, 64 messages per wave step: lists up to kSubList entries go
@@ -446,9 +459,13 @@ __global__ void pack_adj_kernel(const VertexId *nbr, const float *w, EdgeId n,
   }
 }
 
+// Blocks of 256 threads for n items, at most one per four wave slots of
+// the chip (256 CUs x 4 SIMDs x 8 waves): every kernel here loops over its
+// items, and a launch whose mode was not chosen (it returns at once) then
+// costs a few microseconds instead of dispatching 65,536 blocks.
 inline unsigned grid_for(unsigned long long n) {
   unsigned long long g = (n + 255) / 256;
-  return (unsigned)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
+  return (unsigned)(g < 1 ? 1 : (g > 2048 ? 2048 : g));
 }
 
 }  // namespace gg_facade
