@@ -56,8 +56,10 @@ constexpr size_t kBfsSetWords = (size_t)kBfsSlots * kCountStride;
 // examined counters of one iteration: 64 slots, one 128-byte line each
 constexpr size_t kExamWords = 64 * 16;
 
-// Device control block of one traversal (done and iter first: the host
-// reads those 8 bytes after every decision).
+// Device control block of one traversal (done first: the allocation-free
+// variant reads it back after every round of iterations; gg_bfs polls a
+// flag in pinned host memory instead and copies the whole block once, at
+// the end).
 struct BfsCtl {
   uint32_t done, iter;  // iter = iterations started
   uint32_t mode, level, push_iters, V;
