@@ -355,6 +355,29 @@ def test_bfs_degrees_past_16_bits(gg, reorder, light_max):
     g.close()
 
 
+@pytest.mark.parametrize("waves", [4, 64, 0])
+def test_bfs_step_grids(gg, waves):
+    # the step launch (decision, then the bottom-up step or the frontier
+    # binning) runs on max(bottom-up grid, binning grid) blocks: a grid of 4
+    # waves (one block, where the binning of 2^17 vertices takes three), 64,
+    # and one wave per group (0); the heavy-chunk step takes half of it
+    e = oracle.rmat_edges(17, seed=31)
+    V = 1 << 17
+    o = oracle.Graph(e, V)
+    g = gg.Graph.build(e, V, reorder=True)
+    for root in roots_of(o.out_degree(), 3, 31):
+        _, rd, rf = o.bfs(root)
+        reached = rd != 0xFFFFFFFF
+        for mode in ("auto", "push", "pull"):
+            say("bfs step grids", waves, root, mode)
+            with gg.options(force_mode=MODES[mode], bfs_pull_waves=waves):
+                parent, depth, fr, st = g.bfs(root)
+            assert np.array_equal(depth, rd) and np.array_equal(fr, rf), (root, mode)
+            check_parents(e, V, root, parent, depth)
+            assert st["m_reach"] == int(o.out_degree()[reached].astype(np.int64).sum())
+    g.close()
+
+
 @pytest.mark.parametrize("record_starts", [1, 0])
 @pytest.mark.parametrize("reorder", [False, True])
 def test_bfs_list_starts(gg, reorder, record_starts):
