@@ -20,19 +20,42 @@ struct BitmapView {
   }
   // Lanes of a wave that set the bits of one word in lane order (a wave over
   // 64 consecutive vertices, as process_vertices runs) take one atomic for
-  // the word; any other pattern one atomic per lane.
+  // the word.  Otherwise the calling lanes are grouped by word, up to four
+  // groups (the first pending lane's word each time; sorted vertex runs,
+  // such as dense slots over packed messages, span a few words): a group's
+  // bits are gathered with LDS atomics in a per-wave word and its first lane
+  // takes the one global atomic; lanes left after four groups take one
+  // atomic each.  Blocks of at most 1024 threads.
   __device__ void set_bit(size_t i) const {
+    __shared__ unsigned long long s_or[16];
     const size_t w = WORD_OFFSET(i);
+    const unsigned long long bit = 1ull << BIT_OFFSET(i);
     const unsigned lane = __lane_id();
-    const unsigned long long act = __ballot(1);
-    const int first = __ffsll((long long)act) - 1;
+    unsigned long long todo = __ballot(1);
+    const int first = __ffsll((long long)todo) - 1;
     const size_t w0 = ((size_t)(unsigned)__shfl((int)(w >> 32), first) << 32) |
                       (unsigned)__shfl((int)(unsigned)w, first);
     if (__ballot(w != w0 || BIT_OFFSET(i) != lane) == 0) {
-      if ((int)lane == first) atomicOr((unsigned long long *)(data + w0), act);
-    } else {
-      atomicOr((unsigned long long *)(data + w), 1ull << BIT_OFFSET(i));
+      if ((int)lane == first) atomicOr((unsigned long long *)(data + w0), todo);
+      return;
     }
+    unsigned long long *acc = &s_or[(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y *
+                                                                  threadIdx.z)) >> 6 & 15];
+    for (int g = 0; g < 4 && todo; g++) {  // todo is the same in every lane
+      const int lead = __ffsll((long long)todo) - 1;
+      const size_t lw = ((size_t)(unsigned)__shfl((int)(w >> 32), lead) << 32) |
+                        (unsigned)__shfl((int)(unsigned)w, lead);
+      const bool mine = ((todo >> lane) & 1ull) && w == lw;
+      const unsigned long long grp = __ballot(mine);
+      if ((int)lane == lead) *acc = 0;
+      __builtin_amdgcn_wave_barrier();
+      if (mine) atomicOr(acc, bit);
+      __builtin_amdgcn_wave_barrier();
+      if ((int)lane == lead) atomicOr((unsigned long long *)(data + lw), *acc);
+      __builtin_amdgcn_wave_barrier();
+      todo &= ~grp;
+    }
+    if ((todo >> lane) & 1ull) atomicOr((unsigned long long *)(data + w), bit);
   }
 };
 
