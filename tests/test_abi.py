@@ -68,6 +68,14 @@ def test_last_error_and_options_without_gpu():
     for bad in (-1, 32):
         with pytest.raises(gg.GGError):
             gg.set_option("serve_bfs_workers", bad)
+    # BFS list starts from the vertex records: a flag, on by default
+    assert gg.get_option("bfs_record_starts") == 1
+    with gg.options(bfs_record_starts=0):
+        assert gg.get_option("bfs_record_starts") == 0
+    assert gg.get_option("bfs_record_starts") == 1
+    for bad in (-1, 2):
+        with pytest.raises(gg.GGError):
+            gg.set_option("bfs_record_starts", bad)
 
 
 @pytest.mark.parametrize("bad", [-0.5, float("nan"), -float("inf")])
