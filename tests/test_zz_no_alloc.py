@@ -89,6 +89,25 @@ def test_noalloc_matches_bfs_two_calls_then_smaller(gg):
     small.close()
 
 
+def test_noalloc_degrees_past_16_bits(gg):
+    # vertex records keep degrees in 16 bits (65535 and above read from the
+    # degree arrays): a hub with 70000 out-arcs and 70000 in-arcs, its one
+    # reachable in-neighbour last in its list; roots before and at the hub
+    n = 140010
+    pairs = [(0, 2), (2, 3), (3, 140009), (140009, 1)]
+    pairs += [(1, 10 + i) for i in range(70000)]
+    pairs += [(70010 + i, 1) for i in range(69999)]
+    e = oracle.edges_from_pairs(pairs)
+    g = gg.Graph.build(e, n)
+    o = oracle.Graph(e, n)
+    for root in (0, 1):
+        for mode in (0, 1, 2):
+            say("noalloc 16-bit degrees", root, mode)
+            with gg.options(force_mode=mode):
+                run_both(gg, g, e, o, root)
+    g.close()
+
+
 def test_noalloc_forced_modes(gg):
     e = oracle.rmat_edges(14, 16, seed=44)
     V = 1 << 14
