@@ -1006,8 +1006,10 @@ def main():
     extras["bfs_reference_region"] = {
         "ms_per_bfs": round(1e3 * float(np.mean(ref_ts)), 4),
         "gteps": round(ref_m / sum(ref_ts) / 1e9, 2),
-        "timed_region": "each call allocates its parent array (hipMalloc) inside the timer, as "
-                        "bfs.cpp:23-29 does with alloc_vertex_array; the traversal buffers, which "
+        "timed_region": "each call allocates its parent array (gg_malloc, which hands back the "
+                        "buffer the previous call freed) inside the timer and synchronises at its "
+                        "end, as bfs.cpp:23-29 allocates with alloc_vertex_array; the traversal "
+                        "buffers, which "
                         "bfs.cpp also allocates per call (alloc_vertex_subset), stay with the graph"}
     log(f"BFS, reference region: {extras['bfs_reference_region']}")
     # the same roots with the depth array written as well (8 bytes per vertex

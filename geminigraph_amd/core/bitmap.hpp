@@ -9,6 +9,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include "gg.h"
+
 #define WORD_OFFSET(i) ((i) >> 6)
 #define BIT_OFFSET(i) ((i) & 0x3f)
 
@@ -92,13 +94,15 @@ class Bitmap {
   size_t size;
   unsigned long *data;  // device words, WORD_OFFSET(size)+1 of them
   Bitmap() : size(0), data(nullptr) {}
-  explicit Bitmap(size_t size) : size(size), data(nullptr) {
-    gg_facade::hip_check(hipMalloc((void **)&data, sizeof(unsigned long) * (WORD_OFFSET(size) + 1)),
-                         "Bitmap alloc");
+  explicit Bitmap(size_t size) : size(size), data(nullptr) {  // gg_malloc: kept when freed
+    if (gg_malloc((void **)&data, sizeof(unsigned long) * (WORD_OFFSET(size) + 1)) != GG_OK) {
+      fprintf(stderr, "gg: Bitmap alloc: %s\n", gg_last_error());
+      abort();
+    }
     clear();
   }
   ~Bitmap() {
-    if (data) (void)hipFree(data);
+    if (data) (void)gg_free(data);
   }
   Bitmap(const Bitmap &) = delete;
   Bitmap &operator=(const Bitmap &) = delete;

@@ -562,9 +562,12 @@ class Graph {
 
   // ---- vertex arrays (core/graph.hpp:219-324) ----
   template <typename T>
-  T *alloc_vertex_array() {
+  T *alloc_vertex_array() {  // gg_malloc: freed arrays are kept for the next call
     T *p = nullptr;
-    gg_facade::hip_check(hipMalloc((void **)&p, sizeof(T) * (size_t)vertices), "alloc_vertex_array");
+    if (gg_malloc((void **)&p, sizeof(T) * (size_t)vertices) != GG_OK) {
+      fprintf(stderr, "gg: alloc_vertex_array: %s\n", gg_last_error());
+      abort();
+    }
     return p;
   }
   template <typename T>
@@ -579,7 +582,7 @@ class Graph {
   }
   template <typename T>
   void dealloc_vertex_array(T *array) {
-    (void)hipFree(array);
+    (void)gg_free(array);
   }
   // one partition: nothing to move between ranks
   template <typename T>

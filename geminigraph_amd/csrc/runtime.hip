@@ -2,6 +2,7 @@
 // profiling and vertex subsets (device bitmaps).
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 
@@ -141,15 +142,78 @@ int gg_synchronize(void) {
   return GG_OK;
 }
 
+// Buffers of gg_malloc / gg_free (the callers' vertex arrays and subsets)
+// are cached: a freed buffer waits, per device and exact size, for the next
+// gg_malloc of that size, up to alloc_cache_bytes in all; a toolkit that
+// allocates its outputs in every call (as compute() does) then pays no
+// hipMalloc / hipFree after the first.  gg_free still waits for the work
+// queued on the device, as hipFree does, so a buffer handed out again is
+// idle.  When hipMalloc fails, the cache is released and the call retried.
+namespace {
+struct AllocCache {
+  std::mutex m;
+  std::multimap<std::pair<int, size_t>, void *> idle;  // (device, bytes) -> buffer
+  std::map<void *, std::pair<int, size_t>> owned;      // every live or idle buffer
+  size_t idle_bytes = 0;
+};
+AllocCache &alloc_cache() {
+  static AllocCache c;
+  return c;
+}
+void release_idle(AllocCache &c) {  // with c.m held
+  for (auto &kv : c.idle) {
+    c.owned.erase(kv.second);
+    (void)hipFree(kv.second);
+  }
+  c.idle.clear();
+  c.idle_bytes = 0;
+}
+}  // namespace
+
 int gg_malloc(void **d_ptr, size_t bytes) {
   if (!d_ptr) return fail(GG_EINVAL, "gg_malloc: null out pointer");
   GG_TRY(ensure_init());
-  return device_alloc(d_ptr, bytes);
+  if (bytes == 0) bytes = 256;
+  int dev = 0;
+  GG_HIP(hipGetDevice(&dev));
+  AllocCache &c = alloc_cache();
+  std::lock_guard<std::mutex> lk(c.m);
+  auto it = c.idle.find({dev, bytes});
+  if (it != c.idle.end()) {
+    *d_ptr = it->second;
+    c.idle.erase(it);
+    c.idle_bytes -= bytes;
+    return GG_OK;
+  }
+  int r = device_alloc(d_ptr, bytes);
+  if (r != GG_OK && c.idle_bytes) {
+    release_idle(c);
+    r = device_alloc(d_ptr, bytes);
+  }
+  if (r == GG_OK) c.owned[*d_ptr] = {dev, bytes};
+  return r;
 }
 
 int gg_free(void *d_ptr) {
   GG_TRY(ensure_init());
-  if (d_ptr) GG_HIP(hipFree(d_ptr));
+  if (!d_ptr) return GG_OK;
+  AllocCache &c = alloc_cache();
+  std::lock_guard<std::mutex> lk(c.m);
+  auto it = c.owned.find(d_ptr);
+  if (it == c.owned.end()) {  // not from gg_malloc
+    GG_HIP(hipFree(d_ptr));
+    return GG_OK;
+  }
+  const std::pair<int, size_t> key = it->second;
+  const size_t cap = (size_t)std::max<int64_t>(0, options().alloc_cache_bytes);
+  if (c.idle_bytes + key.second > cap) {
+    c.owned.erase(it);
+    GG_HIP(hipFree(d_ptr));
+    return GG_OK;
+  }
+  GG_HIP(hipDeviceSynchronize());  // the queued work that may use it has finished
+  c.idle.insert({key, d_ptr});
+  c.idle_bytes += key.second;
   return GG_OK;
 }
 
@@ -212,6 +276,7 @@ static int64_t *option_slot(const char *name) {
   if (!strcmp(name, "batch_device_loop")) return &o.batch_device_loop;
   if (!strcmp(name, "bfs_ahead")) return &o.bfs_ahead;
   if (!strcmp(name, "bfs_record_starts")) return &o.bfs_record_starts;
+  if (!strcmp(name, "alloc_cache_bytes")) return &o.alloc_cache_bytes;
   if (!strcmp(name, "relabel_hubs")) return &o.relabel_hubs;
   if (!strcmp(name, "serve_bfs_workers")) return &o.serve_bfs_workers;
   return nullptr;
@@ -232,7 +297,7 @@ int gg_set_option(const char *name, int64_t value) {
     return fail(GG_EINVAL, "bfs_alpha must be in [0, 2^20]");
   const bool zero_ok = !strcmp(name, "bfs_pull_waves") || !strcmp(name, "pr_hot_sources") ||
                        !strcmp(name, "bfs_alpha") || !strcmp(name, "relabel_hubs") ||
-                       !strcmp(name, "serve_bfs_workers") ||
+                       !strcmp(name, "serve_bfs_workers") || !strcmp(name, "alloc_cache_bytes") ||
                        !strcmp(name, "heavy_waves");
   if (zero_ok && value < 0) return fail(GG_EINVAL, std::string(name) + " must be >= 0");
   if (strcmp(name, "force_mode") && !zero_ok && !flag && value <= 0)

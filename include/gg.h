@@ -44,7 +44,11 @@ int gg_init(int device);
 /* Block until all work queued by the library has finished. */
 int gg_synchronize(void);
 /* Device memory helpers (alloc_vertex_array / dealloc_vertex_array,
- * core/graph.hpp:229-243; gather_vertex_array, core/graph.hpp:310-324). */
+ * core/graph.hpp:229-243; gather_vertex_array, core/graph.hpp:310-324).
+ * gg_free waits for the work queued on the device (as hipFree does) and
+ * keeps the buffer, per device and size, for the next gg_malloc of the same
+ * size (up to the "alloc_cache_bytes" option in all; hipMalloc failures
+ * release the kept buffers first).  Contents are not cleared. */
 int gg_malloc(void **d_ptr, size_t bytes);
 int gg_free(void *d_ptr);
 int gg_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
@@ -107,6 +111,8 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *                      decided one (1..8, default 1); more hides the host's
  *                      reaction time on short levels, each one past the end
  *                      costs launches that return at once
+ *   "alloc_cache_bytes" bytes of freed gg_malloc buffers kept for reuse
+ *                      (default 4 GiB; 0 = every gg_free calls hipFree)
  *   "bfs_record_starts" BFS bottom-up step: 1 = take each in-list's start
  *                      from the vertex's 16-byte record, read anyway, when
  *                      the graph has fewer than 2^32 arcs (default); 0 =

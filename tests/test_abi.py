@@ -76,6 +76,12 @@ def test_last_error_and_options_without_gpu():
     for bad in (-1, 2):
         with pytest.raises(gg.GGError):
             gg.set_option("bfs_record_starts", bad)
+    # freed gg_malloc buffers kept for reuse: bytes, 0 disables, 4 GiB by default
+    assert gg.get_option("alloc_cache_bytes") == 4 << 30
+    with gg.options(alloc_cache_bytes=0):
+        assert gg.get_option("alloc_cache_bytes") == 0
+    with pytest.raises(gg.GGError):
+        gg.set_option("alloc_cache_bytes", -1)
 
 
 @pytest.mark.parametrize("bad", [-0.5, float("nan"), -float("inf")])

@@ -106,3 +106,42 @@ def test_vertex_array_dump_restore_gather(gg, tmp_path):
         g.restore_vertex_array(small, p)
     with pytest.raises(ValueError):
         g.dump_vertex_array(gg.DeviceArray(V - 1, np.uint32), tmp_path / "x.bin")
+
+
+def test_device_arrays_reused_by_size(gg):
+    # gg_free keeps a buffer for the next gg_malloc of the same size (up to
+    # alloc_cache_bytes): the same address comes back, other sizes get their
+    # own, and a kept buffer carries new contents correctly
+    V = 1 << 16
+    a = gg.DeviceArray(V, np.uint32)
+    pa = a.ptr
+    a.upload(np.arange(V, dtype=np.uint32))
+    a.free()
+    b = gg.DeviceArray(V, np.uint32)
+    assert b.ptr == pa
+    c = gg.DeviceArray(V + 1, np.uint32)
+    assert c.ptr != pa
+    want = np.arange(V, dtype=np.uint32)[::-1].copy()
+    b.upload(want)
+    assert np.array_equal(b.numpy(), want)
+    b.free()
+    c.free()
+    with gg.options(alloc_cache_bytes=0):
+        d = gg.DeviceArray(V + 2, np.uint32)
+        d.free()  # released at once, not kept
+        e = gg.DeviceArray(V + 2, np.uint32)
+        assert np.array_equal(e.upload(np.ones(V + 2, np.uint32)).numpy(), np.ones(V + 2, np.uint32))
+        e.free()
+    # BFS into a reused output array matches BFS into a fresh one
+    pairs = [(i, i + 1) for i in range(999)] + [(0, 500)]
+    edges = oracle.edges_from_pairs(pairs)
+    g = gg.Graph.build(edges, 1000)
+    out1 = gg.DeviceArray(1000, np.uint32)
+    g.bfs_into(0, out1)
+    r1 = out1.numpy()
+    out1.free()
+    out2 = gg.DeviceArray(1000, np.uint32)
+    g.bfs_into(0, out2)
+    assert np.array_equal(out2.numpy(), r1)
+    out2.free()
+    g.close()
