@@ -211,10 +211,23 @@ int gg_free(void *d_ptr) {
     GG_HIP(hipFree(d_ptr));
     return GG_OK;
   }
-  GG_HIP(hipDeviceSynchronize());  // the queued work that may use it has finished
+  // the queued work that may use it has finished, on the buffer's device
+  int cur = key.first;
+  GG_HIP(hipGetDevice(&cur));
+  if (cur != key.first) GG_HIP(hipSetDevice(key.first));
+  const hipError_t e = hipDeviceSynchronize();
+  if (cur != key.first) GG_HIP(hipSetDevice(cur));
+  GG_HIP(e);
   c.idle.insert({key, d_ptr});
   c.idle_bytes += key.second;
   return GG_OK;
+}
+
+// A lower alloc_cache_bytes takes effect at once: kept buffers are released.
+void trim_alloc_cache() {
+  AllocCache &c = alloc_cache();
+  std::lock_guard<std::mutex> lk(c.m);
+  if (c.idle_bytes > (size_t)std::max<int64_t>(0, options().alloc_cache_bytes)) release_idle(c);
 }
 
 int gg_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes) {
@@ -315,6 +328,7 @@ int gg_set_option(const char *name, int64_t value) {
   if (!strcmp(name, "pull_light_max") && value > (1 << 20))
     return fail(GG_EINVAL, "pull_light_max must be <= 2^20");
   *s = value;
+  if (!strcmp(name, "alloc_cache_bytes")) trim_alloc_cache();
   return GG_OK;
 }
 

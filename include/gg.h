@@ -112,7 +112,9 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *                      reaction time on short levels, each one past the end
  *                      costs launches that return at once
  *   "alloc_cache_bytes" bytes of freed gg_malloc buffers kept for reuse
- *                      (default 4 GiB; 0 = every gg_free calls hipFree)
+ *                      (default 4 GiB; 0 = every gg_free calls hipFree);
+ *                      lowering it below what is kept releases the kept
+ *                      buffers at once (gg_mem_info counts them as used)
  *   "bfs_record_starts" BFS bottom-up step: 1 = take each in-list's start
  *                      from the vertex's 16-byte record, read anyway, when
  *                      the graph has fewer than 2^32 arcs (default); 0 =
