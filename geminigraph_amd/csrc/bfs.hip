@@ -35,6 +35,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <mutex>
 #include <type_traits>
@@ -180,13 +181,17 @@ __device__ __forceinline__ void bfs_claim2(const BfsOut &o, uint32_t level, bool
 }
 
 // Progress the host polls, in coherent pinned memory: the deciding thread
-// publishes each decision with a relaxed system-scope store.  The host uses
-// it only to pace its launches; the control block it reads at the end is
-// copied in stream order, so no release (an L2 write-back per level) is
-// needed.
+// publishes each decision with a relaxed system-scope store, which the host
+// uses only to pace its launches (no release, an L2 write-back, per level).
+// At the end, the deciding wave copies the control block to final_copy and
+// then publishes done with a release: gg_bfs reads its statistics there and
+// returns without copying or waiting for the launches still queued (they
+// return at once), so the next call's launches reach the GPU sooner.
+struct BfsCtl;
 struct BfsHostFlag {
   uint32_t decided;  // iterations decided so far
   uint32_t done;
+  BfsCtl *final_copy;  // pinned, coherent; null: the caller copies the block itself
 };
 
 // The push/pull switch (core/graph.hpp:1509): sparse iff the frontier's
@@ -211,7 +216,7 @@ __device__ __forceinline__ uint32_t bfs_mode(int32_t force, uint64_t threshold, 
 __device__ uint32_t bfs_decide_block(BfsCtl *c, uint32_t n, const uint64_t *set,
                                      const unsigned long long *exam, uint32_t *lv,
                                      BfsHostFlag *hf, uint32_t *level_out = nullptr) {
-  __shared__ uint32_t s_mode, s_level;
+  __shared__ uint32_t s_mode, s_level, s_finish;
   if (threadIdx.x < 64) {
     const int lane = lane_id();
     const uint32_t done = c->done;
@@ -236,6 +241,7 @@ __device__ uint32_t bfs_decide_block(BfsCtl *c, uint32_t n, const uint64_t *set,
       if (!done && cnt) mode = bfs_mode(force, threshold, alpha, edg, m_left);
       s_mode = mode;
       s_level = n + n0 + 1;
+      s_finish = blockIdx.x == 0 && !done && !cnt;
       if (blockIdx.x == 0 && !done) {
         const uint32_t i = n + n0;  // the level this iteration runs
         if (i > 0 && i - 1 < GG_STATS_RECORDS) {
@@ -246,7 +252,7 @@ __device__ uint32_t bfs_decide_block(BfsCtl *c, uint32_t n, const uint64_t *set,
         if (!cnt) {
           c->done = 1;
           c->mode = kBfsDone;
-          __hip_atomic_store(&hf->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+          // published below, after the control block's copy
         } else {
           const uint64_t reached = c->reached + cnt;
           c->reached = reached;
@@ -266,6 +272,21 @@ __device__ uint32_t bfs_decide_block(BfsCtl *c, uint32_t n, const uint64_t *set,
         }
       }
     }
+  }
+  // the end of the traversal: wave 0 of block 0 copies the final control
+  // block to host memory, then publishes done (release, system scope)
+  if (blockIdx.x == 0 && threadIdx.x < 64 && s_finish) {
+    __threadfence();  // lane 0's last records are performed before they are read
+    BfsCtl *dst = hf->final_copy;
+    if (dst) {
+      const uint64_t *from = (const uint64_t *)c;
+      uint64_t *to = (uint64_t *)dst;
+      for (size_t k = threadIdx.x; k < sizeof(BfsCtl) / 8; k += 64) to[k] = from[k];
+      __threadfence_system();
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (threadIdx.x == 0)
+      __hip_atomic_store(&hf->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   __syncthreads();
   if (level_out) *level_out = s_level;
@@ -983,7 +1004,7 @@ static int bfs_state(Work *w, uint32_t lv_need, BfsState **out) {
     GG_TRY(device_alloc((void **)&s->sets, 2 * kBfsSetWords * 8));
     GG_TRY(device_alloc((void **)&s->qc, 2 * sizeof(Counters)));
     GG_TRY(device_alloc((void **)&s->exam, 2 * kExamWords * 8));
-    GG_HIP(hipHostMalloc((void **)&s->h_ctl, sizeof(BfsCtl)));
+    GG_HIP(hipHostMalloc((void **)&s->h_ctl, sizeof(BfsCtl), hipHostMallocCoherent));
     GG_HIP(hipHostMalloc((void **)&s->h_flag, sizeof(BfsHostFlag), hipHostMallocCoherent));
   }
   if (lv_need > s->lv_cap) {
@@ -1223,7 +1244,8 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
   volatile BfsHostFlag *hf = s->h_flag;
   hf->decided = 0;
   hf->done = 0;
-  DeviceTimer tm;
+  hf->final_copy = s->h_ctl;  // the device leaves the final control block there
+  const auto t0 = std::chrono::steady_clock::now();
   bfs_issue_start(g, p, b, st);
   // iterations queued ahead of the device: before issuing iteration n, wait
   // until iteration n - ahead has been decided (iteration n - ahead - 1 has
@@ -1246,8 +1268,12 @@ extern "C" int gg_bfs(gg_graph *g, uint32_t root, uint32_t *d_parent, uint32_t *
     bfs_issue_iteration(g, p, b, n, st);
     GG_HIP(hipGetLastError());
   }
-  GG_HIP(hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, st));
-  const double ms = tm.stop();
+  // done was published (release) after the device copied the control block
+  // into h_ctl: no copy and no wait for the launches still queued, which
+  // return at once
+  std::atomic_thread_fence(std::memory_order_acquire);
+  const double ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   const BfsCtl &c = *s->h_ctl;
   if (h_frontier && cap > 0) {
     const uint32_t levels = std::min<uint32_t>(c.iter, (uint32_t)cap);

@@ -238,7 +238,9 @@ typedef struct gg_run_stats {
   uint64_t edges_examined; /* adjacency entries read by edge kernels */
   uint64_t m_reach;        /* sum of out-degrees of reached vertices */
   uint64_t reached;        /* reached vertices (BFS/SSSP/BC) or components (CC) */
-  double device_ms;        /* device time of the compute, by HIP events */
+  double device_ms;        /* device time of the compute, by HIP events
+                              (gg_bfs: host wall time of the call, until the
+                              device published the traversal's end) */
   /* per process_edges call (the first GG_STATS_RECORDS calls): */
   uint32_t modes[GG_STATS_RECORDS];          /* 0 = push (sparse), 1 = pull (dense) */
   uint32_t it_active[GG_STATS_RECORDS];      /* active vertices passed in */

@@ -110,9 +110,10 @@ def test_vertex_array_dump_restore_gather(gg, tmp_path):
 
 def test_device_arrays_reused_by_size(gg):
     # gg_free keeps a buffer for the next gg_malloc of the same size (up to
-    # alloc_cache_bytes): the same address comes back, other sizes get their
-    # own, and a kept buffer carries new contents correctly
-    V = 1 << 16
+    # alloc_cache_bytes): the same address comes back (a size no other test
+    # allocates, so no other kept buffer matches), other sizes get their own,
+    # and a kept buffer carries new contents correctly
+    V = (1 << 16) + 4321
     a = gg.DeviceArray(V, np.uint32)
     pa = a.ptr
     a.upload(np.arange(V, dtype=np.uint32))
