@@ -90,7 +90,7 @@ struct Options {
   int64_t bfs_pull_waves = 8192;
   int64_t batch_graphs = 0;      // batched BFS levels replayed from captured hipGraphs
   int64_t batch_device_loop = 1; // batched BFS/SSSP iterations decided on the device
-  int64_t bfs_ahead = 1;         // BFS: iterations the host issues ahead of the decisions
+  int64_t bfs_ahead = 2;         // BFS: iterations the host issues ahead of the decisions
   int64_t bfs_record_starts = 1; // BFS pull: in-list starts from the vertex records (arcs < 2^32)
   int64_t alloc_cache_bytes = int64_t(4) << 30;  // gg_free keeps up to this much for gg_malloc
   // relabeling (fixed at build): the relabel_hubs highest-out-degree vertices

@@ -108,7 +108,7 @@ int gg_device_copy(void *d_dst, const void *d_src, size_t bytes);
  *                      ahead without waiting (default); 0 = the host reads
  *                      the frontier counts back after every iteration
  *   "bfs_ahead"        BFS: iterations the host issues beyond the last
- *                      decided one (1..8, default 1); more hides the host's
+ *                      decided one (1..8, default 2); more hides the host's
  *                      reaction time on short levels, each one past the end
  *                      costs launches that return at once
  *   "alloc_cache_bytes" bytes of freed gg_malloc buffers kept for reuse

@@ -378,6 +378,27 @@ def test_bfs_step_grids(gg, waves):
     g.close()
 
 
+@pytest.mark.parametrize("ahead", [1, 2, 8])
+def test_bfs_issue_ahead(gg, ahead):
+    # the host queues up to `ahead` iterations past the last decided one;
+    # those issued past the end return at once and must leave the output,
+    # the level sizes and the next call's state alone (roots in a row)
+    e = oracle.rmat_edges(15, seed=33)
+    V = 1 << 15
+    o = oracle.Graph(e, V)
+    g = gg.Graph.build(e, V, reorder=True)
+    for root in roots_of(o.out_degree(), 3, 33):
+        _, rd, rf = o.bfs(root)
+        for mode in ("auto", "push", "pull"):
+            say("bfs issue ahead", ahead, root, mode)
+            with gg.options(force_mode=MODES[mode], bfs_ahead=ahead):
+                parent, depth, fr, st = g.bfs(root)
+            assert np.array_equal(depth, rd) and np.array_equal(fr, rf), (root, mode)
+            assert st["iterations"] == len(rf)
+            check_parents(e, V, root, parent, depth)
+    g.close()
+
+
 @pytest.mark.parametrize("record_starts", [1, 0])
 @pytest.mark.parametrize("reorder", [False, True])
 def test_bfs_list_starts(gg, reorder, record_starts):
